@@ -1,0 +1,130 @@
+/* zandhip: MI355X (gfx950) propagation engine for the auxiliary-mode-expansion
+ * (AME) equations of motion and their DOP54 adaptive Runge-Kutta driver.
+ *
+ * C ABI.  Complex arrays are interleaved (re, im) doubles in the reference's
+ * array layout (numpy complex128, C order):
+ *   sigma (NK,n,n)   psi (NK,NL,NX,NOT,n)   omega (NK,NL,NX,NOT,NL,NX,NOT)
+ *   mode constants (NK,NL,NX,NOT)
+ * Host pointers are borrowed for the duration of a call only; device memory is
+ * owned by the problem handle.  Every call returns 0 on success and a negative
+ * ZH_E* code on failure; zh_last_error() gives the message (thread-local).
+ * A handle is not thread-safe: the caller serialises calls on one handle;
+ * distinct handles may be used concurrently.
+ */
+#ifndef ZANDHIP_H
+#define ZANDHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZH_ABI_VERSION 1
+#define ZH_MAX_LEADS 8
+
+enum { ZH_FP64 = 0, ZH_MIXED = 1 };
+enum { ZH_OK = 0, ZH_EINVAL = -1, ZH_EHIP = -2, ZH_ENOMEM = -3, ZH_ESTATE = -4, ZH_ENONFINITE = -5,
+       ZH_ELIMIT = -6 };
+/* bias waveform kinds (Zandpack/Pulses.py:52-94 shapes) */
+enum { ZH_PULSE_NONE = 0, ZH_PULSE_STEP = 1, ZH_PULSE_GAUSS = 2, ZH_PULSE_RAMP = 3, ZH_PULSE_SIN = 4,
+       ZH_PULSE_BOX = 5 };
+/* time bookkeeping after an accepted step */
+enum { ZH_TIME_REFERENCE = 0, /* t += h_next, as Zandpack/mpi/zand:406-407,427 */
+       ZH_TIME_CORRECT = 1 }; /* t += h of the step just taken */
+
+typedef struct zh_problem zh_problem;
+
+typedef struct {
+  int NK, n, NL, NX, NOT; /* k points (or spin), orbitals, leads, pole slots, channels */
+  int precision;          /* ZH_FP64 or ZH_MIXED */
+  double hbar;            /* eV*fs; Zandpack/td_constants.py:3 uses 0.6582119569 */
+  int device;             /* HIP device ordinal */
+} zh_dims;
+
+typedef struct {
+  double t0, h, eps; /* in: step start, step size, absolute error tolerance */
+  int force_accept;  /* in: 1 = accept regardless of TE (fixed-step / replay) */
+  double te;         /* out: sqrt(sum |sum_j CT_j k_j|^2) over sigma, Psi, Omega */
+  double h_next;     /* out: 0.9 h (eps/TE)^(1/5) */
+  int accepted;      /* out */
+} zh_attempt_info;
+
+/* Version / diagnostics. */
+int zh_abi_version(void);
+const char* zh_last_error(void);
+
+/* Replaces the problem set-up done by Zandpack/mpi/zand:113-132, 568-678
+ * (array shapes, NOT cut, orbital pruning, constant products). */
+int zh_problem_create(const zh_dims* dims, zh_problem** out);
+void zh_problem_destroy(zh_problem* p);
+
+/* Constants, reference layout, complex128 interleaved unless noted:
+ *   H0 (NK,n,n); xi, Ixi (NK,NL,NX,NOT,n); GL_P, dGP=GG_P-GL_P, dGM=GG_M-GL_M,
+ *   Xpp, Xpm (NK,NL,NX,NOT).
+ * Replaces the Arrays/ loads of Zandpack/mpi/zand:218, 568-576. */
+int zh_set_constants(zh_problem* p, const double* H0, const double* xi, const double* Ixi,
+                     const double* GL_P, const double* dGP, const double* dGM, const double* Xpp,
+                     const double* Xpm);
+
+/* Device-evaluated H(t, sigma) = H0 + U diag(sigma[partner]_ii - n0[partner]_i) for NK = 2
+ * spin blocks (examples/Hubbard/Calculation/Bias.py:13-20).  U = 0 disables it.
+ * n0 is (NK, n) real. */
+int zh_set_hubbard(zh_problem* p, double U, const double* n0);
+
+/* Device-evaluated bias Delta_a(t) = weights[a] * V(t) (Bias.bias(t, a) hook,
+ * Zandpack/mpi/zand:320-321).  kind: ZH_PULSE_*; amp, p1, p2 per kind:
+ *   STEP amp for t >= p1; GAUSS amp exp(-((t-p1)/p2)^2); RAMP amp (1 - exp(-t/p1));
+ *   SIN amp sin(p1 t / hbar); BOX amp (tanh(t/p2) - tanh((t-p1)/p2))/2. */
+int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, const double* weights);
+
+/* Resident state y0 = (sigma, Psi, Omega), reference layout, complex128. */
+int zh_state_put(zh_problem* p, const double* sig, const double* psi, const double* omg);
+int zh_state_get(zh_problem* p, double* sig, double* psi, double* omg);
+int zh_sigma_get(zh_problem* p, double* sig);
+
+/* One evaluation of the equations of motion, the reference's
+ * f(t, sig, psi, omg, dH, delta, dH_given) (Zandpack/TimedependentTransport.py:2605-2660,
+ * with the zand formulation Zandpack/mpi/zand:340-387, 746-809).  Ht is the full
+ * Hamiltonian at this time (NK,n,n), delta the per-lead bias (NL).  Outputs are
+ * the time derivatives; J (NL,NK) receives the lead currents of the input state
+ * (Zandpack/mpi/zand:368-375) when non-NULL.  The resident state is untouched. */
+int zh_rhs(zh_problem* p, const double* Ht, const double* delta, const double* sig,
+           const double* psi, const double* omg, double* dsig, double* dpsi, double* domg,
+           double* J);
+
+/* One DOP54 attempt from the resident state (Zandpack/mpi/zand:301-408):
+ * seven stages with device-evaluated H(t,sigma) and bias, the embedded error
+ * norm and the step controller, all on the device.  When accepted the new
+ * state replaces the resident one.  J (NL,NK) receives the stage-0 currents. */
+int zh_attempt(zh_problem* p, zh_attempt_info* info, double* J);
+
+/* Adaptive propagation loop (Zandpack/mpi/zand:297-431) from t0 while t < t1.
+ * Writes up to cap accepted steps: times[i] (time after step i), J (cap,NL,NK)
+ * stage-0 currents of step i, h_used[i].  *n_steps / *n_rejected are outputs.
+ * Stops with ZH_ELIMIT after max_attempts attempts and ZH_ENONFINITE on a
+ * non-finite error estimate. */
+int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int time_update,
+                 long max_attempts, long cap, double* times, double* J, double* h_used,
+                 long* n_steps, long* n_rejected, double* h_final);
+
+/* Use hipGraphs for zh_attempt / zh_propagate (1, default) or direct launches (0). */
+int zh_set_graph(zh_problem* p, int enable);
+
+/* Per-kernel-class HIP event timing on the problem's stream (1) or off (0, default).
+ * While on, attempts run without graphs. */
+int zh_set_profile(zh_problem* p, int enable);
+
+/* Average device time per attempt and per kernel class, measured with HIP
+ * events on the problem's stream since the last reset (ms).  names/ms get up
+ * to cap entries; returns the number of entries. */
+int zh_timing(zh_problem* p, int reset, int cap, char* names, double* ms, long* counts);
+
+/* Building-block check: C = op(A) op(B) (complex128 in/out, host pointers) with
+ * the engine's MFMA GEMM in FP64 (precision ZH_FP64) or FP32 (ZH_MIXED).
+ * A is (M,K) or (K,M) when transA, B is (K,N) or (N,K) when transB. */
+int zh_selftest_gemm(int precision, int M, int N, int K, int transA, int transB, const double* A,
+                     const double* B, double* C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

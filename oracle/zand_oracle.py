@@ -1,0 +1,413 @@
+"""CPU FP64 oracle for the auxiliary-mode-expansion (AME) propagation hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in ``zandpack_amd`` imports this module; only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` use it, and only as the checker / the timed CPU baseline.
+
+It is a plain numpy (complex128) restatement of what the reference computes in
+its orthogonal MPI propagator, written as one process:
+
+* σ master path            -- Zandpack/mpi/zand:300-408 (``SIGMA``)
+* Ψ/Ω worker path          -- Zandpack/mpi/zand:697-821 (``Worker``)
+* Π/Q, currents            -- Zandpack/mpi/zand:91-110, 363-375
+* RK helpers               -- Zandpack/mpi_funcs.py:23-25, 91-102, 247-259
+* DOP54 tables, controller -- Zandpack/mpi_RK4pars_dev.py:118-149
+* loop form of the RHS     -- Zandpack/TimedependentTransport.py:2798-2873
+* Hu poles / Fermi expansion -- Zandpack/PadeDecomp.py:36-42, 84-152
+
+Array layout is the reference's (per k):
+``sigma (NK,n,n)``, ``psi (NK,NL,NX,NOT,n)``, ``omega (NK,NL,NX,NOT,NL,NX,NOT)``,
+mode constants ``(NK,NL,NX,NOT)``.
+
+Pinning (what makes this oracle trustworthy; see tests/test_oracle_*.py):
+
+* ``dop54_tableau``   -- exact Dormand-Prince rationals, order conditions, row sums.
+* ``hu_poles``        -- reproduces the reference's golden accuracy table
+                         ``Pade_data/JieHu2011_accuracy_table.npz``.
+* ``rhs``             -- agrees with the independent loop form ``rhs_loop``;
+                         invariants: dσ Hermitian, Tr dσ = Σ_α J_α, the Fermi x Fermi
+                         block of dΩ vanishes, von-Neumann closed form when ξ=0,
+                         closed-form stationary Ω makes dΩ=0.
+* ``propagate``       -- von-Neumann closed form σ(t)=U σ0 U† to the DOP54 error level.
+"""
+from __future__ import annotations
+
+from fractions import Fraction as Fr
+
+import numpy as np
+
+# Zandpack/td_constants.py:3
+HBAR = 6.582119569e-1  # eV * fs
+
+
+# ---------------------------------------------------------------------------
+# DOP54 tableau and controller -- Zandpack/mpi_RK4pars_dev.py:118-149
+# ---------------------------------------------------------------------------
+def dop54_rationals():
+    """Exact Dormand-Prince 5(4) coefficients (mpi_RK4pars_dev.py:123-132)."""
+    c = [Fr(0), Fr(1, 5), Fr(3, 10), Fr(4, 5), Fr(8, 9), Fr(1), Fr(1)]
+    a = [
+        [],
+        [Fr(1, 5)],
+        [Fr(3, 40), Fr(9, 40)],
+        [Fr(44, 45), Fr(-56, 15), Fr(32, 9)],
+        [Fr(19372, 6561), Fr(-25360, 2187), Fr(64448, 6561), Fr(-212, 729)],
+        [Fr(9017, 3168), Fr(-355, 33), Fr(46732, 5247), Fr(49, 176), Fr(-5103, 18656)],
+        [Fr(35, 384), Fr(0), Fr(500, 1113), Fr(125, 192), Fr(-2187, 6784), Fr(11, 84)],
+    ]
+    b5 = [Fr(35, 384), Fr(0), Fr(500, 1113), Fr(125, 192), Fr(-2187, 6784), Fr(11, 84), Fr(0)]
+    b4 = [Fr(5179, 57600), Fr(0), Fr(7571, 16695), Fr(393, 640), Fr(-92097, 339200),
+          Fr(187, 2100), Fr(1, 40)]
+    return c, a, b5, b4
+
+
+def dop54_tableau():
+    """Float tableau in the reference's slot convention.
+
+    Slots are ``[k1..k7, y0]``; ``B`` is 7x8 with ``B[i,7] = 1`` for i>=1
+    (mpi_RK4pars_dev.py:134-137), ``CT = b5-b4`` (:138), ``CH = [b5, 1]`` (:139-140).
+    """
+    c, a, b5, b4 = dop54_rationals()
+    A = np.array([float(x) for x in c])
+    B = np.zeros((7, 8))
+    for i in range(1, 7):
+        for j, v in enumerate(a[i]):
+            B[i, j] = float(v)
+        B[i, 7] = 1.0
+    b5f = np.array([float(x) for x in b5])
+    b4f = np.array([float(x) for x in b4])
+    CT = b5f - b4f
+    CH = np.hstack((b5f, [1.0]))
+    return dict(A=A, B=B, CT=CT, CH=CH, b5=b5f, b4=b4f, N=7)
+
+
+def hnew(h, te, eps):
+    """Step update, mpi_RK4pars_dev.py:148-149 (no clamp)."""
+    return 0.9 * h * ((eps / te) ** 0.2)
+
+
+# ---------------------------------------------------------------------------
+# Fermi-function pole expansion -- Zandpack/PadeDecomp.py:36-42, 84-152
+# ---------------------------------------------------------------------------
+def _hu_tridiag(m):
+    """Hu et al. 2011 matrix: off-diagonal 1/sqrt(b_i b_j), b_i = 2i-1 (PadeDecomp.py:90-99)."""
+    b = 2.0 * np.arange(1, m + 1) - 1.0
+    off = 1.0 / np.sqrt(b[:-1] * b[1:])
+    return np.diag(off, 1) + np.diag(off, -1)
+
+
+def hu_poles(N):
+    """Poles z_p (i * positive real) and residues R_p of the N-pole Hu expansion.
+
+    Restates PadeDecomp.py:101-127,151-152: roots q = 2/λ from the positive
+    eigenvalues λ of the 2N tridiagonal matrix (complex Hermitian eigh, as the
+    reference), the auxiliary roots from its trailing (2N-1) block, and the
+    residues from the product formula.  For N <= 40 the products are float64
+    (PadeDecomp.py:120-126); for N > 40 the reference switches to sympy
+    (:115-116, 131-147), restated here as exact rational products of the same
+    float64 roots.
+    """
+    if N == 0:
+        return np.zeros(0, complex), np.zeros(0)
+    m = 2 * N
+    T = _hu_tridiag(m).astype(complex)
+    eq = np.linalg.eigh(T)[0]
+    Q = 2.0 / np.sort(eq[eq > 1e-15])[::-1]
+    ep = np.linalg.eigh(T[1:, 1:])[0]
+    P = 2.0 / ep[ep > 1e-15]
+    const = N * (2 * (N + 1) - 1) / 2.0
+    R = np.empty(N)
+    if N <= 40:
+        for i in range(N):
+            p1 = Q ** 2 - Q[i] ** 2
+            p1[np.abs(p1) < 1e-15] = 1.0
+            R[i] = const * np.prod(P ** 2 - Q[i] ** 2) / np.prod(p1)
+    else:
+        Qf = [Fr(float(q)) for q in Q]
+        Pf = [Fr(float(p)) for p in P]
+        cf = Fr(N * (2 * (N + 1) - 1), 2)
+        for i in range(N):
+            p1 = Fr(1)
+            for j in range(N):
+                if j != i:
+                    p1 *= Qf[j] ** 2 - Qf[i] ** 2
+            p2 = Fr(1)
+            for p in Pf:
+                p2 *= p ** 2 - Qf[i] ** 2
+            R[i] = float(cf * p2 / p1)
+    return 1j * Q, R
+
+
+def fd_expanded(E, zi, R, beta, mu=0.0):
+    """f(E) ~ 1/2 - (1/β) Σ_p R_p [1/(E-μ-z_p/β) + 1/(E-μ+z_p/β)]  (PadeDecomp.py:36-42)."""
+    E = np.asarray(E, dtype=complex)
+    xpp = mu + zi / beta
+    xpm = mu - zi / beta
+    d = (1.0 / beta) * (1.0 / np.subtract.outer(E, xpp) + 1.0 / np.subtract.outer(E, xpm)) * R
+    return 0.5 - d.sum(axis=-1)
+
+
+# ---------------------------------------------------------------------------
+# Right-hand side (zand master + worker) -- one RK stage, k = h * f
+# ---------------------------------------------------------------------------
+def _flat(consts):
+    xi = consts["xi"]
+    NK, NL, NX, NOT, n = xi.shape
+    M = NL * NX * NOT
+    return NK, NL, NX, NOT, n, M
+
+
+def lead_of_mode(NL, NX, NOT):
+    return np.repeat(np.arange(NL), NX * NOT)
+
+
+def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False):
+    """One stage of zand: returns (h*dσ/dt, h*dΨ/dt, h*dΩ/dt).
+
+    σ: mpi/zand:340-387; Ψ: mpi/zand:746-778; Ω: mpi/zand:784-809.
+    ``Ht`` is the full Hamiltonian at the stage (H0 + dH(ts, σ)); ``delta`` is
+    Δ_α(ts) per lead.  Dense in the orbital index (no pruning): the reference's
+    pruning drops only orbitals where ξ is zero, which leaves the sums unchanged.
+    """
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    xi = consts["xi"].reshape(NK, M, n)
+    Ixi = consts["Ixi"].reshape(NK, M, n)
+    glp = consts["GL_P"].reshape(NK, M)
+    dgp = consts["dGP"].reshape(NK, M)
+    dgm = consts["dGM"].reshape(NK, M)
+    lead = lead_of_mode(NL, NX, NOT)
+    d_mode = np.asarray(delta, float)[lead]
+    xpp = consts["Xpp"].reshape(NK, M) + d_mode[None, :]
+    xpm = consts["Xpm"].reshape(NK, M) + d_mode[None, :]
+    p = psi.reshape(NK, M, n)
+    W = omg.reshape(NK, M, M)
+
+    # σ: D = Hσ; D -= D†; D *= -i  (mpi/zand:340-342)
+    D = Ht @ sig
+    D = -1j * (D - D.conj().transpose(0, 2, 1))
+    # Q_α = (Ψ_αᵀ Ixi_α + h.c.)/ħ  (mpi/zand:91-110)
+    Q = np.zeros((NK, NL, n, n), complex)
+    for a in range(NL):
+        sl = slice(a * NX * NOT, (a + 1) * NX * NOT)
+        r = p[:, sl, :].transpose(0, 2, 1) @ Ixi[:, sl, :]
+        Q[:, a] = (r + r.conj().transpose(0, 2, 1)) / hbar
+    dsig = (h / hbar) * (D + Q.sum(axis=1))
+
+    # Ψ (mpi/zand:746-778)
+    dpsi = p @ (Ht / hbar).transpose(0, 2, 1)
+    dpsi -= (xpp / hbar)[..., None] * p
+    dpsi += glp[..., None] * xi
+    dpsi += dgp[..., None] * (xi @ sig.transpose(0, 2, 1))
+    dpsi += W @ (xi / hbar ** 2)
+    dpsi *= -1j * h
+
+    # Ω (mpi/zand:784-809)
+    domg = h * (p @ (dgm[..., None] * Ixi).transpose(0, 2, 1))            # ΔM_j (ψ_i·Ixi_j)
+    domg += h * ((dgp[..., None] * xi) @ p.conj().transpose(0, 2, 1))     # ΔP_i (ξ_i·ψ_j*)
+    domg += (1j * h / hbar) * (xpp[:, :, None] - xpm[:, None, :]) * W    # outer subtraction
+    out = (dsig, dpsi.reshape(psi.shape), domg.reshape(omg.shape))
+    if return_pi:
+        return out, Q
+    return out
+
+
+def rhs_loop(consts, sig, psi, omg, Ht, delta, hbar=HBAR):
+    """Brute-force loop form (TimedependentTransport.py:2816-2873, with the Ixi
+    convention of zand).  Returns the derivatives (h = 1).  Tiny shapes only."""
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    xi = consts["xi"]
+    Ixi = consts["Ixi"]
+    dsig = np.zeros_like(sig)
+    dpsi = np.zeros_like(psi)
+    domg = np.zeros_like(omg)
+    for k in range(NK):
+        H = Ht[k]
+        s = sig[k]
+        for i in range(n):
+            for j in range(n):
+                acc = 0j
+                for o in range(n):
+                    acc += H[i, o] * s[o, j] - s[i, o] * H[o, j]
+                dsig[k, i, j] = -1j * acc
+        for a in range(NL):
+            pi = np.zeros((n, n), complex)
+            for x in range(NX):
+                for c in range(NOT):
+                    for i in range(n):
+                        for j in range(n):
+                            pi[i, j] += psi[k, a, x, c, i] * Ixi[k, a, x, c, j]
+            pi /= hbar
+            dsig[k] += pi + pi.conj().T
+        dsig[k] /= hbar
+        for a in range(NL):
+            for x in range(NX):
+                for c in range(NOT):
+                    xp = consts["Xpp"][k, a, x, c] + delta[a]
+                    v = np.zeros(n, complex)
+                    for i in range(n):
+                        acc = 0j
+                        for o in range(n):
+                            acc += H[i, o] * psi[k, a, x, c, o]
+                        acc -= xp * psi[k, a, x, c, i]
+                        v[i] = acc / hbar
+                        v[i] += consts["GL_P"][k, a, x, c] * xi[k, a, x, c, i]
+                        ss = 0j
+                        for o in range(n):
+                            ss += s[i, o] * xi[k, a, x, c, o]
+                        v[i] += consts["dGP"][k, a, x, c] * ss
+                        ow = 0j
+                        for a2 in range(NL):
+                            for x2 in range(NX):
+                                for c2 in range(NOT):
+                                    ow += omg[k, a, x, c, a2, x2, c2] * xi[k, a2, x2, c2, i]
+                        v[i] += ow / hbar ** 2
+                    dpsi[k, a, x, c] = -1j * v
+                    for a2 in range(NL):
+                        for x2 in range(NX):
+                            for c2 in range(NOT):
+                                xm = consts["Xpm"][k, a2, x2, c2] + delta[a2]
+                                t1 = 0j
+                                t2 = 0j
+                                for o in range(n):
+                                    t1 += Ixi[k, a2, x2, c2, o] * psi[k, a, x, c, o]
+                                    t2 += np.conj(psi[k, a2, x2, c2, o]) * xi[k, a, x, c, o]
+                                domg[k, a, x, c, a2, x2, c2] = (
+                                    -1j * (xm - xp) * omg[k, a, x, c, a2, x2, c2] / hbar
+                                    + consts["dGM"][k, a2, x2, c2] * t1
+                                    + consts["dGP"][k, a, x, c] * t2)
+    return dsig, dpsi, domg
+
+
+def currents(consts, psi, hbar=HBAR):
+    """J_α[k] = Tr(Q_α[k]).real/ħ = 2 Re Tr Π_α / ħ²  (mpi/zand:368-375)."""
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    J = np.zeros((NL, NK))
+    for a in range(NL):
+        tr = np.einsum("kxco,kxco->k", psi[:, a], consts["Ixi"][:, a])
+        J[a] = 2.0 * tr.real / hbar ** 2
+    return J
+
+
+# ---------------------------------------------------------------------------
+# Hamiltonian / bias models evaluated per stage
+# ---------------------------------------------------------------------------
+def hubbard_dH(U, n0):
+    """Mean-field Hubbard dH (examples/Hubbard/Calculation/Bias.py:13-20):
+    dH[0,ii] = U(σ[1,ii]-n0[1,i]); dH[1,ii] = U(σ[0,ii]-n0[0,i])."""
+    n0 = np.asarray(n0)
+
+    def dH(t, sig):
+        no = sig.shape[-1]
+        out = np.zeros(sig.shape, complex)
+        idx = np.arange(no)
+        out[0, idx, idx] = U * (sig[1, idx, idx] - n0[1])
+        out[1, idx, idx] = U * (sig[0, idx, idx] - n0[0])
+        return out
+
+    return dH
+
+
+def zero_dH(t, sig):
+    return np.zeros(sig.shape, complex)
+
+
+# ---------------------------------------------------------------------------
+# DOP54 attempt / propagation with zand's control semantics
+# ---------------------------------------------------------------------------
+def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None):
+    """One 7-stage attempt (mpi/zand:301-396).  Returns (ks, TE, J_stage0, y_new).
+
+    ``y0`` = (σ, Ψ, Ω); ``ks[i]`` are the h-scaled stage derivatives.  J is taken
+    from the stage-0 state (mpi/zand:368-375).  ``y_new`` = y0 + Σ b5_j k_j
+    (step_fourth_*, mpi_funcs.py:254-259).
+    """
+    tab = tab or dop54_tableau()
+    A, B, CT = tab["A"], tab["B"], tab["CT"]
+    ks = []
+    J0 = None
+    for i in range(7):
+        ts = t0 + A[i] * h
+        yi = []
+        for part in range(3):
+            v = y0[part].copy()
+            for j in range(i):
+                v = v + B[i, j] * ks[j][part]
+            yi.append(v)
+        Ht = H0 + dH(ts, yi[0])
+        delta = np.array([bias(ts, a) for a in range(NL)], float)
+        k = rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar)
+        if i == 0:
+            J0 = currents(consts, yi[1], hbar=hbar)
+        ks.append(k)
+    te2 = 0.0
+    for part in range(3):
+        e = sum(CT[j] * ks[j][part] for j in range(7))
+        te2 += float(np.sum(e.real ** 2 + e.imag ** 2))
+    TE = te2 ** 0.5
+    b5 = tab["b5"]
+    y_new = tuple(y0[p] + sum(b5[j] * ks[j][p] for j in range(7)) for p in range(3))
+    return ks, TE, J0, y_new
+
+
+def propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
+              time_update="reference", max_attempts=100000, fixed_h=False):
+    """zand's outer loop (mpi/zand:297-431).
+
+    Accept iff TE < eps; h <- 0.9 h (eps/TE)^0.2 after every attempt; on accept
+    t advances by the *new* h when ``time_update='reference'`` (mpi/zand:406-407,427),
+    by the step just taken when ``'correct'``.  ``fixed_h`` accepts every attempt
+    and keeps h (the reference's ``fixed_mode``).
+    Returns dict(times, sigma, J (list of (NL,NK)), h_accepted, TE, n_reject).
+    """
+    tab = dop54_tableau()
+    y = tuple(np.array(v, complex) for v in y0)
+    times = [t0]
+    sigmas = [y[0].copy()]
+    Js, hs, tes = [], [], []
+    n_rej = 0
+    attempts = 0
+    t = t0
+    while t < t1:
+        while True:
+            attempts += 1
+            if attempts > max_attempts:
+                raise RuntimeError("attempt limit reached")
+            ks, TE, J0, y_new = dop54_attempt(consts, y, t, h, H0, dH, bias, NL, hbar, tab)
+            if not np.isfinite(TE):
+                raise FloatingPointError("non-finite error estimate")
+            accept = fixed_h or (TE < eps)
+            h_old = h
+            if not fixed_h:
+                h = hnew(h, TE, eps)
+            if accept:
+                break
+            n_rej += 1
+        y = y_new
+        Js.append(J0)
+        hs.append(h_old)
+        tes.append(TE)
+        t = t + (h if time_update == "reference" else h_old)
+        times.append(t)
+        sigmas.append(y[0].copy())
+    return dict(times=np.array(times), sigma=np.array(sigmas), J=np.array(Js),
+                h_accepted=np.array(hs), TE=np.array(tes), n_reject=n_rej, y=y)
+
+
+# ---------------------------------------------------------------------------
+# Closed forms used to pin the RHS
+# ---------------------------------------------------------------------------
+def stationary_omega(consts, psi, delta, hbar=HBAR):
+    """Ω that makes dΩ = 0 for a fixed Ψ (cmdtools/psinought:447-449 form):
+    Ω_ij = -(ΔM_j(ψ_i·Ixi_j) + ΔP_i(ξ_i·ψ_j*)) * iħ/(χ⁻_j - χ⁺_i)  (with Δ shifts)."""
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    lead = lead_of_mode(NL, NX, NOT)
+    d_mode = np.asarray(delta, float)[lead]
+    xpp = consts["Xpp"].reshape(NK, M) + d_mode
+    xpm = consts["Xpm"].reshape(NK, M) + d_mode
+    p = psi.reshape(NK, M, n)
+    xi = consts["xi"].reshape(NK, M, n)
+    Ixi = consts["Ixi"].reshape(NK, M, n)
+    src = (p @ (consts["dGM"].reshape(NK, M)[..., None] * Ixi).transpose(0, 2, 1)
+           + (consts["dGP"].reshape(NK, M)[..., None] * xi) @ p.conj().transpose(0, 2, 1))
+    om = -src * 1j * hbar / (xpm[:, None, :] - xpp[:, :, None])
+    return om.reshape(NK, NL, NX, NOT, NL, NX, NOT)

@@ -1,0 +1,205 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle on the
+same seeded inputs.  FP64 bar: 1e-12 relative (max-norm) for one evaluation,
+1e-10 for propagated currents; mixed precision (complex64 Ψ/Ω, FP32 MFMA) bar:
+1e-5 for one evaluation, 1e-6 relative on J(t) over a propagation."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import zand_oracle as O
+from zandpack_amd import builder
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _rand_state(p, seed, scale=0.1):
+    rng = np.random.default_rng(seed)
+    NK, n, NL, NX, NOT = p.dims
+    psi = scale * (rng.normal(size=p.xi.shape) + 1j * rng.normal(size=p.xi.shape))
+    shp = (NK, NL, NX, NOT, NL, NX, NOT)
+    omg = scale * (rng.normal(size=shp) + 1j * rng.normal(size=shp))
+    A = rng.normal(size=(NK, n, n)) + 1j * rng.normal(size=(NK, n, n))
+    sig = p.sigma0 + 0.05 * (A + A.conj().transpose(0, 2, 1))
+    return sig, psi, omg
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 3e-6)])
+@pytest.mark.parametrize("shape", [(37, 53, 29), (64, 64, 16), (130, 70, 200), (5, 3, 1)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_building_block(native, precision, tol, shape, ta, tb):
+    from zandpack_amd.engine import selftest_gemm
+    M, N, K = shape
+    rng = np.random.default_rng(M * 1000 + N * 10 + K)
+    A = rng.normal(size=(M, K)) + 1j * rng.normal(size=(M, K))
+    B = rng.normal(size=(K, N)) + 1j * rng.normal(size=(K, N))
+    say("gemm", precision, shape, ta, tb)
+    C = selftest_gemm(A.T.copy() if ta else A, B.T.copy() if tb else B, bool(ta), bool(tb),
+                      precision)
+    ref = A @ B
+    assert np.abs(C - ref).max() <= tol * np.sqrt(K) * np.abs(A).max() * np.abs(B).max() * 4
+
+
+CASES = {
+    "small": lambda: builder.config_small(seed=1),
+    "small_spin_hubbard": lambda: builder.config_small(NK=2, hubbard_U=2.0, seed=2),
+    "chain20": lambda: builder.config_chain(),
+    "wide": lambda: builder.config_small(n_w=6, n_l=7, N_L=5, slots=6, NOT=3, seed=4),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_rhs_parity(native, case, precision):
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    sig, psi, omg = _rand_state(p, 7)
+    Ht = p.H0 + p.dH(0.2, sig)
+    delta = np.array([p.bias(0.2, a) for a in range(2)]) + np.array([0.05, -0.03])
+    ref = O.rhs(p.consts(), sig, psi, omg, Ht, delta)
+    Jref = O.currents(p.consts(), psi)
+    say("rhs", case, precision, p.dims)
+    prop = Propagator(p, precision)
+    ds, dp, do, J = prop.rhs(Ht, delta, sig, psi, omg)
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    for got, want in zip((ds, dp, do), ref):
+        assert rel(got, want) < tol, (case, precision, rel(got, want))
+    assert rel(J, Jref) < (1e-12 if precision == "fp64" else 1e-5)
+    prop.close()
+
+
+@pytest.mark.parametrize("case", ["small", "small_spin_hubbard", "chain20"])
+def test_attempt_parity(native, case):
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    y0 = _rand_state(p, 11, scale=0.02)
+    say("attempt", case)
+    prop = Propagator(p, "fp64", state=y0)
+    t0, h, eps = 0.3, 0.02, 1e-3
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2)
+    r = prop.attempt(t0, h, eps)
+    assert abs(r["te"] - TE) <= 1e-10 * TE
+    assert r["accepted"] == (TE < eps)
+    np.testing.assert_allclose(r["h_next"], O.hnew(h, TE, eps), rtol=1e-10)
+    assert rel(r["J"], J0) < 1e-12
+    s, ps, om = prop.get_state()
+    want = y_new if r["accepted"] else y0
+    for got, w in zip((s, ps, om), want):
+        assert rel(got, w) < 1e-12
+    prop.close()
+
+
+def test_attempt_reject_keeps_state(native):
+    from zandpack_amd.engine import Propagator
+    p = CASES["small"]()
+    y0 = _rand_state(p, 12, scale=0.02)
+    prop = Propagator(p, "fp64", state=y0)
+    r = prop.attempt(0.0, 0.05, 1e-14)
+    assert not r["accepted"]
+    s, ps, om = prop.get_state()
+    assert rel(s, y0[0]) == 0 and rel(ps, y0[1]) == 0 and rel(om, y0[2]) == 0
+    prop.close()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_propagate_chain20(native, graph):
+    """Config 1 (n=20, M=96): adaptive DOP54 with zand's control semantics, 0 -> 3 fs."""
+    from zandpack_amd.engine import Propagator
+    p = builder.config_chain()
+    y0 = p.zero_state()
+    say("propagate chain20 graph", graph)
+    ref = O.propagate(p.consts(), y0, 0.0, 3.0, 0.01, 1e-8, p.H0, p.dH, p.bias, 2)
+    prop = Propagator(p, "fp64", graph=graph)
+    out = prop.propagate(0.0, 3.0, 0.01, 1e-8)
+    assert out["n_steps"] == len(ref["times"]) - 1
+    assert out["n_rejected"] == ref["n_reject"]
+    np.testing.assert_allclose(out["times"], ref["times"], rtol=1e-10)
+    np.testing.assert_allclose(out["h"], ref["h_accepted"], rtol=1e-10)
+    assert rel(out["J"], ref["J"]) < 1e-10
+    assert rel(prop.sigma(), ref["y"][0]) < 1e-10
+    prop.close()
+
+
+def test_graph_replays_match_direct(native):
+    """hipGraph replays (both parities, several times) equal direct launches bit for bit."""
+    from zandpack_amd.engine import Propagator
+    p = CASES["small_spin_hubbard"]()
+    y0 = _rand_state(p, 21, scale=0.02)
+    res = []
+    for graph in (True, False):
+        prop = Propagator(p, "fp64", graph=graph, state=y0)
+        Js = [prop.attempt(0.1 * i, 0.01, 1.0)["J"] for i in range(5)]
+        res.append((np.array(Js), prop.get_state()))
+        prop.close()
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    for a, b in zip(res[0][1], res[1][1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_drop_in_f_and_adaptive_dop(native):
+    """The reference call surface: make_f + AdaptiveDOP vs the oracle driver."""
+    from zandpack_amd.drop_in import AdaptiveDOP, make_f
+    from zandpack_amd.engine import Propagator
+    p = CASES["small_spin_hubbard"]()
+    prop = Propagator(p, "fp64")
+    f = make_f(prop)
+    y0 = p.zero_state()
+    times, data = AdaptiveDOP(f, *y0, 1e-7, 0.0, 0.3, p.dH, p.bias, p.Ixi, h_guess=0.01)
+    ref = O.propagate(p.consts(), y0, 0.0, 0.3, 0.01, 1e-7, p.H0, p.dH, p.bias, 2)
+    np.testing.assert_allclose(times, ref["times"][:-1], rtol=1e-10)
+    J = np.array([data["current_left"], data["current_right"]]).transpose(1, 0, 2)
+    assert rel(J, ref["J"]) < 1e-10
+    assert rel(data["state"][0], ref["y"][0]) < 1e-10
+    prop.close()
+
+
+def test_mixed_precision_currents(native):
+    """Mixed precision follows the FP64 run's accepted (t, h) sequence (replay
+    mode); J(t) relative error (normalised by max |J|) stays below 1e-6."""
+    from zandpack_amd.engine import Propagator
+    p = builder.config_small(n_w=6, n_l=7, N_L=5, slots=6, NOT=3, seed=4)
+    say("mixed replay")
+    a = Propagator(p, "fp64")
+    out = a.propagate(0.0, 2.0, 0.01, 1e-7)
+    b = Propagator(p, "mixed")
+    t = 0.0
+    Jm = []
+    for h in out["h"]:
+        r = b.attempt(t, h, 1.0, force_accept=True)
+        Jm.append(r["J"])
+        t += h
+    Jm = np.array(Jm)
+    err = np.abs(Jm - out["J"]).max() / np.abs(out["J"]).max()
+    say("mixed J rel err", err)
+    assert err < 1e-6
+    a.close()
+    b.close()
+
+
+def test_c2_size_properties(native):
+    """Config 2 shapes (n=400, M=1600): one evaluation against the oracle, plus
+    the size-independent invariants (dσ Hermitian, Tr dσ = Σ J)."""
+    from zandpack_amd.engine import Propagator
+    p = builder.config_c2()
+    sig, psi, omg = _rand_state(p, 3, scale=0.01)
+    Ht = p.H0.copy()
+    delta = np.array([0.25, -0.25])
+    say("c2 rhs", p.dims, p.M)
+    prop = Propagator(p, "fp64")
+    ds, dp, do, J = prop.rhs(Ht, delta, sig, psi, omg)
+    ref = O.rhs(p.consts(), sig, psi, omg, Ht, delta)
+    for got, want in zip((ds, dp, do), ref):
+        assert rel(got, want) < 1e-12
+    np.testing.assert_allclose(ds, ds.conj().transpose(0, 2, 1), atol=1e-12 * np.abs(ds).max())
+    tr = np.trace(ds, axis1=1, axis2=2).real
+    np.testing.assert_allclose(tr, J.sum(axis=0), rtol=1e-10, atol=1e-12 * np.abs(ds).max())
+    prop.close()

@@ -1,0 +1,137 @@
+"""The oracle's right-hand side and DOP54 driver, pinned by an independent loop
+restatement, invariants and closed forms (no stored outputs of the reference
+exist for this path)."""
+import numpy as np
+import pytest
+
+from oracle import zand_oracle as O
+from zandpack_amd import builder
+
+
+def _rand_state(p, rng, scale=0.1):
+    NK, n, NL, NX, NOT = p.dims
+    psi = scale * (rng.normal(size=p.xi.shape) + 1j * rng.normal(size=p.xi.shape))
+    shp = (NK, NL, NX, NOT, NL, NX, NOT)
+    omg = scale * (rng.normal(size=shp) + 1j * rng.normal(size=shp))
+    A = rng.normal(size=(NK, n, n)) + 1j * rng.normal(size=(NK, n, n))
+    sig = p.sigma0 + 0.05 * (A + A.conj().transpose(0, 2, 1))
+    return sig, psi, omg
+
+
+@pytest.mark.parametrize("NK,U", [(1, 0.0), (2, 2.0)])
+def test_vectorised_matches_loop(NK, U):
+    p = builder.config_small(NK=NK, hubbard_U=U, seed=3)
+    rng = np.random.default_rng(0)
+    sig, psi, omg = _rand_state(p, rng)
+    Ht = p.H0 + p.dH(0.3, sig)
+    delta = np.array([0.11, -0.07])
+    a = O.rhs(p.consts(), sig, psi, omg, Ht, delta)
+    b = O.rhs_loop(p.consts(), sig, psi, omg, Ht, delta)
+    for u, v in zip(a, b):
+        np.testing.assert_allclose(u, v, rtol=0, atol=1e-12 * np.abs(v).max())
+
+
+def test_hermiticity_and_continuity():
+    """dσ is Hermitian and Tr dσ/dt = Σ_α J_α (particle conservation, equations.py:73-74)."""
+    p = builder.config_small(seed=5)
+    rng = np.random.default_rng(1)
+    sig, psi, omg = _rand_state(p, rng)
+    Ht = p.H0.copy()
+    (ds, dp, do), Q = O.rhs(p.consts(), sig, psi, omg, Ht, [0.2, -0.1], return_pi=True)
+    np.testing.assert_allclose(ds, ds.conj().transpose(0, 2, 1), atol=1e-13)
+    J = O.currents(p.consts(), psi)
+    tr = np.trace(ds, axis1=1, axis2=2)
+    np.testing.assert_allclose(tr.real, J.sum(axis=0), rtol=1e-12)
+    assert abs(tr.imag).max() < 1e-12
+    # J_α = Tr(Q_α)/ħ with Q_α the lead term of the σ equation (mpi/zand:368-375)
+    np.testing.assert_allclose(J, np.trace(Q, axis1=2, axis2=3).real.T / O.HBAR, rtol=1e-12)
+
+
+def test_fermi_block_of_domega_vanishes():
+    """ΔP = ΔM = 0 on Fermi slots, so dΩ on the Fermi x Fermi block is the
+    outer-subtraction term alone and vanishes when that block of Ω is zero."""
+    p = builder.config_small(seed=7)
+    NK, n, NL, NX, NOT = p.dims
+    N_L = 3
+    np.testing.assert_array_equal(p.dGP[:, :, N_L:], 0)
+    np.testing.assert_array_equal(p.dGM[:, :, N_L:], 0)
+    rng = np.random.default_rng(2)
+    sig, psi, omg = _rand_state(p, rng)
+    omg[:, :, N_L:, :, :, N_L:, :] = 0
+    _, _, do = O.rhs(p.consts(), sig, psi, omg, p.H0, [0.3, 0.1])
+    assert np.abs(do[:, :, N_L:, :, :, N_L:, :]).max() == 0.0
+    assert np.abs(do[:, :, :N_L, :, :, N_L:, :]).max() > 0
+
+
+def test_stationary_omega():
+    """Ω from the closed form of cmdtools/psinought:447-449 makes dΩ = 0."""
+    p = builder.config_small(seed=9)
+    rng = np.random.default_rng(3)
+    sig, psi, _ = _rand_state(p, rng)
+    delta = np.array([0.05, -0.02])
+    om = O.stationary_omega(p.consts(), psi, delta)
+    _, _, do = O.rhs(p.consts(), sig, psi, om, p.H0, delta)
+    assert np.abs(do).max() < 1e-12 * max(np.abs(om).max(), 1)
+
+
+def test_von_neumann_closed_form():
+    """With ξ = 0 the device is isolated: σ(t) = U σ0 U†, U = exp(-iHt/ħ)."""
+    p = builder.config_small(seed=11)
+    p.xi[:] = 0
+    p.Ixi[:] = 0
+    rng = np.random.default_rng(4)
+    sig0, psi, omg = _rand_state(p, rng)
+    psi[:] = 0
+    omg[:] = 0
+    T = 0.8
+    out = O.propagate(p.consts(), (sig0, psi, omg), 0.0, T, 0.01, 1e-11, p.H0,
+                      O.zero_dH, lambda t, a: 0.0, 2, time_update="correct")
+    t_end = out["times"][-1]
+    e, v = np.linalg.eigh(p.H0[0])
+    U = (v * np.exp(-1j * e * t_end / O.HBAR)) @ v.conj().T
+    exact = U @ sig0[0] @ U.conj().T
+    assert np.abs(out["sigma"][-1][0] - exact).max() < 1e-9
+    assert out["n_reject"] >= 0 and len(out["times"]) > 5
+
+
+def test_time_update_quirk():
+    """Reference bookkeeping advances t by the *new* h (mpi/zand:406-407,427)."""
+    p = builder.config_small(seed=12)
+    y = p.zero_state()
+    a = O.propagate(p.consts(), y, 0.0, 0.05, 0.01, 1e-6, p.H0, p.dH, p.bias, 2)
+    h = a["h_accepted"]
+    dt = np.diff(a["times"])
+    # t_{k+1} - t_k is the step proposed after step k, i.e. h used by step k+1
+    np.testing.assert_allclose(dt[:-1], h[1:], rtol=1e-14)
+
+
+def test_landauer_steady_state():
+    """Textbook closed form: a resonant level between two Lorentzian leads.
+
+    At steady state J_L = (1/2πħ) ∫ Γ_L Γ_R |G^r|² (f̃_L - f̃_R) dE with
+    Σ^r_α(E) = W γ / (2 (E - Δ_α - ε + iγ)) and the same pole-expanded f̃.  Pins
+    the ħ / 2π normalisation of the currents and the sign and placement of the
+    bias shifts.
+    """
+    eps0, W, gam, eps_l, kT, V = 0.2, 0.5, 1.0, 0.0, 0.1, 0.6
+    N_F = 10
+    leads = [dict(orbitals=np.array([0]), centres=np.array([eps_l]), widths=np.array([gam]),
+                  G=[np.array([[[W]]], complex)]) for _ in range(2)]
+    p = builder.assemble(np.array([[[eps0]]], complex), leads, N_F, 1, mu=[0.0, 0.0], kT=kT,
+                         sigma0=np.array([[[0.5]]], complex),
+                         pulse=builder.Pulse("step", V, 0.0, 0.0, (0.5, -0.5)))
+    out = O.propagate(p.consts(), p.zero_state(), 0.0, 40.0, 0.01, 1e-9, p.H0, O.zero_dH,
+                      p.bias, 2, time_update="correct")
+    J_end = out["J"][-1][:, 0]
+    z, R = O.hu_poles(N_F)
+    E = np.linspace(-8, 8, 400001)
+    dl, dr = V / 2, -V / 2
+    S = lambda d: W * gam / (2 * (E - d - eps_l + 1j * gam))
+    Gam = lambda d: W * gam ** 2 / ((E - d - eps_l) ** 2 + gam ** 2)
+    Gr = 1.0 / (E - eps0 - S(dl) - S(dr))
+    T = Gam(dl) * Gam(dr) * np.abs(Gr) ** 2
+    f = lambda d: O.fd_expanded(E - d, z, R, 1.0 / kT).real
+    JL = np.trapezoid(T * (f(dl) - f(dr)), E) / (2 * np.pi * O.HBAR)
+    assert JL > 1e-3
+    np.testing.assert_allclose(J_end[0], JL, rtol=2e-4)
+    np.testing.assert_allclose(J_end[1], -JL, rtol=2e-4)

@@ -1,0 +1,196 @@
+// Batched complex GEMM on CDNA4 matrix cores with fused epilogues.
+//
+//   acc = op(A) * op(B)      (complex, K-loop on MFMA 16x16x4, FP64 or FP32)
+//   epi(batch, row, col, acc)
+//
+// Complex operands are stored interleaved (re, im) as in numpy complex128 /
+// complex64.  Tiles are staged through LDS split into real and imaginary
+// planes, k-major, so that the MFMA operand fetch of a wavefront reads 16
+// consecutive elements per k row.  One complex product is four real MFMAs:
+//   Cre += Are*Bre ; Cre += (-Aim)*Bim ; Cim += Are*Bim ; Cim += Aim*Bre
+//
+// v_mfma_{f64_16x16x4f64, f32_16x16x4f32} lane layout (wave64):
+//   A fragment: lane l holds A[l%16][l/16]     (16 x 4)
+//   B fragment: lane l holds B[l/16][l%16]     (4 x 16)
+//   D fragment: lane l, item r holds D[4*(l/16)+r][l%16]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zh {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef v4d acc_t;
+  __device__ static inline acc_t zero() { acc_t z = {0.0, 0.0, 0.0, 0.0}; return z; }
+  __device__ static inline acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  typedef v4f acc_t;
+  __device__ static inline acc_t zero() { acc_t z = {0.f, 0.f, 0.f, 0.f}; return z; }
+  __device__ static inline acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// Interleaved complex element types as stored in memory.
+template <typename R> struct Cplx;
+template <> struct Cplx<double> { typedef double2 type; };
+template <> struct Cplx<float> { typedef float2 type; };
+
+// Operand descriptor.  Element (m,k) of op(A):
+//   !TRANS_A : A[m*lda + k]        TRANS_A : A[k*lda + m]
+// Element (k,n) of op(B):
+//   !TRANS_B : B[k*ldb + n]        TRANS_B : B[n*ldb + k]
+template <typename RA, typename RB>
+struct GemmArgs {
+  int M, N, K;
+  const typename Cplx<RA>::type* A; long lda; long sA;
+  const typename Cplx<RB>::type* B; long ldb; long sB;
+};
+
+constexpr int kThreads = 256;
+
+template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
+          int BM, int BN, int BK, typename Epi>
+__global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi) {
+  static_assert(BM % 32 == 0 && BN % 32 == 0 && BK % 4 == 0, "tile shape");
+  constexpr int TM = BM / 32;  // 16x16 MFMA tiles per wave along M (2x2 waves)
+  constexpr int TN = BN / 32;
+  constexpr int PAD = (sizeof(T) == 8) ? 2 : 4;
+  constexpr int LA = (BM * BK) / kThreads;  // elements each thread stages
+  constexpr int LB = (BN * BK) / kThreads;
+  static_assert((BM * BK) % kThreads == 0 && (BN * BK) % kThreads == 0, "loader");
+  typedef typename Mfma<T>::acc_t acc_t;
+  typedef typename Cplx<RA>::type ca_t;
+  typedef typename Cplx<RB>::type cb_t;
+
+  __shared__ T sAr[BK][BM + PAD];
+  __shared__ T sAi[BK][BM + PAD];
+  __shared__ T sBr[BK][BN + PAD];
+  __shared__ T sBi[BK][BN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int batch = blockIdx.z;
+  const int m0 = blockIdx.y * BM;
+  const int n0 = blockIdx.x * BN;
+  const ca_t* __restrict__ A = g.A + (long)batch * g.sA;
+  const cb_t* __restrict__ B = g.B + (long)batch * g.sB;
+
+  acc_t cr[TM][TN], ci[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) { cr[i][j] = Mfma<T>::zero(); ci[i][j] = Mfma<T>::zero(); }
+
+  ca_t ra[LA];
+  cb_t rb[LB];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < LA; ++it) {
+      const int e = tid + it * kThreads;
+      int m, k;
+      if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
+      const int gm = m0 + m, gk = k0 + k;
+      ca_t v; v.x = 0; v.y = 0;
+      if (gm < g.M && gk < g.K) v = TRANS_A ? A[(long)gk * g.lda + gm] : A[(long)gm * g.lda + gk];
+      ra[it] = v;
+    }
+#pragma unroll
+    for (int it = 0; it < LB; ++it) {
+      const int e = tid + it * kThreads;
+      int n, k;
+      if (!TRANS_B) { k = e / BN; n = e % BN; } else { n = e / BK; k = e % BK; }
+      const int gn = n0 + n, gk = k0 + k;
+      cb_t v; v.x = 0; v.y = 0;
+      if (gn < g.N && gk < g.K) v = TRANS_B ? B[(long)gn * g.ldb + gk] : B[(long)gk * g.ldb + gn];
+      rb[it] = v;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int it = 0; it < LA; ++it) {
+      const int e = tid + it * kThreads;
+      int m, k;
+      if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
+      sAr[k][m] = (T)ra[it].x;
+      sAi[k][m] = (T)ra[it].y;
+    }
+#pragma unroll
+    for (int it = 0; it < LB; ++it) {
+      const int e = tid + it * kThreads;
+      int n, k;
+      if (!TRANS_B) { k = e / BN; n = e % BN; } else { n = e / BK; k = e % BK; }
+      sBr[k][n] = (T)rb[it].x;
+      sBi[k][n] = (T)rb[it].y;
+    }
+  };
+
+  const int ktiles = (g.K + BK - 1) / BK;
+  if (ktiles > 0) load_tiles(0);
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int kt = 0; kt < ktiles; ++kt) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (kt + 1 < ktiles) load_tiles((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int m = wm * (BM / 2) + i * 16 + lr;
+        ar[i] = sAr[kk + lk][m];
+        ai[i] = sAi[kk + lk][m];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int n = wn * (BN / 2) + j * 16 + lr;
+        br[j] = sBr[kk + lk][n];
+        bi[j] = sBi[kk + lk][n];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
+          ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
+          cr[i][j] = Mfma<T>::op(nai[i], bi[j], cr[i][j]);
+          ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
+        }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * (BM / 2) + i * 16 + 4 * lk + r;
+        const int col = n0 + wn * (BN / 2) + j * 16 + lr;
+        if (row < g.M && col < g.N) epi(batch, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
+      }
+}
+
+template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, int BM, int BN,
+          int BK, typename Epi>
+inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& epi,
+                               hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, batch);
+  hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, BM, BN, BK, Epi>), grid,
+                     dim3(kThreads), 0, s, g, epi);
+  return hipGetLastError();
+}
+
+}  // namespace zh

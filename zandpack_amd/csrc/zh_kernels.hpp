@@ -1,0 +1,377 @@
+// Elementwise / reduction kernels and GEMM epilogues of the AME right-hand side
+// and the DOP54 attempt.  All are HBM-streaming: one pass over their operands,
+// coalesced along the fastest (orbital / mode) index.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "zh_gemm.hpp"
+#include "../../include/zandhip.h"
+
+namespace zh {
+
+struct StageParams {
+  double ts, h;
+  double delta[ZH_MAX_LEADS];
+};
+
+struct Ctrl {
+  double t0, h, eps;
+  int force;
+  int accept;
+  double te2, te, h_next;
+};
+
+struct PulseDesc {
+  int kind;
+  double amp, p1, p2, hbar;
+  double w[ZH_MAX_LEADS];
+};
+
+// Dormand-Prince 5(4) (Zandpack/mpi_RK4pars_dev.py:121-138): nodes c, stage
+// matrix a (row i = coefficients of k_0..k_{i-1}), error weights ct = b5 - b4.
+struct Tableau {
+  double c[7];
+  double a[7][7];
+  double ct[7];
+};
+
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __host__ __device__ inline double2 to_d2(C v) {
+  return make_double2((double)v.x, (double)v.y);
+}
+template <typename C> __host__ __device__ inline C from_d2(double2 v);
+template <> __host__ __device__ inline double2 from_d2<double2>(double2 v) { return v; }
+template <> __host__ __device__ inline float2 from_d2<float2>(double2 v) {
+  return make_float2((float)v.x, (float)v.y);
+}
+
+__device__ inline double pulse_value(const PulseDesc& p, double t) {
+  switch (p.kind) {
+    case ZH_PULSE_STEP: return t >= p.p1 ? p.amp : 0.0;
+    case ZH_PULSE_GAUSS: { const double x = (t - p.p1) / p.p2; return p.amp * exp(-x * x); }
+    case ZH_PULSE_RAMP: return p.amp * (1.0 - exp(-t / p.p1));
+    case ZH_PULSE_SIN: return p.amp * sin(p.p1 * t / p.hbar);
+    case ZH_PULSE_BOX: return p.amp * (tanh(t / p.p2) - tanh((t - p.p1) / p.p2)) * 0.5;
+    default: return 0.0;
+  }
+}
+
+// Stage times and biases of one attempt from the device control block.
+__global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, Tableau tab, int NL,
+                               StageParams* sp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double t0 = ctrl->t0, h = ctrl->h;
+  for (int s = 0; s < 7; ++s) {
+    const double ts = t0 + tab.c[s] * h;
+    sp[s].ts = ts;
+    sp[s].h = h;
+    const double v = pulse_value(pulse, ts);
+    for (int a = 0; a < NL; ++a) sp[s].delta[a] = pulse.w[a] * v;
+  }
+}
+
+// dst = y0 + sum_j a_j k_j over a (rows x cols) block; sources are dense with
+// row length cols, destinations have their own leading dimension.
+template <typename C>
+struct CombineArgs {
+  const C* y0;
+  const C* k[7];
+  double a[7];
+  int nterms;
+  C* d1; long ld1;
+  C* d2; long ld2;
+  long rows; int cols;
+};
+
+template <typename C>
+__global__ void k_combine(CombineArgs<C> g) {
+  const long total = g.rows * (long)g.cols;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    double2 v = to_d2(g.y0[idx]);
+    for (int j = 0; j < g.nterms; ++j) {
+      const double2 kv = to_d2(g.k[j][idx]);
+      v.x += g.a[j] * kv.x;
+      v.y += g.a[j] * kv.y;
+    }
+    const long r = idx / g.cols, c = idx - r * g.cols;
+    const C out = from_d2<C>(v);
+    g.d1[r * g.ld1 + c] = out;
+    if (g.d2) g.d2[r * g.ld2 + c] = out;
+  }
+}
+
+// Per-stage gathers of the stage input onto the coupled orbitals:
+//   Aomg[:, :nc] = psi[:, cidx], Bomg[:, nc:] = conj(psi[:, cidx]),
+//   Bp[n + c, o] = hbar * sigma[o, cidx[c]], dvec = Hubbard diagonal.
+template <typename C>
+struct GatherArgs {
+  int NK, n, M, nc;
+  const C* Ap; long ldAp; long sAp;          // stage psi (left n columns)
+  C* Aomg; C* Bomg;                          // (NK, M, 2nc)
+  C* Bp;                                     // (NK, n+nc, n)
+  const double2* sig;                        // (NK, n, n)
+  const int* cidx;
+  double2* dvec;                             // (NK, n)
+  const double* n0;                          // (NK, n)
+  double U, hbar;
+  int hubbard;
+};
+
+template <typename C>
+__global__ void k_gather(GatherArgs<C> g) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long tid0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (blockIdx.y == 0) {
+    const long total = (long)g.NK * g.M * g.nc;
+    for (long idx = tid0; idx < total; idx += stride) {
+      const int c = (int)(idx % g.nc);
+      const long bm = idx / g.nc;                   // b*M + m
+      const int b = (int)(bm / g.M);
+      const long m = bm - (long)b * g.M;
+      const C v = g.Ap[b * g.sAp + m * g.ldAp + g.cidx[c]];
+      g.Aomg[bm * 2 * g.nc + c] = v;
+      C cv = v; cv.y = -cv.y;
+      g.Bomg[bm * 2 * g.nc + g.nc + c] = cv;
+    }
+  } else if (blockIdx.y == 1) {
+    const long total = (long)g.NK * g.nc * g.n;
+    for (long idx = tid0; idx < total; idx += stride) {
+      const int o = (int)(idx % g.n);
+      const long bc = idx / g.n;
+      const int b = (int)(bc / g.nc);
+      const int c = (int)(bc - (long)b * g.nc);
+      double2 s = g.sig[((long)b * g.n + o) * g.n + g.cidx[c]];
+      s.x *= g.hbar; s.y *= g.hbar;
+      g.Bp[((long)b * (g.n + g.nc) + g.n + c) * g.n + o] = from_d2<C>(s);
+    }
+  } else {
+    const long total = (long)g.NK * g.n;
+    for (long idx = tid0; idx < total; idx += stride) {
+      const int b = (int)(idx / g.n);
+      const int i = (int)(idx - (long)b * g.n);
+      double2 d = make_double2(0.0, 0.0);
+      if (g.hubbard) {
+        const int pb = g.NK - 1 - b;  // partner spin block
+        const double2 s = g.sig[((long)pb * g.n + i) * g.n + i];
+        d = make_double2(g.U * (s.x - g.n0[(long)pb * g.n + i]), g.U * s.y);
+      }
+      g.dvec[idx] = d;
+    }
+  }
+}
+
+// Bp[b][o'][o] = H[b][o][o'] (top n rows of the Psi GEMM's B operand).
+template <typename C>
+__global__ void k_transpose_h(int n, long ldBp_batch, const double2* H, C* Bp) {
+  __shared__ double2 tile[32][33];
+  const int b = blockIdx.z;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    if (i < n && j < n) tile[r][tx] = H[((long)b * n + i) * n + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = bx + r, i = by + tx;  // out[j][i] = H[i][j]
+    if (i < n && j < n) Bp[b * ldBp_batch + (long)j * n + i] = from_d2<C>(tile[tx][r]);
+  }
+}
+
+// dsig = (h/hbar) (G + G^H)   (Zandpack/mpi/zand:340-342, 384-387 with G = -iHσ + Π/ħ)
+__global__ void k_sigma_sym(int n, const double2* G, double2* out, const StageParams* sp,
+                            double inv_hbar) {
+  __shared__ double2 tile[32][33];
+  const int b = blockIdx.z;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const double2* Gb = G + (long)b * n * n;
+  // load the transposed block G[bx.., by..]
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bx + r, j = by + tx;
+    if (i < n && j < n) tile[r][tx] = Gb[(long)i * n + j];
+  }
+  __syncthreads();
+  const double f = sp->h * inv_hbar;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    if (i < n && j < n) {
+      const double2 g = Gb[(long)i * n + j];
+      const double2 gt = tile[tx][r];  // G[j][i]
+      out[((long)b * n + i) * n + j] = make_double2(f * (g.x + gt.x), f * (g.y - gt.y));
+    }
+  }
+}
+
+// J_a[b] = 2 Re Tr(Π_a) / ħ² from the gathered stage psi (Zandpack/mpi/zand:368-375).
+template <typename C>
+__global__ void k_currents(int NK, int M, int Ma, int nc, const C* Aomg, const double2* Ixic,
+                           double inv_hbar2, double* J) {
+  const int a = blockIdx.x, b = blockIdx.y;
+  double acc = 0.0;
+  const long total = (long)Ma * nc;
+  for (long idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const long m = (long)a * Ma + idx / nc;
+    const int c = (int)(idx % nc);
+    const double2 p = to_d2(Aomg[((long)b * M + m) * 2 * nc + c]);
+    const double2 x = Ixic[((long)b * M + m) * nc + c];
+    acc += p.x * x.x - p.y * x.y;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J[a * NK + b] = 2.0 * red[0] * inv_hbar2;
+}
+
+// Partial sums of |sum_j ct_j k_j|^2 (Zandpack/mpi_funcs.py:247-252).
+template <typename C>
+struct TEArgs {
+  const C* k[7];
+  double ct[7];
+  long count;
+};
+
+constexpr int kTEBlocks = 1024;
+
+template <typename C>
+__global__ void k_te(TEArgs<C> g, double* partial) {
+  double acc = 0.0;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < g.count;
+       idx += (long)gridDim.x * blockDim.x) {
+    double vr = 0.0, vi = 0.0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      if (g.ct[j] != 0.0) {
+        const double2 kv = to_d2(g.k[j][idx]);
+        vr += g.ct[j] * kv.x;
+        vi += g.ct[j] * kv.y;
+      }
+    }
+    acc += vr * vr + vi * vi;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// TE = sqrt(sum), accept iff TE < eps, h_next = 0.9 h (eps/TE)^0.2
+// (Zandpack/mpi/zand:396-407, Zandpack/mpi_RK4pars_dev.py:148-149).
+__global__ void k_controller(const double* partial, int np, Ctrl* ctrl) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double te2 = red[0];
+    const double te = sqrt(te2);
+    ctrl->te2 = te2;
+    ctrl->te = te;
+    ctrl->accept = (ctrl->force || te < ctrl->eps) ? 1 : 0;
+    ctrl->h_next = 0.9 * ctrl->h * pow(ctrl->eps / te, 0.2);
+  }
+}
+
+// ---------------------------------------------------------------- epilogues
+// C = alpha * acc (complex alpha), batched, dense.
+template <typename C>
+struct EpiStore {
+  C* out; long ldc; long sC;
+  double ar, ai;
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    out[b * sC + (long)row * ldc + col] =
+        from_d2<C>(make_double2(ar * re - ai * im, ar * im + ai * re));
+  }
+};
+
+// G = -i (H0 σ) - i d_row σ + Π_c[row, cpos[col]] / ħ
+struct EpiSigma {
+  double2* G; const double2* sig; const double2* dvec; const double2* Pic; const int* cpos;
+  int n, nc; double inv_hbar;
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    const long off = ((long)b * n + row) * n + col;
+    const double2 s = sig[off];
+    const double2 d = dvec[(long)b * n + row];
+    const double2 ds = cmul(d, s);
+    double gr = im + ds.y, gi = -re - ds.x;  // -i (acc + d s)
+    const int c = cpos[col];
+    if (c >= 0) {
+      const double2 p = Pic[((long)b * n + row) * nc + c];
+      gr += p.x * inv_hbar;
+      gi += p.y * inv_hbar;
+    }
+    G[off] = make_double2(gr, gi);
+  }
+};
+
+// kΨ = -i h [ (acc - (χ⁺+Δ)ψ + d_col ψ)/ħ + (GL_P ξ + W/ħ²) on coupled columns ]
+// with acc = ψ H0ᵀ + ħ (ΔP ξ)_c σ_cᵀ  (Zandpack/mpi/zand:746-778)
+template <typename C>
+struct EpiPsi {
+  C* out;                       // (NK, M, n)
+  const C* Ap; long ldAp;       // stage psi
+  const C* xic; const C* W;     // (NK, M, nc)
+  const double2* glp; const double2* chip;  // (NK, M)
+  const double2* dvec;          // (NK, n)
+  const int* lead; const int* cpos;
+  const StageParams* sp;
+  int M, n, nc; double inv_hbar, inv_hbar2;
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    const long bm = (long)b * M + row;
+    const double2 p = to_d2(Ap[bm * ldAp + col]);
+    double2 chi = chip[bm];
+    chi.x += sp->delta[lead[row]];
+    const double2 d = dvec[(long)b * n + col];
+    const double2 cp = cmul(chi, p), dp = cmul(d, p);
+    double vr = (re - cp.x + dp.x) * inv_hbar;
+    double vi = (im - cp.y + dp.y) * inv_hbar;
+    const int c = cpos[col];
+    if (c >= 0) {
+      const double2 gx = cmul(glp[bm], to_d2(xic[bm * nc + c]));
+      const double2 w = to_d2(W[bm * nc + c]);
+      vr += gx.x + w.x * inv_hbar2;
+      vi += gx.y + w.y * inv_hbar2;
+    }
+    const double h = sp->h;
+    out[bm * n + col] = from_d2<C>(make_double2(h * vi, -h * vr));
+  }
+};
+
+// kΩ = h acc + i (h/ħ)(χ⁺_i + Δ_α(i) - χ⁻_j - Δ_α(j)) Ω_ij   (Zandpack/mpi/zand:784-809)
+template <typename C>
+struct EpiOmega {
+  C* out; const C* om;          // (NK, M, M)
+  const double2* chip; const double2* chim;
+  const int* lead;
+  const StageParams* sp;
+  int M; double inv_hbar;
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    const long off = ((long)b * M + row) * M + col;
+    const double h = sp->h;
+    const double2 a = chip[(long)b * M + row], c = chim[(long)b * M + col];
+    const double dr = (a.x + sp->delta[lead[row]]) - (c.x + sp->delta[lead[col]]);
+    const double di = a.y - c.y;
+    const double2 o = to_d2(om[off]);
+    // i (h/ħ) (dr + i di) o
+    const double2 t = cmul(make_double2(-di * h * inv_hbar, dr * h * inv_hbar), o);
+    out[off] = from_d2<C>(make_double2(h * re + t.x, h * im + t.y));
+  }
+};
+
+}  // namespace zh

@@ -1,0 +1,99 @@
+"""The reference's in-process call surface, backed by the GPU engine.
+
+* ``make_f(propagator)`` returns ``f(t, old_sig, old_psi, old_omega, dH,
+  delta_variant, dH_given=True) -> (D_sig, D_psi, D_omega)``, the signature of
+  ``TD_Transport.make_f`` / ``make_f_purenp`` (Zandpack/TimedependentTransport.py:2767-2777,
+  2605-2660): H = H0 + dH(t, σ) when ``dH_given`` else dH(t, σ); Δ_α = delta_variant(t, α).
+  The derivatives follow the zand formulation (Zandpack/mpi/zand:340-387, 746-809).
+* ``AdaptiveDOP(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi, ...)``
+  mirrors the driver signature of Zandpack/TimedependentTransport.py:3392 with
+  the step semantics of the production propagator (Zandpack/mpi/zand:297-431):
+  seven stages (k7 from stage 7, not the ``k7 = h*k6`` slip at :3537), absolute
+  error norm over σ, Ψ and Ω, h <- 0.9 h (eps/TE)^(1/5) after every attempt,
+  currents from the stage-0 state, and ``t += h_next`` on acceptance unless
+  ``time_update='correct'``.
+
+Both run every evaluation on the GPU (host arrays in and out per call); the
+device-resident path with no per-stage transfers is ``Propagator.propagate``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .engine import Propagator
+
+_TAB_C = np.array([0.0, 1 / 5, 3 / 10, 4 / 5, 8 / 9, 1.0, 1.0])
+_TAB_A = [
+    [],
+    [1 / 5],
+    [3 / 40, 9 / 40],
+    [44 / 45, -56 / 15, 32 / 9],
+    [19372 / 6561, -25360 / 2187, 64448 / 6561, -212 / 729],
+    [9017 / 3168, -355 / 33, 46732 / 5247, 49 / 176, -5103 / 18656],
+    [35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84],
+]
+_B5 = np.array([35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84, 0.0])
+_B4 = np.array([5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187 / 2100, 1 / 40])
+_CT = _B5 - _B4
+
+
+def make_f(prop: Propagator):
+    """Reference-style RHS closure over the constants held by ``prop``."""
+    H0 = prop.problem.H0
+    NL = prop.NL
+
+    def f(t, old_sig, old_psi, old_omega, dH, delta_variant, dH_given=True):
+        Ht = H0 + dH(t, old_sig) if dH_given else dH(t, old_sig)
+        delta = np.array([delta_variant(t, a) for a in range(NL)], dtype=np.float64)
+        ds, dp, do, J = prop.rhs(Ht, delta, old_sig, old_psi, old_omega)
+        f.last_J = J
+        return ds, dp, do
+
+    f.last_J = None
+    f.propagator = prop
+    return f
+
+
+def AdaptiveDOP(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_guess=None,
+                dH_given=True, fixed_mode=False, elec_names=("left", "right"),
+                time_update="reference", max_attempts=10 ** 6, **_unused):
+    """Adaptive DOP54 over an ``f`` made by ``make_f``.  Returns (times, data) with
+    ``data['current_<elec>']`` (list of (NK,) arrays) and ``data['density matrix']``."""
+    h = 0.01 if h_guess is None else float(h_guess)
+    y = [np.array(sig0, complex), np.array(psi0, complex), np.array(omega0, complex)]
+    data = {f"current_{e}": [] for e in elec_names}
+    dms, times = [], []
+    t = float(t0)
+    attempts = 0
+    while t < t1:
+        while True:
+            attempts += 1
+            if attempts > max_attempts:
+                raise RuntimeError("attempt limit reached")
+            ks = []
+            J0 = None
+            for i in range(7):
+                yi = [y[p] + sum(_TAB_A[i][j] * ks[j][p] for j in range(i)) for p in range(3)]
+                d = f(t + _TAB_C[i] * h, yi[0], yi[1], yi[2], dH, delta_func, dH_given=dH_given)
+                if i == 0:
+                    J0 = f.last_J
+                ks.append(tuple(h * v for v in d))
+            te = np.sqrt(sum(float(np.sum(np.abs(sum(_CT[j] * ks[j][p] for j in range(7))) ** 2))
+                             for p in range(3)))
+            if not np.isfinite(te):
+                raise FloatingPointError("non-finite error estimate")
+            h_old = h
+            if fixed_mode:
+                break
+            h = 0.9 * h * (eps / te) ** 0.2
+            if te < eps:
+                break
+        dms.append(y[0].copy())
+        for a, e in enumerate(elec_names):
+            data[f"current_{e}"].append(J0[a].copy())
+        y = [y[p] + sum(_B5[j] * ks[j][p] for j in range(7)) for p in range(3)]
+        times.append(t)
+        t += h if (time_update == "reference" and not fixed_mode) else h_old
+    data["density matrix"] = np.array(dms)
+    data["state"] = tuple(y)
+    return np.array(times), data

@@ -1,0 +1,151 @@
+"""Host-side handle over the native propagation engine (libzandhip.so).
+
+``Propagator`` owns one ``zh_problem``: constants, device-evaluated H(t, σ)
+model and bias, and the device-resident state (σ, Ψ, Ω).  It replaces one run of
+the reference's ``mpirun -np P zand`` (Zandpack/mpi/zand:200-495, 498-853) on a
+single GPU, and exposes the explicit one-evaluation RHS that the reference's
+in-process ``f`` closures compute (Zandpack/TimedependentTransport.py:2605-2660).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .builder import AMEProblem
+
+PRECISIONS = {"fp64": N.ZH_FP64, "mixed": N.ZH_MIXED}
+
+
+class Propagator:
+    def __init__(self, problem: AMEProblem, precision: str = "fp64", device: int = 0,
+                 graph: bool = True, state=None):
+        self.lib = N.lib()
+        self.problem = problem
+        self.precision = precision
+        NK, n, NL, NX, NOT = problem.dims
+        self.NK, self.n, self.NL, self.NX, self.NOT = NK, n, NL, NX, NOT
+        self.M = NL * NX * NOT
+        d = N.zh_dims(NK, n, NL, NX, NOT, PRECISIONS[precision], float(problem.hbar), device)
+        h = C.c_void_p()
+        N.check(self.lib.zh_problem_create(C.byref(d), C.byref(h)))
+        self.h = h
+        p = problem
+        self._keep = [N.c128(a) for a in (p.H0, p.xi, p.Ixi, p.GL_P, p.dGP, p.dGM, p.Xpp, p.Xpm)]
+        N.check(self.lib.zh_set_constants(self.h, *[N.ptr(a) for a in self._keep]))
+        if p.hubbard_U:
+            n0 = np.ascontiguousarray(p.hubbard_n0, dtype=np.float64)
+            N.check(self.lib.zh_set_hubbard(self.h, float(p.hubbard_U), N.ptr(n0)))
+        self.set_pulse(p.pulse)
+        N.check(self.lib.zh_set_graph(self.h, 1 if graph else 0))
+        self.put_state(*(state if state is not None else p.zero_state()))
+
+    # --------------------------------------------------------------- set-up
+    def set_pulse(self, pulse):
+        w = np.zeros(max(self.NL, 1))
+        w[: len(pulse.weights)] = pulse.weights[: self.NL]
+        w = np.ascontiguousarray(w[: self.NL])
+        N.check(self.lib.zh_set_pulse(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
+                                      float(pulse.p1), float(pulse.p2), N.ptr(w)))
+
+    def set_graph(self, on: bool):
+        N.check(self.lib.zh_set_graph(self.h, 1 if on else 0))
+
+    def set_profile(self, on: bool):
+        N.check(self.lib.zh_set_profile(self.h, 1 if on else 0))
+
+    # --------------------------------------------------------------- state
+    def put_state(self, sig, psi, omg):
+        s, p, o = N.c128(sig), N.c128(psi), N.c128(omg)
+        assert s.shape == (self.NK, self.n, self.n)
+        assert p.shape == (self.NK, self.NL, self.NX, self.NOT, self.n)
+        assert o.shape == (self.NK, self.NL, self.NX, self.NOT, self.NL, self.NX, self.NOT)
+        N.check(self.lib.zh_state_put(self.h, N.ptr(s), N.ptr(p), N.ptr(o)))
+
+    def get_state(self):
+        s = np.empty((self.NK, self.n, self.n), complex)
+        p = np.empty((self.NK, self.NL, self.NX, self.NOT, self.n), complex)
+        o = np.empty((self.NK, self.NL, self.NX, self.NOT, self.NL, self.NX, self.NOT), complex)
+        N.check(self.lib.zh_state_get(self.h, N.ptr(s), N.ptr(p), N.ptr(o)))
+        return s, p, o
+
+    def sigma(self):
+        s = np.empty((self.NK, self.n, self.n), complex)
+        N.check(self.lib.zh_sigma_get(self.h, N.ptr(s)))
+        return s
+
+    # --------------------------------------------------------------- compute
+    def rhs(self, Ht, delta, sig, psi, omg, want_omega=True):
+        """Time derivatives (dσ, dΨ, dΩ) and currents J (NL, NK) of one state."""
+        Ht, s, p, o = (N.c128(a) for a in (Ht, sig, psi, omg))
+        d = np.ascontiguousarray(delta, dtype=np.float64)
+        assert d.shape == (self.NL,)
+        ds = np.empty_like(s)
+        dp = np.empty_like(p)
+        do = np.empty_like(o) if want_omega else None
+        J = np.empty((self.NL, self.NK))
+        N.check(self.lib.zh_rhs(self.h, N.ptr(Ht), N.ptr(d), N.ptr(s), N.ptr(p), N.ptr(o),
+                                N.ptr(ds), N.ptr(dp), N.ptr(do), N.ptr(J)))
+        return ds, dp, do, J
+
+    def attempt(self, t0, h, eps, force_accept=False):
+        """One DOP54 attempt from the resident state; commits when accepted."""
+        info = N.zh_attempt_info(float(t0), float(h), float(eps), 1 if force_accept else 0,
+                                 0.0, 0.0, 0)
+        J = np.empty((self.NL, self.NK))
+        N.check(self.lib.zh_attempt(self.h, C.byref(info), N.ptr(J)))
+        return dict(te=info.te, h_next=info.h_next, accepted=bool(info.accepted), J=J)
+
+    def propagate(self, t0, t1, h0, eps, time_update="reference", max_attempts=10 ** 7,
+                  cap=100000):
+        """zand's adaptive loop on the device; returns times, J, h of accepted steps."""
+        times = np.empty(cap)
+        J = np.empty((cap, self.NL, self.NK))
+        hs = np.empty(cap)
+        ns, nr = C.c_long(0), C.c_long(0)
+        hf = C.c_double(0.0)
+        tu = N.ZH_TIME_REFERENCE if time_update == "reference" else N.ZH_TIME_CORRECT
+        N.check(self.lib.zh_propagate(self.h, float(t0), float(t1), float(h0), float(eps), tu,
+                                      int(max_attempts), int(cap), N.ptr(times), N.ptr(J),
+                                      N.ptr(hs), C.byref(ns), C.byref(nr), C.byref(hf)))
+        k = min(ns.value, cap)
+        return dict(times=np.concatenate(([t0], times[:k])), J=J[:k], h=hs[:k],
+                    n_steps=ns.value, n_rejected=nr.value, h_final=hf.value)
+
+    def timing(self, reset=False, cap=16):
+        names = C.create_string_buffer(32 * cap)
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        k = self.lib.zh_timing(self.h, 1 if reset else 0, cap, names,
+                               N.ptr(ms), cnt.ctypes.data_as(C.POINTER(C.c_long)))
+        raw = names.raw
+        out = {}
+        for i in range(k):
+            nm = raw[32 * i: 32 * (i + 1)].split(b"\0")[0].decode()
+            out[nm] = (ms[i], int(cnt[i]))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.zh_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def selftest_gemm(A, B, transA=False, transB=False, precision="fp64"):
+    """C = op(A) op(B) on the device with the engine's MFMA GEMM."""
+    A = N.c128(A)
+    B = N.c128(B)
+    M, K = (A.shape[1], A.shape[0]) if transA else A.shape
+    K2, Nn = (B.shape[1], B.shape[0]) if transB else B.shape
+    assert K == K2
+    Cm = np.empty((M, Nn), complex)
+    N.check(N.lib().zh_selftest_gemm(PRECISIONS[precision], M, Nn, K, int(transA), int(transB),
+                                     N.ptr(A), N.ptr(B), N.ptr(Cm)))
+    return Cm
