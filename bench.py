@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Headline benchmark: accepted DOP54 steps per second of the AME propagation
+at n=1000 orbitals, M=4000 auxiliary modes per spin block (NK=2, mean-field
+Hubbard U=2 eV, two leads x 20 channels, 50 Lorentzians + 50 Fermi slots),
+FP64 drop-in vs mixed precision on one MI355X.
+
+One "step" is one accepted DOP54 step (seven right-hand-side evaluations, the
+embedded error norm and the controller) of the whole spin-polarised system,
+advanced from device-resident state.  ``value`` is the mixed-precision rate;
+the FP64 rate, the mixed/FP64 speedup and the mixed-vs-FP64 current error in
+replay mode (mixed forced onto the FP64 run's accepted (t, h) sequence) are
+reported beside it, with the roofline of the dominant kernel and the CPU
+oracle timed on the host.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK45 steps/s @n=1000,M=4000: FP64 vs mixed at ≤1e-6 J err; p50/p99 latency"
+PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X dense FP64 / FP32 matrix
+
+
+def algorithmic_flops(p):
+    """Real flops per launch of each GEMM class (8 per complex MAC), per the
+    algorithmic count C_RHS = n^3 + M n^2 + M n n_c + Σ n n_a M_a + M^2 n_c
+    + Σ M M_a n_a + M^2 n_c per k block."""
+    NK, n, NL, NX, NOT = p.dims
+    M = p.M
+    Ma = NX * NOT
+    xi = p.xi.reshape(NK, NL, Ma, n)
+    na = [int(np.count_nonzero(np.any(xi[:, a] != 0, axis=(0, 1)))) for a in range(NL)]
+    nc = int(np.count_nonzero(np.any(p.xi.reshape(-1, n) != 0, axis=0)))
+    c = {
+        "gemm_hsig": n ** 3,
+        "gemm_psi": M * n * n + M * n * nc,
+        "gemm_pi": sum(n * a * Ma for a in na),
+        "gemm_omgxi": M * M * nc,
+        "gemm_omega": sum(M * Ma * a for a in na) + M * M * nc,
+    }
+    return {k: 8.0 * NK * v for k, v in c.items()}, na, nc
+
+
+def run_leg(p, precision, K, W, eps, h0):
+    from zandpack_amd.engine import Propagator
+    prop = Propagator(p, precision)
+    t, h = 0.0, h0
+
+    def one():
+        nonlocal t, h
+        r = prop.attempt(t, h, eps)
+        if not np.isfinite(r["te"]):
+            raise FloatingPointError("non-finite error estimate")
+        h_old, h = h, r["h_next"]
+        if r["accepted"]:
+            t += h  # reference bookkeeping (t += h_next)
+        return r, h_old
+
+    for _ in range(W):
+        one()
+    prop.timing(reset=True)
+    acc = rej = 0
+    t_start = time.perf_counter()
+    for _ in range(K):
+        r, _ = one()
+        acc += r["accepted"]
+        rej += not r["accepted"]
+    el = time.perf_counter() - t_start
+    dev = prop.timing()
+    out = dict(steps_per_s=acc / el, attempts_per_s=K / el, ms_per_attempt=1e3 * el / K,
+               device_ms_per_attempt=dev["attempt"][0], accepted=acc, rejected=rej,
+               h_last=h, t_reached=t)
+    prop.close()
+    return out
+
+
+def profile_leg(p, precision, attempts, h0):
+    from zandpack_amd.engine import Propagator
+    prop = Propagator(p, precision)
+    prop.set_profile(True)
+    prop.attempt(0.0, h0, 1.0, force_accept=True)
+    prop.timing(reset=True)
+    t = 0.0
+    for _ in range(attempts):
+        prop.attempt(t, h0, 1.0, force_accept=True)
+        t += h0
+    tm = prop.timing()
+    prop.close()
+    return {k: v[0] for k, v in tm.items() if v[1] > 0}
+
+
+def replay_error(p, eps, h0, n_steps):
+    """Mixed precision forced onto the FP64 run's accepted (t, h) sequence."""
+    from zandpack_amd.engine import Propagator
+    a = Propagator(p, "fp64")
+    seq, J64 = [], []
+    t, h = 0.0, h0
+    while len(seq) < n_steps:
+        r = a.attempt(t, h, eps)
+        h_old, h = h, r["h_next"]
+        if r["accepted"]:
+            seq.append((t, h_old))
+            J64.append(r["J"])
+            t += h
+    s64 = a.sigma()
+    a.close()
+    b = Propagator(p, "mixed")
+    Jm = [b.attempt(ts, hs, 1.0, force_accept=True)["J"] for ts, hs in seq]
+    sm = b.sigma()
+    b.close()
+    J64, Jm = np.array(J64), np.array(Jm)
+    jerr = float(np.abs(Jm - J64).max() / max(np.abs(J64).max(), 1e-300))
+    serr = float(np.linalg.norm(sm - s64) / np.linalg.norm(s64))
+    return jerr, serr, float(np.abs(J64).max())
+
+
+def cpu_baseline(p):
+    """The CPU oracle (numpy/BLAS, zand-style orbital pruning) on one RHS
+    evaluation of the same workload, scaled to steps/s (7 evaluations per step)."""
+    from oracle import zand_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    sig, psi, omg = p.zero_state()
+    psi = 1e-3 * (rng.normal(size=psi.shape) + 1j * rng.normal(size=psi.shape))
+    Ht = p.H0 + p.dH(0.0, sig)
+    t0 = time.perf_counter()
+    O.rhs(p.consts(), sig, psi, omg, Ht, [0.15, -0.15], h=0.01, prune=True)
+    el = time.perf_counter() - t0
+    return {"value": 1.0 / (7 * el), "unit": "steps/s", "cores": int(threads), "kind": "port",
+            "sample": f"one RHS evaluation of the same n=1000, M=4000, NK=2 state "
+                      f"({el:.2f} s), x7 per DOP54 step"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--h0", type=float, default=0.002)
+    ap.add_argument("--replay-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "mode sharding is a design, not a build", file=sys.stderr)
+        sys.exit(2)
+
+    from zandpack_amd import builder
+    p = builder.config_c3(seed=0)
+    NK, n, NL, NX, NOT = p.dims
+    print(f"# workload n={n} M={p.M} NK={NK}", file=sys.stderr, flush=True)
+
+    legs = {}
+    for prec in ("fp64", "mixed"):
+        print(f"# timing {prec}", file=sys.stderr, flush=True)
+        legs[prec] = run_leg(p, prec, args.steps, args.warmup, args.eps, args.h0)
+    print("# replay error", file=sys.stderr, flush=True)
+    jerr, serr, jmax = replay_error(p, args.eps, args.h0, args.replay_steps)
+
+    flops, na, nc = algorithmic_flops(p)
+    roof = {}
+    for prec in ("fp64", "mixed"):
+        print(f"# kernel profile {prec}", file=sys.stderr, flush=True)
+        tm = profile_leg(p, prec, 3, args.h0)
+        gemms = {k: v for k, v in tm.items() if k in flops}
+        dom = max(gemms, key=lambda k: gemms[k])
+        # kernel-class time is summed over NK-batched launches of one stage
+        ach = flops[dom] / (gemms[dom] * 1e-3) / 1e12
+        peak = PEAK[prec] if dom not in ("gemm_hsig", "gemm_pi") else PEAK["fp64"]
+        roof[prec] = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": peak,
+                      "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                      "avg_ms": gemms[dom], "kernel_ms": tm}
+
+    value = legs["mixed"]["steps_per_s"]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / value if value else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "mixed: c64 Psi/Omega on f32 MFMA, c128 sigma/Hsigma/Pi/error/controller",
+        "data": "synthetic",
+        "config": {"workload": "honeycomb 40x25 n=1000, NK=2 spin, Hubbard U=2 eV, 2 leads x "
+                               "20 ch, N_L=50 + 50 Fermi slots, M=4000 per k, step bias 0.3 eV",
+                   "n": n, "M": p.M, "NK": NK, "n_c": nc, "n_alpha": na, "eps": args.eps,
+                   "h0": args.h0},
+        "fp64": legs["fp64"],
+        "mixed": legs["mixed"],
+        "speedup_mixed_vs_fp64": value / legs["fp64"]["steps_per_s"],
+        "replay": {"steps": args.replay_steps, "J_rel_err_mixed": jerr,
+                   "sigma_rel_err_mixed": serr, "J_max": jmax},
+        "roofline": {k: v for k, v in roof["mixed"].items() if k != "kernel_ms"},
+        "roofline_fp64": {k: v for k, v in roof["fp64"].items() if k != "kernel_ms"},
+        "kernel_ms": {"fp64": roof["fp64"]["kernel_ms"], "mixed": roof["mixed"]["kernel_ms"]},
+    }
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(p)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,13 +161,22 @@ def lead_of_mode(NL, NX, NOT):
     return np.repeat(np.arange(NL), NX * NOT)
 
 
-def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False):
+def coupled_orbitals(consts):
+    """Orbitals where any ξ or Ixi entry is non-zero (mpi/zand:274-280, 589-602)."""
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    nz = (consts["xi"] != 0) | (consts["Ixi"] != 0)
+    return np.where(nz.reshape(-1, n).any(axis=0))[0]
+
+
+def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, prune=False):
     """One stage of zand: returns (h*dσ/dt, h*dΨ/dt, h*dΩ/dt).
 
     σ: mpi/zand:340-387; Ψ: mpi/zand:746-778; Ω: mpi/zand:784-809.
     ``Ht`` is the full Hamiltonian at the stage (H0 + dH(ts, σ)); ``delta`` is
-    Δ_α(ts) per lead.  Dense in the orbital index (no pruning): the reference's
-    pruning drops only orbitals where ξ is zero, which leaves the sums unchanged.
+    Δ_α(ts) per lead.  ``prune=False`` is dense in the orbital index;
+    ``prune=True`` restricts every ξ/Ixi contraction to the coupled orbitals as
+    zand does (mpi/zand:589-602, 641-658) -- the same sums, since ξ vanishes
+    elsewhere, at the reference's cost.
     """
     NK, NL, NX, NOT, n, M = _flat(consts)
     xi = consts["xi"].reshape(NK, M, n)
@@ -181,6 +190,8 @@ def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False):
     xpm = consts["Xpm"].reshape(NK, M) + d_mode[None, :]
     p = psi.reshape(NK, M, n)
     W = omg.reshape(NK, M, M)
+    cidx = coupled_orbitals(consts) if prune else slice(None)
+    xc, ixc, pc = xi[:, :, cidx], Ixi[:, :, cidx], p[:, :, cidx]
 
     # σ: D = Hσ; D -= D†; D *= -i  (mpi/zand:340-342)
     D = Ht @ sig
@@ -189,7 +200,8 @@ def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False):
     Q = np.zeros((NK, NL, n, n), complex)
     for a in range(NL):
         sl = slice(a * NX * NOT, (a + 1) * NX * NOT)
-        r = p[:, sl, :].transpose(0, 2, 1) @ Ixi[:, sl, :]
+        r = np.zeros((NK, n, n), complex)
+        r[:, :, cidx] = p[:, sl, :].transpose(0, 2, 1) @ ixc[:, sl, :]
         Q[:, a] = (r + r.conj().transpose(0, 2, 1)) / hbar
     dsig = (h / hbar) * (D + Q.sum(axis=1))
 
@@ -197,14 +209,14 @@ def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False):
     dpsi = p @ (Ht / hbar).transpose(0, 2, 1)
     dpsi -= (xpp / hbar)[..., None] * p
     dpsi += glp[..., None] * xi
-    dpsi += dgp[..., None] * (xi @ sig.transpose(0, 2, 1))
-    dpsi += W @ (xi / hbar ** 2)
+    dpsi += dgp[..., None] * (xc @ sig[:, :, cidx].transpose(0, 2, 1))
+    dpsi[:, :, cidx] += W @ (xc / hbar ** 2)
     dpsi *= -1j * h
 
     # Ω (mpi/zand:784-809)
-    domg = h * (p @ (dgm[..., None] * Ixi).transpose(0, 2, 1))            # ΔM_j (ψ_i·Ixi_j)
-    domg += h * ((dgp[..., None] * xi) @ p.conj().transpose(0, 2, 1))     # ΔP_i (ξ_i·ψ_j*)
-    domg += (1j * h / hbar) * (xpp[:, :, None] - xpm[:, None, :]) * W    # outer subtraction
+    domg = h * (pc @ (dgm[..., None] * ixc).transpose(0, 2, 1))            # ΔM_j (ψ_i·Ixi_j)
+    domg += h * ((dgp[..., None] * xc) @ pc.conj().transpose(0, 2, 1))     # ΔP_i (ξ_i·ψ_j*)
+    domg += (1j * h / hbar) * (xpp[:, :, None] - xpm[:, None, :]) * W     # outer subtraction
     out = (dsig, dpsi.reshape(psi.shape), domg.reshape(omg.shape))
     if return_pi:
         return out, Q

@@ -125,7 +125,10 @@ struct Engine : EngineBase {
   Ctrl* ctrl = nullptr;
   StageParams* sp = nullptr;
   double* Jd = nullptr;
+  double* Jpart = nullptr;
   int iy0 = 7, iyn = 8;
+  static constexpr int kPiParts = 8;  // split-K of Π_c = ψᵀ Ixi_c (K = M)
+  static constexpr int kWParts = 2;   // split-K of W = Ω ξ_c (K = M)
   // pinned host mailboxes
   Ctrl* ctrl_h = nullptr;
   double* J_h = nullptr;
@@ -172,6 +175,7 @@ struct Engine : EngineBase {
     ctrl = dalloc<Ctrl>(1);
     sp = dalloc<StageParams>(7);
     Jd = dalloc<double>((size_t)NL * NK);
+    Jpart = dalloc<double>((size_t)NL * NK * kCurChunks);
     std::vector<int> ld(M);
     for (int m = 0; m < M; ++m) ld[m] = m / Ma;
     HIPCHK(hipMemcpyAsync(lead, ld.data(), sizeof(int) * M, hipMemcpyHostToDevice, stream));
@@ -220,9 +224,9 @@ struct Engine : EngineBase {
       cidx = dalloc<int>(nc);
       cpos = dalloc<int>(n);
       Ixic = dalloc<double2>(mc);
-      Pic = dalloc<double2>((size_t)NK * n * nc);
+      Pic = dalloc<double2>((size_t)NK * kPiParts * n * nc);
       xic = dalloc<cs>(mc);
-      W = dalloc<cs>(mc);
+      W = dalloc<cs>(mc * kWParts);
       Ap = dalloc<cs>((size_t)NK * M * (n + nc));
       Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
       Aomg = dalloc<cs>((size_t)NK * M * 2 * nc);
@@ -407,26 +411,25 @@ struct Engine : EngineBase {
     if (nc > 0) {
       timed(KC_GEMM_PI, [&] {
         // Π_c = ψᵀ Ixi_c  (n x M) (M x nc), FP64
-        GemmArgs<R, double> g{n, nc, M, Ap, n + nc, (long)M * (n + nc), Ixic, nc, (long)M * nc};
+        GemmArgs<R, double> g{n, nc, M, Ap, n + nc, (long)M * (n + nc), Ixic, nc, (long)M * nc,
+                              kPiParts};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
         HIPCHK((launch_zgemm<double, R, double, true, false, 64, 32, 16>(g, NK, e, stream)));
       });
-      if (currents) {
-        timed(KC_CURRENTS, [&] {
-          hipLaunchKernelGGL(k_currents<cs>, dim3(NL, NK), dim3(256), 0, stream, NK, M, Ma, nc,
-                             (const cs*)Aomg, (const double2*)Ixic, ih * ih, Jd);
-        });
-        HIPCHK(hipGetLastError());
-      }
-    } else if (currents) {
-      // no coupled orbitals: the reduction over an empty range writes J = 0
-      hipLaunchKernelGGL(k_currents<cs>, dim3(NL, NK), dim3(256), 0, stream, NK, M, Ma, 0,
-                         (const cs*)Aomg, (const double2*)Ixic, ih * ih, Jd);
+    }
+    if (currents) {
+      // with no coupled orbitals the reductions run over empty ranges and give J = 0
+      timed(KC_CURRENTS, [&] {
+        hipLaunchKernelGGL(k_currents<cs>, dim3(kCurChunks, NL, NK), dim3(256), 0, stream, NK, M,
+                           Ma, nc, (const cs*)Aomg, (const double2*)Ixic, Jpart);
+        hipLaunchKernelGGL(k_currents_final, dim3(1), dim3(((NL * NK + 63) / 64) * 64), 0, stream,
+                           NL * NK, (const double*)Jpart, ih * ih, Jd);
+      });
       HIPCHK(hipGetLastError());
     }
     timed(KC_GEMM_HSIG, [&] {
       GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n};
-      EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih};
+      EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih, kPiParts};
       HIPCHK((launch_zgemm<double, double, double, false, false, 64, 64, 16>(g, NK, e, stream)));
     });
     timed(KC_SIGSYM, [&] {
@@ -438,7 +441,7 @@ struct Engine : EngineBase {
     if (nc > 0) {
       timed(KC_GEMM_OMGXI, [&] {
         // W = Ω ξ_c  (M x M)(M x nc)
-        GemmArgs<R, R> g{M, nc, M, om_in, M, (long)M * M, xic, nc, (long)M * nc};
+        GemmArgs<R, R> g{M, nc, M, om_in, M, (long)M * M, xic, nc, (long)M * nc, kWParts};
         EpiStore<cs> e{W, nc, (long)M * nc, 1.0, 0.0};
         HIPCHK((launch_zgemm<R, R, R, false, false, 64, 32, 16>(g, NK, e, stream)));
       });
@@ -447,7 +450,7 @@ struct Engine : EngineBase {
       // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (M x (n+nc)) ((n+nc) x n)
       GemmArgs<R, R> g{M, n, n + nc, Ap, n + nc, (long)M * (n + nc), Bp, n, (long)(n + nc) * n};
       EpiPsi<cs> e{psi[kslot], Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps, M, n, nc,
-                   ih, ih * ih};
+                   ih, ih * ih, kWParts};
       HIPCHK((launch_zgemm<R, R, R, false, false, 64, 64, 16>(g, NK, e, stream)));
     });
     timed(KC_GEMM_OMEGA, [&] {

@@ -12,7 +12,9 @@
 // v_mfma_{f64_16x16x4f64, f32_16x16x4f32} lane layout (wave64):
 //   A fragment: lane l holds A[l%16][l/16]     (16 x 4)
 //   B fragment: lane l holds B[l/16][l%16]     (4 x 16)
-//   D fragment: lane l, item r holds D[4*(l/16)+r][l%16]
+//   D fragment, item r of lane l:
+//     f32: D[4*(l/16) + r][l%16]   (each lane owns 4 consecutive rows)
+//     f64: D[(l/16) + 4*r][l%16]   (rows interleaved with stride 4)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -29,6 +31,7 @@ template <> struct Mfma<double> {
   __device__ static inline acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int r) { return lk + 4 * r; }
 };
 template <> struct Mfma<float> {
   typedef v4f acc_t;
@@ -36,6 +39,7 @@ template <> struct Mfma<float> {
   __device__ static inline acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int r) { return 4 * lk + r; }
 };
 
 // Interleaved complex element types as stored in memory.
@@ -47,11 +51,15 @@ template <> struct Cplx<float> { typedef float2 type; };
 //   !TRANS_A : A[m*lda + k]        TRANS_A : A[k*lda + m]
 // Element (k,n) of op(B):
 //   !TRANS_B : B[k*ldb + n]        TRANS_B : B[n*ldb + k]
+// splitk > 1 splits the K range over blockIdx.z = batch * splitk + part; the
+// epilogue then receives z (not the batch index) and must keep the partial
+// products apart (they are summed by the consumer in a fixed order).
 template <typename RA, typename RB>
 struct GemmArgs {
   int M, N, K;
   const typename Cplx<RA>::type* A; long lda; long sA;
   const typename Cplx<RB>::type* B; long ldb; long sB;
+  int splitk = 1;
 };
 
 constexpr int kThreads = 256;
@@ -79,7 +87,8 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int batch = blockIdx.z;
+  const int batch = blockIdx.z / g.splitk;
+  const int part = blockIdx.z - batch * g.splitk;
   const int m0 = blockIdx.y * BM;
   const int n0 = blockIdx.x * BN;
   const ca_t* __restrict__ A = g.A + (long)batch * g.sA;
@@ -134,14 +143,18 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
     }
   };
 
-  const int ktiles = (g.K + BK - 1) / BK;
-  if (ktiles > 0) load_tiles(0);
+  // K range of this part, in whole BK tiles
+  const int ktot = (g.K + BK - 1) / BK;
+  const int kper = (ktot + g.splitk - 1) / g.splitk;
+  const int kt0 = part * kper;
+  const int ktiles = max(0, min(ktot, kt0 + kper) - kt0);
+  if (ktiles > 0) load_tiles(kt0 * BK);
   const int lr = lane & 15, lk = lane >> 4;
   for (int kt = 0; kt < ktiles; ++kt) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (kt + 1 < ktiles) load_tiles((kt + 1) * BK);
+    if (kt + 1 < ktiles) load_tiles((kt0 + kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
@@ -176,9 +189,9 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * (BM / 2) + i * 16 + 4 * lk + r;
+        const int row = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, r);
         const int col = n0 + wn * (BN / 2) + j * 16 + lr;
-        if (row < g.M && col < g.N) epi(batch, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
+        if (row < g.M && col < g.N) epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
       }
 }
 
@@ -187,7 +200,7 @@ template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, int 
 inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& epi,
                                hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
-  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, batch);
+  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, batch * g.splitk);
   hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, BM, BN, BK, Epi>), grid,
                      dim3(kThreads), 0, s, g, epi);
   return hipGetLastError();

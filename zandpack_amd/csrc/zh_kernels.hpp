@@ -206,14 +206,19 @@ __global__ void k_sigma_sym(int n, const double2* G, double2* out, const StagePa
   }
 }
 
-// J_a[b] = 2 Re Tr(Π_a) / ħ² from the gathered stage psi (Zandpack/mpi/zand:368-375).
+// J_a[b] = 2 Re Tr(Π_a) / ħ² from the gathered stage psi (Zandpack/mpi/zand:368-375):
+// kCurChunks partial sums per (lead, k), then a fixed-order final sum.
+constexpr int kCurChunks = 64;
+
 template <typename C>
 __global__ void k_currents(int NK, int M, int Ma, int nc, const C* Aomg, const double2* Ixic,
-                           double inv_hbar2, double* J) {
-  const int a = blockIdx.x, b = blockIdx.y;
-  double acc = 0.0;
+                           double* part) {
+  const int chunk = blockIdx.x, a = blockIdx.y, b = blockIdx.z;
   const long total = (long)Ma * nc;
-  for (long idx = threadIdx.x; idx < total; idx += blockDim.x) {
+  const long per = (total + kCurChunks - 1) / kCurChunks;
+  const long lo = chunk * per, hi = min(total, lo + per);
+  double acc = 0.0;
+  for (long idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
     const long m = (long)a * Ma + idx / nc;
     const int c = (int)(idx % nc);
     const double2 p = to_d2(Aomg[((long)b * M + m) * 2 * nc + c]);
@@ -227,7 +232,15 @@ __global__ void k_currents(int NK, int M, int Ma, int nc, const C* Aomg, const d
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) J[a * NK + b] = 2.0 * red[0] * inv_hbar2;
+  if (threadIdx.x == 0) part[((long)a * NK + b) * kCurChunks + chunk] = red[0];
+}
+
+__global__ void k_currents_final(int nlk, const double* part, double inv_hbar2, double* J) {
+  const int i = threadIdx.x;
+  if (i >= nlk) return;
+  double s = 0.0;
+  for (int c = 0; c < kCurChunks; ++c) s += part[(long)i * kCurChunks + c];
+  J[i] = 2.0 * s * inv_hbar2;
 }
 
 // Partial sums of |sum_j ct_j k_j|^2 (Zandpack/mpi_funcs.py:247-252).
@@ -304,6 +317,7 @@ struct EpiStore {
 struct EpiSigma {
   double2* G; const double2* sig; const double2* dvec; const double2* Pic; const int* cpos;
   int n, nc; double inv_hbar;
+  int pi_parts;  // split-K partials of Π_c, layout (NK, pi_parts, n, nc)
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
     const long off = ((long)b * n + row) * n + col;
     const double2 s = sig[off];
@@ -312,9 +326,14 @@ struct EpiSigma {
     double gr = im + ds.y, gi = -re - ds.x;  // -i (acc + d s)
     const int c = cpos[col];
     if (c >= 0) {
-      const double2 p = Pic[((long)b * n + row) * nc + c];
-      gr += p.x * inv_hbar;
-      gi += p.y * inv_hbar;
+      double pr = 0.0, pi = 0.0;
+      for (int s = 0; s < pi_parts; ++s) {
+        const double2 p = Pic[(((long)b * pi_parts + s) * n + row) * nc + c];
+        pr += p.x;
+        pi += p.y;
+      }
+      gr += pr * inv_hbar;
+      gi += pi * inv_hbar;
     }
     G[off] = make_double2(gr, gi);
   }
@@ -332,6 +351,7 @@ struct EpiPsi {
   const int* lead; const int* cpos;
   const StageParams* sp;
   int M, n, nc; double inv_hbar, inv_hbar2;
+  int w_parts;  // split-K partials of W, layout (NK, w_parts, M, nc)
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
     const long bm = (long)b * M + row;
     const double2 p = to_d2(Ap[bm * ldAp + col]);
@@ -344,9 +364,14 @@ struct EpiPsi {
     const int c = cpos[col];
     if (c >= 0) {
       const double2 gx = cmul(glp[bm], to_d2(xic[bm * nc + c]));
-      const double2 w = to_d2(W[bm * nc + c]);
-      vr += gx.x + w.x * inv_hbar2;
-      vi += gx.y + w.y * inv_hbar2;
+      double wr = 0.0, wi = 0.0;
+      for (int s = 0; s < w_parts; ++s) {
+        const double2 w = to_d2(W[(((long)b * w_parts + s) * M + row) * nc + c]);
+        wr += w.x;
+        wi += w.y;
+      }
+      vr += gx.x + wr * inv_hbar2;
+      vi += gx.y + wi * inv_hbar2;
     }
     const double h = sp->h;
     out[bm * n + col] = from_d2<C>(make_double2(h * vi, -h * vr));
