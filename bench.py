@@ -75,7 +75,8 @@ def run_leg(p, precision, K, W, eps, h0):
         rej += not r["accepted"]
     el = time.perf_counter() - t_start
     dev = prop.timing()
-    out = dict(steps_per_s=acc / el, attempts_per_s=K / el, ms_per_attempt=1e3 * el / K,
+    lay = prop.layout_info()
+    out = dict(layout=lay, steps_per_s=acc / el, attempts_per_s=K / el, ms_per_attempt=1e3 * el / K,
                device_ms_per_attempt=dev["attempt"][0], accepted=acc, rejected=rej,
                h_last=h, t_reached=t)
     prop.close()

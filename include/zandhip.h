@@ -57,6 +57,18 @@ const char* zh_last_error(void);
 int zh_problem_create(const zh_dims* dims, zh_problem** out);
 void zh_problem_destroy(zh_problem* p);
 
+/* Device mode order, chosen before zh_set_constants: 0 auto (default), 1 the
+ * reference order, 2 active modes (dGP or dGM non-zero) first, padded to a
+ * 64-mode tile, then the passive Fermi modes.  Order 2 lets the Omega kernels skip
+ * the vanishing source terms per tile and the passive x passive block of Omega,
+ * which stays exactly zero when it starts at zero (the reference's Fermi-block
+ * pruning, Zandpack/mpi/nozand:818-890).  Auto picks 2 when the padding costs at
+ * most 10% of the modes.  Results do not depend on the order. */
+int zh_set_layout(zh_problem* p, int mode);
+/* out[5] = {active-first order in use, passive block skipped, device modes,
+ * active block size, coupled orbitals}. */
+int zh_layout_info(zh_problem* p, int* out);
+
 /* Constants, reference layout, complex128 interleaved unless noted:
  *   H0 (NK,n,n); xi, Ixi (NK,NL,NX,NOT,n); GL_P, dGP=GG_P-GL_P, dGM=GG_M-GL_M,
  *   Xpp, Xpm (NK,NL,NX,NOT).

@@ -49,8 +49,23 @@ def test_gemm_building_block(native, precision, tol, shape, ta, tb):
     assert np.abs(C - ref).max() <= tol * np.sqrt(K) * np.abs(A).max() * np.abs(B).max() * 4
 
 
+def _complex_h(seed=8):
+    """Small case with a complex Hermitian Hamiltonian (Peierls-like phases), so
+    the general complex H products run instead of the real-H fast path."""
+    p = builder.config_small(seed=seed)
+    rng = np.random.default_rng(seed)
+    n = p.H0.shape[-1]
+    A = rng.normal(size=(n, n))
+    p.H0 = p.H0 + 0.2j * (A - A.T)[None]
+    return p
+
+
 CASES = {
     "small": lambda: builder.config_small(seed=1),
+    "complex_H": _complex_h,
+    # 2 leads x 4 Lorentzians x 8 channels = 64 active modes: auto picks the
+    # active-first order with no padding
+    "tile_aligned": lambda: builder.config_small(n_w=8, n_l=3, N_L=4, slots=4, NOT=8, seed=6),
     "small_spin_hubbard": lambda: builder.config_small(NK=2, hubbard_U=2.0, seed=2),
     "chain20": lambda: builder.config_chain(),
     "wide": lambda: builder.config_small(n_w=6, n_l=7, N_L=5, slots=6, NOT=3, seed=4),
@@ -59,7 +74,8 @@ CASES = {
 
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_rhs_parity(native, case, precision):
+@pytest.mark.parametrize("layout", ["reference", "pruned"])
+def test_rhs_parity(native, case, precision, layout):
     from zandpack_amd.engine import Propagator
     p = CASES[case]()
     sig, psi, omg = _rand_state(p, 7)
@@ -67,8 +83,8 @@ def test_rhs_parity(native, case, precision):
     delta = np.array([p.bias(0.2, a) for a in range(2)]) + np.array([0.05, -0.03])
     ref = O.rhs(p.consts(), sig, psi, omg, Ht, delta)
     Jref = O.currents(p.consts(), psi)
-    say("rhs", case, precision, p.dims)
-    prop = Propagator(p, precision)
+    say("rhs", case, precision, layout, p.dims)
+    prop = Propagator(p, precision, layout=layout)
     ds, dp, do, J = prop.rhs(Ht, delta, sig, psi, omg)
     tol = 1e-12 if precision == "fp64" else 2e-5
     for got, want in zip((ds, dp, do), ref):
@@ -77,13 +93,26 @@ def test_rhs_parity(native, case, precision):
     prop.close()
 
 
-@pytest.mark.parametrize("case", ["small", "small_spin_hubbard", "chain20"])
-def test_attempt_parity(native, case):
+@pytest.mark.parametrize("case", ["small", "complex_H", "small_spin_hubbard", "chain20",
+                                  "tile_aligned"])
+@pytest.mark.parametrize("layout", ["reference", "pruned"])
+@pytest.mark.parametrize("zero_ff", [False, True])
+def test_attempt_parity(native, case, layout, zero_ff):
+    """One attempt from a random state; with zero_ff the Fermi x Fermi block of Ω
+    starts at zero, which the active-first order skips."""
     from zandpack_amd.engine import Propagator
     p = CASES[case]()
     y0 = _rand_state(p, 11, scale=0.02)
-    say("attempt", case)
-    prop = Propagator(p, "fp64", state=y0)
+    if zero_ff:
+        passive = (p.dGP == 0) & (p.dGM == 0)
+        pm = passive.all(axis=0).reshape(-1)
+        om = y0[2].reshape(p.dims[0], p.M, p.M)
+        om[:, pm[:, None] & pm[None, :]] = 0
+    say("attempt", case, layout, zero_ff)
+    prop = Propagator(p, "fp64", state=y0, layout=layout)
+    info = prop.layout_info()
+    assert info["active_first"] == (layout == "pruned")
+    assert info["skip_passive_block"] == (layout == "pruned" and zero_ff)
     t0, h, eps = 0.3, 0.02, 1e-3
     ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2)
     r = prop.attempt(t0, h, eps)
@@ -110,15 +139,15 @@ def test_attempt_reject_keeps_state(native):
     prop.close()
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_propagate_chain20(native, graph):
+@pytest.mark.parametrize("graph,layout", [(True, "auto"), (False, "auto"), (True, "pruned")])
+def test_propagate_chain20(native, graph, layout):
     """Config 1 (n=20, M=96): adaptive DOP54 with zand's control semantics, 0 -> 3 fs."""
     from zandpack_amd.engine import Propagator
     p = builder.config_chain()
     y0 = p.zero_state()
-    say("propagate chain20 graph", graph)
+    say("propagate chain20 graph", graph, layout)
     ref = O.propagate(p.consts(), y0, 0.0, 3.0, 0.01, 1e-8, p.H0, p.dH, p.bias, 2)
-    prop = Propagator(p, "fp64", graph=graph)
+    prop = Propagator(p, "fp64", graph=graph, layout=layout)
     out = prop.propagate(0.0, 3.0, 0.01, 1e-8)
     assert out["n_steps"] == len(ref["times"]) - 1
     assert out["n_rejected"] == ref["n_reject"]
@@ -143,6 +172,24 @@ def test_graph_replays_match_direct(native):
     np.testing.assert_array_equal(res[0][0], res[1][0])
     for a, b in zip(res[0][1], res[1][1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_layouts_agree_over_propagation(native):
+    """Reference order vs active-first order with the passive block skipped: the
+    same adaptive run (steps, rejections) and currents to rounding."""
+    from zandpack_amd.engine import Propagator
+    p = CASES["tile_aligned"]()
+    outs = []
+    for layout in ("reference", "pruned"):
+        prop = Propagator(p, "fp64", layout=layout)
+        outs.append((prop.propagate(0.0, 1.0, 0.01, 1e-8), prop.get_state(), prop.layout_info()))
+        prop.close()
+    (a, sa, ia), (b, sb, ib) = outs
+    assert not ia["active_first"] and ib["active_first"] and ib["skip_passive_block"]
+    assert a["n_steps"] == b["n_steps"] and a["n_rejected"] == b["n_rejected"]
+    assert rel(b["J"], a["J"]) < 1e-10
+    for x, y in zip(sb, sa):
+        assert rel(x, y) < 1e-10
 
 
 def test_drop_in_f_and_adaptive_dop(native):

@@ -55,6 +55,8 @@ SIGNATURES = {
     "zh_attempt": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info), _dp]),
     "zh_propagate": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_int, C.c_long, C.c_long, _dp, _dp, _dp, _lp, _lp, _dp]),
+    "zh_set_layout": (C.c_int, [C.c_void_p, C.c_int]),
+    "zh_layout_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_set_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, _dp, _lp]),

@@ -90,6 +90,8 @@ struct EngineBase {
   virtual void rhs(const double* Ht, const double* delta, const double* sig, const double* psi,
                    const double* omg, double* dsig, double* dpsi, double* domg, double* J) = 0;
   virtual void attempt(zh_attempt_info* info, double* J) = 0;
+  virtual void set_layout(int mode) = 0;
+  virtual int layout_info(int* out) = 0;
   bool use_graph = true;
   bool profile = false;
   double kc_ms[KC_N] = {0};
@@ -101,7 +103,14 @@ struct EngineBase {
 template <typename R>  // storage real type of Psi / Omega and compute type of their GEMMs
 struct Engine : EngineBase {
   typedef typename Cplx<R>::type cs;
-  int NK, n, NL, NX, NOT, M, Ma, nc = 0;
+  static constexpr int kTile = 64;    // Ω GEMM / Ωξ GEMM row and column tile
+  static constexpr int kPiParts = 8;  // split-K of Π_c = ψᵀ Ixi_c (K = modes)
+  static constexpr int kWParts = 2;   // split-K of W = Ω ξ_c (K = modes)
+  int NK, n, NL, NX, NOT;
+  int M;        // modes per k in the reference layout, NL*NX*NOT
+  int Md = 0;   // modes per k on the device (active block padded to kTile)
+  int ML = 0;   // active block [0, ML); passive modes in [ML, Md)
+  int nc = 0;   // coupled orbitals
   double hbar;
   int device;
   hipStream_t stream = nullptr;
@@ -110,10 +119,19 @@ struct Engine : EngineBase {
   PulseDesc pulse;
   double U = 0.0;
   bool have_consts = false, have_state = false, h_dirty = false;
-  std::vector<double> H0h;  // host copy of H0 (re-installed after an explicit rhs call)
+  bool h_real = false;     // the Hamiltonian in Hcur / Bp has no imaginary part
+  bool xi_real = false;    // ξ and Ixi on the coupled orbitals are real
+  bool pruned = false;     // active-first device mode order in use
+  bool ff = false;         // passive x passive block of Ω is zero and skipped
+  bool ff_dirty = false;   // that block may hold stale values in some slot
+  int layout_req = 0;      // 0 auto, 1 reference mode order, 2 active-first (pruned)
+  std::vector<double> H0h; // host copy of H0 (re-installed after an explicit rhs call)
+  std::vector<int> perm;   // device mode -> reference mode (-1: inert padding mode)
+  std::vector<int> cidx_h; // coupled orbitals in device column order
 
   // device buffers
   int *cidx = nullptr, *cpos = nullptr, *lead = nullptr;
+  int2* t1 = nullptr;
   double2 *Hcur = nullptr, *Ixic = nullptr, *Pic = nullptr, *G = nullptr, *dvec = nullptr;
   double2 *glp = nullptr, *chip = nullptr, *chim = nullptr;
   double* n0 = nullptr;
@@ -127,8 +145,6 @@ struct Engine : EngineBase {
   double* Jd = nullptr;
   double* Jpart = nullptr;
   int iy0 = 7, iyn = 8;
-  static constexpr int kPiParts = 8;  // split-K of Π_c = ψᵀ Ixi_c (K = M)
-  static constexpr int kWParts = 2;   // split-K of W = Ω ξ_c (K = M)
   // pinned host mailboxes
   Ctrl* ctrl_h = nullptr;
   double* J_h = nullptr;
@@ -149,7 +165,6 @@ struct Engine : EngineBase {
   Engine(const zh_dims& d)
       : NK(d.NK), n(d.n), NL(d.NL), NX(d.NX), NOT(d.NOT), hbar(d.hbar), device(d.device) {
     M = NL * NX * NOT;
-    Ma = NX * NOT;
     std::memset(&pulse, 0, sizeof(pulse));
     pulse.hbar = hbar;
     HIPCHK(hipSetDevice(device));
@@ -157,28 +172,11 @@ struct Engine : EngineBase {
     HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&J_h, sizeof(double) * NL * NK, hipHostMallocDefault));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    const size_t nn = (size_t)NK * n * n, mn = (size_t)NK * M * n, mm = (size_t)NK * M * M;
-    for (int s = 0; s < 10; ++s) {
-      sig[s] = dalloc<double2>(nn);
-      if (s != 9) psi[s] = dalloc<cs>(mn);
-      omg[s] = dalloc<cs>(mm);
-    }
-    lead = dalloc<int>(M);
-    Hcur = dalloc<double2>(nn);
-    G = dalloc<double2>(nn);
-    dvec = dalloc<double2>((size_t)NK * n);
-    glp = dalloc<double2>((size_t)NK * M);
-    chip = dalloc<double2>((size_t)NK * M);
-    chim = dalloc<double2>((size_t)NK * M);
-    n0 = dalloc<double>((size_t)NK * n);
-    te_part = dalloc<double>(3 * kTEBlocks);
+    te_part = dalloc<double>(kTEParts * kTEBlocks);
     ctrl = dalloc<Ctrl>(1);
     sp = dalloc<StageParams>(7);
     Jd = dalloc<double>((size_t)NL * NK);
     Jpart = dalloc<double>((size_t)NL * NK * kCurChunks);
-    std::vector<int> ld(M);
-    for (int m = 0; m < M; ++m) ld[m] = m / Ma;
-    HIPCHK(hipMemcpyAsync(lead, ld.data(), sizeof(int) * M, hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
 
@@ -199,68 +197,184 @@ struct Engine : EngineBase {
   }
 
   // ---------------------------------------------------------------- set-up
+  // Device mode order and coupled-orbital order, decided once from the constants.
+  void plan_layout(const std::complex<double>* xi, const std::complex<double>* Ixi,
+                   const std::complex<double>* dgp, const std::complex<double>* dgm) {
+    // coupled orbitals (the reference's pruning, Zandpack/mpi/zand:589-602), grouped by
+    // the first lead that couples them so each lead's orbitals form a short K range
+    std::vector<int> first_lead(n, -1);
+    for (int b = 0; b < NK; ++b)
+      for (int m = 0; m < M; ++m) {
+        const int a = m / (NX * NOT);
+        for (int o = 0; o < n; ++o) {
+          const long off = ((long)b * M + m) * n + o;
+          if ((xi[off] != 0.0 || Ixi[off] != 0.0) && (first_lead[o] < 0 || a < first_lead[o]))
+            first_lead[o] = a;
+        }
+      }
+    cidx_h.clear();
+    for (int a = 0; a < NL; ++a)
+      for (int o = 0; o < n; ++o)
+        if (first_lead[o] == a) cidx_h.push_back(o);
+    nc = (int)cidx_h.size();
+    // active modes: ΔP or ΔM non-zero for some k (Fermi slots have both zero)
+    std::vector<char> active(M, 0);
+    for (int b = 0; b < NK; ++b)
+      for (int m = 0; m < M; ++m)
+        if (dgp[(long)b * M + m] != 0.0 || dgm[(long)b * M + m] != 0.0) active[m] = 1;
+    int na = 0;
+    for (int m = 0; m < M; ++m) na += active[m];
+    const int np = M - na;
+    const int ml_pad = ((na + kTile - 1) / kTile) * kTile;
+    bool use = false;
+    if (layout_req == 2) use = np > 0;
+    else if (layout_req == 0) use = np > 0 && (ml_pad + np) <= M + M / 10;
+    perm.clear();
+    if (use) {
+      for (int m = 0; m < M; ++m)
+        if (active[m]) perm.push_back(m);
+      while ((int)perm.size() < ml_pad) perm.push_back(-1);
+      ML = ml_pad;
+      for (int m = 0; m < M; ++m)
+        if (!active[m]) perm.push_back(m);
+    } else {
+      for (int m = 0; m < M; ++m) perm.push_back(m);
+      ML = M;
+    }
+    Md = (int)perm.size();
+    pruned = use;
+  }
+
+  void allocate() {
+    const size_t nn = (size_t)NK * n * n, mn = (size_t)NK * Md * n, mm = (size_t)NK * Md * Md;
+    for (int s = 0; s < 10; ++s) {
+      sig[s] = dalloc<double2>(nn);
+      if (s != 9) psi[s] = dalloc<cs>(mn);
+      omg[s] = dalloc<cs>(mm);
+    }
+    lead = dalloc<int>(Md);
+    Hcur = dalloc<double2>(nn);
+    G = dalloc<double2>(nn);
+    dvec = dalloc<double2>((size_t)NK * n);
+    glp = dalloc<double2>((size_t)NK * Md);
+    chip = dalloc<double2>((size_t)NK * Md);
+    chim = dalloc<double2>((size_t)NK * Md);
+    n0 = dalloc<double>((size_t)NK * n);
+    const size_t mc = (size_t)NK * Md * nc;
+    cidx = dalloc<int>(nc);
+    cpos = dalloc<int>(n);
+    t1 = dalloc<int2>((Md + kTile - 1) / kTile);
+    Ixic = dalloc<double2>(mc);
+    Pic = dalloc<double2>((size_t)NK * kPiParts * n * nc);
+    xic = dalloc<cs>(mc);
+    W = dalloc<cs>(mc * kWParts);
+    Ap = dalloc<cs>((size_t)NK * Md * (n + nc));
+    Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
+    Aomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
+    Bomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
+  }
+
   void set_constants(const double* H0, const double* xi_, const double* Ixi_, const double* glp_,
                      const double* dgp_, const double* dgm_, const double* xpp_,
                      const double* xpm_) override {
     HIPCHK(hipSetDevice(device));
     drop_graphs();
-    const std::complex<double>* xi = (const std::complex<double>*)xi_;
-    const std::complex<double>* Ixi = (const std::complex<double>*)Ixi_;
-    const std::complex<double>* dgp = (const std::complex<double>*)dgp_;
-    const std::complex<double>* dgm = (const std::complex<double>*)dgm_;
-    // coupled orbitals: where any xi or Ixi entry is non-zero (the reference
-    // prunes orbitals the same way, Zandpack/mpi/zand:589-602)
-    std::vector<int> cp(n, -1), ci;
-    for (int o = 0; o < n; ++o) {
-      bool nz = false;
-      for (long bm = 0; bm < (long)NK * M && !nz; ++bm)
-        nz = (xi[bm * n + o] != 0.0) || (Ixi[bm * n + o] != 0.0);
-      if (nz) { cp[o] = (int)ci.size(); ci.push_back(o); }
-    }
-    require(!have_consts || (int)ci.size() == nc, "coupled-orbital set changed; create a new problem");
+    typedef std::complex<double> cd;
+    const cd* xi = (const cd*)xi_;
+    const cd* Ixi = (const cd*)Ixi_;
+    const cd* dgp = (const cd*)dgp_;
+    const cd* dgm = (const cd*)dgm_;
+    const cd* glpc = (const cd*)glp_;
+    const cd* xpp = (const cd*)xpp_;
+    const cd* xpm = (const cd*)xpm_;
     if (!have_consts) {
-      nc = (int)ci.size();
-      const size_t mc = (size_t)NK * M * nc;
-      cidx = dalloc<int>(nc);
-      cpos = dalloc<int>(n);
-      Ixic = dalloc<double2>(mc);
-      Pic = dalloc<double2>((size_t)NK * kPiParts * n * nc);
-      xic = dalloc<cs>(mc);
-      W = dalloc<cs>(mc * kWParts);
-      Ap = dalloc<cs>((size_t)NK * M * (n + nc));
-      Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
-      Aomg = dalloc<cs>((size_t)NK * M * 2 * nc);
-      Bomg = dalloc<cs>((size_t)NK * M * 2 * nc);
+      plan_layout(xi, Ixi, dgp, dgm);
+      allocate();
+    } else {
+      const int nc0 = nc, Md0 = Md, ML0 = ML;
+      std::vector<int> perm0 = perm, cidx0 = cidx_h;
+      plan_layout(xi, Ixi, dgp, dgm);
+      require(nc == nc0 && Md == Md0 && ML == ML0 && perm == perm0 && cidx_h == cidx0,
+              "coupling structure changed; create a new problem");
     }
-    HIPCHK(hipMemcpyAsync(cidx, ci.data(), sizeof(int) * std::max(nc, 1) * (nc > 0), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(cpos, cp.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
-    const size_t mc = (size_t)NK * M * nc;
-    std::vector<double2> ixic(mc);
-    std::vector<cs> xc(mc), apr(mc), aor(mc), bol(mc);
-    for (long bm = 0; bm < (long)NK * M; ++bm) {
-      for (int c = 0; c < nc; ++c) {
-        const std::complex<double> x = xi[bm * n + ci[c]], ix = Ixi[bm * n + ci[c]];
-        ixic[bm * nc + c] = make_double2(ix.real(), ix.imag());
-        xc[bm * nc + c] = from_d2<cs>(make_double2(x.real(), x.imag()));
-        const std::complex<double> px = dgp[bm] * x, mx = dgm[bm] * ix;
-        apr[bm * nc + c] = from_d2<cs>(make_double2(px.real(), px.imag()));
-        bol[bm * nc + c] = from_d2<cs>(make_double2(mx.real(), mx.imag()));
+    std::vector<int> cp(n, -1);
+    for (int c = 0; c < nc; ++c) cp[cidx_h[c]] = c;
+    // per lead: range of its orbitals in the coupled order (term1 K range)
+    std::vector<int> llo(NL, nc), lhi(NL, 0);
+    for (int b = 0; b < NK; ++b)
+      for (int m = 0; m < M; ++m) {
+        const int a = m / (NX * NOT);
+        for (int c = 0; c < nc; ++c) {
+          const long off = ((long)b * M + m) * n + cidx_h[c];
+          if (xi[off] != 0.0 || Ixi[off] != 0.0) { llo[a] = std::min(llo[a], c); lhi[a] = std::max(lhi[a], c + 1); }
+        }
       }
+    const int ntile = (Md + kTile - 1) / kTile;
+    std::vector<int2> tr(ntile);
+    std::vector<int> ld(Md, 0);
+    for (int d = 0; d < Md; ++d) ld[d] = perm[d] >= 0 ? perm[d] / (NX * NOT) : 0;
+    for (int t = 0; t < ntile; ++t) {
+      int lo = nc, hi = 0;
+      for (int d = t * kTile; d < std::min(Md, (t + 1) * kTile); ++d) {
+        const int m = perm[d];
+        if (m < 0) continue;
+        bool nzm = false;
+        for (int b = 0; b < NK; ++b) nzm = nzm || dgm[(long)b * M + m] != 0.0;
+        if (!nzm) continue;  // term1 vanishes on this column
+        lo = std::min(lo, llo[ld[d]]);
+        hi = std::max(hi, lhi[ld[d]]);
+      }
+      tr[t] = make_int2(lo < hi ? lo : 0, lo < hi ? hi : 0);
     }
+    const size_t mc = (size_t)NK * Md * nc;
+    std::vector<double2> ixic(mc), g1((size_t)NK * Md), x1((size_t)NK * Md), x2((size_t)NK * Md);
+    std::vector<cs> xc(mc), apr(mc), bol(mc);
+    for (int b = 0; b < NK; ++b)
+      for (int d = 0; d < Md; ++d) {
+        const int m = perm[d];
+        const long bd = (long)b * Md + d;
+        if (m < 0) {  // inert padding mode
+          g1[bd] = x1[bd] = x2[bd] = make_double2(0, 0);
+          for (int c = 0; c < nc; ++c) {
+            ixic[bd * nc + c] = make_double2(0, 0);
+            xc[bd * nc + c] = apr[bd * nc + c] = bol[bd * nc + c] = from_d2<cs>(make_double2(0, 0));
+          }
+          continue;
+        }
+        const long bm = (long)b * M + m;
+        g1[bd] = make_double2(glpc[bm].real(), glpc[bm].imag());
+        x1[bd] = make_double2(xpp[bm].real(), xpp[bm].imag());
+        x2[bd] = make_double2(xpm[bm].real(), xpm[bm].imag());
+        for (int c = 0; c < nc; ++c) {
+          const cd x = xi[bm * n + cidx_h[c]], ix = Ixi[bm * n + cidx_h[c]];
+          ixic[bd * nc + c] = make_double2(ix.real(), ix.imag());
+          xc[bd * nc + c] = from_d2<cs>(make_double2(x.real(), x.imag()));
+          const cd px = dgp[bm] * x, mx = dgm[bm] * ix;
+          apr[bd * nc + c] = from_d2<cs>(make_double2(px.real(), px.imag()));
+          bol[bd * nc + c] = from_d2<cs>(make_double2(mx.real(), mx.imag()));
+        }
+      }
+    xi_real = true;
+    for (size_t i = 0; i < mc && xi_real; ++i)
+      xi_real = ixic[i].y == 0.0 && (double)to_d2(xc[i]).y == 0.0;
     if (nc > 0) {
+      HIPCHK(hipMemcpyAsync(cidx, cidx_h.data(), sizeof(int) * nc, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(Ixic, ixic.data(), sizeof(double2) * mc, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(xic, xc.data(), sizeof(cs) * mc, hipMemcpyHostToDevice, stream));
-      // right block of Ap, right block of Aomg, left block of Bomg (constant columns)
+      // constant column blocks of Ap (right), Aomg (right) and Bomg (left)
       HIPCHK(hipMemcpy2DAsync(Ap + n, sizeof(cs) * (n + nc), apr.data(), sizeof(cs) * nc,
-                              sizeof(cs) * nc, (size_t)NK * M, hipMemcpyHostToDevice, stream));
+                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpy2DAsync(Aomg + nc, sizeof(cs) * 2 * nc, apr.data(), sizeof(cs) * nc,
-                              sizeof(cs) * nc, (size_t)NK * M, hipMemcpyHostToDevice, stream));
+                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpy2DAsync(Bomg, sizeof(cs) * 2 * nc, bol.data(), sizeof(cs) * nc,
-                              sizeof(cs) * nc, (size_t)NK * M, hipMemcpyHostToDevice, stream));
+                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyHostToDevice, stream));
     }
-    HIPCHK(hipMemcpyAsync(glp, glp_, sizeof(double2) * NK * M, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(chip, xpp_, sizeof(double2) * NK * M, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(chim, xpm_, sizeof(double2) * NK * M, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(cpos, cp.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(t1, tr.data(), sizeof(int2) * ntile, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(lead, ld.data(), sizeof(int) * Md, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(glp, g1.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(chip, x1.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(chim, x2.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
     H0h.assign(H0, H0 + 2 * (size_t)NK * n * n);
     install_h(H0);
     HIPCHK(hipStreamSynchronize(stream));
@@ -269,6 +383,11 @@ struct Engine : EngineBase {
   }
 
   void install_h(const double* H) {
+    // a real Hamiltonian halves the MFMA work of the Hσ and ψHᵀ products
+    bool real = true;
+    for (size_t i = 0; i < (size_t)NK * n * n && real; ++i) real = H[2 * i + 1] == 0.0;
+    if (real != h_real) drop_graphs();
+    h_real = real;
     HIPCHK(hipMemcpyAsync(Hcur, H, sizeof(double2) * NK * n * n, hipMemcpyHostToDevice, stream));
     dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
     hipLaunchKernelGGL(k_transpose_h<cs>, grid, dim3(32, 8), 0, stream, n,
@@ -278,6 +397,7 @@ struct Engine : EngineBase {
 
   void set_hubbard(double U_, const double* n0_) override {
     require(U_ == 0.0 || NK == 2, "the Hubbard model couples two spin blocks: NK must be 2");
+    require(have_consts, "set the constants first", ZH_ESTATE);
     drop_graphs();
     U = U_;
     if (U != 0.0) {
@@ -293,7 +413,7 @@ struct Engine : EngineBase {
   }
 
   // ------------------------------------------------------------- transfers
-  // host complex128 (rows x cols dense) -> device storage C (leading dim ld)
+  // host complex128 sigma-like (rows x cols dense) -> device storage C (leading dim ld)
   template <typename C>
   void upload(C* dst, long ld, const double* src, long rows, long cols) {
     if (sizeof(C) == sizeof(double2)) {
@@ -324,20 +444,110 @@ struct Engine : EngineBase {
       }
     }
   }
+  // Ψ (NK, M, n) reference rows -> device rows (leading dim ld), permuted
+  void upload_psi(cs* dst, long ld, const double* src) {
+    std::vector<cs> tmp((size_t)NK * Md * n);
+    for (int b = 0; b < NK; ++b)
+      for (int d = 0; d < Md; ++d) {
+        cs* row = tmp.data() + ((size_t)b * Md + d) * n;
+        const int m = perm[d];
+        for (int o = 0; o < n; ++o) {
+          if (m < 0) { row[o] = from_d2<cs>(make_double2(0, 0)); continue; }
+          const double* s = src + 2 * (((size_t)b * M + m) * n + o);
+          row[o] = from_d2<cs>(make_double2(s[0], s[1]));
+        }
+      }
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(cs) * ld, tmp.data(), sizeof(cs) * n, sizeof(cs) * n,
+                            (size_t)NK * Md, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void download_psi(double* dst, const cs* src) {
+    std::vector<cs> tmp((size_t)NK * Md * n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, sizeof(cs) * tmp.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int b = 0; b < NK; ++b)
+      for (int d = 0; d < Md; ++d) {
+        const int m = perm[d];
+        if (m < 0) continue;
+        for (int o = 0; o < n; ++o) {
+          const cs v = tmp[((size_t)b * Md + d) * n + o];
+          double* t = dst + 2 * (((size_t)b * M + m) * n + o);
+          t[0] = (double)v.x;
+          t[1] = (double)v.y;
+        }
+      }
+  }
+  // Ω (NK, M, M) reference -> device (NK, Md, Md), both axes permuted.
+  // Returns whether the passive x passive block is exactly zero.
+  bool upload_omega(cs* dst, const double* src) {
+    std::vector<cs> tmp((size_t)NK * Md * Md);
+    bool ffz = true;
+    for (int b = 0; b < NK; ++b)
+      for (int d1 = 0; d1 < Md; ++d1) {
+        cs* row = tmp.data() + ((size_t)b * Md + d1) * Md;
+        const int m1 = perm[d1];
+        for (int d2 = 0; d2 < Md; ++d2) {
+          const int m2 = perm[d2];
+          if (m1 < 0 || m2 < 0) { row[d2] = from_d2<cs>(make_double2(0, 0)); continue; }
+          const double* s = src + 2 * (((size_t)b * M + m1) * M + m2);
+          row[d2] = from_d2<cs>(make_double2(s[0], s[1]));
+          if (d1 >= ML && d2 >= ML && (s[0] != 0.0 || s[1] != 0.0)) ffz = false;
+        }
+      }
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), sizeof(cs) * tmp.size(), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return ffz;
+  }
+  void download_omega(double* dst, const cs* src) {
+    std::vector<cs> tmp((size_t)NK * Md * Md);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, sizeof(cs) * tmp.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int b = 0; b < NK; ++b)
+      for (int d1 = 0; d1 < Md; ++d1) {
+        const int m1 = perm[d1];
+        if (m1 < 0) continue;
+        for (int d2 = 0; d2 < Md; ++d2) {
+          const int m2 = perm[d2];
+          if (m2 < 0) continue;
+          const cs v = tmp[((size_t)b * Md + d1) * Md + d2];
+          double* t = dst + 2 * (((size_t)b * M + m1) * M + m2);
+          t[0] = (double)v.x;
+          t[1] = (double)v.y;
+        }
+      }
+  }
+  // zero the passive x passive block of every Ω slot
+  void clear_ff() {
+    if (ML >= Md) return;
+    for (int s = 0; s < 10; ++s)
+      for (int b = 0; b < NK; ++b)
+        HIPCHK(hipMemset2DAsync(omg[s] + ((size_t)b * Md + ML) * Md + ML, sizeof(cs) * Md, 0,
+                                sizeof(cs) * (Md - ML), Md - ML, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void set_ff(bool on) {
+    on = on && pruned;
+    if (on != ff) drop_graphs();
+    if (on && (!ff || ff_dirty)) { clear_ff(); ff_dirty = false; }
+    ff = on;
+  }
 
   void state_put(const double* s, const double* p, const double* o) override {
+    require(have_consts, "set the constants first", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
     upload(sig[iy0], n, s, (long)NK * n, n);
-    upload(psi[iy0], n, p, (long)NK * M, n);
-    upload(omg[iy0], M, o, (long)NK * M, M);
+    upload_psi(psi[iy0], n, p);
+    const bool ffz = upload_omega(omg[iy0], o);
+    if (!ffz) ff_dirty = true;
+    set_ff(ffz);
     have_state = true;
   }
   void state_get(double* s, double* p, double* o) override {
     require(have_state, "no resident state", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
     if (s) download(s, sig[iy0], (size_t)NK * n * n);
-    if (p) download(p, psi[iy0], (size_t)NK * M * n);
-    if (o) download(o, omg[iy0], (size_t)NK * M * M);
+    if (p) download_psi(p, psi[iy0]);
+    if (o) download_omega(o, omg[iy0]);
   }
   void sigma_get(double* s) override { state_get(s, nullptr, nullptr); }
 
@@ -361,35 +571,45 @@ struct Engine : EngineBase {
     if (profile) accum(kc);
   }
 
-  void combine(int s, bool first, const double2* sig_dst, cs* omg_dst, bool last) {
+  void set_layout(int mode) override {
+    require(!have_consts, "the layout is fixed once the constants are set", ZH_ESTATE);
+    require(mode >= 0 && mode <= 2, "layout: 0 auto, 1 reference order, 2 active-first");
+    layout_req = mode;
+  }
+  int layout_info(int* out) override {
+    out[0] = pruned ? 1 : 0; out[1] = ff ? 1 : 0; out[2] = Md; out[3] = ML; out[4] = nc;
+    return 5;
+  }
+
+  MapOmega omap() const { return MapOmega{NK, Md, ML, ff ? 1 : 0}; }
+
+  void combine(int s, bool first, double2* sig_dst, cs* omg_dst, bool last) {
     const int nt = first ? 0 : s;
-    auto fill = [&](auto& g, auto* const* ks) {
-      g.nterms = nt;
-      for (int j = 0; j < nt; ++j) { g.k[j] = ks[j]; g.a[j] = tab.a[s][j]; }
-    };
     const int blocks = 2048;
     timed(KC_COMBINE, [&] {
       if (!first) {
         CombineArgs<double2> gs{};
         gs.y0 = sig[iy0];
-        fill(gs, sig);
-        gs.d1 = (double2*)sig_dst; gs.ld1 = n; gs.d2 = nullptr; gs.ld2 = 0;
-        gs.rows = (long)NK * n; gs.cols = n;
-        hipLaunchKernelGGL(k_combine<double2>, dim3(blocks), dim3(256), 0, stream, gs);
+        gs.nterms = nt;
+        for (int j = 0; j < nt; ++j) { gs.k[j] = sig[j]; gs.a[j] = tab.a[s][j]; }
+        gs.d1 = sig_dst; gs.d2 = nullptr;
+        MapDense ms{NK * n, n, n, n, n};
+        launch_combine(gs, ms, blocks, stream);
         CombineArgs<cs> go{};
         go.y0 = omg[iy0];
-        fill(go, omg);
-        go.d1 = omg_dst; go.ld1 = M; go.d2 = nullptr; go.ld2 = 0;
-        go.rows = (long)NK * M; go.cols = M;
-        hipLaunchKernelGGL(k_combine<cs>, dim3(blocks), dim3(256), 0, stream, go);
+        go.nterms = nt;
+        for (int j = 0; j < nt; ++j) { go.k[j] = omg[j]; go.a[j] = tab.a[s][j]; }
+        go.d1 = omg_dst; go.d2 = nullptr;
+        launch_combine(go, omap(), blocks, stream);
       }
       CombineArgs<cs> gp{};
       gp.y0 = psi[iy0];
-      fill(gp, psi);
-      gp.d1 = Ap; gp.ld1 = n + nc;
-      gp.d2 = last ? psi[iyn] : nullptr; gp.ld2 = n;
-      gp.rows = (long)NK * M; gp.cols = n;
-      hipLaunchKernelGGL(k_combine<cs>, dim3(blocks), dim3(256), 0, stream, gp);
+      gp.nterms = nt;
+      for (int j = 0; j < nt; ++j) { gp.k[j] = psi[j]; gp.a[j] = tab.a[s][j]; }
+      gp.d1 = Ap;
+      gp.d2 = last ? psi[iyn] : nullptr;
+      MapDense mp{NK * Md, n, n, n + nc, n};
+      launch_combine(gp, mp, blocks, stream);
     });
     HIPCHK(hipGetLastError());
   }
@@ -401,8 +621,8 @@ struct Engine : EngineBase {
     const double ih = 1.0 / hbar;
     timed(KC_GATHER, [&] {
       GatherArgs<cs> g;
-      g.NK = NK; g.n = n; g.M = M; g.nc = nc;
-      g.Ap = Ap; g.ldAp = n + nc; g.sAp = (long)M * (n + nc);
+      g.NK = NK; g.n = n; g.M = Md; g.nc = nc;
+      g.Ap = Ap; g.ldAp = n + nc; g.sAp = (long)Md * (n + nc);
       g.Aomg = Aomg; g.Bomg = Bomg; g.Bp = Bp; g.sig = sig_in; g.cidx = cidx;
       g.dvec = dvec; g.n0 = n0; g.U = U; g.hbar = hbar; g.hubbard = hub && U != 0.0;
       hipLaunchKernelGGL(k_gather<cs>, dim3(512, 3), dim3(256), 0, stream, g);
@@ -410,9 +630,9 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
     if (nc > 0) {
       timed(KC_GEMM_PI, [&] {
-        // Π_c = ψᵀ Ixi_c  (n x M) (M x nc), FP64
-        GemmArgs<R, double> g{n, nc, M, Ap, n + nc, (long)M * (n + nc), Ixic, nc, (long)M * nc,
-                              kPiParts};
+        // Π_c = ψᵀ Ixi_c  (n x Md) (Md x nc), FP64, split over K
+        GemmArgs<R, double> g{n, nc, Md, Ap, n + nc, (long)Md * (n + nc), Ixic, nc,
+                              (long)Md * nc, kPiParts, 0, xi_real ? Md : 0};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
         HIPCHK((launch_zgemm<double, R, double, true, false, 64, 32, 16>(g, NK, e, stream)));
       });
@@ -420,15 +640,16 @@ struct Engine : EngineBase {
     if (currents) {
       // with no coupled orbitals the reductions run over empty ranges and give J = 0
       timed(KC_CURRENTS, [&] {
-        hipLaunchKernelGGL(k_currents<cs>, dim3(kCurChunks, NL, NK), dim3(256), 0, stream, NK, M,
-                           Ma, nc, (const cs*)Aomg, (const double2*)Ixic, Jpart);
+        hipLaunchKernelGGL(k_currents<cs>, dim3(kCurChunks, NL, NK), dim3(256), 0, stream, NK, Md,
+                           nc, (const int*)lead, (const cs*)Aomg, (const double2*)Ixic, Jpart);
         hipLaunchKernelGGL(k_currents_final, dim3(1), dim3(((NL * NK + 63) / 64) * 64), 0, stream,
                            NL * NK, (const double*)Jpart, ih * ih, Jd);
       });
       HIPCHK(hipGetLastError());
     }
     timed(KC_GEMM_HSIG, [&] {
-      GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n};
+      GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n, 1,
+                                 h_real ? n : 0, 0};
       EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih, kPiParts};
       HIPCHK((launch_zgemm<double, double, double, false, false, 64, 64, 16>(g, NK, e, stream)));
     });
@@ -440,53 +661,58 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
     if (nc > 0) {
       timed(KC_GEMM_OMGXI, [&] {
-        // W = Ω ξ_c  (M x M)(M x nc)
-        GemmArgs<R, R> g{M, nc, M, om_in, M, (long)M * M, xic, nc, (long)M * nc, kWParts};
-        EpiStore<cs> e{W, nc, (long)M * nc, 1.0, 0.0};
-        HIPCHK((launch_zgemm<R, R, R, false, false, 64, 32, 16>(g, NK, e, stream)));
+        // W = Ω ξ_c  (Md x Md)(Md x nc)
+        GemmArgs<R, R> g{Md, nc, Md, om_in, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts,
+                         0, xi_real ? Md : 0};
+        EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
+        KSegOmgXi ks{ML, kTile, ff ? 1 : 0};
+        HIPCHK((launch_zgemm<R, R, R, false, false, kTile, 32, 16>(g, NK, e, stream, ks)));
       });
     }
     timed(KC_GEMM_PSI, [&] {
-      // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (M x (n+nc)) ((n+nc) x n)
-      GemmArgs<R, R> g{M, n, n + nc, Ap, n + nc, (long)M * (n + nc), Bp, n, (long)(n + nc) * n};
-      EpiPsi<cs> e{psi[kslot], Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps, M, n, nc,
+      // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (Md x (n+nc)) ((n+nc) x n)
+      GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n,
+                       1, 0, h_real ? n : 0};
+      EpiPsi<cs> e{psi[kslot], Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps, Md, n, nc,
                    ih, ih * ih, kWParts};
       HIPCHK((launch_zgemm<R, R, R, false, false, 64, 64, 16>(g, NK, e, stream)));
     });
     timed(KC_GEMM_OMEGA, [&] {
-      // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (M x 2nc)(2nc x M), outer-subtraction epilogue
-      EpiOmega<cs> e{omg[kslot], om_in, chip, chim, lead, sps, M, ih};
-      if (nc > 0) {
-        GemmArgs<R, R> g{M, M, 2 * nc, Aomg, 2 * nc, (long)M * 2 * nc, Bomg, 2 * nc,
-                         (long)M * 2 * nc};
-        HIPCHK((launch_zgemm<R, R, R, false, true, 64, 64, 16>(g, NK, e, stream)));
-      } else {
-        GemmArgs<R, R> g{M, M, 0, Aomg, 1, 0, Bomg, 1, 0};
-        HIPCHK((launch_zgemm<R, R, R, false, true, 64, 64, 16>(g, NK, e, stream)));
-      }
+      // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), outer-subtraction epilogue
+      EpiOmega<cs> e{omg[kslot], om_in, chip, chim, lead, sps, Md, ih};
+      KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};
+      GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
+                       (long)Md * 2 * nc};
+      HIPCHK((launch_zgemm<R, R, R, false, true, kTile, kTile, 16>(g, NK, e, stream, ks)));
     });
   }
 
   void te_and_control() {
     timed(KC_TE, [&] {
+      // slots with a non-zero error weight (b5 - b4 vanishes for k_2)
       TEArgs<double2> ts{};
       TEArgs<cs> tp{}, to{};
+      int nw = 0;
       for (int j = 0; j < 7; ++j) {
-        ts.k[j] = sig[j]; tp.k[j] = psi[j]; to.k[j] = omg[j];
-        ts.ct[j] = tp.ct[j] = to.ct[j] = tab.ct[j];
+        if (tab.ct[j] == 0.0) continue;
+        ts.k[nw] = sig[j]; tp.k[nw] = psi[j]; to.k[nw] = omg[j];
+        ts.ct[nw] = tp.ct[nw] = to.ct[nw] = tab.ct[j];
+        ++nw;
       }
-      ts.count = (long)NK * n * n;
-      tp.count = (long)NK * M * n;
-      to.count = (long)NK * M * M;
-      hipLaunchKernelGGL(k_te<double2>, dim3(kTEBlocks), dim3(256), 0, stream, ts, te_part);
-      hipLaunchKernelGGL(k_te<cs>, dim3(kTEBlocks), dim3(256), 0, stream, tp, te_part + kTEBlocks);
-      hipLaunchKernelGGL(k_te<cs>, dim3(kTEBlocks), dim3(256), 0, stream, to,
-                         te_part + 2 * kTEBlocks);
+      require(nw == 6, "error weights: expected six non-zero entries");
+      MapDense ms{NK * n, n, n, n, n};
+      MapDense mp{NK * Md, n, n, n, n};
+      hipLaunchKernelGGL((k_te<double2, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, ts,
+                         ms, te_part);
+      hipLaunchKernelGGL((k_te<cs, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, tp, mp,
+                         te_part + kTEBlocks);
+      hipLaunchKernelGGL((k_te<cs, MapOmega, 6>), dim3(kTEBlocks), dim3(256), 0, stream, to,
+                         omap(), te_part + 2 * kTEBlocks);
     });
     HIPCHK(hipGetLastError());
     timed(KC_CTRL, [&] {
       hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, stream, (const double*)te_part,
-                         3 * kTEBlocks, ctrl);
+                         kTEParts * kTEBlocks, ctrl);
     });
     HIPCHK(hipGetLastError());
   }
@@ -524,6 +750,7 @@ struct Engine : EngineBase {
     require(have_state, "no resident state", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
     ensure_h0();
+    if (ff && ff_dirty) { clear_ff(); ff_dirty = false; }
     ctrl_h->t0 = info->t0;
     ctrl_h->h = info->h;
     ctrl_h->eps = info->eps;
@@ -572,13 +799,18 @@ struct Engine : EngineBase {
     for (int a = 0; a < NL; ++a) st.delta[a] = delta[a];
     HIPCHK(hipMemcpyAsync(sp, &st, sizeof(st), hipMemcpyHostToDevice, stream));
     upload(sig[9], n, s, (long)NK * n, n);
-    upload(Ap, n + nc, p, (long)NK * M, n);
-    upload(omg[9], M, o, (long)NK * M, M);
+    upload_psi(Ap, n + nc, p);
+    upload_omega(omg[9], o);
+    // the input's passive block may be non-zero: evaluate it in full
+    const bool ff_keep = ff;
+    ff = false;
     rhs_stage(0, sig[9], omg[9], 0, J != nullptr, false);
     HIPCHK(hipStreamSynchronize(stream));
+    ff = ff_keep;
+    ff_dirty = true;
     if (ds) download(ds, sig[0], (size_t)NK * n * n);
-    if (dp) download(dp, psi[0], (size_t)NK * M * n);
-    if (dom) download(dom, omg[0], (size_t)NK * M * M);
+    if (dp) download_psi(dp, psi[0]);
+    if (dom) download_omega(dom, omg[0]);
     if (J) {
       HIPCHK(hipMemcpy(J, Jd, sizeof(double) * NL * NK, hipMemcpyDeviceToHost));
     }
@@ -772,6 +1004,20 @@ int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int
     *n_steps = steps;
     *n_rejected = rej;
     if (h_final) *h_final = h;
+  });
+}
+
+int zh_set_layout(zh_problem* p, int mode) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_layout(mode);
+  });
+}
+
+int zh_layout_info(zh_problem* p, int* out) {
+  return zh_try([&] {
+    require(p && out, "null argument");
+    p->eng->layout_info(out);
   });
 }
 

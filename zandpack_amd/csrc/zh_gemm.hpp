@@ -54,19 +54,33 @@ template <> struct Cplx<float> { typedef float2 type; };
 // splitk > 1 splits the K range over blockIdx.z = batch * splitk + part; the
 // epilogue then receives z (not the batch index) and must keep the partial
 // products apart (they are summed by the consumer in a fixed order).
+// kreal_a / kreal_b: K tiles lying entirely below these bounds have a real A
+// (resp. B) operand there, and the complex product drops the vanishing real
+// MFMAs (2 instead of 4 per tile pair).
 template <typename RA, typename RB>
 struct GemmArgs {
   int M, N, K;
   const typename Cplx<RA>::type* A; long lda; long sA;
   const typename Cplx<RB>::type* B; long ldb; long sB;
   int splitk = 1;
+  int kreal_a = 0, kreal_b = 0;
 };
 
 constexpr int kThreads = 256;
 
+// K-range selection per output tile (tm = row tile, tn = column tile): up to two
+// half-open segments of the K axis; tiles whose operands are known to vanish are
+// skipped, and a tile with no segment gets acc = 0 (its epilogue still runs).
+struct KFull {
+  __device__ inline int get(int, int, int K, int* lo, int* hi) const {
+    lo[0] = 0; hi[0] = K; return 1;
+  }
+  __device__ inline bool skip(int, int) const { return false; }
+};
+
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
-          int BM, int BN, int BK, typename Epi>
-__global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi) {
+          int BM, int BN, int BK, typename Epi, typename KSeg>
+__global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi, KSeg ks) {
   static_assert(BM % 32 == 0 && BN % 32 == 0 && BK % 4 == 0, "tile shape");
   constexpr int TM = BM / 32;  // 16x16 MFMA tiles per wave along M (2x2 waves)
   constexpr int TN = BN / 32;
@@ -87,6 +101,7 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
+  if (ks.skip(blockIdx.y, blockIdx.x)) return;  // uniform over the workgroup
   const int batch = blockIdx.z / g.splitk;
   const int part = blockIdx.z - batch * g.splitk;
   const int m0 = blockIdx.y * BM;
@@ -102,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
 
   ca_t ra[LA];
   cb_t rb[LB];
-  auto load_tiles = [&](int k0) {
+  auto load_tiles = [&](int k0, int khi) {
 #pragma unroll
     for (int it = 0; it < LA; ++it) {
       const int e = tid + it * kThreads;
@@ -110,7 +125,7 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
       const int gm = m0 + m, gk = k0 + k;
       ca_t v; v.x = 0; v.y = 0;
-      if (gm < g.M && gk < g.K) v = TRANS_A ? A[(long)gk * g.lda + gm] : A[(long)gm * g.lda + gk];
+      if (gm < g.M && gk < khi) v = TRANS_A ? A[(long)gk * g.lda + gm] : A[(long)gm * g.lda + gk];
       ra[it] = v;
     }
 #pragma unroll
@@ -120,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       if (!TRANS_B) { k = e / BN; n = e % BN; } else { n = e / BK; k = e % BK; }
       const int gn = n0 + n, gk = k0 + k;
       cb_t v; v.x = 0; v.y = 0;
-      if (gn < g.N && gk < g.K) v = TRANS_B ? B[(long)gn * g.ldb + gk] : B[(long)gk * g.ldb + gn];
+      if (gn < g.N && gk < khi) v = TRANS_B ? B[(long)gn * g.ldb + gk] : B[(long)gk * g.ldb + gn];
       rb[it] = v;
     }
   };
@@ -143,18 +158,30 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
     }
   };
 
-  // K range of this part, in whole BK tiles
-  const int ktot = (g.K + BK - 1) / BK;
+  // K tiles of this tile's segments, split evenly over the splitk parts
+  int slo[2] = {0, 0}, shi[2] = {0, 0};
+  const int nseg = ks.get(blockIdx.y, blockIdx.x, g.K, slo, shi);
+  const int nt0 = nseg > 0 ? (shi[0] - slo[0] + BK - 1) / BK : 0;
+  const int nt1 = nseg > 1 ? (shi[1] - slo[1] + BK - 1) / BK : 0;
+  const int ktot = nt0 + nt1;
   const int kper = (ktot + g.splitk - 1) / g.splitk;
   const int kt0 = part * kper;
   const int ktiles = max(0, min(ktot, kt0 + kper) - kt0);
-  if (ktiles > 0) load_tiles(kt0 * BK);
+  auto tile_k0 = [&](int t) { return t < nt0 ? slo[0] + t * BK : slo[1] + (t - nt0) * BK; };
+  auto tile_load = [&](int t) {
+    if (t < nt0) load_tiles(slo[0] + t * BK, shi[0]);
+    else load_tiles(slo[1] + (t - nt0) * BK, shi[1]);
+  };
+  if (ktiles > 0) tile_load(kt0);
   const int lr = lane & 15, lk = lane >> 4;
   for (int kt = 0; kt < ktiles; ++kt) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (kt + 1 < ktiles) load_tiles((kt0 + kt + 1) * BK);
+    const int k0c = tile_k0(kt0 + kt);
+    const bool real_a = k0c + BK <= g.kreal_a;  // uniform over the workgroup
+    const bool real_b = k0c + BK <= g.kreal_b;
+    if (kt + 1 < ktiles) tile_load(kt0 + kt + 1);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
@@ -171,15 +198,33 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
         br[j] = sBr[kk + lk][n];
         bi[j] = sBi[kk + lk][n];
       }
+      if (real_b) {
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
-          ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
-          cr[i][j] = Mfma<T>::op(nai[i], bi[j], cr[i][j]);
-          ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
-        }
+          for (int j = 0; j < TN; ++j) {
+            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
+            ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
+          }
+      } else if (real_a) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
+            ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
+            ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
+            cr[i][j] = Mfma<T>::op(nai[i], bi[j], cr[i][j]);
+            ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
+          }
+      }
     }
   }
 
@@ -196,13 +241,13 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
 }
 
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, int BM, int BN,
-          int BK, typename Epi>
+          int BK, typename Epi, typename KSeg = KFull>
 inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& epi,
-                               hipStream_t s) {
+                               hipStream_t s, const KSeg& ks = KSeg()) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, batch * g.splitk);
-  hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, BM, BN, BK, Epi>), grid,
-                     dim3(kThreads), 0, s, g, epi);
+  hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, BM, BN, BK, Epi, KSeg>), grid,
+                     dim3(kThreads), 0, s, g, epi, ks);
   return hipGetLastError();
 }
 

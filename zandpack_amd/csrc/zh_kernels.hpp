@@ -72,34 +72,75 @@ __global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, Tableau tab, i
   }
 }
 
-// dst = y0 + sum_j a_j k_j over a (rows x cols) block; sources are dense with
-// row length cols, destinations have their own leading dimension.
+// Row maps of the elementwise slot kernels: visited row r -> offsets of its first
+// element in the source slots and in the destination(s), and its visited length.
+struct MapDense {  // rows x cols block, source row stride lds, destination strides ld1 / ld2
+  int rows, cols; long lds, ld1, ld2;
+  __host__ __device__ inline int nrows() const { return rows; }
+  __device__ inline int row(int r, long& src, long& d1, long& d2) const {
+    src = r * lds; d1 = r * ld1; d2 = r * ld2;
+    return cols;
+  }
+};
+// NK blocks of (Md x Md) Ω; with ff the passive x passive corner [ML,Md)^2 is not
+// visited (it is identically zero in every slot).
+struct MapOmega {
+  int NK, Md, ML, ff;
+  __host__ __device__ inline int nrows() const { return NK * Md; }
+  __device__ inline int row(int r, long& src, long& d1, long& d2) const {
+    src = d1 = d2 = (long)r * Md;
+    const int i = r % Md;
+    return (ff && i >= ML) ? ML : Md;
+  }
+};
+
+// dst = y0 + sum_j a_j k_j   (Zandpack/mpi/zand:306-313, 722-735)
 template <typename C>
 struct CombineArgs {
   const C* y0;
   const C* k[7];
   double a[7];
   int nterms;
-  C* d1; long ld1;
-  C* d2; long ld2;
-  long rows; int cols;
+  C* d1;
+  C* d2;
 };
 
-template <typename C>
-__global__ void k_combine(CombineArgs<C> g) {
-  const long total = g.rows * (long)g.cols;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
-    double2 v = to_d2(g.y0[idx]);
-    for (int j = 0; j < g.nterms; ++j) {
-      const double2 kv = to_d2(g.k[j][idx]);
-      v.x += g.a[j] * kv.x;
-      v.y += g.a[j] * kv.y;
+// NT = number of stage terms, a compile-time constant so that all NT+1 slot
+// loads of an element are in flight together.
+template <typename C, typename Map, int NT>
+__global__ __launch_bounds__(256) void k_combine(CombineArgs<C> g, Map mp) {
+  const int nr = mp.nrows();
+  for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+    long src, o1, o2;
+    const int cols = mp.row(r, src, o1, o2);
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+      C kv[NT + 1];
+      kv[0] = g.y0[src + c];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) kv[j + 1] = g.k[j][src + c];
+      double vr = (double)kv[0].x, vi = (double)kv[0].y;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        vr += g.a[j] * (double)kv[j + 1].x;
+        vi += g.a[j] * (double)kv[j + 1].y;
+      }
+      const C out = from_d2<C>(make_double2(vr, vi));
+      g.d1[o1 + c] = out;
+      if (g.d2) g.d2[o2 + c] = out;
     }
-    const long r = idx / g.cols, c = idx - r * g.cols;
-    const C out = from_d2<C>(v);
-    g.d1[r * g.ld1 + c] = out;
-    if (g.d2) g.d2[r * g.ld2 + c] = out;
+  }
+}
+
+template <typename C, typename Map>
+inline void launch_combine(const CombineArgs<C>& g, const Map& mp, int blocks, hipStream_t s) {
+  switch (g.nterms) {
+#define ZH_COMBINE_CASE(N)                                                                   \
+  case N:                                                                                    \
+    hipLaunchKernelGGL((k_combine<C, Map, N>), dim3(blocks), dim3(256), 0, s, g, mp); break;
+    ZH_COMBINE_CASE(0) ZH_COMBINE_CASE(1) ZH_COMBINE_CASE(2) ZH_COMBINE_CASE(3)
+    ZH_COMBINE_CASE(4) ZH_COMBINE_CASE(5) ZH_COMBINE_CASE(6)
+#undef ZH_COMBINE_CASE
+    default: break;
   }
 }
 
@@ -211,15 +252,16 @@ __global__ void k_sigma_sym(int n, const double2* G, double2* out, const StagePa
 constexpr int kCurChunks = 64;
 
 template <typename C>
-__global__ void k_currents(int NK, int M, int Ma, int nc, const C* Aomg, const double2* Ixic,
-                           double* part) {
+__global__ void k_currents(int NK, int M, int nc, const int* lead, const C* Aomg,
+                           const double2* Ixic, double* part) {
   const int chunk = blockIdx.x, a = blockIdx.y, b = blockIdx.z;
-  const long total = (long)Ma * nc;
+  const long total = (long)M * nc;
   const long per = (total + kCurChunks - 1) / kCurChunks;
   const long lo = chunk * per, hi = min(total, lo + per);
   double acc = 0.0;
   for (long idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
-    const long m = (long)a * Ma + idx / nc;
+    const long m = idx / nc;
+    if (lead[m] != a) continue;
     const int c = (int)(idx % nc);
     const double2 p = to_d2(Aomg[((long)b * M + m) * 2 * nc + c]);
     const double2 x = Ixic[((long)b * M + m) * nc + c];
@@ -243,31 +285,37 @@ __global__ void k_currents_final(int nlk, const double* part, double inv_hbar2, 
   J[i] = 2.0 * s * inv_hbar2;
 }
 
-// Partial sums of |sum_j ct_j k_j|^2 (Zandpack/mpi_funcs.py:247-252).
+// Partial sums of |sum_j ct_j k_j|^2 over the visited set of one state part
+// (Zandpack/mpi_funcs.py:247-252).
 template <typename C>
 struct TEArgs {
   const C* k[7];
   double ct[7];
-  long count;
 };
 
 constexpr int kTEBlocks = 1024;
+constexpr int kTEParts = 3;  // sigma, Psi, Omega
 
-template <typename C>
-__global__ void k_te(TEArgs<C> g, double* partial) {
+// The slots with a non-zero error weight are packed in front (k[0..NW)).
+template <typename C, typename Map, int NW>
+__global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial) {
   double acc = 0.0;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < g.count;
-       idx += (long)gridDim.x * blockDim.x) {
-    double vr = 0.0, vi = 0.0;
+  const int nr = mp.nrows();
+  for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+    long src, o1, o2;
+    const int cols = mp.row(r, src, o1, o2);
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+      C kv[NW];
 #pragma unroll
-    for (int j = 0; j < 7; ++j) {
-      if (g.ct[j] != 0.0) {
-        const double2 kv = to_d2(g.k[j][idx]);
-        vr += g.ct[j] * kv.x;
-        vi += g.ct[j] * kv.y;
+      for (int j = 0; j < NW; ++j) kv[j] = g.k[j][src + c];
+      double vr = 0.0, vi = 0.0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        vr += g.ct[j] * (double)kv[j].x;
+        vi += g.ct[j] * (double)kv[j].y;
       }
+      acc += vr * vr + vi * vi;
     }
-    acc += vr * vr + vi * vi;
   }
   __shared__ double red[256];
   red[threadIdx.x] = acc;
@@ -397,6 +445,40 @@ struct EpiOmega {
     const double2 t = cmul(make_double2(-di * h * inv_hbar, dr * h * inv_hbar), o);
     out[off] = from_d2<C>(make_double2(h * re + t.x, h * im + t.y));
   }
+};
+
+// ------------------------------------------------------------ K segments
+// Device mode order: active modes (ΔP or ΔM non-zero) in [0, ML), ML a multiple
+// of the Ω tiles, passive (Fermi) modes after.  For the Ω sources
+//   term1 (K = [0, nc))   vanishes on passive columns (ΔM_j = 0) and needs only
+//                          the coupled orbitals of the column's lead(s);
+//   term2 (K = [nc, 2nc)) vanishes on passive rows (ΔP_i = 0);
+// the passive x passive block of Ω stays exactly zero when it starts at zero
+// (ff = 1) and is skipped.
+struct KSegOmega {
+  const int2* t1;  // per column tile: term1 K range [x, y)
+  int nc, ML, BM, BN, ff;
+  __device__ inline int get(int tm, int tn, int, int* lo, int* hi) const {
+    int n = 0;
+    const int2 r = t1[tn];
+    if (r.y > r.x) { lo[n] = r.x; hi[n] = r.y; ++n; }
+    if (tm * BM < ML) { lo[n] = nc; hi[n] = 2 * nc; ++n; }
+    return n;
+  }
+  __device__ inline bool skip(int tm, int tn) const {
+    return ff && tm * BM >= ML && tn * BN >= ML;
+  }
+};
+
+// W = Ω ξ_c: passive rows see only active columns when the passive block is zero.
+struct KSegOmgXi {
+  int ML, BM, ff;
+  __device__ inline int get(int tm, int, int K, int* lo, int* hi) const {
+    lo[0] = 0;
+    hi[0] = (ff && tm * BM >= ML) ? ML : K;
+    return 1;
+  }
+  __device__ inline bool skip(int, int) const { return false; }
 };
 
 }  // namespace zh

@@ -16,11 +16,12 @@ from . import _native as N
 from .builder import AMEProblem
 
 PRECISIONS = {"fp64": N.ZH_FP64, "mixed": N.ZH_MIXED}
+LAYOUTS = {"auto": 0, "reference": 1, "pruned": 2}
 
 
 class Propagator:
     def __init__(self, problem: AMEProblem, precision: str = "fp64", device: int = 0,
-                 graph: bool = True, state=None):
+                 graph: bool = True, state=None, layout: str = "auto"):
         self.lib = N.lib()
         self.problem = problem
         self.precision = precision
@@ -31,6 +32,7 @@ class Propagator:
         h = C.c_void_p()
         N.check(self.lib.zh_problem_create(C.byref(d), C.byref(h)))
         self.h = h
+        N.check(self.lib.zh_set_layout(self.h, LAYOUTS[layout]))
         p = problem
         self._keep = [N.c128(a) for a in (p.H0, p.xi, p.Ixi, p.GL_P, p.dGP, p.dGM, p.Xpp, p.Xpm)]
         N.check(self.lib.zh_set_constants(self.h, *[N.ptr(a) for a in self._keep]))
@@ -48,6 +50,12 @@ class Propagator:
         w = np.ascontiguousarray(w[: self.NL])
         N.check(self.lib.zh_set_pulse(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
                                       float(pulse.p1), float(pulse.p2), N.ptr(w)))
+
+    def layout_info(self):
+        out = (C.c_int * 5)()
+        N.check(self.lib.zh_layout_info(self.h, out))
+        return dict(active_first=bool(out[0]), skip_passive_block=bool(out[1]),
+                    device_modes=out[2], active_block=out[3], coupled_orbitals=out[4])
 
     def set_graph(self, on: bool):
         N.check(self.lib.zh_set_graph(self.h, 1 if on else 0))
