@@ -100,6 +100,23 @@ struct EngineBase {
   long attempt_count = 0;
 };
 
+// GEMM tiles per product, picked by a sweep over tile shapes at n=1000, M=4048
+// on MI355X: small single-buffered tiles keep 2-3 workgroups per CU, which hides
+// the global-load latency better than larger or double-buffered tiles.
+template <typename R> struct Tiles;
+template <> struct Tiles<double> {
+  typedef Tile<64, 64, 8, 2> psi;
+  typedef Tile<64, 64, 16, 2> omega;
+  typedef Tile<64, 32, 16, 2> omgxi;
+};
+template <> struct Tiles<float> {
+  typedef Tile<64, 64, 32, 2> psi;
+  typedef Tile<64, 64, 16, 2> omega;
+  typedef Tile<64, 32, 32, 2> omgxi;
+};
+typedef Tile<64, 32, 16, 2> TileHsig;
+typedef Tile<64, 32, 16, 2> TilePi;
+
 template <typename R>  // storage real type of Psi / Omega and compute type of their GEMMs
 struct Engine : EngineBase {
   typedef typename Cplx<R>::type cs;
@@ -634,7 +651,7 @@ struct Engine : EngineBase {
         GemmArgs<R, double> g{n, nc, Md, Ap, n + nc, (long)Md * (n + nc), Ixic, nc,
                               (long)Md * nc, kPiParts, 0, xi_real ? Md : 0};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
-        HIPCHK((launch_zgemm<double, R, double, true, false, 64, 32, 16>(g, NK, e, stream)));
+        HIPCHK((launch_zgemm<double, R, double, true, false, TilePi>(g, NK, e, stream)));
       });
     }
     if (currents) {
@@ -651,7 +668,7 @@ struct Engine : EngineBase {
       GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n, 1,
                                  h_real ? n : 0, 0};
       EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih, kPiParts};
-      HIPCHK((launch_zgemm<double, double, double, false, false, 64, 64, 16>(g, NK, e, stream)));
+      HIPCHK((launch_zgemm<double, double, double, false, false, TileHsig>(g, NK, e, stream)));
     });
     timed(KC_SIGSYM, [&] {
       dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
@@ -665,8 +682,9 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, nc, Md, om_in, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts,
                          0, xi_real ? Md : 0};
         EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
+        static_assert(Tiles<R>::omgxi::BM == kTile, "omgxi tile");
         KSegOmgXi ks{ML, kTile, ff ? 1 : 0};
-        HIPCHK((launch_zgemm<R, R, R, false, false, kTile, 32, 16>(g, NK, e, stream, ks)));
+        HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::omgxi>(g, NK, e, stream, ks)));
       });
     }
     timed(KC_GEMM_PSI, [&] {
@@ -675,15 +693,16 @@ struct Engine : EngineBase {
                        1, 0, h_real ? n : 0};
       EpiPsi<cs> e{psi[kslot], Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps, Md, n, nc,
                    ih, ih * ih, kWParts};
-      HIPCHK((launch_zgemm<R, R, R, false, false, 64, 64, 16>(g, NK, e, stream)));
+      HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi>(g, NK, e, stream)));
     });
     timed(KC_GEMM_OMEGA, [&] {
       // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), outer-subtraction epilogue
       EpiOmega<cs> e{omg[kslot], om_in, chip, chim, lead, sps, Md, ih};
+      static_assert(Tiles<R>::omega::BM == kTile && Tiles<R>::omega::BN == kTile, "omega tile");
       KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
-      HIPCHK((launch_zgemm<R, R, R, false, true, kTile, kTile, 16>(g, NK, e, stream, ks)));
+      HIPCHK((launch_zgemm<R, R, R, false, true, typename Tiles<R>::omega>(g, NK, e, stream, ks)));
     });
   }
 
@@ -834,7 +853,7 @@ static void selftest_impl(int M, int N, int K, const double* A, const double* B,
   HIPCHK(hipMemcpy(db, hb.data(), sizeof(cs) * nb, hipMemcpyHostToDevice));
   GemmArgs<R, R> g{M, N, K, da, TA ? M : K, 0, db, TB ? K : N, 0};
   EpiStore<double2> e{dc, N, 0, 1.0, 0.0};
-  HIPCHK((launch_zgemm<R, R, R, TA, TB, 64, 64, 16>(g, 1, e, nullptr)));
+  HIPCHK((launch_zgemm<R, R, R, TA, TB, typename Tiles<R>::psi>(g, 1, e, nullptr)));
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(C, dc, sizeof(double2) * ncn, hipMemcpyDeviceToHost));
   (void)hipFree(da);

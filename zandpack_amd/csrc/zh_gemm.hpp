@@ -72,18 +72,25 @@ constexpr int kThreads = 256;
 // half-open segments of the K axis; tiles whose operands are known to vanish are
 // skipped, and a tile with no segment gets acc = 0 (its epilogue still runs).
 struct KFull {
-  __device__ inline int get(int, int, int K, int* lo, int* hi) const {
-    lo[0] = 0; hi[0] = K; return 1;
+  __device__ inline int get(int, int, int K, int& lo0, int& hi0, int&, int&) const {
+    lo0 = 0; hi0 = K; return 1;
   }
   __device__ inline bool skip(int, int) const { return false; }
 };
 
+// Workgroup: 256 threads = 4 waves laid out WM x (4/WM) over a BM x BN output
+// tile; each wave owns a (BM/WM) x (BN*WM/4) block of 16x16 MFMA tiles.  The
+// global loads of K tile t+1 are issued before the MFMAs of tile t.  With DB the
+// LDS holds two K tiles and one barrier per tile separates the stages; without,
+// the single LDS tile costs a second barrier but keeps more workgroups per CU.
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
-          int BM, int BN, int BK, typename Epi, typename KSeg>
+          int BM, int BN, int BK, int WM, bool DB, typename Epi, typename KSeg>
 __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi, KSeg ks) {
-  static_assert(BM % 32 == 0 && BN % 32 == 0 && BK % 4 == 0, "tile shape");
-  constexpr int TM = BM / 32;  // 16x16 MFMA tiles per wave along M (2x2 waves)
-  constexpr int TN = BN / 32;
+  constexpr int NBUF = DB ? 2 : 1;
+  constexpr int WN = 4 / WM;
+  constexpr int WTM = BM / WM, WTN = BN / WN;  // wave tile
+  constexpr int TM = WTM / 16, TN = WTN / 16;  // 16x16 MFMA tiles per wave
+  static_assert(WM * WN == 4 && WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0, "tile shape");
   constexpr int PAD = (sizeof(T) == 8) ? 2 : 4;
   constexpr int LA = (BM * BK) / kThreads;  // elements each thread stages
   constexpr int LB = (BN * BK) / kThreads;
@@ -92,16 +99,16 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
   typedef typename Cplx<RA>::type ca_t;
   typedef typename Cplx<RB>::type cb_t;
 
-  __shared__ T sAr[BK][BM + PAD];
-  __shared__ T sAi[BK][BM + PAD];
-  __shared__ T sBr[BK][BN + PAD];
-  __shared__ T sBi[BK][BN + PAD];
+  __shared__ T sAr[NBUF][BK][BM + PAD];
+  __shared__ T sAi[NBUF][BK][BM + PAD];
+  __shared__ T sBr[NBUF][BK][BN + PAD];
+  __shared__ T sBi[NBUF][BK][BN + PAD];
 
+  if (ks.skip(blockIdx.y, blockIdx.x)) return;  // uniform over the workgroup
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  if (ks.skip(blockIdx.y, blockIdx.x)) return;  // uniform over the workgroup
+  const int wm = wave / WN, wn = wave % WN;
   const int batch = blockIdx.z / g.splitk;
   const int part = blockIdx.z - batch * g.splitk;
   const int m0 = blockIdx.y * BM;
@@ -139,80 +146,87 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       rb[it] = v;
     }
   };
-  auto store_tiles = [&]() {
+  auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int it = 0; it < LA; ++it) {
       const int e = tid + it * kThreads;
       int m, k;
       if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
-      sAr[k][m] = (T)ra[it].x;
-      sAi[k][m] = (T)ra[it].y;
+      sAr[buf][k][m] = (T)ra[it].x;
+      sAi[buf][k][m] = (T)ra[it].y;
     }
 #pragma unroll
     for (int it = 0; it < LB; ++it) {
       const int e = tid + it * kThreads;
       int n, k;
       if (!TRANS_B) { k = e / BN; n = e % BN; } else { n = e / BK; k = e % BK; }
-      sBr[k][n] = (T)rb[it].x;
-      sBi[k][n] = (T)rb[it].y;
+      sBr[buf][k][n] = (T)rb[it].x;
+      sBi[buf][k][n] = (T)rb[it].y;
     }
   };
 
   // K tiles of this tile's segments, split evenly over the splitk parts
-  int slo[2] = {0, 0}, shi[2] = {0, 0};
-  const int nseg = ks.get(blockIdx.y, blockIdx.x, g.K, slo, shi);
-  const int nt0 = nseg > 0 ? (shi[0] - slo[0] + BK - 1) / BK : 0;
-  const int nt1 = nseg > 1 ? (shi[1] - slo[1] + BK - 1) / BK : 0;
+  int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+  const int nseg = ks.get(blockIdx.y, blockIdx.x, g.K, lo0, hi0, lo1, hi1);
+  const int nt0 = nseg > 0 ? (hi0 - lo0 + BK - 1) / BK : 0;
+  const int nt1 = nseg > 1 ? (hi1 - lo1 + BK - 1) / BK : 0;
   const int ktot = nt0 + nt1;
   const int kper = (ktot + g.splitk - 1) / g.splitk;
   const int kt0 = part * kper;
   const int ktiles = max(0, min(ktot, kt0 + kper) - kt0);
-  auto tile_k0 = [&](int t) { return t < nt0 ? slo[0] + t * BK : slo[1] + (t - nt0) * BK; };
+  auto tile_k0 = [&](int t) { return t < nt0 ? lo0 + t * BK : lo1 + (t - nt0) * BK; };
   auto tile_load = [&](int t) {
-    if (t < nt0) load_tiles(slo[0] + t * BK, shi[0]);
-    else load_tiles(slo[1] + (t - nt0) * BK, shi[1]);
+    if (t < nt0) load_tiles(lo0 + t * BK, hi0);
+    else load_tiles(lo1 + (t - nt0) * BK, hi1);
   };
-  if (ktiles > 0) tile_load(kt0);
   const int lr = lane & 15, lk = lane >> 4;
+  if (ktiles > 0) {
+    tile_load(kt0);
+    if (DB) store_tiles(0);
+  }
+  if (DB) __syncthreads();
   for (int kt = 0; kt < ktiles; ++kt) {
-    __syncthreads();
-    store_tiles();
-    __syncthreads();
+    // double buffer: tile kt is in buffer kt&1, tile kt+1 goes to the other one
+    // after the MFMAs; single buffer: store, barrier, prefetch, MFMAs, barrier
+    const int buf = DB ? (kt & 1) : 0;
+    if (!DB) {
+      store_tiles(0);
+      __syncthreads();
+    }
     const int k0c = tile_k0(kt0 + kt);
     const bool real_a = k0c + BK <= g.kreal_a;  // uniform over the workgroup
     const bool real_b = k0c + BK <= g.kreal_b;
-    if (kt + 1 < ktiles) tile_load(kt0 + kt + 1);
+    const bool more = kt + 1 < ktiles;
+    if (more) tile_load(kt0 + kt + 1);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
-        const int m = wm * (BM / 2) + i * 16 + lr;
-        ar[i] = sAr[kk + lk][m];
-        ai[i] = sAi[kk + lk][m];
+        const int m = wm * WTM + i * 16 + lr;
+        ar[i] = sAr[buf][kk + lk][m];
+        ai[i] = sAi[buf][kk + lk][m];
         nai[i] = -ai[i];
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int n = wn * (BN / 2) + j * 16 + lr;
-        br[j] = sBr[kk + lk][n];
-        bi[j] = sBi[kk + lk][n];
+        const int n = wn * WTN + j * 16 + lr;
+        br[j] = sBr[buf][kk + lk][n];
+        bi[j] = sBi[buf][kk + lk][n];
       }
-      if (real_b) {
+      if (real_a || real_b) {
+        // half product: Cre += Ar Br, Cim += (real B ? Ai Br : Ar Bi)
+        T x2[TM], y2[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) x2[i] = real_b ? ai[i] : ar[i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) y2[j] = real_b ? br[j] : bi[j];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
             cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
-            ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
-          }
-      } else if (real_a) {
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
-            ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
+            ci[i][j] = Mfma<T>::op(x2[i], y2[j], ci[i][j]);
           }
       } else {
 #pragma unroll
@@ -226,6 +240,8 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
           }
       }
     }
+    if (DB && more) store_tiles(buf ^ 1);
+    __syncthreads();
   }
 
 #pragma unroll
@@ -234,20 +250,29 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, r);
-        const int col = n0 + wn * (BN / 2) + j * 16 + lr;
+        const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lk, r);
+        const int col = n0 + wn * WTN + j * 16 + lr;
         if (row < g.M && col < g.N) epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
       }
 }
 
-template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, int BM, int BN,
-          int BK, typename Epi, typename KSeg = KFull>
+// Tile configuration of one GEMM: BM x BN output tile, BK K tile, WM waves along M,
+// double-buffered LDS or not.
+template <int BM_, int BN_, int BK_, int WM_, bool DB_ = false>
+struct Tile {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_;
+  static constexpr bool DB = DB_;
+};
+
+template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, typename TL,
+          typename Epi, typename KSeg = KFull>
 inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& epi,
                                hipStream_t s, const KSeg& ks = KSeg()) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
-  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, batch * g.splitk);
-  hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, BM, BN, BK, Epi, KSeg>), grid,
-                     dim3(kThreads), 0, s, g, epi, ks);
+  dim3 grid((g.N + TL::BN - 1) / TL::BN, (g.M + TL::BM - 1) / TL::BM, batch * g.splitk);
+  hipLaunchKernelGGL(
+      (zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, TL::BM, TL::BN, TL::BK, TL::WM, TL::DB, Epi, KSeg>),
+      grid, dim3(kThreads), 0, s, g, epi, ks);
   return hipGetLastError();
 }
 

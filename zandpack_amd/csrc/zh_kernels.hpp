@@ -458,12 +458,12 @@ struct EpiOmega {
 struct KSegOmega {
   const int2* t1;  // per column tile: term1 K range [x, y)
   int nc, ML, BM, BN, ff;
-  __device__ inline int get(int tm, int tn, int, int* lo, int* hi) const {
-    int n = 0;
+  __device__ inline int get(int tm, int tn, int, int& lo0, int& hi0, int& lo1, int& hi1) const {
     const int2 r = t1[tn];
-    if (r.y > r.x) { lo[n] = r.x; hi[n] = r.y; ++n; }
-    if (tm * BM < ML) { lo[n] = nc; hi[n] = 2 * nc; ++n; }
-    return n;
+    const bool s1 = r.y > r.x, s2 = tm * BM < ML;
+    if (s1) { lo0 = r.x; hi0 = r.y; lo1 = nc; hi1 = 2 * nc; }
+    else { lo0 = nc; hi0 = 2 * nc; }
+    return (int)s1 + (int)s2;
   }
   __device__ inline bool skip(int tm, int tn) const {
     return ff && tm * BM >= ML && tn * BN >= ML;
@@ -473,9 +473,9 @@ struct KSegOmega {
 // W = Ω ξ_c: passive rows see only active columns when the passive block is zero.
 struct KSegOmgXi {
   int ML, BM, ff;
-  __device__ inline int get(int tm, int, int K, int* lo, int* hi) const {
-    lo[0] = 0;
-    hi[0] = (ff && tm * BM >= ML) ? ML : K;
+  __device__ inline int get(int tm, int, int K, int& lo0, int& hi0, int&, int&) const {
+    lo0 = 0;
+    hi0 = (ff && tm * BM >= ML) ? ML : K;
     return 1;
   }
   __device__ inline bool skip(int, int) const { return false; }
