@@ -144,6 +144,59 @@ def cpu_baseline(p):
                       f"({el:.2f} s), x7 per DOP54 step"}
 
 
+def replay_fp64_error(req):
+    """FP64 run of one served request forced onto its accepted (t, h) sequence:
+    max |J_mixed - J_fp64| / max |J_fp64|."""
+    from zandpack_amd.engine import Propagator
+    prop = Propagator(req.problem, "fp64")
+    J = [prop.attempt(t, h, 1.0, force_accept=True)["J"] for t, h in zip(req.times[:-1], req.h)]
+    prop.close()
+    J, Jm = np.array(J), np.array(req.J)
+    return float(np.abs(Jm - J).max() / max(np.abs(J).max(), 1e-300))
+
+
+def batched_leg(n_inst, steps, seed=0):
+    """Config 4: n_inst uneven propagations (n in {100..800}, M = 4n) advanced
+    together, each for ``steps`` accepted steps; aggregate steps/s."""
+    from zandpack_amd import builder
+    from zandpack_amd.serving import Request, Server
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.arange(100, 801, 100), size=n_inst)
+    reqs = [Request(i, builder.config_c4_instance(int(n), seed=seed * 1000 + i), t_end=60.0,
+                    eps=1e-6, h0=0.005, precision="mixed", max_steps=steps)
+            for i, n in enumerate(ns)]
+    st = Server(hbm_bytes=200 << 30, max_live=n_inst).run(reqs, realtime=False)
+    return {"instances": n_inst, "n_values": sorted(int(x) for x in ns), "steps_each": steps,
+            "steps_per_s": st["steps_per_s"], "wall_s": st["wall_s"], "failed": st["failed"],
+            "precision": "mixed"}
+
+
+def serving_leg(n_req, rate, t_scale, seed=0):
+    """Config 5: Poisson arrivals at ``rate``/s; n ~ U[50,1200], M ~ U[2n,5n],
+    step/Gaussian/sine bias, t_end ~ U[20,200] fs x t_scale; mixed precision,
+    continuous batching.  p50/p99 time-to-completion, aggregate steps/s, and the
+    FP64 current error of a 5% sample."""
+    from zandpack_amd import builder
+    from zandpack_amd.serving import Request, Server
+    rng = np.random.default_rng(seed)
+    reqs, t = [], 0.0
+    for i in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        n = int(rng.integers(50, 1201))
+        M = int(rng.integers(2 * n, 5 * n + 1))
+        p = builder.config_sized(n, M, seed=seed * 1000 + i, pulse=builder.random_pulse(rng))
+        reqs.append(Request(i, p, t_end=float(rng.uniform(20, 200)) * t_scale, eps=1e-6,
+                            h0=0.005, arrival=t, precision="mixed"))
+    st = Server(hbm_bytes=200 << 30, max_live=64).run(reqs, realtime=True)
+    k = max(1, n_req // 20)
+    sample = [r for r in st["done"] if r.error is None][:k]
+    errs = [replay_fp64_error(r) for r in sample]
+    return {"requests": n_req, "rate_per_s": rate, "t_end_scale": t_scale,
+            "p50_s": st["p50_s"], "p99_s": st["p99_s"], "steps_per_s": st["steps_per_s"],
+            "wall_s": st["wall_s"], "failed": st["failed"], "steps": st["steps"],
+            "sampled_J_rel_err_vs_fp64": errs}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +206,12 @@ def main():
     ap.add_argument("--h0", type=float, default=0.002)
     ap.add_argument("--replay-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batched-instances", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-t-scale", type=float, default=0.005)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -212,6 +271,12 @@ def main():
         "roofline_fp64": {k: v for k, v in roof["fp64"].items() if k != "kernel_ms"},
         "kernel_ms": {"fp64": roof["fp64"]["kernel_ms"], "mixed": roof["mixed"]["kernel_ms"]},
     }
+    if not args.no_serving:
+        print("# batched (config 4)", file=sys.stderr, flush=True)
+        line["batched"] = batched_leg(args.batched_instances, args.batched_steps)
+        print("# serving (config 5)", file=sys.stderr, flush=True)
+        line["serving"] = serving_leg(args.serving_requests, args.serving_rate,
+                                      args.serving_t_scale)
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(p)

@@ -109,6 +109,17 @@ int zh_rhs(zh_problem* p, const double* Ht, const double* delta, const double* s
  * state replaces the resident one.  J (NL,NK) receives the stage-0 currents. */
 int zh_attempt(zh_problem* p, zh_attempt_info* info, double* J);
 
+/* The same attempt split in two, so attempts of several problems (each with its
+ * own HIP stream) overlap on the device: launch enqueues it and returns; collect
+ * waits for it, fills info (te, h_next, accepted) and J, and commits the state
+ * when accepted.  One attempt per problem may be in flight.  No counterpart in
+ * the reference (its ranks run one problem); used by the serving loop. */
+int zh_attempt_launch(zh_problem* p, const zh_attempt_info* info);
+int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J);
+
+/* Device memory held by the problem (bytes), for admission by HBM footprint. */
+int zh_problem_bytes(zh_problem* p, long* bytes);
+
 /* Adaptive propagation loop (Zandpack/mpi/zand:297-431) from t0 while t < t1.
  * Writes up to cap accepted steps: times[i] (time after step i), J (cap,NL,NK)
  * stage-0 currents of step i, h_used[i].  *n_steps / *n_rejected are outputs.

@@ -1,0 +1,77 @@
+"""Serving loop: continuous batching of device-resident propagations.
+
+Each request's result is bitwise the same alone and in the mix (same kernels,
+own buffers and stream, fixed reduction orders); admission respects the HBM
+budget; a failing request is retired without disturbing the others."""
+import sys
+
+import numpy as np
+import pytest
+
+from zandpack_amd import builder
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _requests(precision="mixed"):
+    from zandpack_amd.serving import Request
+    rng = np.random.default_rng(5)
+    out = []
+    for i, (n, M) in enumerate([(40, 80), (70, 210), (100, 400)]):
+        p = builder.config_sized(n, M, seed=10 + i, pulse=builder.random_pulse(rng))
+        out.append(Request(i, p, t_end=0.15 + 0.05 * i, eps=1e-6, h0=0.005, arrival=0.01 * i,
+                           precision=precision))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_alone_equals_mix(native, precision):
+    from zandpack_amd.serving import Server
+    say("serving alone vs mix", precision)
+    alone = []
+    for r in _requests(precision):
+        st = Server().run([r], realtime=False)
+        assert st["failed"] == 0
+        alone.append(st["done"][0])
+    mixed = Server().run(_requests(precision), realtime=True)
+    assert mixed["failed"] == 0 and mixed["requests"] == 3
+    by_id = {r.rid: r for r in mixed["done"]}
+    for a in alone:
+        b = by_id[a.rid]
+        assert a.steps == b.steps and a.rejected == b.rejected
+        np.testing.assert_array_equal(np.array(a.J), np.array(b.J))
+        np.testing.assert_array_equal(np.array(a.times), np.array(b.times))
+        np.testing.assert_array_equal(a.sigma, b.sigma)
+    assert mixed["p50_s"] is not None and mixed["p99_s"] >= mixed["p50_s"]
+
+
+def test_admission_by_footprint(native):
+    from zandpack_amd.serving import Server, footprint_bytes
+    from zandpack_amd.engine import Propagator
+    reqs = _requests()
+    fps = [footprint_bytes(r.problem, r.precision) for r in reqs]
+    # the estimate covers what the engine allocates
+    prop = Propagator(reqs[2].problem, "mixed")
+    assert prop.device_bytes() <= fps[2]
+    prop.close()
+    budget = max(fps) + min(fps) // 2  # never room for the two largest together
+    srv = Server(hbm_bytes=budget)
+    st = srv.run(reqs, realtime=False)
+    assert st["failed"] == 0 and st["requests"] == 3
+    assert srv.peak_bytes <= budget
+
+
+def test_failing_request_is_retired(native):
+    from zandpack_amd.serving import Server
+    reqs = _requests()
+    reqs[1].eps = 1e-30      # never accepted
+    reqs[1].max_attempts = 5
+    st = Server().run(reqs, realtime=False)
+    errs = {r.rid: r.error for r in st["done"]}
+    assert errs[1] == "attempt limit reached"
+    assert errs[0] is None and errs[2] is None

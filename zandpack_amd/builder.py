@@ -393,3 +393,45 @@ def config_small(n_w=4, n_l=3, N_L=3, slots=4, NOT=2, NK=1, seed=0, hubbard_U=0.
     return config_honeycomb(n_w, n_l, N_L, slots, NOT, NK=NK, seed=seed,
                             pulse=pulse or Pulse("gauss", 0.3, 1.0, 0.8, (0.5, -0.5)),
                             hubbard_U=hubbard_U, name="small", t_end=1.0, eps=1e-7, h0=0.01)
+
+
+def config_sized(n_target, M_target, seed=0, pulse=None, hubbard_U=0.0, NK=1, t_end=1.0,
+                 eps=1e-6, h0=0.005, name=None):
+    """Honeycomb device of about ``n_target`` orbitals with about ``M_target`` modes
+    (two leads, edge-localised rank-NOT couplings, half Lorentzian / half Fermi
+    slots), for the batched and serving workloads."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    width = int(np.clip(round(np.sqrt(n_target / 1.5)), 4, 40))
+    length = max(2, int(round(n_target / width)))
+    NOT = int(min(width, max(1, round(M_target / 100.0))))
+    NX = max(4, int(round(M_target / (2.0 * NOT))))
+    slots = 2 * max(1, (NX // 2) // 2)
+    N_L = max(1, NX - slots)
+    return config_honeycomb(width, length, N_L, slots, NOT, NK=NK, seed=int(rng.integers(1 << 30)),
+                            pulse=pulse or Pulse("none"), hubbard_U=hubbard_U,
+                            name=name or f"sized_n{width * length}", t_end=t_end, eps=eps, h0=h0)
+
+
+def random_pulse(rng):
+    """Step, Gaussian or sinusoidal bias at 0.1-1 eV (sine frequency 0.1-2 eV/ħ)."""
+    kind = ["step", "gauss", "sin"][int(rng.integers(3))]
+    amp = float(rng.uniform(0.1, 1.0))
+    if kind == "step":
+        return Pulse("step", amp, 0.0, 0.0, (0.5, -0.5))
+    if kind == "gauss":
+        return Pulse("gauss", amp, float(rng.uniform(0.0, 1.0)), float(rng.uniform(0.2, 2.0)),
+                     (0.5, -0.5))
+    return Pulse("sin", amp, float(rng.uniform(0.1, 2.0)), 0.0, (0.5, -0.5))
+
+
+def config_c4_instance(n, seed=0, t_end=60.0):
+    """One instance of the batched workload: honeycomb of n orbitals (edge 20),
+    two leads with n/50 channels, N_L = 50 Lorentzians + 50 Fermi slots, so
+    M = 4n; Gaussian pulse with amplitude U[0.05, 0.5] eV and width U[2, 20] fs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    NOT = max(1, n // 50)
+    amp = float(rng.uniform(0.05, 0.5))
+    width = float(rng.uniform(2.0, 20.0))
+    return config_honeycomb(20, max(2, n // 20), 50, 50, NOT, NK=1, seed=seed,
+                            pulse=Pulse("gauss", amp, 2.0 * width, width, (0.5, -0.5)),
+                            name=f"c4_n{n}", t_end=t_end, eps=1e-6, h0=0.005)

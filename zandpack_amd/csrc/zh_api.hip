@@ -90,6 +90,9 @@ struct EngineBase {
   virtual void rhs(const double* Ht, const double* delta, const double* sig, const double* psi,
                    const double* omg, double* dsig, double* dpsi, double* domg, double* J) = 0;
   virtual void attempt(zh_attempt_info* info, double* J) = 0;
+  virtual void launch(const zh_attempt_info* info) = 0;
+  virtual void collect(zh_attempt_info* info, double* J) = 0;
+  virtual long bytes() const = 0;
   virtual void set_layout(int mode) = 0;
   virtual int layout_info(int* out) = 0;
   bool use_graph = true;
@@ -168,6 +171,10 @@ struct Engine : EngineBase {
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   hipEvent_t ev[2 * KC_N + 2];
 
+  long nbytes = 0;
+  bool pending = false;  // an attempt was launched and not collected yet
+  zh_attempt_info pend_info;
+
   template <typename T> T* dalloc(size_t count) {
     void* p = nullptr;
     if (count == 0) count = 1;
@@ -175,6 +182,7 @@ struct Engine : EngineBase {
     if (e != hipSuccess)
       throw Error{ZH_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
     allocs.push_back(p);
+    nbytes += (long)(count * sizeof(T));
     HIPCHK(hipMemsetAsync(p, 0, count * sizeof(T), stream));
     return (T*)p;
   }
@@ -764,9 +772,13 @@ struct Engine : EngineBase {
     }
   }
 
-  void attempt(zh_attempt_info* info, double* J) override {
+  long bytes() const override { return nbytes; }
+
+  // Enqueue one attempt on the problem's stream and return; collect() waits.
+  void launch(const zh_attempt_info* info) override {
     require(have_consts, "constants not set", ZH_ESTATE);
     require(have_state, "no resident state", ZH_ESTATE);
+    require(!pending, "an attempt is already in flight on this problem", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
     ensure_h0();
     if (ff && ff_dirty) { clear_ff(); ff_dirty = false; }
@@ -793,16 +805,30 @@ struct Engine : EngineBase {
     HIPCHK(hipEventRecord(ev[2 * KC_N + 1], stream));
     HIPCHK(hipMemcpyAsync(ctrl_h, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(J_h, Jd, sizeof(double) * NL * NK, hipMemcpyDeviceToHost, stream));
+    pend_info = *info;
+    pending = true;
+  }
+
+  void collect(zh_attempt_info* info, double* J) override {
+    require(pending, "no attempt in flight on this problem", ZH_ESTATE);
+    HIPCHK(hipSetDevice(device));
+    pending = false;
     HIPCHK(hipStreamSynchronize(stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ev[2 * KC_N], ev[2 * KC_N + 1]));
     attempt_ms += ms;
     attempt_count += 1;
+    *info = pend_info;
     info->te = ctrl_h->te;
     info->h_next = ctrl_h->h_next;
     info->accepted = ctrl_h->accept;
     if (J) std::memcpy(J, J_h, sizeof(double) * NL * NK);
     if (info->accepted) std::swap(iy0, iyn);
+  }
+
+  void attempt(zh_attempt_info* info, double* J) override {
+    launch(info);
+    collect(info, J);
   }
 
   void rhs(const double* Ht, const double* delta, const double* s, const double* p,
@@ -1023,6 +1049,29 @@ int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int
     *n_steps = steps;
     *n_rejected = rej;
     if (h_final) *h_final = h;
+  });
+}
+
+int zh_attempt_launch(zh_problem* p, const zh_attempt_info* info) {
+  return zh_try([&] {
+    require(p && info, "null argument");
+    require(info->h > 0 && std::isfinite(info->h), "step size must be positive and finite");
+    p->eng->launch(info);
+  });
+}
+
+int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J) {
+  return zh_try([&] {
+    require(p && info, "null argument");
+    p->eng->collect(info, J);
+    if (!std::isfinite(info->te)) throw Error{ZH_ENONFINITE, "non-finite error estimate"};
+  });
+}
+
+int zh_problem_bytes(zh_problem* p, long* bytes) {
+  return zh_try([&] {
+    require(p && bytes, "null argument");
+    *bytes = p->eng->bytes();
   });
 }
 

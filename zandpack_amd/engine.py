@@ -105,6 +105,24 @@ class Propagator:
         N.check(self.lib.zh_attempt(self.h, C.byref(info), N.ptr(J)))
         return dict(te=info.te, h_next=info.h_next, accepted=bool(info.accepted), J=J)
 
+    def launch(self, t0, h, eps, force_accept=False):
+        """Enqueue one attempt and return at once (see ``collect``)."""
+        self._pending = N.zh_attempt_info(float(t0), float(h), float(eps),
+                                          1 if force_accept else 0, 0.0, 0.0, 0)
+        N.check(self.lib.zh_attempt_launch(self.h, C.byref(self._pending)))
+
+    def collect(self):
+        """Wait for the launched attempt; commits the state when accepted."""
+        info = self._pending
+        J = np.empty((self.NL, self.NK))
+        N.check(self.lib.zh_attempt_collect(self.h, C.byref(info), N.ptr(J)))
+        return dict(te=info.te, h_next=info.h_next, accepted=bool(info.accepted), J=J)
+
+    def device_bytes(self):
+        b = C.c_long(0)
+        N.check(self.lib.zh_problem_bytes(self.h, C.byref(b)))
+        return b.value
+
     def propagate(self, t0, t1, h0, eps, time_update="reference", max_attempts=10 ** 7,
                   cap=100000):
         """zand's adaptive loop on the device; returns times, J, h of accepted steps."""

@@ -1,0 +1,173 @@
+"""Continuous-batching serving loop over device-resident propagations.
+
+Requests (one AME propagation each: a device, its lead constants, a bias pulse
+and an end time) arrive over time.  The server admits them in arrival order as
+HBM frees up (by the footprint of σ, Ψ, Ω, the seven RK stages and the GEMM
+operands at the request's precision), keeps every admitted state resident on
+the device, and advances all live requests together one DOP54 attempt per
+tick: each attempt is launched on its request's own HIP stream before any is
+collected, so the attempts of different requests run concurrently on the GPU.
+Finished requests are retired at once and their memory admits the next ones.
+
+Every request's arithmetic is the same kernels on its own buffers in its own
+order, so its result is bitwise the same alone and in any mix.
+
+There is no counterpart in the reference (one `mpirun zand` runs one problem);
+the step semantics per request are zand's (Zandpack/mpi/zand:297-431).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .builder import AMEProblem
+from .engine import Propagator
+
+
+def footprint_bytes(problem: AMEProblem, precision: str = "mixed") -> int:
+    """Upper estimate of the device memory one request holds (bytes)."""
+    NK, n, NL, NX, NOT = problem.dims
+    M = problem.M
+    Md = M + 64  # active-block padding bound
+    cs = 16 if precision == "fp64" else 8
+    nc = int(np.count_nonzero(np.any(problem.xi.reshape(-1, n) != 0, axis=0)))
+    slots = 10 * NK * (Md * Md * cs + n * n * 16) + 9 * NK * Md * n * cs
+    ops = NK * (Md * (n + nc) * cs + (n + nc) * n * cs + 4 * Md * nc * cs + 3 * Md * nc * cs
+                + Md * nc * 16 + 8 * n * nc * 16 + 3 * n * n * 16 + 3 * Md * 16)
+    return int(slots + ops + (1 << 20))
+
+
+@dataclass
+class Request:
+    rid: int
+    problem: AMEProblem
+    t_end: float
+    eps: float
+    h0: float
+    arrival: float = 0.0          # seconds after the server starts
+    precision: str = "mixed"
+    time_update: str = "reference"
+    max_attempts: int = 10 ** 6
+    max_steps: int | None = None  # finish after this many accepted steps (bounded runs)
+    # filled in by the server
+    times: list = field(default_factory=list)
+    h: list = field(default_factory=list)
+    J: list = field(default_factory=list)
+    steps: int = 0
+    rejected: int = 0
+    admitted: float = float("nan")
+    finished: float = float("nan")
+    sigma: np.ndarray | None = None
+    error: str | None = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+class _Live:
+    def __init__(self, req: Request, graph: bool):
+        self.req = req
+        self.prop = Propagator(req.problem, req.precision, graph=graph)
+        self.bytes = footprint_bytes(req.problem, req.precision)
+        self.t = 0.0
+        self.h = req.h0
+        self.attempts = 0
+        req.times.append(0.0)
+
+
+class Server:
+    """Admission by HBM footprint, step-granular continuous batching."""
+
+    def __init__(self, hbm_bytes: int = 64 << 30, max_live: int = 64, graph: bool = True,
+                 keep_sigma: bool = True):
+        self.budget = int(hbm_bytes)
+        self.max_live = int(max_live)
+        self.graph = graph
+        self.keep_sigma = keep_sigma
+
+    def run(self, requests, realtime: bool = True):
+        """Serve ``requests`` (arrival times in seconds).  With ``realtime`` the
+        arrivals follow the wall clock; otherwise all are present at start."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        for r in pending:
+            if footprint_bytes(r.problem, r.precision) > self.budget:
+                raise ValueError(f"request {r.rid} alone exceeds the HBM budget")
+        live: list[_Live] = []
+        done: list[Request] = []
+        used = 0
+        self.peak_bytes = 0
+        ticks = 0
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+        while pending or live:
+            now = clock()
+            while (pending and pending[0].arrival <= now and len(live) < self.max_live
+                   and used + footprint_bytes(pending[0].problem, pending[0].precision)
+                   <= self.budget):
+                r = pending.pop(0)
+                r.admitted = time.perf_counter() - t0
+                lv = _Live(r, self.graph)
+                used += lv.bytes
+                self.peak_bytes = max(self.peak_bytes, used)
+                live.append(lv)
+            if not live:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                continue
+            finished = []
+            launched = []
+            for lv in live:
+                try:
+                    lv.prop.launch(lv.t, lv.h, lv.req.eps)
+                    launched.append(lv)
+                except Exception as e:  # e.g. a step size driven to zero or infinity
+                    lv.req.error = str(e)
+                    finished.append(lv)
+            for lv in launched:
+                try:
+                    res = lv.prop.collect()
+                except Exception as e:  # a failed request is retired, the rest go on
+                    lv.req.error = str(e)
+                    finished.append(lv)
+                    continue
+                lv.attempts += 1
+                h_old, lv.h = lv.h, res["h_next"]
+                if res["accepted"]:
+                    lv.req.J.append(res["J"])
+                    lv.req.h.append(h_old)
+                    lv.t += lv.h if lv.req.time_update == "reference" else h_old
+                    lv.req.times.append(lv.t)
+                    lv.req.steps += 1
+                else:
+                    lv.req.rejected += 1
+                if lv.t >= lv.req.t_end or (lv.req.max_steps is not None
+                                              and lv.req.steps >= lv.req.max_steps):
+                    finished.append(lv)
+                elif lv.attempts >= lv.req.max_attempts:
+                    lv.req.error = "attempt limit reached"
+                    finished.append(lv)
+            ticks += 1
+            for lv in finished:
+                live.remove(lv)
+                lv.req.finished = time.perf_counter() - t0
+                if self.keep_sigma and lv.req.error is None:
+                    lv.req.sigma = lv.prop.sigma()
+                lv.prop.close()
+                used -= lv.bytes
+                done.append(lv.req)
+        wall = time.perf_counter() - t0
+        return self.stats(done, wall, ticks)
+
+    @staticmethod
+    def stats(done, wall, ticks):
+        ok = [r for r in done if r.error is None]
+        lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
+        steps = int(sum(r.steps for r in done))
+        return dict(requests=len(done), failed=len(done) - len(ok), wall_s=wall, ticks=ticks,
+                    steps=steps, steps_per_s=steps / wall if wall > 0 else 0.0,
+                    p50_s=float(np.percentile(lat, 50)) if lat.size else None,
+                    p99_s=float(np.percentile(lat, 99)) if lat.size else None,
+                    done=done)
