@@ -30,23 +30,31 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X dense FP64 / FP32 matri
 
 
 def algorithmic_flops(p):
-    """Real flops per launch of each GEMM class (8 per complex MAC), per the
-    algorithmic count C_RHS = n^3 + M n^2 + M n n_c + Σ n n_a M_a + M^2 n_c
-    + Σ M M_a n_a + M^2 n_c per k block."""
+    """Real flops per launch of each GEMM class for the work the path performs
+    (SURVEY count C_RHS per k, with 8 flops per complex MAC and 4 per complex x
+    real MAC): a real Hamiltonian halves ψHᵀ and Hσ; Fermi-block pruning keeps
+    only the non-vanishing Ω source terms and skips the passive x passive block
+    of Ω ξ (the terms the kernels execute)."""
     NK, n, NL, NX, NOT = p.dims
     M = p.M
     Ma = NX * NOT
     xi = p.xi.reshape(NK, NL, Ma, n)
     na = [int(np.count_nonzero(np.any(xi[:, a] != 0, axis=(0, 1)))) for a in range(NL)]
     nc = int(np.count_nonzero(np.any(p.xi.reshape(-1, n) != 0, axis=0)))
+    h_real = not np.any(p.H0.imag)
+    hmac = 4.0 if h_real else 8.0
+    act = ((p.dGP != 0) | (p.dGM != 0)).any(axis=0).reshape(NL, Ma)  # per lead, per mode
+    colM = (p.dGM != 0).any(axis=0).reshape(NL, Ma)                  # term1 columns
+    rowP = int(((p.dGP != 0).any(axis=0)).sum())                    # term2 rows
+    ML = int(act.sum())
     c = {
-        "gemm_hsig": n ** 3,
-        "gemm_psi": M * n * n + M * n * nc,
-        "gemm_pi": sum(n * a * Ma for a in na),
-        "gemm_omgxi": M * M * nc,
-        "gemm_omega": sum(M * Ma * a for a in na) + M * M * nc,
+        "gemm_hsig": hmac * n ** 3,
+        "gemm_psi": hmac * M * n * n + 8.0 * M * n * nc,
+        "gemm_pi": 8.0 * sum(n * a * Ma for a in na),
+        "gemm_omgxi": 8.0 * (M * M - (M - ML) ** 2) * nc,
+        "gemm_omega": 8.0 * (sum(M * int(colM[a].sum()) * na[a] for a in range(NL)) + rowP * M * nc),
     }
-    return {k: 8.0 * NK * v for k, v in c.items()}, na, nc
+    return {k: NK * v for k, v in c.items()}, na, nc
 
 
 def run_leg(p, precision, K, W, eps, h0):

@@ -69,6 +69,8 @@ CASES = {
     "small_spin_hubbard": lambda: builder.config_small(NK=2, hubbard_U=2.0, seed=2),
     "chain20": lambda: builder.config_chain(),
     "wide": lambda: builder.config_small(n_w=6, n_l=7, N_L=5, slots=6, NOT=3, seed=4),
+    # 72 coupled orbitals: the 80-column tiles of the Ωξ and Π products
+    "wide_edges": lambda: builder.config_small(n_w=36, n_l=3, N_L=2, slots=2, NOT=4, seed=9),
 }
 
 

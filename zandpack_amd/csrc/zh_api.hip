@@ -125,7 +125,7 @@ struct Engine : EngineBase {
   typedef typename Cplx<R>::type cs;
   static constexpr int kTile = 64;    // Ω GEMM / Ωξ GEMM row and column tile
   static constexpr int kPiParts = 8;  // split-K of Π_c = ψᵀ Ixi_c (K = modes)
-  static constexpr int kWParts = 2;   // split-K of W = Ω ξ_c (K = modes)
+  static constexpr int kWParts = 4;   // split-K of W = Ω ξ_c (K = modes)
   int NK, n, NL, NX, NOT;
   int M;        // modes per k in the reference layout, NL*NX*NOT
   int Md = 0;   // modes per k on the device (active block padded to kTile)
@@ -153,6 +153,8 @@ struct Engine : EngineBase {
   int *cidx = nullptr, *cpos = nullptr, *lead = nullptr;
   int2* t1 = nullptr;
   double2 *Hcur = nullptr, *Ixic = nullptr, *Pic = nullptr, *G = nullptr, *dvec = nullptr;
+  double* Hre = nullptr;  // Re H (A operand of Hσ when H is real)
+  R* HreT = nullptr;      // Re Hᵀ (B operand of ψHᵀ when H is real)
   double2 *glp = nullptr, *chip = nullptr, *chim = nullptr;
   double* n0 = nullptr;
   cs *Bp = nullptr, *Ap = nullptr, *Aomg = nullptr, *Bomg = nullptr, *xic = nullptr, *W = nullptr;
@@ -279,6 +281,8 @@ struct Engine : EngineBase {
     }
     lead = dalloc<int>(Md);
     Hcur = dalloc<double2>(nn);
+    Hre = dalloc<double>(nn);
+    HreT = dalloc<R>(nn);
     G = dalloc<double2>(nn);
     dvec = dalloc<double2>((size_t)NK * n);
     glp = dalloc<double2>((size_t)NK * Md);
@@ -415,8 +419,9 @@ struct Engine : EngineBase {
     h_real = real;
     HIPCHK(hipMemcpyAsync(Hcur, H, sizeof(double2) * NK * n * n, hipMemcpyHostToDevice, stream));
     dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
-    hipLaunchKernelGGL(k_transpose_h<cs>, grid, dim3(32, 8), 0, stream, n,
-                       (long)(n + nc) * n, (const double2*)Hcur, Bp);
+    hipLaunchKernelGGL((k_transpose_h<cs, R>), grid, dim3(32, 8), 0, stream, n,
+                       (long)(n + nc) * n, (const double2*)Hcur, Bp, real ? Hre : nullptr,
+                       real ? HreT : nullptr);
     HIPCHK(hipGetLastError());
   }
 
@@ -657,7 +662,7 @@ struct Engine : EngineBase {
       timed(KC_GEMM_PI, [&] {
         // Π_c = ψᵀ Ixi_c  (n x Md) (Md x nc), FP64, split over K
         GemmArgs<R, double> g{n, nc, Md, Ap, n + nc, (long)Md * (n + nc), Ixic, nc,
-                              (long)Md * nc, kPiParts, 0, xi_real ? Md : 0};
+                              (long)Md * nc, kPiParts};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
         HIPCHK((launch_zgemm<double, R, double, true, false, TilePi>(g, NK, e, stream)));
       });
@@ -673,10 +678,14 @@ struct Engine : EngineBase {
       HIPCHK(hipGetLastError());
     }
     timed(KC_GEMM_HSIG, [&] {
-      GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n, 1,
-                                 h_real ? n : 0, 0};
+      GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n};
       EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih, kPiParts};
-      HIPCHK((launch_zgemm<double, double, double, false, false, TileHsig>(g, NK, e, stream)));
+      if (h_real) {  // real H: the A operand is Re H
+        g.Areal = Hre; g.ldar = n; g.sAr = (long)n * n;
+        HIPCHK((launch_zgemm<double, double, double, false, false, TileHsig, 1>(g, NK, e, stream)));
+      } else {
+        HIPCHK((launch_zgemm<double, double, double, false, false, TileHsig>(g, NK, e, stream)));
+      }
     });
     timed(KC_SIGSYM, [&] {
       dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
@@ -686,22 +695,41 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
     if (nc > 0) {
       timed(KC_GEMM_OMGXI, [&] {
-        // W = Ω ξ_c  (Md x Md)(Md x nc)
-        GemmArgs<R, R> g{Md, nc, Md, om_in, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts,
-                         0, xi_real ? Md : 0};
+        // W = Ω ξ_c  (Md x Md)(Md x nc); the column tile follows nc so that Ω is
+        // streamed once and few padded columns are computed
+        GemmArgs<R, R> g{Md, nc, Md, om_in, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
         EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
-        static_assert(Tiles<R>::omgxi::BM == kTile, "omgxi tile");
         KSegOmgXi ks{ML, kTile, ff ? 1 : 0};
-        HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::omgxi>(g, NK, e, stream, ks)));
+        if (sizeof(R) == 4 && nc > 64 && nc <= 80)  // FP64 at this width drops to 1 wave/SIMD
+          HIPCHK((launch_zgemm<R, R, R, false, false, Tile<kTile, 80, 16, 4>>(g, NK, e, stream, ks)));
+        else if (nc > 32 && nc <= 64)
+          HIPCHK((launch_zgemm<R, R, R, false, false, Tile<kTile, 64, 16, 2>>(g, NK, e, stream, ks)));
+        else
+          HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::omgxi>(g, NK, e, stream, ks)));
       });
     }
     timed(KC_GEMM_PSI, [&] {
-      // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (Md x (n+nc)) ((n+nc) x n)
-      GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n,
-                       1, 0, h_real ? n : 0};
-      EpiPsi<cs> e{psi[kslot], Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps, Md, n, nc,
-                   ih, ih * ih, kWParts};
-      HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi>(g, NK, e, stream)));
+      if (h_real) {
+        // [ΔPξ_c] [ħ σ_cᵀ] into the k slot, then ψ Re(H)ᵀ with the epilogue adding it
+        EpiStore<cs> e0{psi[kslot], n, (long)Md * n, 1.0, 0.0};
+        if (nc > 0) {
+          GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
+                            (long)(n + nc) * n};
+          HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi>(g0, NK, e0, stream)));
+        }
+        GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
+        g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
+        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, W, glp, chip,
+                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, kWParts};
+        HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi, 2>(g, NK, e, stream)));
+      } else {
+        // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (Md x (n+nc)) ((n+nc) x n)
+        GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
+                         (long)(n + nc) * n};
+        EpiPsi<cs> e{psi[kslot], nullptr, Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps,
+                     Md, n, nc, ih, ih * ih, kWParts};
+        HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi>(g, NK, e, stream)));
+      }
     });
     timed(KC_GEMM_OMEGA, [&] {
       // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), outer-subtraction epilogue

@@ -54,16 +54,19 @@ template <> struct Cplx<float> { typedef float2 type; };
 // splitk > 1 splits the K range over blockIdx.z = batch * splitk + part; the
 // epilogue then receives z (not the batch index) and must keep the partial
 // products apart (they are summed by the consumer in a fixed order).
-// kreal_a / kreal_b: K tiles lying entirely below these bounds have a real A
-// (resp. B) operand there, and the complex product drops the vanishing real
-// MFMAs (2 instead of 4 per tile pair).
+// Real-operand modes (template RMODE of the kernel): 0 complex x complex; 1 the
+// A operand is the real matrix Areal (element (m,k) at Areal[m*ldar + k], or
+// [k*ldar + m] transposed); 2 the B operand is the real matrix Breal ((k,n) at
+// Breal[k*ldbr + n], or [n*ldbr + k]).  A real operand moves half the bytes and
+// its product needs two real MFMAs per tile pair instead of four.
 template <typename RA, typename RB>
 struct GemmArgs {
   int M, N, K;
   const typename Cplx<RA>::type* A; long lda; long sA;
   const typename Cplx<RB>::type* B; long ldb; long sB;
   int splitk = 1;
-  int kreal_a = 0, kreal_b = 0;
+  const RA* Areal = nullptr; long ldar = 0; long sAr = 0;
+  const RB* Breal = nullptr; long ldbr = 0; long sBr = 0;
 };
 
 constexpr int kThreads = 256;
@@ -84,8 +87,9 @@ struct KFull {
 // LDS holds two K tiles and one barrier per tile separates the stages; without,
 // the single LDS tile costs a second barrier but keeps more workgroups per CU.
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
-          int BM, int BN, int BK, int WM, bool DB, typename Epi, typename KSeg>
+          int BM, int BN, int BK, int WM, bool DB, int RMODE, typename Epi, typename KSeg>
 __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi, KSeg ks) {
+  constexpr bool RA_ = RMODE == 1, RB_ = RMODE == 2;
   constexpr int NBUF = DB ? 2 : 1;
   constexpr int WN = 4 / WM;
   constexpr int WTM = BM / WM, WTN = BN / WN;  // wave tile
@@ -115,6 +119,8 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
   const int n0 = blockIdx.x * BN;
   const ca_t* __restrict__ A = g.A + (long)batch * g.sA;
   const cb_t* __restrict__ B = g.B + (long)batch * g.sB;
+  const RA* __restrict__ Ar = g.Areal ? g.Areal + (long)batch * g.sAr : nullptr;
+  const RB* __restrict__ Br = g.Breal ? g.Breal + (long)batch * g.sBr : nullptr;
 
   acc_t cr[TM][TN], ci[TM][TN];
 #pragma unroll
@@ -132,7 +138,10 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
       const int gm = m0 + m, gk = k0 + k;
       ca_t v; v.x = 0; v.y = 0;
-      if (gm < g.M && gk < khi) v = TRANS_A ? A[(long)gk * g.lda + gm] : A[(long)gm * g.lda + gk];
+      if (gm < g.M && gk < khi) {
+        if (RA_) v.x = TRANS_A ? Ar[(long)gk * g.ldar + gm] : Ar[(long)gm * g.ldar + gk];
+        else v = TRANS_A ? A[(long)gk * g.lda + gm] : A[(long)gm * g.lda + gk];
+      }
       ra[it] = v;
     }
 #pragma unroll
@@ -142,7 +151,10 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       if (!TRANS_B) { k = e / BN; n = e % BN; } else { n = e / BK; k = e % BK; }
       const int gn = n0 + n, gk = k0 + k;
       cb_t v; v.x = 0; v.y = 0;
-      if (gn < g.N && gk < khi) v = TRANS_B ? B[(long)gn * g.ldb + gk] : B[(long)gk * g.ldb + gn];
+      if (gn < g.N && gk < khi) {
+        if (RB_) v.x = TRANS_B ? Br[(long)gn * g.ldbr + gk] : Br[(long)gk * g.ldbr + gn];
+        else v = TRANS_B ? B[(long)gn * g.ldb + gk] : B[(long)gk * g.ldb + gn];
+      }
       rb[it] = v;
     }
   };
@@ -153,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       int m, k;
       if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
       sAr[buf][k][m] = (T)ra[it].x;
-      sAi[buf][k][m] = (T)ra[it].y;
+      if (!RA_) sAi[buf][k][m] = (T)ra[it].y;
     }
 #pragma unroll
     for (int it = 0; it < LB; ++it) {
@@ -161,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       int n, k;
       if (!TRANS_B) { k = e / BN; n = e % BN; } else { n = e / BK; k = e % BK; }
       sBr[buf][k][n] = (T)rb[it].x;
-      sBi[buf][k][n] = (T)rb[it].y;
+      if (!RB_) sBi[buf][k][n] = (T)rb[it].y;
     }
   };
 
@@ -193,42 +205,43 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       store_tiles(0);
       __syncthreads();
     }
-    const int k0c = tile_k0(kt0 + kt);
-    const bool real_a = k0c + BK <= g.kreal_a;  // uniform over the workgroup
-    const bool real_b = k0c + BK <= g.kreal_b;
     const bool more = kt + 1 < ktiles;
     if (more) tile_load(kt0 + kt + 1);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
+      T ar[TM], ai[TM], br[TN], bi[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
         const int m = wm * WTM + i * 16 + lr;
         ar[i] = sAr[buf][kk + lk][m];
-        ai[i] = sAi[buf][kk + lk][m];
-        nai[i] = -ai[i];
+        if (!RA_) ai[i] = sAi[buf][kk + lk][m];
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         const int n = wn * WTN + j * 16 + lr;
         br[j] = sBr[buf][kk + lk][n];
-        bi[j] = sBi[buf][kk + lk][n];
+        if (!RB_) bi[j] = sBi[buf][kk + lk][n];
       }
-      if (real_a || real_b) {
-        // half product: Cre += Ar Br, Cim += (real B ? Ai Br : Ar Bi)
-        T x2[TM], y2[TN];
-#pragma unroll
-        for (int i = 0; i < TM; ++i) x2[i] = real_b ? ai[i] : ar[i];
-#pragma unroll
-        for (int j = 0; j < TN; ++j) y2[j] = real_b ? br[j] : bi[j];
+      if (RA_) {  // Cre += Ar Br, Cim += Ar Bi
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
             cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
-            ci[i][j] = Mfma<T>::op(x2[i], y2[j], ci[i][j]);
+            ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
+          }
+      } else if (RB_) {  // Cre += Ar Br, Cim += Ai Br
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
+            ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
           }
       } else {
+        T nai[TM];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) nai[i] = -ai[i];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -265,14 +278,14 @@ struct Tile {
 };
 
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, typename TL,
-          typename Epi, typename KSeg = KFull>
+          int RMODE = 0, typename Epi, typename KSeg = KFull>
 inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& epi,
                                hipStream_t s, const KSeg& ks = KSeg()) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   dim3 grid((g.N + TL::BN - 1) / TL::BN, (g.M + TL::BM - 1) / TL::BM, batch * g.splitk);
-  hipLaunchKernelGGL(
-      (zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, TL::BM, TL::BN, TL::BK, TL::WM, TL::DB, Epi, KSeg>),
-      grid, dim3(kThreads), 0, s, g, epi, ks);
+  hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, TL::BM, TL::BN, TL::BK, TL::WM,
+                                   TL::DB, RMODE, Epi, KSeg>),
+                     grid, dim3(kThreads), 0, s, g, epi, ks);
   return hipGetLastError();
 }
 

@@ -204,21 +204,30 @@ __global__ void k_gather(GatherArgs<C> g) {
   }
 }
 
-// Bp[b][o'][o] = H[b][o][o'] (top n rows of the Psi GEMM's B operand).
-template <typename C>
-__global__ void k_transpose_h(int n, long ldBp_batch, const double2* H, C* Bp) {
+// Bp[b][o'][o] = H[b][o][o'] (top n rows of the Psi GEMM's B operand); when the
+// Hamiltonian is real also its real copies Hre = Re H and HreT = Re Hᵀ.
+template <typename C, typename R>
+__global__ void k_transpose_h(int n, long ldBp_batch, const double2* H, C* Bp, double* Hre,
+                              R* HreT) {
   __shared__ double2 tile[32][33];
   const int b = blockIdx.z;
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
   const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
   for (int r = ty; r < 32; r += 8) {
     const int i = by + r, j = bx + tx;
-    if (i < n && j < n) tile[r][tx] = H[((long)b * n + i) * n + j];
+    if (i < n && j < n) {
+      const double2 v = H[((long)b * n + i) * n + j];
+      tile[r][tx] = v;
+      if (Hre) Hre[((long)b * n + i) * n + j] = v.x;
+    }
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
     const int j = bx + r, i = by + tx;  // out[j][i] = H[i][j]
-    if (i < n && j < n) Bp[b * ldBp_batch + (long)j * n + i] = from_d2<C>(tile[tx][r]);
+    if (i < n && j < n) {
+      Bp[b * ldBp_batch + (long)j * n + i] = from_d2<C>(tile[tx][r]);
+      if (HreT) HreT[((long)b * n + j) * n + i] = (R)tile[tx][r].x;
+    }
   }
 }
 
@@ -388,10 +397,13 @@ struct EpiSigma {
 };
 
 // kΨ = -i h [ (acc - (χ⁺+Δ)ψ + d_col ψ)/ħ + (GL_P ξ + W/ħ²) on coupled columns ]
-// with acc = ψ H0ᵀ + ħ (ΔP ξ)_c σ_cᵀ  (Zandpack/mpi/zand:746-778)
+// with acc = ψ H0ᵀ + ħ (ΔP ξ)_c σ_cᵀ  (Zandpack/mpi/zand:746-778).  With acc0 the
+// GEMM computed only part of acc; the rest is read from acc0 (same layout as out,
+// the element is read and then overwritten by the same thread).
 template <typename C>
 struct EpiPsi {
   C* out;                       // (NK, M, n)
+  const C* acc0;
   const C* Ap; long ldAp;       // stage psi
   const C* xic; const C* W;     // (NK, M, nc)
   const double2* glp; const double2* chip;  // (NK, M)
@@ -402,6 +414,11 @@ struct EpiPsi {
   int w_parts;  // split-K partials of W, layout (NK, w_parts, M, nc)
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
     const long bm = (long)b * M + row;
+    if (acc0) {
+      const double2 a0 = to_d2(acc0[bm * n + col]);
+      re += a0.x;
+      im += a0.y;
+    }
     const double2 p = to_d2(Ap[bm * ldAp + col]);
     double2 chi = chip[bm];
     chi.x += sp->delta[lead[row]];
