@@ -59,7 +59,7 @@ def algorithmic_flops(p):
 
 def run_leg(p, precision, K, W, eps, h0):
     from zandpack_amd.engine import Propagator
-    prop = Propagator(p, precision)
+    prop = Propagator(p, precision, autotune=True)
     t, h = 0.0, h0
 
     def one():
@@ -84,7 +84,7 @@ def run_leg(p, precision, K, W, eps, h0):
     el = time.perf_counter() - t_start
     dev = prop.timing()
     lay = prop.layout_info()
-    out = dict(layout=lay, steps_per_s=acc / el, attempts_per_s=K / el, ms_per_attempt=1e3 * el / K,
+    out = dict(layout=lay, tiles=prop.tiles, steps_per_s=acc / el, attempts_per_s=K / el, ms_per_attempt=1e3 * el / K,
                device_ms_per_attempt=dev["attempt"][0], accepted=acc, rejected=rej,
                h_last=h, t_reached=t)
     prop.close()
@@ -93,7 +93,7 @@ def run_leg(p, precision, K, W, eps, h0):
 
 def profile_leg(p, precision, attempts, h0):
     from zandpack_amd.engine import Propagator
-    prop = Propagator(p, precision)
+    prop = Propagator(p, precision, autotune=True)
     prop.set_profile(True)
     prop.attempt(0.0, h0, 1.0, force_accept=True)
     prop.timing(reset=True)
@@ -109,7 +109,7 @@ def profile_leg(p, precision, attempts, h0):
 def replay_error(p, eps, h0, n_steps):
     """Mixed precision forced onto the FP64 run's accepted (t, h) sequence."""
     from zandpack_amd.engine import Propagator
-    a = Propagator(p, "fp64")
+    a = Propagator(p, "fp64", autotune=True)
     seq, J64 = [], []
     t, h = 0.0, h0
     while len(seq) < n_steps:
@@ -121,7 +121,7 @@ def replay_error(p, eps, h0, n_steps):
             t += h
     s64 = a.sigma()
     a.close()
-    b = Propagator(p, "mixed")
+    b = Propagator(p, "mixed", autotune=True)
     Jm = [b.attempt(ts, hs, 1.0, force_accept=True)["J"] for ts, hs in seq]
     sm = b.sigma()
     b.close()

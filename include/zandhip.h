@@ -117,6 +117,17 @@ int zh_attempt(zh_problem* p, zh_attempt_info* info, double* J);
 int zh_attempt_launch(zh_problem* p, const zh_attempt_info* info);
 int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J);
 
+/* Tile autotuner: times a few compiled tile shapes of each GEMM of the right-hand
+ * side on rejected attempts from the resident state (nothing is committed) and
+ * keeps the fastest; choices are cached per (precision, size bucket) for the
+ * process, so later problems of the same bucket reuse them (force = 1 re-times).
+ * choice (optional) receives the 5 chosen candidate indices.  No counterpart in
+ * the reference. */
+int zh_autotune(zh_problem* p, int force, int* choice);
+
+/* Set the 5 candidate indices directly (0..2 each), e.g. to reuse a tuning. */
+int zh_set_tiles(zh_problem* p, const int* choice);
+
 /* Device memory held by the problem (bytes), for admission by HBM footprint. */
 int zh_problem_bytes(zh_problem* p, long* bytes);
 

@@ -251,3 +251,30 @@ def test_c2_size_properties(native):
     tr = np.trace(ds, axis1=1, axis2=2).real
     np.testing.assert_allclose(tr, J.sum(axis=0), rtol=1e-10, atol=1e-12 * np.abs(ds).max())
     prop.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["wide_edges", "complex_H"])
+def test_every_tile_candidate_matches_oracle(native, precision, case):
+    """The autotuner may pick any compiled tile shape: every candidate gives the
+    same attempt as the oracle."""
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    y0 = _rand_state(p, 31, scale=0.02)
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, p.dH, p.bias, 2)
+    # the error norm is a difference of nearly equal stage sums: its FP32 relative
+    # error is larger than the state's
+    tol = 1e-11 if precision == "fp64" else 1e-4
+    prop = Propagator(p, precision, state=y0, autotune=True)
+    tiles = prop.autotune(force=True)
+    assert all(0 <= v < 3 for v in tiles.values())
+    for c in range(3):
+        say("tiles", case, precision, c)
+        prop.put_state(*y0)
+        prop.set_tiles({k: c for k in tiles})
+        r = prop.attempt(0.1, 0.01, 1.0)
+        assert abs(r["te"] - TE) <= tol * TE, (c, r["te"], TE)
+        s, ps, om = prop.get_state()
+        for got, w in zip((s, ps, om), y_new):
+            assert rel(got, w) < (1e-12 if precision == "fp64" else 1e-5), c
+    prop.close()

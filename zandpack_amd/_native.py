@@ -56,6 +56,8 @@ SIGNATURES = {
     "zh_attempt_launch": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info)]),
     "zh_attempt_collect": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info), _dp]),
     "zh_problem_bytes": (C.c_int, [C.c_void_p, _lp]),
+    "zh_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "zh_set_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_propagate": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_int, C.c_long, C.c_long, _dp, _dp, _dp, _lp, _lp, _dp]),
     "zh_set_layout": (C.c_int, [C.c_void_p, C.c_int]),

@@ -20,7 +20,11 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -93,6 +97,8 @@ struct EngineBase {
   virtual void launch(const zh_attempt_info* info) = 0;
   virtual void collect(zh_attempt_info* info, double* J) = 0;
   virtual long bytes() const = 0;
+  virtual void autotune(int force, int* out) = 0;
+  virtual void set_tiles(const int* sel) = 0;
   virtual void set_layout(int mode) = 0;
   virtual int layout_info(int* out) = 0;
   bool use_graph = true;
@@ -103,22 +109,42 @@ struct EngineBase {
   long attempt_count = 0;
 };
 
-// GEMM tiles per product, picked by a sweep over tile shapes at n=1000, M=4048
-// on MI355X: small single-buffered tiles keep 2-3 workgroups per CU, which hides
-// the global-load latency better than larger or double-buffered tiles.
-template <typename R> struct Tiles;
-template <> struct Tiles<double> {
-  typedef Tile<64, 64, 8, 2> psi;
-  typedef Tile<64, 64, 16, 2> omega;
-  typedef Tile<64, 32, 16, 2> omgxi;
+// Candidate tiles per product (index 0 is the default).  The autotuner
+// (zh_autotune) times them on the problem's own resident data and keeps the
+// fastest per product; choices are cached per (precision, size bucket).
+enum TuneSlot { TS_PSI, TS_HSIG, TS_OMEGA, TS_OMGXI, TS_PI, TS_N };
+constexpr int kCands = 3;
+template <typename R> struct Cands;
+template <> struct Cands<double> {
+  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 32, 16, 2>> psi;
+  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> psiR;
+  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> omega;
+  typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 32, 32, 2>, Tile<64, 64, 16, 2>> omgxi;
 };
-template <> struct Tiles<float> {
-  typedef Tile<64, 64, 32, 2> psi;
-  typedef Tile<64, 64, 16, 2> omega;
-  typedef Tile<64, 32, 32, 2> omgxi;
+template <> struct Cands<float> {
+  typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 64, 16, 2>> psi;
+  typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 128, 16, 2>> psiR;
+  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> omega;
+  typedef TileList<Tile<64, 32, 32, 2>, Tile<64, 32, 16, 2>, Tile<64, 80, 16, 4>> omgxi;
 };
-typedef Tile<64, 32, 16, 2> TileHsig;
-typedef Tile<64, 32, 16, 2> TilePi;
+// σ-path products are FP64 in both precision modes
+typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>> CandHsig;
+typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> CandHsigR;
+typedef TileList<Tile<64, 32, 16, 2>, Tile<32, 32, 16, 2>, Tile<64, 32, 32, 2>> CandPi;
+
+struct TuneKey {
+  int prec, nb, mb, cb, hr;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(prec, nb, mb, cb, hr) < std::tie(o.prec, o.nb, o.mb, o.cb, o.hr);
+  }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::array<int, TS_N>> g_tune_cache;
+static int pow2_bucket(int v) {
+  int b = 1;
+  while (b < v) b <<= 1;
+  return b;
+}
 
 template <typename R>  // storage real type of Psi / Omega and compute type of their GEMMs
 struct Engine : EngineBase {
@@ -167,6 +193,7 @@ struct Engine : EngineBase {
   double* Jd = nullptr;
   double* Jpart = nullptr;
   int iy0 = 7, iyn = 8;
+  int tsel[TS_N] = {0, 0, 0, 0, 0};  // chosen candidate tile per product
   // pinned host mailboxes
   Ctrl* ctrl_h = nullptr;
   double* J_h = nullptr;
@@ -664,7 +691,8 @@ struct Engine : EngineBase {
         GemmArgs<R, double> g{n, nc, Md, Ap, n + nc, (long)Md * (n + nc), Ixic, nc,
                               (long)Md * nc, kPiParts};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
-        HIPCHK((launch_zgemm<double, R, double, true, false, TilePi>(g, NK, e, stream)));
+        HIPCHK((launch_sel<double, R, double, true, false, 0>(tsel[TS_PI], g, NK, e, stream, KFull(),
+                                                              CandPi())));
       });
     }
     if (currents) {
@@ -682,9 +710,11 @@ struct Engine : EngineBase {
       EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih, kPiParts};
       if (h_real) {  // real H: the A operand is Re H
         g.Areal = Hre; g.ldar = n; g.sAr = (long)n * n;
-        HIPCHK((launch_zgemm<double, double, double, false, false, TileHsig, 1>(g, NK, e, stream)));
+        HIPCHK((launch_sel<double, double, double, false, false, 1>(tsel[TS_HSIG], g, NK, e, stream,
+                                                                    KFull(), CandHsigR())));
       } else {
-        HIPCHK((launch_zgemm<double, double, double, false, false, TileHsig>(g, NK, e, stream)));
+        HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, stream,
+                                                                    KFull(), CandHsig())));
       }
     });
     timed(KC_SIGSYM, [&] {
@@ -700,12 +730,8 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, nc, Md, om_in, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
         EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
         KSegOmgXi ks{ML, kTile, ff ? 1 : 0};
-        if (sizeof(R) == 4 && nc > 64 && nc <= 80)  // FP64 at this width drops to 1 wave/SIMD
-          HIPCHK((launch_zgemm<R, R, R, false, false, Tile<kTile, 80, 16, 4>>(g, NK, e, stream, ks)));
-        else if (nc > 32 && nc <= 64)
-          HIPCHK((launch_zgemm<R, R, R, false, false, Tile<kTile, 64, 16, 2>>(g, NK, e, stream, ks)));
-        else
-          HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::omgxi>(g, NK, e, stream, ks)));
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_OMGXI], g, NK, e, stream, ks,
+                                                     typename Cands<R>::omgxi())));
       });
     }
     timed(KC_GEMM_PSI, [&] {
@@ -715,30 +741,33 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
-          HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi>(g0, NK, e0, stream)));
+          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, stream, KFull(),
+                                                       typename Cands<R>::psi())));
         }
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, W, glp, chip,
                      dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, kWParts};
-        HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi, 2>(g, NK, e, stream)));
+        HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, stream, KFull(),
+                                                     typename Cands<R>::psiR())));
       } else {
         // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (Md x (n+nc)) ((n+nc) x n)
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
         EpiPsi<cs> e{psi[kslot], nullptr, Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps,
                      Md, n, nc, ih, ih * ih, kWParts};
-        HIPCHK((launch_zgemm<R, R, R, false, false, typename Tiles<R>::psi>(g, NK, e, stream)));
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, stream, KFull(),
+                                                     typename Cands<R>::psi())));
       }
     });
     timed(KC_GEMM_OMEGA, [&] {
       // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), outer-subtraction epilogue
       EpiOmega<cs> e{omg[kslot], om_in, chip, chim, lead, sps, Md, ih};
-      static_assert(Tiles<R>::omega::BM == kTile && Tiles<R>::omega::BN == kTile, "omega tile");
-      KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};
+      KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};  // every omega candidate is 64 x 64
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
-      HIPCHK((launch_zgemm<R, R, R, false, true, typename Tiles<R>::omega>(g, NK, e, stream, ks)));
+      HIPCHK((launch_sel<R, R, R, false, true, 0>(tsel[TS_OMEGA], g, NK, e, stream, ks,
+                                                  typename Cands<R>::omega())));
     });
   }
 
@@ -801,6 +830,70 @@ struct Engine : EngineBase {
   }
 
   long bytes() const override { return nbytes; }
+
+  void set_tiles(const int* sel) override {
+    for (int i = 0; i < TS_N; ++i)
+      require(sel[i] >= 0 && sel[i] < kCands, "tile candidate index out of range");
+    for (int i = 0; i < TS_N; ++i) tsel[i] = sel[i];
+    drop_graphs();
+  }
+
+  TuneKey tune_key() const {
+    return TuneKey{(int)sizeof(R), pow2_bucket(n), pow2_bucket(Md), pow2_bucket(std::max(nc, 1)),
+                   h_real ? 1 : 0};
+  }
+
+  // Time each candidate tile of every product on rejected attempts from the
+  // resident state (eps = 0: nothing is committed) and keep the fastest.
+  void autotune(int force, int* out) override {
+    require(have_consts && have_state, "autotune needs constants and a resident state", ZH_ESTATE);
+    ensure_h0();
+    const TuneKey key = tune_key();
+    bool cached = false;
+    if (!force) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) {
+        for (int i = 0; i < TS_N; ++i) tsel[i] = it->second[i];
+        cached = true;
+      }
+    }
+    if (!cached) {
+      static const int kc_of[TS_N] = {KC_GEMM_PSI, KC_GEMM_HSIG, KC_GEMM_OMEGA, KC_GEMM_OMGXI,
+                                      KC_GEMM_PI};
+      double best[TS_N];
+      int arg[TS_N];
+      for (int i = 0; i < TS_N; ++i) { best[i] = 1e300; arg[i] = 0; }
+      const bool prof0 = profile;
+      profile = true;
+      zh_attempt_info info;
+      std::memset(&info, 0, sizeof(info));
+      info.h = 1e-6;
+      info.eps = 0.0;  // never accepted: the resident state is untouched
+      for (int c = 0; c < kCands; ++c) {
+        for (int i = 0; i < TS_N; ++i) tsel[i] = c;
+        attempt(&info, nullptr);  // warm-up (code load)
+        for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
+        for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
+        for (int i = 0; i < TS_N; ++i) {
+          const int k = kc_of[i];
+          const double t = kc_count[k] ? kc_ms[k] / kc_count[k] : 0.0;
+          if (t < best[i]) { best[i] = t; arg[i] = c; }
+        }
+      }
+      for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
+      attempt_ms = 0; attempt_count = 0;
+      profile = prof0;
+      for (int i = 0; i < TS_N; ++i) tsel[i] = arg[i];
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      std::array<int, TS_N> a;
+      for (int i = 0; i < TS_N; ++i) a[i] = tsel[i];
+      g_tune_cache[key] = a;
+    }
+    drop_graphs();
+    if (out)
+      for (int i = 0; i < TS_N; ++i) out[i] = tsel[i];
+  }
 
   // Enqueue one attempt on the problem's stream and return; collect() waits.
   void launch(const zh_attempt_info* info) override {
@@ -907,7 +1000,7 @@ static void selftest_impl(int M, int N, int K, const double* A, const double* B,
   HIPCHK(hipMemcpy(db, hb.data(), sizeof(cs) * nb, hipMemcpyHostToDevice));
   GemmArgs<R, R> g{M, N, K, da, TA ? M : K, 0, db, TB ? K : N, 0};
   EpiStore<double2> e{dc, N, 0, 1.0, 0.0};
-  HIPCHK((launch_zgemm<R, R, R, TA, TB, typename Tiles<R>::psi>(g, 1, e, nullptr)));
+  HIPCHK((launch_zgemm<R, R, R, TA, TB, Tile<64, 64, 16, 2>>(g, 1, e, nullptr)));
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(C, dc, sizeof(double2) * ncn, hipMemcpyDeviceToHost));
   (void)hipFree(da);
@@ -1093,6 +1186,20 @@ int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J) {
     require(p && info, "null argument");
     p->eng->collect(info, J);
     if (!std::isfinite(info->te)) throw Error{ZH_ENONFINITE, "non-finite error estimate"};
+  });
+}
+
+int zh_autotune(zh_problem* p, int force, int* choice) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->autotune(force, choice);
+  });
+}
+
+int zh_set_tiles(zh_problem* p, const int* choice) {
+  return zh_try([&] {
+    require(p && choice, "null argument");
+    p->eng->set_tiles(choice);
   });
 }
 

@@ -289,4 +289,23 @@ inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& 
   return hipGetLastError();
 }
 
+// Candidate tile configurations of one GEMM; the engine's autotuner picks one by
+// index at run time (launch_sel).
+template <typename... TLs> struct TileList {};
+
+template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, int RMODE,
+          typename Epi, typename KSeg, typename TL0>
+inline hipError_t launch_sel(int, const GemmArgs<RA, RB>& g, int batch, const Epi& epi,
+                             hipStream_t s, const KSeg& ks, TileList<TL0>) {
+  return launch_zgemm<T, RA, RB, TRANS_A, TRANS_B, TL0, RMODE>(g, batch, epi, s, ks);
+}
+template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, int RMODE,
+          typename Epi, typename KSeg, typename TL0, typename TL1, typename... TLs>
+inline hipError_t launch_sel(int idx, const GemmArgs<RA, RB>& g, int batch, const Epi& epi,
+                             hipStream_t s, const KSeg& ks, TileList<TL0, TL1, TLs...>) {
+  if (idx <= 0) return launch_zgemm<T, RA, RB, TRANS_A, TRANS_B, TL0, RMODE>(g, batch, epi, s, ks);
+  return launch_sel<T, RA, RB, TRANS_A, TRANS_B, RMODE>(idx - 1, g, batch, epi, s, ks,
+                                                        TileList<TL1, TLs...>());
+}
+
 }  // namespace zh

@@ -21,7 +21,7 @@ LAYOUTS = {"auto": 0, "reference": 1, "pruned": 2}
 
 class Propagator:
     def __init__(self, problem: AMEProblem, precision: str = "fp64", device: int = 0,
-                 graph: bool = True, state=None, layout: str = "auto"):
+                 graph: bool = True, state=None, layout: str = "auto", autotune: bool = False):
         self.lib = N.lib()
         self.problem = problem
         self.precision = precision
@@ -42,6 +42,7 @@ class Propagator:
         self.set_pulse(p.pulse)
         N.check(self.lib.zh_set_graph(self.h, 1 if graph else 0))
         self.put_state(*(state if state is not None else p.zero_state()))
+        self.tiles = self.autotune() if autotune else None
 
     # --------------------------------------------------------------- set-up
     def set_pulse(self, pulse):
@@ -117,6 +118,16 @@ class Propagator:
         J = np.empty((self.NL, self.NK))
         N.check(self.lib.zh_attempt_collect(self.h, C.byref(info), N.ptr(J)))
         return dict(te=info.te, h_next=info.h_next, accepted=bool(info.accepted), J=J)
+
+    def autotune(self, force=False):
+        """Pick the fastest tile shape of each GEMM on this problem's resident data."""
+        out = (C.c_int * 5)()
+        N.check(self.lib.zh_autotune(self.h, 1 if force else 0, out))
+        return dict(zip(("psi", "hsig", "omega", "omgxi", "pi"), list(out)))
+
+    def set_tiles(self, choice):
+        arr = (C.c_int * 5)(*[int(choice[k]) for k in ("psi", "hsig", "omega", "omgxi", "pi")])
+        N.check(self.lib.zh_set_tiles(self.h, arr))
 
     def device_bytes(self):
         b = C.c_long(0)

@@ -68,9 +68,11 @@ class Request:
 
 
 class _Live:
-    def __init__(self, req: Request, graph: bool):
+    def __init__(self, req: Request, graph: bool, autotune: bool):
         self.req = req
-        self.prop = Propagator(req.problem, req.precision, graph=graph)
+        # tile choices are cached per (precision, size bucket): only the first
+        # request of a bucket pays for the timing attempts
+        self.prop = Propagator(req.problem, req.precision, graph=graph, autotune=autotune)
         self.bytes = footprint_bytes(req.problem, req.precision)
         self.t = 0.0
         self.h = req.h0
@@ -82,11 +84,12 @@ class Server:
     """Admission by HBM footprint, step-granular continuous batching."""
 
     def __init__(self, hbm_bytes: int = 64 << 30, max_live: int = 64, graph: bool = True,
-                 keep_sigma: bool = True):
+                 keep_sigma: bool = True, autotune: bool = True):
         self.budget = int(hbm_bytes)
         self.max_live = int(max_live)
         self.graph = graph
         self.keep_sigma = keep_sigma
+        self.autotune = autotune
 
     def run(self, requests, realtime: bool = True):
         """Serve ``requests`` (arrival times in seconds).  With ``realtime`` the
@@ -109,7 +112,7 @@ class Server:
                    <= self.budget):
                 r = pending.pop(0)
                 r.admitted = time.perf_counter() - t0
-                lv = _Live(r, self.graph)
+                lv = _Live(r, self.graph, self.autotune)
                 used += lv.bytes
                 self.peak_bytes = max(self.peak_bytes, used)
                 live.append(lv)
