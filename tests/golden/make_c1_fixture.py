@@ -1,0 +1,34 @@
+"""Writes c1_chain20.npz: the oracle's propagation of config 1 (20-site chain,
+two one-channel leads, 8 Lorentzians + 20 Hu poles, M = 96, ramp bias
+V_L = 0.1 (1 - exp(-t/5 fs)) eV) from the isolated-device equilibrium with
+Ψ = Ω = 0, DOP54 with zand's control (absolute eps = 1e-8, t += h_next) to
+t = 50 fs.  Run from the repository root: python tests/golden/make_c1_fixture.py"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import zand_oracle as O  # noqa: E402
+from zandpack_amd import builder  # noqa: E402
+
+T_END, H0, EPS = 50.0, 0.01, 1e-8
+
+
+def run():
+    p = builder.config_chain()
+    out = O.propagate(p.consts(), p.zero_state(), 0.0, T_END, H0, EPS, p.H0, p.dH, p.bias, 2)
+    return p, out
+
+
+if __name__ == "__main__":
+    p, out = run()
+    every = 50
+    np.savez_compressed(
+        os.path.join(os.path.dirname(os.path.abspath(__file__)), "c1_chain20.npz"),
+        times=out["times"], J=out["J"], h=out["h_accepted"], TE=out["TE"],
+        sigma_every=out["sigma"][::every], every=every, n_reject=out["n_reject"],
+        psi_final=out["y"][1], omega_final=out["y"][2], t_end=T_END, h0=H0, eps=EPS)
+    print(len(out["times"]), "steps written")

@@ -278,3 +278,33 @@ def test_every_tile_candidate_matches_oracle(native, precision, case):
         for got, w in zip((s, ps, om), y_new):
             assert rel(got, w) < (1e-12 if precision == "fp64" else 1e-5), c
     prop.close()
+
+
+def test_config1_golden_fixture(native):
+    """Config 1 to 50 fs against tests/golden/c1_chain20.npz: replay of the
+    fixture's (t, h) sequence reproduces J(t) and σ(t) (FP64 ≤ 1e-10, mixed J ≤ 1e-6),
+    and the free-running adaptive run takes the same steps."""
+    import os
+    from zandpack_amd.engine import Propagator
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_chain20.npz"))
+    p = builder.config_chain()
+    ts, hs = f["times"][:-1], f["h"]
+    every = int(f["every"])
+    for precision, tol in (("fp64", 1e-10), ("mixed", 1e-6)):
+        say("c1 replay", precision)
+        prop = Propagator(p, precision)
+        J, sig = [], []
+        for i, (t, h) in enumerate(zip(ts, hs)):
+            if i % every == 0:
+                sig.append(prop.sigma())
+            J.append(prop.attempt(t, h, 1.0, force_accept=True)["J"])
+        J = np.array(J)
+        assert np.abs(J - f["J"]).max() / np.abs(f["J"]).max() < tol
+        if precision == "fp64":
+            assert np.abs(np.array(sig) - f["sigma_every"][:len(sig)]).max() < 1e-10
+        prop.close()
+    prop = Propagator(p, "fp64")
+    out = prop.propagate(0.0, float(f["t_end"]), float(f["h0"]), float(f["eps"]))
+    assert out["n_steps"] == len(f["times"]) - 1 and out["n_rejected"] == int(f["n_reject"])
+    assert rel(out["J"], f["J"]) < 1e-10
+    prop.close()
