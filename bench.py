@@ -57,6 +57,18 @@ def algorithmic_flops(p):
     return {k: NK * v for k, v in c.items()}, na, nc
 
 
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_c3.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)["per_launch"][precision][kernel]
+        return d["fetch_bytes"] + d["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_leg(p, precision, K, W, eps, h0):
     from zandpack_amd.engine import Propagator
     prop = Propagator(p, precision, autotune=True)
@@ -249,7 +261,8 @@ def main():
         ach = flops[dom] / (gemms[dom] * 1e-3) / 1e12
         peak = PEAK[prec] if dom not in ("gemm_hsig", "gemm_pi") else PEAK["fp64"]
         roof[prec] = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": peak,
-                      "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                      "unit": "TFLOP/s", "frac": ach / peak,
+                      "traffic": measured_traffic(dom, prec),
                       "avg_ms": gemms[dom], "kernel_ms": tm}
 
     value = legs["mixed"]["steps_per_s"]

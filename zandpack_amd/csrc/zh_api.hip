@@ -119,7 +119,7 @@ template <> struct Cands<double> {
   typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 32, 16, 2>> psi;
   typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> psiR;
   typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> omega;
-  typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 32, 32, 2>, Tile<64, 64, 16, 2>> omgxi;
+  typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 32, 32, 2>, Tile<64, 80, 16, 4>> omgxi;
 };
 template <> struct Cands<float> {
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 64, 16, 2>> psi;
@@ -766,6 +766,7 @@ struct Engine : EngineBase {
       KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};  // every omega candidate is 64 x 64
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
+      g.xcd_order = 0;  // the skipped passive block would unbalance contiguous XCD bands
       HIPCHK((launch_sel<R, R, R, false, true, 0>(tsel[TS_OMEGA], g, NK, e, stream, ks,
                                                   typename Cands<R>::omega())));
     });

@@ -65,6 +65,7 @@ struct GemmArgs {
   const typename Cplx<RA>::type* A; long lda; long sA;
   const typename Cplx<RB>::type* B; long ldb; long sB;
   int splitk = 1;
+  int xcd_order = 1;  // XCD-aware tile order (off where tiles are skipped unevenly)
   const RA* Areal = nullptr; long ldar = 0; long sAr = 0;
   const RB* Breal = nullptr; long ldbr = 0; long sBr = 0;
 };
@@ -108,15 +109,29 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
   __shared__ T sBr[NBUF][BK][BN + PAD];
   __shared__ T sBi[NBUF][BK][BN + PAD];
 
-  if (ks.skip(blockIdx.y, blockIdx.x)) return;  // uniform over the workgroup
+  // XCD-aware tile order: consecutive workgroups go round-robin to the 8 XCDs,
+  // so give XCD x the contiguous tile range x*per.. (whole row blocks with all
+  // their column tiles), keeping each A row block in one XCD's L2.
+  int ty = blockIdx.y, tx = blockIdx.x;
+  {
+    const int nx = gridDim.x, total = gridDim.x * gridDim.y;
+    if (g.xcd_order && total >= 16) {
+      const int lin = blockIdx.y * nx + blockIdx.x;
+      const int xcd = lin & 7, k = lin >> 3, q = total >> 3, r = total & 7;
+      const int tile = xcd * q + min(xcd, r) + k;
+      ty = tile / nx;
+      tx = tile - ty * nx;
+    }
+  }
+  if (ks.skip(ty, tx)) return;  // uniform over the workgroup
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int batch = blockIdx.z / g.splitk;
   const int part = blockIdx.z - batch * g.splitk;
-  const int m0 = blockIdx.y * BM;
-  const int n0 = blockIdx.x * BN;
+  const int m0 = ty * BM;
+  const int n0 = tx * BN;
   const ca_t* __restrict__ A = g.A + (long)batch * g.sA;
   const cb_t* __restrict__ B = g.B + (long)batch * g.sB;
   const RA* __restrict__ Ar = g.Areal ? g.Areal + (long)batch * g.sAr : nullptr;
@@ -179,7 +194,7 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
 
   // K tiles of this tile's segments, split evenly over the splitk parts
   int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
-  const int nseg = ks.get(blockIdx.y, blockIdx.x, g.K, lo0, hi0, lo1, hi1);
+  const int nseg = ks.get(ty, tx, g.K, lo0, hi0, lo1, hi1);
   const int nt0 = nseg > 0 ? (hi0 - lo0 + BK - 1) / BK : 0;
   const int nt1 = nseg > 1 ? (hi1 - lo1 + BK - 1) / BK : 0;
   const int ktot = nt0 + nt1;

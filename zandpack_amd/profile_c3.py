@@ -1,0 +1,41 @@
+"""Profiling target: a few DOP54 attempts of the n=1000, M=4000, NK=2 workload
+(the headline configuration) in one precision, for rocprofv3 runs, e.g.
+
+    rocprofv3 --kernel-trace --stats -d out -o c3 -- python3 -m zandpack_amd.profile_c3 --precision fp64
+    rocprofv3 --pmc FETCH_SIZE -d out -o fetch -- python3 -m zandpack_amd.profile_c3 --copy-mib 256
+
+``--copy-mib`` also runs one device copy of that many MiB (torch), whose counters
+calibrate FETCH_SIZE / WRITE_SIZE against a known byte count.
+"""
+import argparse
+import sys
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--attempts", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true")
+    a = ap.parse_args()
+    from zandpack_amd import builder
+    from zandpack_amd.engine import Propagator
+    if a.copy_mib:
+        import torch
+        x = torch.ones(a.copy_mib << 20, dtype=torch.uint8, device="cuda")
+        y = x.clone()
+        torch.cuda.synchronize()
+        del x, y
+    p = builder.config_c3(seed=0)
+    prop = Propagator(p, a.precision, graph=not a.no_graph, autotune=True)
+    t, h = 0.0, 0.002
+    for _ in range(a.attempts):
+        r = prop.attempt(t, h, 1.0, force_accept=True)
+        t += h
+    print(f"profile_c3 {a.precision}: {a.attempts} attempts, tiles {prop.tiles}, "
+          f"TE {r['te']:.3e}", flush=True)
+    prop.close()
+
+
+if __name__ == "__main__":
+    sys.exit(main())
