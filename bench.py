@@ -30,11 +30,13 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X dense FP64 / FP32 matri
 
 
 def algorithmic_flops(p):
-    """Real flops per launch of each GEMM class for the work the path performs
-    (SURVEY count C_RHS per k, with 8 flops per complex MAC and 4 per complex x
-    real MAC): a real Hamiltonian halves ψHᵀ and Hσ; Fermi-block pruning keeps
-    only the non-vanishing Ω source terms and skips the passive x passive block
-    of Ω ξ (the terms the kernels execute)."""
+    """Real flops per launch of each GEMM class for the work the path performs.
+    Dense complex count per k: n³ (Hσ) + M n² + M n n_c (ψHᵀ, ξσᵀ) + Σ_α n n_α M_α
+    (Π) + M² n_c (Ωξ) + Σ_α M M_α n_α + M² n_c (Ω sources), 8 flops per complex MAC
+    and 4 per complex x real MAC.  A real Hamiltonian halves ψHᵀ and Hσ, a sparse
+    one reduces them to its non-zeros; Fermi-block pruning keeps only the
+    non-vanishing Ω source terms and skips the passive x passive block of Ωξ (the
+    terms the kernels execute)."""
     NK, n, NL, NX, NOT = p.dims
     M = p.M
     Ma = NX * NOT
@@ -43,13 +45,18 @@ def algorithmic_flops(p):
     nc = int(np.count_nonzero(np.any(p.xi.reshape(-1, n) != 0, axis=0)))
     h_real = not np.any(p.H0.imag)
     hmac = 4.0 if h_real else 8.0
+    # H products as gathers over the row-compressed H (the engine's automatic choice
+    # for at most 64 and at most n/8 non-zeros per row): nnz MACs per column of σ / row of ψ
+    nnz_row = (p.H0 != 0).sum(axis=2)
+    sparse = nnz_row.max() <= 64 and 8 * nnz_row.max() <= n
+    nnz = float(nnz_row.sum(axis=1).max())
     act = ((p.dGP != 0) | (p.dGM != 0)).any(axis=0).reshape(NL, Ma)  # per lead, per mode
     colM = (p.dGM != 0).any(axis=0).reshape(NL, Ma)                  # term1 columns
     rowP = int(((p.dGP != 0).any(axis=0)).sum())                    # term2 rows
     ML = int(act.sum())
     c = {
-        "gemm_hsig": hmac * n ** 3,
-        "gemm_psi": hmac * M * n * n + 8.0 * M * n * nc,
+        "gemm_hsig": hmac * (nnz * n if sparse else n ** 3),
+        "gemm_psi": hmac * M * (nnz if sparse else n * n) + 8.0 * M * n * nc,
         "gemm_pi": 8.0 * sum(n * a * Ma for a in na),
         "gemm_omgxi": 8.0 * (M * M - (M - ML) ** 2) * nc,
         "gemm_omega": 8.0 * (sum(M * int(colM[a].sum()) * na[a] for a in range(NL)) + rowP * M * nc),

@@ -69,6 +69,16 @@ int zh_set_layout(zh_problem* p, int mode);
  * active block size, coupled orbitals}. */
 int zh_layout_info(zh_problem* p, int* out);
 
+/* How the two H products of the right-hand side run (H sigma,
+ * Zandpack/mpi/zand:340-343, and psi H^T, Zandpack/mpi/zand:746-778, both dense
+ * matrix products there): 0 auto (default), 1 dense MFMA products, 2 gathers over
+ * a row-compressed copy of H (at most 64 non-zeros per row; ZH_EINVAL otherwise).
+ * Auto takes 2 when every row holds at most 64 non-zeros and at most n/8.  The
+ * choice is made again whenever H is installed.  Results agree to rounding. */
+int zh_set_sparse(zh_problem* p, int mode);
+/* out[3] = {H real, row-compressed H products in use, entries per compressed row}. */
+int zh_hamiltonian_info(zh_problem* p, int* out);
+
 /* Constants, reference layout, complex128 interleaved unless noted:
  *   H0 (NK,n,n); xi, Ixi (NK,NL,NX,NOT,n); GL_P, dGP=GG_P-GL_P, dGM=GG_M-GL_M,
  *   Xpp, Xpm (NK,NL,NX,NOT).

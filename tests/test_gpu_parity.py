@@ -77,7 +77,8 @@ CASES = {
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("layout", ["reference", "pruned"])
-def test_rhs_parity(native, case, precision, layout):
+@pytest.mark.parametrize("hproducts", ["dense", "sparse"])
+def test_rhs_parity(native, case, precision, layout, hproducts):
     from zandpack_amd.engine import Propagator
     p = CASES[case]()
     sig, psi, omg = _rand_state(p, 7)
@@ -85,9 +86,10 @@ def test_rhs_parity(native, case, precision, layout):
     delta = np.array([p.bias(0.2, a) for a in range(2)]) + np.array([0.05, -0.03])
     ref = O.rhs(p.consts(), sig, psi, omg, Ht, delta)
     Jref = O.currents(p.consts(), psi)
-    say("rhs", case, precision, layout, p.dims)
-    prop = Propagator(p, precision, layout=layout)
+    say("rhs", case, precision, layout, hproducts, p.dims)
+    prop = Propagator(p, precision, layout=layout, hproducts=hproducts)
     ds, dp, do, J = prop.rhs(Ht, delta, sig, psi, omg)
+    assert prop.hamiltonian_info()["sparse"] == (hproducts == "sparse")
     tol = 1e-12 if precision == "fp64" else 2e-5
     for got, want in zip((ds, dp, do), ref):
         assert rel(got, want) < tol, (case, precision, rel(got, want))
@@ -127,6 +129,74 @@ def test_attempt_parity(native, case, layout, zero_ff):
     for got, w in zip((s, ps, om), want):
         assert rel(got, w) < 1e-12
     prop.close()
+
+
+@pytest.mark.parametrize("case", ["small", "complex_H", "small_spin_hubbard", "chain20"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_attempt_sparse_h(native, case, precision):
+    """One attempt with the H products as gathers over the row-compressed H."""
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    y0 = _rand_state(p, 13, scale=0.02)
+    say("attempt sparse-H", case, precision)
+    prop = Propagator(p, precision, state=y0, hproducts="sparse")
+    assert prop.hamiltonian_info()["sparse"]
+    t0, h, eps = 0.3, 0.02, 1e-3
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2)
+    r = prop.attempt(t0, h, eps)
+    tol = 1e-10 if precision == "fp64" else 1e-4
+    assert abs(r["te"] - TE) <= tol * TE
+    assert rel(r["J"], J0) < (1e-12 if precision == "fp64" else 1e-5)
+    if precision == "fp64":
+        assert r["accepted"] == (TE < eps)
+        s, ps, om = prop.get_state()
+        want = y_new if r["accepted"] else y0
+        for got, w in zip((s, ps, om), want):
+            assert rel(got, w) < 1e-12
+    prop.close()
+
+
+def test_sparse_h_choice(native):
+    """Auto takes the row-compressed H for a sparse tight-binding H and the dense
+    products for a dense one; forcing it on a row with more than 64 entries fails."""
+    from zandpack_amd import _native as N
+    from zandpack_amd.engine import Propagator
+    p = builder.config_c2()
+    say("sparse choice", p.dims)
+    prop = Propagator(p, "mixed")
+    info = prop.hamiltonian_info()
+    assert info["real"] and info["sparse"] and 1 <= info["row_entries"] <= 8
+    prop.set_hproducts("dense")
+    assert not prop.hamiltonian_info()["sparse"]
+    prop.close()
+    q = builder.config_small(n_w=10, n_l=10, seed=3)
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=q.H0.shape[1:])
+    q.H0 = q.H0 + 0.01 * (A + A.T)[None]
+    prop = Propagator(q, "fp64")
+    assert not prop.hamiltonian_info()["sparse"]
+    with pytest.raises(N.ZHError):
+        prop.set_hproducts("sparse")
+    assert not prop.hamiltonian_info()["sparse"]
+    prop.close()
+
+
+def test_sparse_dense_propagation_agree(native):
+    """Adaptive propagation of config 1 with either form of the H products."""
+    from zandpack_amd.engine import Propagator
+    outs = []
+    for mode in ("dense", "sparse"):
+        p = builder.config_chain()
+        say("propagate", mode)
+        prop = Propagator(p, "fp64", hproducts=mode)
+        outs.append((prop.propagate(0.0, 2.0, 0.01, 1e-8), prop.get_state()))
+        prop.close()
+    (a, sa), (b, sb) = outs
+    assert a["n_steps"] == b["n_steps"] and a["n_rejected"] == b["n_rejected"]
+    np.testing.assert_allclose(a["times"], b["times"], rtol=1e-12)
+    assert rel(a["J"], b["J"]) < 1e-10
+    for x, y in zip(sa, sb):
+        assert rel(x, y) < 1e-10
 
 
 def test_attempt_reject_keeps_state(native):

@@ -62,6 +62,8 @@ SIGNATURES = {
                                C.c_int, C.c_long, C.c_long, _dp, _dp, _dp, _lp, _lp, _dp]),
     "zh_set_layout": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_layout_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "zh_set_sparse": (C.c_int, [C.c_void_p, C.c_int]),
+    "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_set_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, _dp, _lp]),

@@ -101,6 +101,8 @@ struct EngineBase {
   virtual void set_tiles(const int* sel) = 0;
   virtual void set_layout(int mode) = 0;
   virtual int layout_info(int* out) = 0;
+  virtual void set_sparse(int mode) = 0;
+  virtual void hamiltonian_info(int* out) = 0;
   bool use_graph = true;
   bool profile = false;
   double kc_ms[KC_N] = {0};
@@ -152,6 +154,7 @@ struct Engine : EngineBase {
   static constexpr int kTile = 64;    // Ω GEMM / Ωξ GEMM row and column tile
   static constexpr int kPiParts = 8;  // split-K of Π_c = ψᵀ Ixi_c (K = modes)
   static constexpr int kWParts = 4;   // split-K of W = Ω ξ_c (K = modes)
+  static constexpr int kEllMax = 64;  // most entries per row of the row-compressed H
   int NK, n, NL, NX, NOT;
   int M;        // modes per k in the reference layout, NL*NX*NOT
   int Md = 0;   // modes per k on the device (active block padded to kTile)
@@ -171,6 +174,10 @@ struct Engine : EngineBase {
   bool ff = false;         // passive x passive block of Ω is zero and skipped
   bool ff_dirty = false;   // that block may hold stale values in some slot
   int layout_req = 0;      // 0 auto, 1 reference mode order, 2 active-first (pruned)
+  int sparse_req = 0;      // 0 auto, 1 dense H products, 2 row-compressed H products
+  bool h_sparse = false;   // the H products run as gathers over the row-compressed H
+  int ell_R = 0;           // entries per row of the row-compressed H
+  int hpsi_rb = 0;         // mode rows per workgroup of the sparse ψHᵀ kernel
   std::vector<double> H0h; // host copy of H0 (re-installed after an explicit rhs call)
   std::vector<int> perm;   // device mode -> reference mode (-1: inert padding mode)
   std::vector<int> cidx_h; // coupled orbitals in device column order
@@ -180,6 +187,10 @@ struct Engine : EngineBase {
   int2* t1 = nullptr;
   double2 *Hcur = nullptr, *Ixic = nullptr, *Pic = nullptr, *G = nullptr, *dvec = nullptr;
   double* Hre = nullptr;  // Re H (A operand of Hσ when H is real)
+  int* ell_col = nullptr;     // row-compressed H: (NK, kEllMax, n) column indices
+  double2* ell_val = nullptr; // and values
+  std::vector<int> ell_col_h;
+  std::vector<double2> ell_val_h;
   R* HreT = nullptr;      // Re Hᵀ (B operand of ψHᵀ when H is real)
   double2 *glp = nullptr, *chip = nullptr, *chim = nullptr;
   double* n0 = nullptr;
@@ -309,6 +320,8 @@ struct Engine : EngineBase {
     lead = dalloc<int>(Md);
     Hcur = dalloc<double2>(nn);
     Hre = dalloc<double>(nn);
+    ell_col = dalloc<int>((size_t)NK * kEllMax * n);
+    ell_val = dalloc<double2>((size_t)NK * kEllMax * n);
     HreT = dalloc<R>(nn);
     G = dalloc<double2>(nn);
     dvec = dalloc<double2>((size_t)NK * n);
@@ -438,19 +451,75 @@ struct Engine : EngineBase {
     have_consts = true;
   }
 
+  // Mode rows per workgroup of k_hpsi_ell: those rows live in LDS (at most 32 KiB,
+  // or one row of up to 64 KiB); 0 when a row does not fit.
+  int hpsi_rows() const {
+    const size_t row = (size_t)n * sizeof(cs);
+    for (int rb : {4, 2, 1})
+      if (rb * row <= 32768) return rb;
+    return row <= 65536 ? 1 : 0;
+  }
+
   void install_h(const double* H) {
     // a real Hamiltonian halves the MFMA work of the Hσ and ψHᵀ products
     bool real = true;
     for (size_t i = 0; i < (size_t)NK * n * n && real; ++i) real = H[2 * i + 1] == 0.0;
-    if (real != h_real) drop_graphs();
+    // entries per row: with few, both H products run as gathers (row-compressed H)
+    int rmax = 0;
+    for (long r = 0; r < (long)NK * n; ++r) {
+      int c = 0;
+      const double* row = H + 2 * r * n;
+      for (int o = 0; o < n; ++o) c += (row[2 * o] != 0.0 || row[2 * o + 1] != 0.0);
+      rmax = std::max(rmax, c);
+    }
+    const int rb = hpsi_rows();
+    const bool fits = rmax <= kEllMax && rb > 0;
+    bool sparse = false;
+    if (sparse_req == 2) {
+      require(fits, "row-compressed H needs at most " + std::to_string(kEllMax) +
+                        " non-zeros per row and a Psi row of at most 64 KiB (rows hold " +
+                        std::to_string(rmax) + ")");
+      sparse = true;
+    } else if (sparse_req == 0) {
+      sparse = fits && 8 * rmax <= n;
+    }
+    const int R_ = sparse ? std::max(rmax, 1) : 0;
+    if (real != h_real || sparse != h_sparse || R_ != ell_R) drop_graphs();
     h_real = real;
+    h_sparse = sparse;
+    ell_R = R_;
+    hpsi_rb = rb;
     HIPCHK(hipMemcpyAsync(Hcur, H, sizeof(double2) * NK * n * n, hipMemcpyHostToDevice, stream));
+    if (sparse) {
+      ell_col_h.assign((size_t)NK * R_ * n, 0);
+      ell_val_h.assign((size_t)NK * R_ * n, make_double2(0.0, 0.0));
+      for (int b = 0; b < NK; ++b)
+        for (int o = 0; o < n; ++o) {
+          const double* row = H + 2 * ((long)b * n + o) * n;
+          int r = 0;
+          for (int c = 0; c < n; ++c)
+            if (row[2 * c] != 0.0 || row[2 * c + 1] != 0.0) {
+              const size_t at = ((size_t)b * R_ + r) * n + o;
+              ell_col_h[at] = c;
+              ell_val_h[at] = make_double2(row[2 * c], row[2 * c + 1]);
+              ++r;
+            }
+          for (; r < R_; ++r) ell_col_h[((size_t)b * R_ + r) * n + o] = o;  // zero padding
+        }
+      HIPCHK(hipMemcpyAsync(ell_col, ell_col_h.data(), sizeof(int) * ell_col_h.size(),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(ell_val, ell_val_h.data(), sizeof(double2) * ell_val_h.size(),
+                            hipMemcpyHostToDevice, stream));
+    }
     dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
     hipLaunchKernelGGL((k_transpose_h<cs, R>), grid, dim3(32, 8), 0, stream, n,
                        (long)(n + nc) * n, (const double2*)Hcur, Bp, real ? Hre : nullptr,
                        real ? HreT : nullptr);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));  // the host copies above are read asynchronously
   }
+
+  Ell ell() const { return Ell{ell_col, ell_val, ell_R, n}; }
 
   void set_hubbard(double U_, const double* n0_) override {
     require(U_ == 0.0 || NK == 2, "the Hubbard model couples two spin blocks: NK must be 2");
@@ -637,6 +706,25 @@ struct Engine : EngineBase {
     out[0] = pruned ? 1 : 0; out[1] = ff ? 1 : 0; out[2] = Md; out[3] = ML; out[4] = nc;
     return 5;
   }
+  void set_sparse(int mode) override {
+    require(mode >= 0 && mode <= 2, "H products: 0 auto, 1 dense, 2 row-compressed");
+    const int old = sparse_req;
+    sparse_req = mode;
+    if (have_consts) {
+      try {
+        drop_graphs();
+        install_h(H0h.data());
+        h_dirty = false;
+      } catch (...) {
+        sparse_req = old;
+        install_h(H0h.data());
+        throw;
+      }
+    }
+  }
+  void hamiltonian_info(int* out) override {
+    out[0] = h_real ? 1 : 0; out[1] = h_sparse ? 1 : 0; out[2] = ell_R;
+  }
 
   MapOmega omap() const { return MapOmega{NK, Md, ML, ff ? 1 : 0}; }
 
@@ -708,7 +796,12 @@ struct Engine : EngineBase {
     timed(KC_GEMM_HSIG, [&] {
       GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n};
       EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih, kPiParts};
-      if (h_real) {  // real H: the A operand is Re H
+      if (h_sparse) {
+        constexpr int rb = 8;
+        hipLaunchKernelGGL((k_hsig_ell<rb, EpiSigma>), dim3((n + 255) / 256, (n + rb - 1) / rb, NK),
+                           dim3(256), 0, stream, sig_in, ell(), e);
+        HIPCHK(hipGetLastError());
+      } else if (h_real) {  // real H: the A operand is Re H
         g.Areal = Hre; g.ldar = n; g.sAr = (long)n * n;
         HIPCHK((launch_sel<double, double, double, false, false, 1>(tsel[TS_HSIG], g, NK, e, stream,
                                                                     KFull(), CandHsigR())));
@@ -735,7 +828,32 @@ struct Engine : EngineBase {
       });
     }
     timed(KC_GEMM_PSI, [&] {
-      if (h_real) {
+      if (h_sparse) {
+        // [ΔPξ_c] [ħ σ_cᵀ] into the k slot (MFMA, K = nc), then the gather ψHᵀ with
+        // the Ψ epilogue adding it
+        EpiStore<cs> e0{psi[kslot], n, (long)Md * n, 1.0, 0.0};
+        if (nc > 0) {
+          GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
+                            (long)(n + nc) * n};
+          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, stream, KFull(),
+                                                       typename Cands<R>::psi())));
+        }
+        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, W, glp, chip,
+                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, kWParts};
+        const size_t lds = (size_t)hpsi_rb * n * sizeof(cs);
+        const long sAp = (long)Md * (n + nc);
+        const dim3 grid((Md + hpsi_rb - 1) / hpsi_rb, NK);
+        if (hpsi_rb == 4)
+          hipLaunchKernelGGL((k_hpsi_ell<cs, 4, EpiPsi<cs>>), grid, dim3(256), lds, stream, Md,
+                             (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
+        else if (hpsi_rb == 2)
+          hipLaunchKernelGGL((k_hpsi_ell<cs, 2, EpiPsi<cs>>), grid, dim3(256), lds, stream, Md,
+                             (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
+        else
+          hipLaunchKernelGGL((k_hpsi_ell<cs, 1, EpiPsi<cs>>), grid, dim3(256), lds, stream, Md,
+                             (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
+        HIPCHK(hipGetLastError());
+      } else if (h_real) {
         // [ΔPξ_c] [ħ σ_cᵀ] into the k slot, then ψ Re(H)ᵀ with the epilogue adding it
         EpiStore<cs> e0{psi[kslot], n, (long)Md * n, 1.0, 0.0};
         if (nc > 0) {
@@ -841,7 +959,7 @@ struct Engine : EngineBase {
 
   TuneKey tune_key() const {
     return TuneKey{(int)sizeof(R), pow2_bucket(n), pow2_bucket(Md), pow2_bucket(std::max(nc, 1)),
-                   h_real ? 1 : 0};
+                   h_sparse ? 2 : (h_real ? 1 : 0)};
   }
 
   // Time each candidate tile of every product on rejected attempts from the
@@ -1222,6 +1340,20 @@ int zh_layout_info(zh_problem* p, int* out) {
   return zh_try([&] {
     require(p && out, "null argument");
     p->eng->layout_info(out);
+  });
+}
+
+int zh_set_sparse(zh_problem* p, int mode) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_sparse(mode);
+  });
+}
+
+int zh_hamiltonian_info(zh_problem* p, int* out) {
+  return zh_try([&] {
+    require(p && out, "null argument");
+    p->eng->hamiltonian_info(out);
   });
 }
 

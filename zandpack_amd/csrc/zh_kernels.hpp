@@ -498,4 +498,82 @@ struct KSegOmgXi {
   __device__ inline bool skip(int, int) const { return false; }
 };
 
+// ------------------------------------------------------------ sparse H
+// Row-compressed Hamiltonian with a fixed count R of entries per row:
+// col[(b R + r) n + o], val[(b R + r) n + o] for row o of H[b]; rows with fewer
+// non-zeros are padded with (col = o, val = 0).  Tight-binding and most DFT
+// Hamiltonians hold a few entries per row, so the two H products of the right-hand
+// side (Hσ and ψHᵀ, Zandpack/mpi/zand:340-343, 746-778, dense there) become
+// gathers bounded by HBM instead of n³-class matrix products.
+struct Ell {
+  const int* col;
+  const double2* val;
+  int R, n;
+};
+
+// acc[i, o] = (ψ Hᵀ)[i, o] = Σ_r ψ[i, col_r(o)] H_r(o), for RB mode rows per
+// workgroup with those rows staged in LDS; acc goes to epi (the Ψ right-hand side).
+template <typename C, int RB, typename Epi>
+__global__ __launch_bounds__(256) void k_hpsi_ell(int Md, const C* Ap, long ldAp, long sAp,
+                                                  Ell e, Epi epi) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C* rows = reinterpret_cast<C*>(lds_raw);  // RB x n
+  const int b = blockIdx.y;
+  const int i0 = blockIdx.x * RB;
+  const int n = e.n;
+  for (int q = 0; q < RB; ++q) {
+    const int i = i0 + q;
+    for (int o = threadIdx.x; o < n; o += blockDim.x)
+      rows[q * n + o] = (i < Md) ? Ap[b * sAp + (long)i * ldAp + o] : from_d2<C>(make_double2(0, 0));
+  }
+  __syncthreads();
+  const int* col = e.col + (long)b * e.R * n;
+  const double2* val = e.val + (long)b * e.R * n;
+  for (int o = threadIdx.x; o < n; o += blockDim.x) {
+    double ar[RB], ai[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
+    for (int r = 0; r < e.R; ++r) {
+      const int c = col[(long)r * n + o];
+      const double2 v = val[(long)r * n + o];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const double2 p = to_d2(rows[q * n + c]);
+        ar[q] += p.x * v.x - p.y * v.y;
+        ai[q] += p.x * v.y + p.y * v.x;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RB; ++q)
+      if (i0 + q < Md) epi(b, i0 + q, o, ar[q], ai[q]);
+  }
+}
+
+// acc[o, j] = (H σ)[o, j] = Σ_r H_r(o) σ[col_r(o), j] for RB rows o per workgroup
+// and 256 consecutive columns j; acc goes to epi (the σ right-hand side).
+template <int RB, typename Epi>
+__global__ __launch_bounds__(256) void k_hsig_ell(const double2* sig, Ell e, Epi epi) {
+  const int b = blockIdx.z;
+  const int n = e.n;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int o0 = blockIdx.y * RB;
+  if (j >= n) return;
+  const int* col = e.col + (long)b * e.R * n;
+  const double2* val = e.val + (long)b * e.R * n;
+  const double2* s = sig + (long)b * n * n;
+  for (int q = 0; q < RB; ++q) {
+    const int o = o0 + q;
+    if (o >= n) break;
+    double ar = 0.0, ai = 0.0;
+    for (int r = 0; r < e.R; ++r) {
+      const int c = col[(long)r * n + o];
+      const double2 v = val[(long)r * n + o];
+      const double2 x = s[(long)c * n + j];
+      ar += v.x * x.x - v.y * x.y;
+      ai += v.x * x.y + v.y * x.x;
+    }
+    epi(b, o, j, ar, ai);
+  }
+}
+
 }  // namespace zh

@@ -17,11 +17,13 @@ from .builder import AMEProblem
 
 PRECISIONS = {"fp64": N.ZH_FP64, "mixed": N.ZH_MIXED}
 LAYOUTS = {"auto": 0, "reference": 1, "pruned": 2}
+SPARSE = {"auto": 0, "dense": 1, "sparse": 2}
 
 
 class Propagator:
     def __init__(self, problem: AMEProblem, precision: str = "fp64", device: int = 0,
-                 graph: bool = True, state=None, layout: str = "auto", autotune: bool = False):
+                 graph: bool = True, state=None, layout: str = "auto", autotune: bool = False,
+                 hproducts: str = "auto"):
         self.lib = N.lib()
         self.problem = problem
         self.precision = precision
@@ -33,6 +35,7 @@ class Propagator:
         N.check(self.lib.zh_problem_create(C.byref(d), C.byref(h)))
         self.h = h
         N.check(self.lib.zh_set_layout(self.h, LAYOUTS[layout]))
+        N.check(self.lib.zh_set_sparse(self.h, SPARSE[hproducts]))
         p = problem
         self._keep = [N.c128(a) for a in (p.H0, p.xi, p.Ixi, p.GL_P, p.dGP, p.dGM, p.Xpp, p.Xpm)]
         N.check(self.lib.zh_set_constants(self.h, *[N.ptr(a) for a in self._keep]))
@@ -57,6 +60,16 @@ class Propagator:
         N.check(self.lib.zh_layout_info(self.h, out))
         return dict(active_first=bool(out[0]), skip_passive_block=bool(out[1]),
                     device_modes=out[2], active_block=out[3], coupled_orbitals=out[4])
+
+    def set_hproducts(self, mode: str):
+        """H products as dense MFMA GEMMs ("dense"), gathers over a row-compressed H
+        ("sparse") or chosen from H's non-zeros ("auto")."""
+        N.check(self.lib.zh_set_sparse(self.h, SPARSE[mode]))
+
+    def hamiltonian_info(self):
+        out = (C.c_int * 3)()
+        N.check(self.lib.zh_hamiltonian_info(self.h, out))
+        return dict(real=bool(out[0]), sparse=bool(out[1]), row_entries=out[2])
 
     def set_graph(self, on: bool):
         N.check(self.lib.zh_set_graph(self.h, 1 if on else 0))
