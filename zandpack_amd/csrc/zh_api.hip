@@ -118,21 +118,32 @@ enum TuneSlot { TS_PSI, TS_HSIG, TS_OMEGA, TS_OMGXI, TS_PI, TS_N };
 constexpr int kCands = 3;
 template <typename R> struct Cands;
 template <> struct Cands<double> {
-  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 32, 16, 2>> psi;
-  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> psiR;
-  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> omega;
-  typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 32, 32, 2>, Tile<64, 80, 16, 4>> omgxi;
+  // FP64 tiles carry occupancy hints (waves per SIMD): without them the compiler keeps
+  // the accumulators in AGPRs beside ~120-190 VGPRs and runs 1-2 waves per SIMD,
+  // too few to hide the epilogue's and the next tile's memory latency.
+  typedef TileList<Tile<64, 64, 8, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
+                   Tile<64, 32, 16, 2, false, false, 4>> psi;
+  typedef TileList<Tile<64, 64, 8, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
+                   Tile<64, 64, 16, 2, false, false, 2>> psiR;
+  typedef TileList<Tile<64, 64, 8, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
+                   Tile<64, 64, 8, 2, false, false, 4>> omega;
+  typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<64, 80, 16, 4, false, false, 2>,
+                   Tile<64, 32, 32, 2, false, false, 3>> omgxi;
 };
 template <> struct Cands<float> {
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 64, 16, 2>> psi;
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 128, 16, 2>> psiR;
-  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> omega;
+  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 8, 2, false, false, 5>>
+      omega;
   typedef TileList<Tile<64, 32, 32, 2>, Tile<64, 32, 16, 2>, Tile<64, 80, 16, 4>> omgxi;
 };
 // σ-path products are FP64 in both precision modes
-typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>> CandHsig;
-typedef TileList<Tile<64, 32, 16, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 32, 2>> CandHsigR;
-typedef TileList<Tile<64, 32, 16, 2>, Tile<32, 32, 16, 2>, Tile<64, 32, 32, 2>> CandPi;
+typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<64, 64, 8, 2, false, false, 3>,
+                 Tile<64, 64, 16, 2, false, false, 3>> CandHsig;
+typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
+                 Tile<64, 64, 8, 2, false, false, 4>> CandHsigR;
+typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<32, 32, 16, 2, false, false, 4>,
+                 Tile<64, 80, 16, 4, false, false, 2>> CandPi;
 
 struct TuneKey {
   int prec, nb, mb, cb, hr;
@@ -193,6 +204,7 @@ struct Engine : EngineBase {
   std::vector<double2> ell_val_h;
   R* HreT = nullptr;      // Re Hᵀ (B operand of ψHᵀ when H is real)
   double2 *glp = nullptr, *chip = nullptr, *chim = nullptr;
+  double2 *uvec = nullptr, *vvec = nullptr;  // per-stage (h/ħ)(χ± + Δ) of the Ω epilogue
   double* n0 = nullptr;
   cs *Bp = nullptr, *Ap = nullptr, *Aomg = nullptr, *Bomg = nullptr, *xic = nullptr, *W = nullptr;
   double2* sig[10] = {nullptr};
@@ -328,6 +340,8 @@ struct Engine : EngineBase {
     glp = dalloc<double2>((size_t)NK * Md);
     chip = dalloc<double2>((size_t)NK * Md);
     chim = dalloc<double2>((size_t)NK * Md);
+    uvec = dalloc<double2>((size_t)NK * Md);
+    vvec = dalloc<double2>((size_t)NK * Md);
     n0 = dalloc<double>((size_t)NK * n);
     const size_t mc = (size_t)NK * Md * nc;
     cidx = dalloc<int>(nc);
@@ -770,7 +784,8 @@ struct Engine : EngineBase {
       g.Ap = Ap; g.ldAp = n + nc; g.sAp = (long)Md * (n + nc);
       g.Aomg = Aomg; g.Bomg = Bomg; g.Bp = Bp; g.sig = sig_in; g.cidx = cidx;
       g.dvec = dvec; g.n0 = n0; g.U = U; g.hbar = hbar; g.hubbard = hub && U != 0.0;
-      hipLaunchKernelGGL(k_gather<cs>, dim3(512, 3), dim3(256), 0, stream, g);
+      g.sp = sps; g.chip = chip; g.chim = chim; g.lead = lead; g.u = uvec; g.v = vvec;
+      hipLaunchKernelGGL(k_gather<cs>, dim3(512, 4), dim3(256), 0, stream, g);
     });
     HIPCHK(hipGetLastError());
     if (nc > 0) {
@@ -787,7 +802,8 @@ struct Engine : EngineBase {
       // with no coupled orbitals the reductions run over empty ranges and give J = 0
       timed(KC_CURRENTS, [&] {
         hipLaunchKernelGGL(k_currents<cs>, dim3(kCurChunks, NL, NK), dim3(256), 0, stream, NK, Md,
-                           nc, (const int*)lead, (const cs*)Aomg, (const double2*)Ixic, Jpart);
+                           nc, (const int*)lead, (const cs*)Ap, (long)(n + nc),
+                           (long)Md * (n + nc), (const int*)cidx, (const double2*)Ixic, Jpart);
         hipLaunchKernelGGL(k_currents_final, dim3(1), dim3(((NL * NK + 63) / 64) * 64), 0, stream,
                            NL * NK, (const double*)Jpart, ih * ih, Jd);
       });
@@ -880,7 +896,7 @@ struct Engine : EngineBase {
     });
     timed(KC_GEMM_OMEGA, [&] {
       // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), outer-subtraction epilogue
-      EpiOmega<cs> e{omg[kslot], om_in, chip, chim, lead, sps, Md, ih};
+      EpiOmega<cs> e{omg[kslot], om_in, uvec, vvec, Md};
       KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};  // every omega candidate is 64 x 64
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};

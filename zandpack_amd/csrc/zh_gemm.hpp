@@ -73,14 +73,26 @@ struct GemmArgs {
 constexpr int kThreads = 256;
 
 // K-range selection per output tile (tm = row tile, tn = column tile): up to two
-// half-open segments of the K axis; tiles whose operands are known to vanish are
-// skipped, and a tile with no segment gets acc = 0 (its epilogue still runs).
+// half-open segments [lo0, hi0), [lo1, hi1) of the K axis (nseg of them); tiles
+// whose operands are known to vanish are skipped, and a tile with no segment gets
+// acc = 0 (its epilogue still runs).  Returned by value and built with selects, so
+// that the ranges stay in registers.
+struct KRange {
+  int lo0, hi0, lo1, hi1, nseg;
+};
 struct KFull {
-  __device__ inline int get(int, int, int K, int& lo0, int& hi0, int&, int&) const {
-    lo0 = 0; hi0 = K; return 1;
-  }
+  __device__ inline KRange get(int, int, int K) const { return KRange{0, K, 0, 0, 1}; }
   __device__ inline bool skip(int, int) const { return false; }
 };
+
+// Epilogue operand loaded ahead of the K loop (PF): an epilogue that reads one
+// element per output (such as the stage input of the Ω right-hand side) provides
+//   typedef ... pre_t;  pre_t load(z, row, col) const;
+//   void operator()(z, row, col, re, im, pre_t) const;
+// and the kernel issues those loads before its first K tile, so their latency
+// hides behind the MFMAs instead of following them.
+template <typename Epi, bool PF> struct PreSlot { typedef char type; };
+template <typename Epi> struct PreSlot<Epi, true> { typedef typename Epi::pre_t type; };
 
 // Workgroup: 256 threads = 4 waves laid out WM x (4/WM) over a BM x BN output
 // tile; each wave owns a (BM/WM) x (BN*WM/4) block of 16x16 MFMA tiles.  The
@@ -88,8 +100,10 @@ struct KFull {
 // LDS holds two K tiles and one barrier per tile separates the stages; without,
 // the single LDS tile costs a second barrier but keeps more workgroups per CU.
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
-          int BM, int BN, int BK, int WM, bool DB, int RMODE, typename Epi, typename KSeg>
-__global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi, KSeg ks) {
+          int BM, int BN, int BK, int WM, bool DB, bool PF, int OCC, int RMODE, typename Epi,
+          typename KSeg>
+__global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi,
+                                                              KSeg ks) {
   constexpr bool RA_ = RMODE == 1, RB_ = RMODE == 2;
   constexpr int NBUF = DB ? 2 : 1;
   constexpr int WN = 4 / WM;
@@ -193,20 +207,33 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
   };
 
   // K tiles of this tile's segments, split evenly over the splitk parts
-  int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
-  const int nseg = ks.get(ty, tx, g.K, lo0, hi0, lo1, hi1);
+  const KRange kr = ks.get(ty, tx, g.K);
+  const int lo0 = kr.lo0, hi0 = kr.hi0, lo1 = kr.lo1, hi1 = kr.hi1, nseg = kr.nseg;
   const int nt0 = nseg > 0 ? (hi0 - lo0 + BK - 1) / BK : 0;
   const int nt1 = nseg > 1 ? (hi1 - lo1 + BK - 1) / BK : 0;
   const int ktot = nt0 + nt1;
   const int kper = (ktot + g.splitk - 1) / g.splitk;
   const int kt0 = part * kper;
   const int ktiles = max(0, min(ktot, kt0 + kper) - kt0);
-  auto tile_k0 = [&](int t) { return t < nt0 ? lo0 + t * BK : lo1 + (t - nt0) * BK; };
   auto tile_load = [&](int t) {
-    if (t < nt0) load_tiles(lo0 + t * BK, hi0);
-    else load_tiles(lo1 + (t - nt0) * BK, hi1);
+    const bool first = t < nt0;
+    load_tiles(first ? lo0 + t * BK : lo1 + (t - nt0) * BK, first ? hi0 : hi1);
   };
   const int lr = lane & 15, lk = lane >> 4;
+  typedef typename PreSlot<Epi, PF>::type pre_t;
+  pre_t pre[PF ? TM : 1][PF ? TN : 1][PF ? 4 : 1];
+  if constexpr (PF) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lk, r);
+          const int col = n0 + wn * WTN + j * 16 + lr;
+          if (row < g.M && col < g.N) pre[i][j][r] = epi.load(blockIdx.z, row, col);
+        }
+  }
   if (ktiles > 0) {
     tile_load(kt0);
     if (DB) store_tiles(0);
@@ -280,16 +307,22 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs<RA, RB> g, Epi
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lk, r);
         const int col = n0 + wn * WTN + j * 16 + lr;
-        if (row < g.M && col < g.N) epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
+        if (row < g.M && col < g.N) {
+          if constexpr (PF)
+            epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r], pre[i][j][r]);
+          else
+            epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
+        }
       }
 }
 
 // Tile configuration of one GEMM: BM x BN output tile, BK K tile, WM waves along M,
-// double-buffered LDS or not.
-template <int BM_, int BN_, int BK_, int WM_, bool DB_ = false>
+// double-buffered LDS or not, epilogue operand loaded ahead of the K loop or not,
+// and the fewest waves per SIMD the register allocation must allow (OCC).
+template <int BM_, int BN_, int BK_, int WM_, bool DB_ = false, bool PF_ = false, int OCC_ = 1>
 struct Tile {
-  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_;
-  static constexpr bool DB = DB_;
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, OCC = OCC_;
+  static constexpr bool DB = DB_, PF = PF_;
 };
 
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, typename TL,
@@ -299,7 +332,7 @@ inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& 
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   dim3 grid((g.N + TL::BN - 1) / TL::BN, (g.M + TL::BM - 1) / TL::BM, batch * g.splitk);
   hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, TL::BM, TL::BN, TL::BK, TL::WM,
-                                   TL::DB, RMODE, Epi, KSeg>),
+                                   TL::DB, TL::PF, TL::OCC, RMODE, Epi, KSeg>),
                      grid, dim3(kThreads), 0, s, g, epi, ks);
   return hipGetLastError();
 }

@@ -145,8 +145,12 @@ inline void launch_combine(const CombineArgs<C>& g, const Map& mp, int blocks, h
 }
 
 // Per-stage gathers of the stage input onto the coupled orbitals:
-//   Aomg[:, :nc] = psi[:, cidx], Bomg[:, nc:] = conj(psi[:, cidx]),
-//   Bp[n + c, o] = hbar * sigma[o, cidx[c]], dvec = Hubbard diagonal.
+//   Aomg[:, :nc] = h psi[:, cidx], Bomg[:, nc:] = h conj(psi[:, cidx])   (the Ω
+//     sources come out of their GEMM already scaled by the step),
+//   Bp[n + c, o] = hbar * sigma[o, cidx[c]], dvec = Hubbard diagonal,
+//   u_i = (h/ħ)(χ⁺_i + Δ_α(i)), v_j = (h/ħ)(χ⁻_j + Δ_α(j))   (real parts shifted),
+// so that the Ω epilogue reads two per-mode values instead of the lead table,
+// the bias and the step.
 template <typename C>
 struct GatherArgs {
   int NK, n, M, nc;
@@ -159,6 +163,10 @@ struct GatherArgs {
   const double* n0;                          // (NK, n)
   double U, hbar;
   int hubbard;
+  const StageParams* sp;
+  const double2* chip; const double2* chim;  // (NK, M)
+  const int* lead;                           // (M)
+  double2* u; double2* v;                    // (NK, M)
 };
 
 template <typename C>
@@ -167,15 +175,24 @@ __global__ void k_gather(GatherArgs<C> g) {
   const long tid0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (blockIdx.y == 0) {
     const long total = (long)g.NK * g.M * g.nc;
+    const double h = g.sp->h;
     for (long idx = tid0; idx < total; idx += stride) {
       const int c = (int)(idx % g.nc);
       const long bm = idx / g.nc;                   // b*M + m
       const int b = (int)(bm / g.M);
       const long m = bm - (long)b * g.M;
-      const C v = g.Ap[b * g.sAp + m * g.ldAp + g.cidx[c]];
-      g.Aomg[bm * 2 * g.nc + c] = v;
-      C cv = v; cv.y = -cv.y;
-      g.Bomg[bm * 2 * g.nc + g.nc + c] = cv;
+      const double2 v = to_d2(g.Ap[b * g.sAp + m * g.ldAp + g.cidx[c]]);
+      g.Aomg[bm * 2 * g.nc + c] = from_d2<C>(make_double2(h * v.x, h * v.y));
+      g.Bomg[bm * 2 * g.nc + g.nc + c] = from_d2<C>(make_double2(h * v.x, -h * v.y));
+    }
+  } else if (blockIdx.y == 3) {
+    const long total = (long)g.NK * g.M;
+    const double s = g.sp->h / g.hbar;
+    for (long idx = tid0; idx < total; idx += stride) {
+      const double d = g.sp->delta[g.lead[idx % g.M]];
+      const double2 a = g.chip[idx], c = g.chim[idx];
+      g.u[idx] = make_double2(s * (a.x + d), s * a.y);
+      g.v[idx] = make_double2(s * (c.x + d), s * c.y);
     }
   } else if (blockIdx.y == 1) {
     const long total = (long)g.NK * g.nc * g.n;
@@ -260,9 +277,10 @@ __global__ void k_sigma_sym(int n, const double2* G, double2* out, const StagePa
 // kCurChunks partial sums per (lead, k), then a fixed-order final sum.
 constexpr int kCurChunks = 64;
 
+// Π_α traces from the stage psi on the coupled orbitals (Ap[:, cidx]).
 template <typename C>
-__global__ void k_currents(int NK, int M, int nc, const int* lead, const C* Aomg,
-                           const double2* Ixic, double* part) {
+__global__ void k_currents(int NK, int M, int nc, const int* lead, const C* Ap, long ldAp,
+                           long sAp, const int* cidx, const double2* Ixic, double* part) {
   const int chunk = blockIdx.x, a = blockIdx.y, b = blockIdx.z;
   const long total = (long)M * nc;
   const long per = (total + kCurChunks - 1) / kCurChunks;
@@ -272,7 +290,7 @@ __global__ void k_currents(int NK, int M, int nc, const int* lead, const C* Aomg
     const long m = idx / nc;
     if (lead[m] != a) continue;
     const int c = (int)(idx % nc);
-    const double2 p = to_d2(Aomg[((long)b * M + m) * 2 * nc + c]);
+    const double2 p = to_d2(Ap[b * sAp + m * ldAp + cidx[c]]);
     const double2 x = Ixic[((long)b * M + m) * nc + c];
     acc += p.x * x.x - p.y * x.y;
   }
@@ -444,23 +462,26 @@ struct EpiPsi {
 };
 
 // kΩ = h acc + i (h/ħ)(χ⁺_i + Δ_α(i) - χ⁻_j - Δ_α(j)) Ω_ij   (Zandpack/mpi/zand:784-809)
+// with h acc from the GEMM (its ψ operands carry h) and the per-mode u, v of k_gather.
 template <typename C>
 struct EpiOmega {
   C* out; const C* om;          // (NK, M, M)
-  const double2* chip; const double2* chim;
-  const int* lead;
-  const StageParams* sp;
-  int M; double inv_hbar;
-  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+  const double2* u; const double2* v;  // (NK, M)
+  int M;
+  typedef C pre_t;  // the stage-input element, loadable ahead of the K loop
+  __device__ inline C load(int b, int row, int col) const {
+    return om[((long)b * M + row) * M + col];
+  }
+  __device__ inline void operator()(int b, int row, int col, double re, double im, C ov) const {
     const long off = ((long)b * M + row) * M + col;
-    const double h = sp->h;
-    const double2 a = chip[(long)b * M + row], c = chim[(long)b * M + col];
-    const double dr = (a.x + sp->delta[lead[row]]) - (c.x + sp->delta[lead[col]]);
-    const double di = a.y - c.y;
-    const double2 o = to_d2(om[off]);
-    // i (h/ħ) (dr + i di) o
-    const double2 t = cmul(make_double2(-di * h * inv_hbar, dr * h * inv_hbar), o);
-    out[off] = from_d2<C>(make_double2(h * re + t.x, h * im + t.y));
+    const double2 a = u[(long)b * M + row], c = v[(long)b * M + col];
+    const double dr = a.x - c.x, di = a.y - c.y;
+    const double2 o = to_d2(ov);
+    const double2 t = cmul(make_double2(-di, dr), o);  // i (dr + i di) o
+    out[off] = from_d2<C>(make_double2(re + t.x, im + t.y));
+  }
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    (*this)(b, row, col, re, im, load(b, row, col));
   }
 };
 
@@ -475,12 +496,11 @@ struct EpiOmega {
 struct KSegOmega {
   const int2* t1;  // per column tile: term1 K range [x, y)
   int nc, ML, BM, BN, ff;
-  __device__ inline int get(int tm, int tn, int, int& lo0, int& hi0, int& lo1, int& hi1) const {
+  __device__ inline KRange get(int tm, int tn, int) const {
     const int2 r = t1[tn];
     const bool s1 = r.y > r.x, s2 = tm * BM < ML;
-    if (s1) { lo0 = r.x; hi0 = r.y; lo1 = nc; hi1 = 2 * nc; }
-    else { lo0 = nc; hi0 = 2 * nc; }
-    return (int)s1 + (int)s2;
+    // term1 first when present, then term2; term2 alone otherwise
+    return KRange{s1 ? r.x : nc, s1 ? r.y : 2 * nc, nc, 2 * nc, (int)s1 + (int)s2};
   }
   __device__ inline bool skip(int tm, int tn) const {
     return ff && tm * BM >= ML && tn * BN >= ML;
@@ -490,10 +510,8 @@ struct KSegOmega {
 // W = Ω ξ_c: passive rows see only active columns when the passive block is zero.
 struct KSegOmgXi {
   int ML, BM, ff;
-  __device__ inline int get(int tm, int, int K, int& lo0, int& hi0, int&, int&) const {
-    lo0 = 0;
-    hi0 = (ff && tm * BM >= ML) ? ML : K;
-    return 1;
+  __device__ inline KRange get(int tm, int, int K) const {
+    return KRange{0, (ff && tm * BM >= ML) ? ML : K, 0, 0, 1};
   }
   __device__ inline bool skip(int, int) const { return false; }
 };
