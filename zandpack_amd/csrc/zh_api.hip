@@ -121,29 +121,29 @@ template <> struct Cands<double> {
   // FP64 tiles carry occupancy hints (waves per SIMD): without them the compiler keeps
   // the accumulators in AGPRs beside ~120-190 VGPRs and runs 1-2 waves per SIMD,
   // too few to hide the epilogue's and the next tile's memory latency.
-  typedef TileList<Tile<64, 64, 8, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
-                   Tile<64, 32, 16, 2, false, false, 4>> psi;
-  typedef TileList<Tile<64, 64, 8, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
-                   Tile<64, 64, 16, 2, false, false, 2>> psiR;
-  typedef TileList<Tile<64, 64, 8, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
-                   Tile<64, 64, 8, 2, false, false, 4>> omega;
-  typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<64, 80, 16, 4, false, false, 2>,
-                   Tile<64, 32, 32, 2, false, false, 3>> omgxi;
+  typedef TileList<Tile<64, 64, 8, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
+                   Tile<64, 32, 16, 2, false, 4>> psi;
+  typedef TileList<Tile<64, 64, 8, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
+                   Tile<64, 64, 16, 2, false, 2>> psiR;
+  typedef TileList<Tile<64, 64, 8, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
+                   Tile<64, 64, 8, 2, false, 4>> omega;
+  typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 80, 16, 4, false, 2>,
+                   Tile<64, 32, 32, 2, false, 3>> omgxi;
 };
 template <> struct Cands<float> {
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 64, 16, 2>> psi;
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 128, 16, 2>> psiR;
-  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 8, 2, false, false, 5>>
+  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 8, 2, false, 5>>
       omega;
   typedef TileList<Tile<64, 32, 32, 2>, Tile<64, 32, 16, 2>, Tile<64, 80, 16, 4>> omgxi;
 };
 // σ-path products are FP64 in both precision modes
-typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<64, 64, 8, 2, false, false, 3>,
-                 Tile<64, 64, 16, 2, false, false, 3>> CandHsig;
-typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<64, 64, 16, 2, false, false, 3>,
-                 Tile<64, 64, 8, 2, false, false, 4>> CandHsigR;
-typedef TileList<Tile<64, 32, 16, 2, false, false, 3>, Tile<32, 32, 16, 2, false, false, 4>,
-                 Tile<64, 80, 16, 4, false, false, 2>> CandPi;
+typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 64, 8, 2, false, 3>,
+                 Tile<64, 64, 16, 2, false, 3>> CandHsig;
+typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
+                 Tile<64, 64, 8, 2, false, 4>> CandHsigR;
+typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<32, 32, 16, 2, false, 4>,
+                 Tile<64, 80, 16, 4, false, 2>> CandPi;
 
 struct TuneKey {
   int prec, nb, mb, cb, hr;
@@ -174,6 +174,12 @@ struct Engine : EngineBase {
   double hbar;
   int device;
   hipStream_t stream = nullptr;
+  // Within a stage the Ω branch (its GEMM), the Ψ branch (Ωξ, Ψ right-hand side) and
+  // the σ branch (Π, currents, Hσ) are independent once the stage input is gathered:
+  // they run on three streams (fork/join events, also inside the captured graph), so
+  // MFMA-heavy and HBM-heavy kernels share the CUs.  All streams at default priority.
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   std::vector<void*> allocs;
   Tableau tab = dop54();
   PulseDesc pulse;
@@ -207,6 +213,9 @@ struct Engine : EngineBase {
   double2 *uvec = nullptr, *vvec = nullptr;  // per-stage (h/ħ)(χ± + Δ) of the Ω epilogue
   double* n0 = nullptr;
   cs *Bp = nullptr, *Ap = nullptr, *Aomg = nullptr, *Bomg = nullptr, *xic = nullptr, *W = nullptr;
+  cs* Pred[7] = {nullptr};  // y0 ξ_c and k_j ξ_c of the current attempt (Ω_s ξ_c by linearity)
+  cs* Wst = nullptr;        // Ω_s ξ_c of the current stage
+  int te_omega_blocks = 0;  // error-norm partials written by the last stage's Ω kernel
   double2* sig[10] = {nullptr};
   cs* psi[10] = {nullptr};
   cs* omg[10] = {nullptr};
@@ -246,10 +255,12 @@ struct Engine : EngineBase {
     pulse.hbar = hbar;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& a : aux) HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    for (auto& e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&J_h, sizeof(double) * NL * NK, hipHostMallocDefault));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    te_part = dalloc<double>(kTEParts * kTEBlocks);
     ctrl = dalloc<Ctrl>(1);
     sp = dalloc<StageParams>(7);
     Jd = dalloc<double>((size_t)NL * NK);
@@ -262,6 +273,11 @@ struct Engine : EngineBase {
     for (auto& g : graph)
       if (g) (void)hipGraphExecDestroy(g);
     for (auto& e : ev) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (auto& e : ev_join)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& a : aux)
+      if (a) (void)hipStreamDestroy(a);
     for (void* p : allocs) (void)hipFree(p);
     if (ctrl_h) (void)hipHostFree(ctrl_h);
     if (J_h) (void)hipHostFree(J_h);
@@ -351,6 +367,11 @@ struct Engine : EngineBase {
     Pic = dalloc<double2>((size_t)NK * kPiParts * n * nc);
     xic = dalloc<cs>(mc);
     W = dalloc<cs>(mc * kWParts);
+    for (auto& p : Pred) p = dalloc<cs>(mc);
+    Wst = dalloc<cs>(mc);
+    const int tiles = (Md + kTile - 1) / kTile;  // every Ω tile candidate is kTile x kTile
+    te_omega_blocks = tiles * tiles * NK;
+    te_part = dalloc<double>((size_t)2 * kTEBlocks + te_omega_blocks);
     Ap = dalloc<cs>((size_t)NK * Md * (n + nc));
     Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
     Aomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
@@ -740,31 +761,30 @@ struct Engine : EngineBase {
     out[0] = h_real ? 1 : 0; out[1] = h_sparse ? 1 : 0; out[2] = ell_R;
   }
 
-  MapOmega omap() const { return MapOmega{NK, Md, ML, ff ? 1 : 0}; }
 
-  void combine(int s, bool first, double2* sig_dst, cs* omg_dst, bool last) {
-    const int nt = first ? 0 : s;
+  // Stage inputs of σ and Ψ (Ω's is formed inside its own right-hand side kernel);
+  // terms with a zero coefficient are left out.
+  void combine(int s, bool first, double2* sig_dst, bool last) {
     const int blocks = 2048;
+    int nt = 0, slot[6];
+    double coef[6];
+    if (!first)
+      for (int j = 0; j < s; ++j)
+        if (tab.a[s][j] != 0.0) { slot[nt] = j; coef[nt] = tab.a[s][j]; ++nt; }
     timed(KC_COMBINE, [&] {
       if (!first) {
         CombineArgs<double2> gs{};
         gs.y0 = sig[iy0];
         gs.nterms = nt;
-        for (int j = 0; j < nt; ++j) { gs.k[j] = sig[j]; gs.a[j] = tab.a[s][j]; }
+        for (int j = 0; j < nt; ++j) { gs.k[j] = sig[slot[j]]; gs.a[j] = coef[j]; }
         gs.d1 = sig_dst; gs.d2 = nullptr;
         MapDense ms{NK * n, n, n, n, n};
         launch_combine(gs, ms, blocks, stream);
-        CombineArgs<cs> go{};
-        go.y0 = omg[iy0];
-        go.nterms = nt;
-        for (int j = 0; j < nt; ++j) { go.k[j] = omg[j]; go.a[j] = tab.a[s][j]; }
-        go.d1 = omg_dst; go.d2 = nullptr;
-        launch_combine(go, omap(), blocks, stream);
       }
       CombineArgs<cs> gp{};
       gp.y0 = psi[iy0];
       gp.nterms = nt;
-      for (int j = 0; j < nt; ++j) { gp.k[j] = psi[j]; gp.a[j] = tab.a[s][j]; }
+      for (int j = 0; j < nt; ++j) { gp.k[j] = psi[slot[j]]; gp.a[j] = coef[j]; }
       gp.d1 = Ap;
       gp.d2 = last ? psi[iyn] : nullptr;
       MapDense mp{NK * Md, n, n, n + nc, n};
@@ -773,11 +793,24 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
   }
 
-  // Right-hand side of stage s from (sig_in, Ap, om_in) into slot kslot.
-  void rhs_stage(int s, const double2* sig_in, const cs* om_in, int kslot, bool currents,
+  // Ω side of one stage: stage input y0 + Σ a_j k_j, where Ω_s goes (ynew, the
+  // last stage), whether k_s is stored and whether the error norm is accumulated.
+  struct OmegaStage {
+    const cs* y0 = nullptr;
+    int nterms = 0;
+    const cs* k[6] = {nullptr};
+    double a[6] = {0};
+    double ct[6] = {0};
+    double ct_self = 0.0;
+    cs* ynew = nullptr;
+    bool store_k = true;
+    bool te = false;
+  };
+
+  // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
+  void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, bool currents,
                  bool hub) {
     const StageParams* sps = sp + s;
-    const double ih = 1.0 / hbar;
     timed(KC_GATHER, [&] {
       GatherArgs<cs> g;
       g.NK = NK; g.n = n; g.M = Md; g.nc = nc;
@@ -788,23 +821,49 @@ struct Engine : EngineBase {
       hipLaunchKernelGGL(k_gather<cs>, dim3(512, 4), dim3(256), 0, stream, g);
     });
     HIPCHK(hipGetLastError());
+    // the three branches (serial on the main stream when profiling, so that the
+    // per-class event timings stay meaningful)
+    const bool conc = !profile;
+    hipStream_t s_psi = stream, s_omg = stream;
+    if (conc) {
+      HIPCHK(hipEventRecord(ev_fork, stream));
+      HIPCHK(hipStreamWaitEvent(aux[0], ev_fork, 0));
+      HIPCHK(hipStreamWaitEvent(aux[1], ev_fork, 0));
+      s_omg = aux[0];
+      s_psi = aux[1];
+    }
+    stage_omega(os, kslot, s_omg);
+    stage_psi(s, os, kslot, s_psi);
+    stage_sigma(s, sig_in, kslot, currents, stream);
+    if (conc) {
+      HIPCHK(hipEventRecord(ev_join[0], aux[0]));
+      HIPCHK(hipEventRecord(ev_join[1], aux[1]));
+      HIPCHK(hipStreamWaitEvent(stream, ev_join[0], 0));
+      HIPCHK(hipStreamWaitEvent(stream, ev_join[1], 0));
+    }
+  }
+
+  // σ branch: Π_c, currents (stage 0), Hσ with the σ right-hand side.
+  void stage_sigma(int s, const double2* sig_in, int kslot, bool currents, hipStream_t st) {
+    const StageParams* sps = sp + s;
+    const double ih = 1.0 / hbar;
     if (nc > 0) {
       timed(KC_GEMM_PI, [&] {
         // Π_c = ψᵀ Ixi_c  (n x Md) (Md x nc), FP64, split over K
         GemmArgs<R, double> g{n, nc, Md, Ap, n + nc, (long)Md * (n + nc), Ixic, nc,
                               (long)Md * nc, kPiParts};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
-        HIPCHK((launch_sel<double, R, double, true, false, 0>(tsel[TS_PI], g, NK, e, stream, KFull(),
+        HIPCHK((launch_sel<double, R, double, true, false, 0>(tsel[TS_PI], g, NK, e, st, KFull(),
                                                               CandPi())));
       });
     }
     if (currents) {
       // with no coupled orbitals the reductions run over empty ranges and give J = 0
       timed(KC_CURRENTS, [&] {
-        hipLaunchKernelGGL(k_currents<cs>, dim3(kCurChunks, NL, NK), dim3(256), 0, stream, NK, Md,
+        hipLaunchKernelGGL(k_currents<cs>, dim3(kCurChunks, NL, NK), dim3(256), 0, st, NK, Md,
                            nc, (const int*)lead, (const cs*)Ap, (long)(n + nc),
                            (long)Md * (n + nc), (const int*)cidx, (const double2*)Ixic, Jpart);
-        hipLaunchKernelGGL(k_currents_final, dim3(1), dim3(((NL * NK + 63) / 64) * 64), 0, stream,
+        hipLaunchKernelGGL(k_currents_final, dim3(1), dim3(((NL * NK + 63) / 64) * 64), 0, st,
                            NL * NK, (const double*)Jpart, ih * ih, Jd);
       });
       HIPCHK(hipGetLastError());
@@ -815,32 +874,48 @@ struct Engine : EngineBase {
       if (h_sparse) {
         constexpr int rb = 8;
         hipLaunchKernelGGL((k_hsig_ell<rb, EpiSigma>), dim3((n + 255) / 256, (n + rb - 1) / rb, NK),
-                           dim3(256), 0, stream, sig_in, ell(), e);
+                           dim3(256), 0, st, sig_in, ell(), e);
         HIPCHK(hipGetLastError());
       } else if (h_real) {  // real H: the A operand is Re H
         g.Areal = Hre; g.ldar = n; g.sAr = (long)n * n;
-        HIPCHK((launch_sel<double, double, double, false, false, 1>(tsel[TS_HSIG], g, NK, e, stream,
+        HIPCHK((launch_sel<double, double, double, false, false, 1>(tsel[TS_HSIG], g, NK, e, st,
                                                                     KFull(), CandHsigR())));
       } else {
-        HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, stream,
+        HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, st,
                                                                     KFull(), CandHsig())));
       }
     });
     timed(KC_SIGSYM, [&] {
       dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
-      hipLaunchKernelGGL(k_sigma_sym, grid, dim3(32, 8), 0, stream, n, (const double2*)G,
+      hipLaunchKernelGGL(k_sigma_sym, grid, dim3(32, 8), 0, st, n, (const double2*)G,
                          sig[kslot], sps, ih);
     });
     HIPCHK(hipGetLastError());
+  }
+
+  // Ψ branch: Ω_s ξ_c by linearity, then the Ψ right-hand side.
+  void stage_psi(int s, const OmegaStage& os, int kslot, hipStream_t st) {
+    const StageParams* sps = sp + s;
+    const double ih = 1.0 / hbar;
     if (nc > 0) {
       timed(KC_GEMM_OMGXI, [&] {
-        // W = Ω ξ_c  (Md x Md)(Md x nc); the column tile follows nc so that Ω is
-        // streamed once and few padded columns are computed
-        GemmArgs<R, R> g{Md, nc, Md, om_in, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
+        // X ξ_c  (Md x Md)(Md x nc) for X = y0 (first stage) or the previous stage's k,
+        // then Ω_s ξ_c = y0 ξ_c + Σ_j a_sj k_j ξ_c: Ω is streamed once per stage and no
+        // stage-input copy of it is needed
+        const cs* X = (s == 0) ? os.y0 : omg[s - 1];
+        GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
         EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
         KSegOmgXi ks{ML, kTile, ff ? 1 : 0};
-        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_OMGXI], g, NK, e, stream, ks,
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_OMGXI], g, NK, e, st, ks,
                                                      typename Cands<R>::omgxi())));
+        WSumArgs<cs> w{};
+        w.part = W;
+        for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
+        w.W = Wst;
+        for (int j = 0; j < 6; ++j) w.a[j] = (j < s) ? tab.a[s][j] : 0.0;
+        w.s = s; w.parts = kWParts; w.NK = NK; w.M = Md; w.nc = nc;
+        hipLaunchKernelGGL(k_wsum<cs>, dim3(1024), dim3(256), 0, st, w);
+        HIPCHK(hipGetLastError());
       });
     }
     timed(KC_GEMM_PSI, [&] {
@@ -851,22 +926,22 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
-          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, stream, KFull(),
+          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, KFull(),
                                                        typename Cands<R>::psi())));
         }
-        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, W, glp, chip,
-                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, kWParts};
+        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
+                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1};
         const size_t lds = (size_t)hpsi_rb * n * sizeof(cs);
         const long sAp = (long)Md * (n + nc);
         const dim3 grid((Md + hpsi_rb - 1) / hpsi_rb, NK);
         if (hpsi_rb == 4)
-          hipLaunchKernelGGL((k_hpsi_ell<cs, 4, EpiPsi<cs>>), grid, dim3(256), lds, stream, Md,
+          hipLaunchKernelGGL((k_hpsi_ell<cs, 4, EpiPsi<cs>>), grid, dim3(256), lds, st, Md,
                              (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
         else if (hpsi_rb == 2)
-          hipLaunchKernelGGL((k_hpsi_ell<cs, 2, EpiPsi<cs>>), grid, dim3(256), lds, stream, Md,
+          hipLaunchKernelGGL((k_hpsi_ell<cs, 2, EpiPsi<cs>>), grid, dim3(256), lds, st, Md,
                              (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
         else
-          hipLaunchKernelGGL((k_hpsi_ell<cs, 1, EpiPsi<cs>>), grid, dim3(256), lds, stream, Md,
+          hipLaunchKernelGGL((k_hpsi_ell<cs, 1, EpiPsi<cs>>), grid, dim3(256), lds, st, Md,
                              (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
         HIPCHK(hipGetLastError());
       } else if (h_real) {
@@ -875,47 +950,72 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
-          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, stream, KFull(),
+          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, KFull(),
                                                        typename Cands<R>::psi())));
         }
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
-        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, W, glp, chip,
-                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, kWParts};
-        HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, stream, KFull(),
+        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
+                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1};
+        HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psiR())));
       } else {
         // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (Md x (n+nc)) ((n+nc) x n)
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
-        EpiPsi<cs> e{psi[kslot], nullptr, Ap, n + nc, xic, W, glp, chip, dvec, lead, cpos, sps,
-                     Md, n, nc, ih, ih * ih, kWParts};
-        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, stream, KFull(),
+        EpiPsi<cs> e{psi[kslot], nullptr, Ap, n + nc, xic, Wst, glp, chip, dvec, lead, cpos, sps,
+                     Md, n, nc, ih, ih * ih, 1};
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
       }
     });
+  }
+
+  // Ω branch: sources, stage input, outer term and (last stage) the error norm.
+  void stage_omega(const OmegaStage& os, int kslot, hipStream_t st) {
     timed(KC_GEMM_OMEGA, [&] {
-      // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), outer-subtraction epilogue
-      EpiOmega<cs> e{omg[kslot], om_in, uvec, vvec, Md};
+      // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), with the stage input
+      // formed, the outer-subtraction term applied and (last stage) the error norm
+      // accumulated in the epilogue
       KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};  // every omega candidate is 64 x 64
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
       g.xcd_order = 0;  // the skipped passive block would unbalance contiguous XCD bands
-      HIPCHK((launch_sel<R, R, R, false, true, 0>(tsel[TS_OMEGA], g, NK, e, stream, ks,
-                                                  typename Cands<R>::omega())));
+      auto fill = [&](auto& e) {
+        e.out = os.store_k ? omg[kslot] : nullptr;
+        e.y0 = os.y0;
+        e.nterms = os.nterms;
+        for (int j = 0; j < 6; ++j) { e.k[j] = os.k[j]; e.a[j] = os.a[j]; e.ct[j] = os.ct[j]; }
+        e.ynew = os.ynew;
+        e.u = uvec; e.v = vvec; e.M = Md;
+        e.ct_self = os.ct_self;
+        e.te_part = te_part + 2 * kTEBlocks;
+      };
+      if (os.te) {
+        EpiOmegaRK<cs, true> e{};
+        fill(e);
+        HIPCHK((launch_sel<R, R, R, false, true, 0>(tsel[TS_OMEGA], g, NK, e, st, ks,
+                                                    typename Cands<R>::omega())));
+      } else {
+        EpiOmegaRK<cs, false> e{};
+        fill(e);
+        HIPCHK((launch_sel<R, R, R, false, true, 0>(tsel[TS_OMEGA], g, NK, e, st, ks,
+                                                    typename Cands<R>::omega())));
+      }
     });
   }
 
   void te_and_control() {
     timed(KC_TE, [&] {
       // slots with a non-zero error weight (b5 - b4 vanishes for k_2)
+      // (the Ω part is accumulated by the last stage's Ω kernel)
       TEArgs<double2> ts{};
-      TEArgs<cs> tp{}, to{};
+      TEArgs<cs> tp{};
       int nw = 0;
       for (int j = 0; j < 7; ++j) {
         if (tab.ct[j] == 0.0) continue;
-        ts.k[nw] = sig[j]; tp.k[nw] = psi[j]; to.k[nw] = omg[j];
-        ts.ct[nw] = tp.ct[nw] = to.ct[nw] = tab.ct[j];
+        ts.k[nw] = sig[j]; tp.k[nw] = psi[j];
+        ts.ct[nw] = tp.ct[nw] = tab.ct[j];
         ++nw;
       }
       require(nw == 6, "error weights: expected six non-zero entries");
@@ -925,13 +1025,11 @@ struct Engine : EngineBase {
                          ms, te_part);
       hipLaunchKernelGGL((k_te<cs, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, tp, mp,
                          te_part + kTEBlocks);
-      hipLaunchKernelGGL((k_te<cs, MapOmega, 6>), dim3(kTEBlocks), dim3(256), 0, stream, to,
-                         omap(), te_part + 2 * kTEBlocks);
     });
     HIPCHK(hipGetLastError());
     timed(KC_CTRL, [&] {
       hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, stream, (const double*)te_part,
-                         kTEParts * kTEBlocks, ctrl);
+                         2 * kTEBlocks + te_omega_blocks, ctrl);
     });
     HIPCHK(hipGetLastError());
   }
@@ -946,11 +1044,28 @@ struct Engine : EngineBase {
     for (int s = 0; s < 7; ++s) {
       const bool last = (s == 6);
       double2* sdst = last ? sig[iyn] : sig[9];
-      cs* odst = last ? omg[iyn] : omg[9];
-      combine(s, s == 0, sdst, odst, last);
+      combine(s, s == 0, sdst, last);
       const double2* sin_ = (s == 0) ? sig[iy0] : sdst;
-      const cs* oin = (s == 0) ? omg[iy0] : odst;
-      rhs_stage(s, sin_, oin, s, s == 0, true);
+      OmegaStage os;
+      os.y0 = omg[iy0];
+      for (int j = 0; j < s; ++j)
+        if (tab.a[s][j] != 0.0) {
+          os.k[os.nterms] = omg[j];
+          os.a[os.nterms] = tab.a[s][j];
+          os.ct[os.nterms] = tab.ct[j];
+          ++os.nterms;
+        }
+      if (last) {
+        // y_new = Ω_7; k_7 is only needed by the error norm, accumulated in place
+        os.ynew = omg[iyn];
+        os.store_k = false;
+        os.te = true;
+        os.ct_self = tab.ct[6];
+        int covered = 0;
+        for (int j = 0; j < 6; ++j) covered += (tab.ct[j] == 0.0 || tab.a[6][j] != 0.0);
+        require(covered == 6, "error weights need a stage-7 term the tableau does not carry");
+      }
+      rhs_stage(s, sin_, os, s, s == 0, true);
     }
     te_and_control();
   }
@@ -1105,7 +1220,9 @@ struct Engine : EngineBase {
     // the input's passive block may be non-zero: evaluate it in full
     const bool ff_keep = ff;
     ff = false;
-    rhs_stage(0, sig[9], omg[9], 0, J != nullptr, false);
+    OmegaStage os;
+    os.y0 = omg[9];
+    rhs_stage(0, sig[9], os, 0, J != nullptr, false);
     HIPCHK(hipStreamSynchronize(stream));
     ff = ff_keep;
     ff_dirty = true;

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace zh {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -85,14 +87,25 @@ struct KFull {
   __device__ inline bool skip(int, int) const { return false; }
 };
 
-// Epilogue operand loaded ahead of the K loop (PF): an epilogue that reads one
-// element per output (such as the stage input of the Ω right-hand side) provides
-//   typedef ... pre_t;  pre_t load(z, row, col) const;
-//   void operator()(z, row, col, re, im, pre_t) const;
-// and the kernel issues those loads before its first K tile, so their latency
-// hides behind the MFMAs instead of following them.
-template <typename Epi, bool PF> struct PreSlot { typedef char type; };
-template <typename Epi> struct PreSlot<Epi, true> { typedef typename Epi::pre_t type; };
+// Reducing epilogues (static constexpr bool kReduce = true) accumulate one double
+// per thread through operator()(z, row, col, re, im, double& acc); the workgroup's
+// sum (a fixed-order tree over its threads) goes to epi.finish(block, sum), where
+// block is the workgroup's linear index in the grid.  Skipped tiles report 0.
+template <typename Epi, typename = void> struct HasReduce : std::false_type {};
+template <typename Epi>
+struct HasReduce<Epi, std::void_t<decltype(Epi::kReduce)>>
+    : std::integral_constant<bool, Epi::kReduce> {};
+
+// Blocked epilogues (static constexpr bool kBlock4 = true) take the four outputs a
+// lane holds in one 16x16 MFMA tile (same column, four rows) in one call,
+//   block(z, rows[4], col, re[4], im[4], double& acc),
+// rows[r] = -1 for rows outside the matrix, so that they can issue every load of
+// the four elements before their first store (stores to a buffer that may alias
+// the inputs otherwise keep the loads of the next element behind them).
+template <typename Epi, typename = void> struct HasBlock4 : std::false_type {};
+template <typename Epi>
+struct HasBlock4<Epi, std::void_t<decltype(Epi::kBlock4)>>
+    : std::integral_constant<bool, Epi::kBlock4> {};
 
 // Workgroup: 256 threads = 4 waves laid out WM x (4/WM) over a BM x BN output
 // tile; each wave owns a (BM/WM) x (BN*WM/4) block of 16x16 MFMA tiles.  The
@@ -100,7 +113,7 @@ template <typename Epi> struct PreSlot<Epi, true> { typedef typename Epi::pre_t 
 // LDS holds two K tiles and one barrier per tile separates the stages; without,
 // the single LDS tile costs a second barrier but keeps more workgroups per CU.
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
-          int BM, int BN, int BK, int WM, bool DB, bool PF, int OCC, int RMODE, typename Epi,
+          int BM, int BN, int BK, int WM, bool DB, int OCC, int RMODE, typename Epi,
           typename KSeg>
 __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi,
                                                               KSeg ks) {
@@ -137,7 +150,14 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
       tx = tile - ty * nx;
     }
   }
-  if (ks.skip(ty, tx)) return;  // uniform over the workgroup
+  constexpr bool RED = HasReduce<Epi>::value;
+  const int blin = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (ks.skip(ty, tx)) {  // uniform over the workgroup
+    if constexpr (RED) {
+      if (threadIdx.x == 0) epi.finish(blin, 0.0);
+    }
+    return;
+  }
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -220,20 +240,6 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
     load_tiles(first ? lo0 + t * BK : lo1 + (t - nt0) * BK, first ? hi0 : hi1);
   };
   const int lr = lane & 15, lk = lane >> 4;
-  typedef typename PreSlot<Epi, PF>::type pre_t;
-  pre_t pre[PF ? TM : 1][PF ? TN : 1][PF ? 4 : 1];
-  if constexpr (PF) {
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lk, r);
-          const int col = n0 + wn * WTN + j * 16 + lr;
-          if (row < g.M && col < g.N) pre[i][j][r] = epi.load(blockIdx.z, row, col);
-        }
-  }
   if (ktiles > 0) {
     tile_load(kt0);
     if (DB) store_tiles(0);
@@ -299,30 +305,61 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
     __syncthreads();
   }
 
+  double red = 0.0;
+  if constexpr (HasBlock4<Epi>::value) {
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lk, r);
+      for (int j = 0; j < TN; ++j) {
         const int col = n0 + wn * WTN + j * 16 + lr;
-        if (row < g.M && col < g.N) {
-          if constexpr (PF)
-            epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r], pre[i][j][r]);
-          else
-            epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
+        int rows[4];
+        double re[4], im[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lk, r);
+          rows[r] = (row < g.M && col < g.N) ? row : -1;
+          re[r] = (double)cr[i][j][r];
+          im[r] = (double)ci[i][j][r];
         }
+        epi.block(blockIdx.z, rows, col, re, im, red);
       }
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lk, r);
+          const int col = n0 + wn * WTN + j * 16 + lr;
+          if (row < g.M && col < g.N) {
+            if constexpr (RED)
+              epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r], red);
+            else
+              epi(blockIdx.z, row, col, (double)cr[i][j][r], (double)ci[i][j][r]);
+          }
+        }
+  }
+  if constexpr (RED) {
+    __shared__ double sred[kThreads];
+    sred[tid] = red;
+    __syncthreads();
+#pragma unroll
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+      if (tid < s) sred[tid] += sred[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) epi.finish(blin, sred[0]);
+  }
 }
 
 // Tile configuration of one GEMM: BM x BN output tile, BK K tile, WM waves along M,
-// double-buffered LDS or not, epilogue operand loaded ahead of the K loop or not,
-// and the fewest waves per SIMD the register allocation must allow (OCC).
-template <int BM_, int BN_, int BK_, int WM_, bool DB_ = false, bool PF_ = false, int OCC_ = 1>
+// double-buffered LDS or not, and the fewest waves per SIMD the register allocation
+// must allow (OCC).
+template <int BM_, int BN_, int BK_, int WM_, bool DB_ = false, int OCC_ = 1>
 struct Tile {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, OCC = OCC_;
-  static constexpr bool DB = DB_, PF = PF_;
+  static constexpr bool DB = DB_;
 };
 
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, typename TL,
@@ -332,7 +369,7 @@ inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& 
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   dim3 grid((g.N + TL::BN - 1) / TL::BN, (g.M + TL::BM - 1) / TL::BM, batch * g.splitk);
   hipLaunchKernelGGL((zgemm_kernel<T, RA, RB, TRANS_A, TRANS_B, TL::BM, TL::BN, TL::BK, TL::WM,
-                                   TL::DB, TL::PF, TL::OCC, RMODE, Epi, KSeg>),
+                                   TL::DB, TL::OCC, RMODE, Epi, KSeg>),
                      grid, dim3(kThreads), 0, s, g, epi, ks);
   return hipGetLastError();
 }

@@ -82,17 +82,6 @@ struct MapDense {  // rows x cols block, source row stride lds, destination stri
     return cols;
   }
 };
-// NK blocks of (Md x Md) Ω; with ff the passive x passive corner [ML,Md)^2 is not
-// visited (it is identically zero in every slot).
-struct MapOmega {
-  int NK, Md, ML, ff;
-  __host__ __device__ inline int nrows() const { return NK * Md; }
-  __device__ inline int row(int r, long& src, long& d1, long& d2) const {
-    src = d1 = d2 = (long)r * Md;
-    const int i = r % Md;
-    return (ff && i >= ML) ? ML : Md;
-  }
-};
 
 // dst = y0 + sum_j a_j k_j   (Zandpack/mpi/zand:306-313, 722-735)
 template <typename C>
@@ -321,7 +310,6 @@ struct TEArgs {
 };
 
 constexpr int kTEBlocks = 1024;
-constexpr int kTEParts = 3;  // sigma, Psi, Omega
 
 // The slots with a non-zero error weight are packed in front (k[0..NW)).
 template <typename C, typename Map, int NW>
@@ -461,29 +449,115 @@ struct EpiPsi {
   }
 };
 
-// kΩ = h acc + i (h/ħ)(χ⁺_i + Δ_α(i) - χ⁻_j - Δ_α(j)) Ω_ij   (Zandpack/mpi/zand:784-809)
-// with h acc from the GEMM (its ψ operands carry h) and the per-mode u, v of k_gather.
-template <typename C>
-struct EpiOmega {
-  C* out; const C* om;          // (NK, M, M)
-  const double2* u; const double2* v;  // (NK, M)
+// The Ω right-hand side of one stage with the stage input formed in place:
+//   Ω_s = y0 + Σ_j a_j k_j                     (Zandpack/mpi/zand:306-313, 722-735)
+//   k_s = h acc + i (u_row - v_col) Ω_s        (Zandpack/mpi/zand:784-809)
+// so no stage-input copy of Ω is written or read back.  out (k_s) may be null when
+// nothing reads it; ynew receives Ω_s (the last stage: the new state).  With TE the
+// epilogue also accumulates |Σ_j ct_j k_j + ct_s k_s|² (Zandpack/mpi_funcs.py:247-252)
+// over the elements it visits, reduced per workgroup into te_part.
+template <typename C, bool TE>
+struct EpiOmegaRK {
+  static constexpr bool kReduce = TE;
+  static constexpr bool kBlock4 = true;
+  static constexpr int kMaxTerms = 6;
+  C* out;                                 // (NK, M, M) k_s, or null
+  const C* y0;                            // (NK, M, M)
+  const C* k[kMaxTerms]; double a[kMaxTerms]; int nterms;
+  C* ynew;                                // Ω_s, or null
+  const double2* u; const double2* v;     // (NK, M), (h/ħ)(χ± + Δ)
   int M;
-  typedef C pre_t;  // the stage-input element, loadable ahead of the K loop
-  __device__ inline C load(int b, int row, int col) const {
-    return om[((long)b * M + row) * M + col];
+  double ct[kMaxTerms]; double ct_self;   // error weights of k[] and of k_s (TE)
+  double* te_part;                        // one partial per workgroup (TE)
+
+  // Four outputs of one column (rows[r] < 0: outside): every slot load first, then
+  // the stores.
+  __device__ inline void block(int b, const int* rows, int col, const double* re,
+                               const double* im, double& acc) const {
+    long off[4];
+    double sr[4], si[4], er[4], ei[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      off[r] = ((long)b * M + (rows[r] < 0 ? 0 : rows[r])) * M + col;
+      const double2 y = rows[r] < 0 ? make_double2(0.0, 0.0) : to_d2(y0[off[r]]);
+      sr[r] = y.x; si[r] = y.y; er[r] = 0.0; ei[r] = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxTerms; ++j)
+      if (j < nterms) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double2 x = rows[r] < 0 ? make_double2(0.0, 0.0) : to_d2(k[j][off[r]]);
+          sr[r] += a[j] * x.x;
+          si[r] += a[j] * x.y;
+          if constexpr (TE) {
+            er[r] += ct[j] * x.x;
+            ei[r] += ct[j] * x.y;
+          }
+        }
+      }
+    const double2 vv = v[(long)b * M + col];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (rows[r] < 0) continue;
+      if (ynew) ynew[off[r]] = from_d2<C>(make_double2(sr[r], si[r]));
+      const double2 uu = u[(long)b * M + rows[r]];
+      const double dr = uu.x - vv.x, di = uu.y - vv.y;
+      // k_s = h acc + i (dr + i di) Ω_s
+      const double2 t = cmul(make_double2(-di, dr), make_double2(sr[r], si[r]));
+      const C ks = from_d2<C>(make_double2(re[r] + t.x, im[r] + t.y));
+      if (out) out[off[r]] = ks;
+      if constexpr (TE) {
+        const double xr = er[r] + ct_self * (double)ks.x, xi = ei[r] + ct_self * (double)ks.y;
+        acc += xr * xr + xi * xi;
+      }
+    }
   }
-  __device__ inline void operator()(int b, int row, int col, double re, double im, C ov) const {
-    const long off = ((long)b * M + row) * M + col;
-    const double2 a = u[(long)b * M + row], c = v[(long)b * M + col];
-    const double dr = a.x - c.x, di = a.y - c.y;
-    const double2 o = to_d2(ov);
-    const double2 t = cmul(make_double2(-di, dr), o);  // i (dr + i di) o
-    out[off] = from_d2<C>(make_double2(re + t.x, im + t.y));
-  }
-  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
-    (*this)(b, row, col, re, im, load(b, row, col));
-  }
+  __device__ inline void finish(int block, double sum) const { te_part[block] = sum; }
 };
+
+// Ω_s ξ_c by linearity: the split-K partials of this stage's product X ξ_c (X = y0
+// at the first stage, else the previous stage's k) are summed into pred[s], then
+//   W = pred[0] + Σ_{j<s} a_j pred[j+1]
+// (pred[0] = y0 ξ_c, pred[j] = k_j ξ_c).
+template <typename C>
+struct WSumArgs {
+  const C* part;            // (NK, parts, M, nc)
+  C* pred[7];               // (NK, M, nc) each
+  C* W;                     // (NK, M, nc)
+  double a[6];
+  int s, parts, NK, M, nc;
+};
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_wsum(WSumArgs<C> g) {
+  const long per = (long)g.M * g.nc, total = (long)g.NK * per;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long b = idx / per, rem = idx - b * per;
+    double pr = 0.0, pi = 0.0;
+    for (int p = 0; p < g.parts; ++p) {
+      const double2 x = to_d2(g.part[(b * g.parts + p) * per + rem]);
+      pr += x.x;
+      pi += x.y;
+    }
+    const C ps = from_d2<C>(make_double2(pr, pi));
+    g.pred[g.s][idx] = ps;
+    double wr, wi;
+    if (g.s == 0) {
+      wr = (double)ps.x; wi = (double)ps.y;
+    } else {
+      const double2 y = to_d2(g.pred[0][idx]);
+      wr = y.x; wi = y.y;
+      for (int j = 0; j < g.s; ++j) {
+        const double2 x = (j + 1 == g.s) ? to_d2(ps) : to_d2(g.pred[j + 1][idx]);
+        wr += g.a[j] * x.x;
+        wi += g.a[j] * x.y;
+      }
+    }
+    g.W[idx] = from_d2<C>(make_double2(wr, wi));
+  }
+}
 
 // ------------------------------------------------------------ K segments
 // Device mode order: active modes (ΔP or ΔM non-zero) in [0, ML), ML a multiple
