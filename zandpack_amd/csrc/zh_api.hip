@@ -127,7 +127,7 @@ template <> struct Cands<double> {
                    Tile<64, 64, 16, 2, false, 2>> psiR;
   typedef TileList<Tile<64, 64, 8, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
                    Tile<64, 64, 8, 2, false, 4>> omega;
-  typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 80, 16, 4, false, 2>,
+  typedef TileList<Tile<64, 48, 16, 4, false, 3>, Tile<64, 32, 16, 2, false, 3>,
                    Tile<64, 32, 32, 2, false, 3>> omgxi;
 };
 template <> struct Cands<float> {
@@ -135,15 +135,15 @@ template <> struct Cands<float> {
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 128, 16, 2>> psiR;
   typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 8, 2, false, 5>>
       omega;
-  typedef TileList<Tile<64, 32, 32, 2>, Tile<64, 32, 16, 2>, Tile<64, 80, 16, 4>> omgxi;
+  typedef TileList<Tile<64, 48, 16, 4>, Tile<64, 48, 32, 4>, Tile<64, 32, 16, 2>> omgxi;
 };
 // σ-path products are FP64 in both precision modes
 typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 64, 8, 2, false, 3>,
                  Tile<64, 64, 16, 2, false, 3>> CandHsig;
 typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
                  Tile<64, 64, 8, 2, false, 4>> CandHsigR;
-typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<32, 32, 16, 2, false, 4>,
-                 Tile<64, 80, 16, 4, false, 2>> CandPi;
+typedef TileList<Tile<64, 48, 16, 4, false, 3>, Tile<64, 32, 16, 2, false, 3>,
+                 Tile<32, 32, 16, 2, false, 4>> CandPi;
 
 struct TuneKey {
   int prec, nb, mb, cb, hr;
@@ -166,11 +166,13 @@ struct Engine : EngineBase {
   static constexpr int kPiParts = 8;  // split-K of Π_c = ψᵀ Ixi_c (K = modes)
   static constexpr int kWParts = 4;   // split-K of W = Ω ξ_c (K = modes)
   static constexpr int kEllMax = 64;  // most entries per row of the row-compressed H
+  static constexpr int kCGroup = 16;  // coupled-orbital column group (lead-aligned)
   int NK, n, NL, NX, NOT;
   int M;        // modes per k in the reference layout, NL*NX*NOT
   int Md = 0;   // modes per k on the device (active block padded to kTile)
   int ML = 0;   // active block [0, ML); passive modes in [ML, Md)
-  int nc = 0;   // coupled orbitals
+  int nc = 0;   // coupled-orbital columns (lead groups padded to kCGroup when NL > 1)
+  int nc_real = 0;  // coupled orbitals
   double hbar;
   int device;
   hipStream_t stream = nullptr;
@@ -201,7 +203,10 @@ struct Engine : EngineBase {
 
   // device buffers
   int *cidx = nullptr, *cpos = nullptr, *lead = nullptr;
-  int2* t1 = nullptr;
+  int2* t1 = nullptr;      // per Ω column tile: term1 K range
+  int2* t2 = nullptr;      // per Ω row tile: term2 K range (relative to nc)
+  int* gmask = nullptr;    // per 16-column group of the coupled order: lead mask
+  int4* lrange = nullptr;  // per lead: device rows in the active and passive blocks
   double2 *Hcur = nullptr, *Ixic = nullptr, *Pic = nullptr, *G = nullptr, *dvec = nullptr;
   double* Hre = nullptr;  // Re H (A operand of Hσ when H is real)
   int* ell_col = nullptr;     // row-compressed H: (NK, kEllMax, n) column indices
@@ -305,10 +310,17 @@ struct Engine : EngineBase {
             first_lead[o] = a;
         }
       }
+    // With several leads each lead's group is padded (cidx -1, zero columns) to a
+    // multiple of 16, so that 16-aligned column tiles of the ξ products hold one lead
+    // and run over that lead's modes only.
     cidx_h.clear();
-    for (int a = 0; a < NL; ++a)
+    nc_real = 0;
+    for (int a = 0; a < NL; ++a) {
       for (int o = 0; o < n; ++o)
-        if (first_lead[o] == a) cidx_h.push_back(o);
+        if (first_lead[o] == a) { cidx_h.push_back(o); ++nc_real; }
+      if (NL > 1)
+        while (cidx_h.size() % kCGroup) cidx_h.push_back(-1);
+    }
     nc = (int)cidx_h.size();
     // active modes: ΔP or ΔM non-zero for some k (Fermi slots have both zero)
     std::vector<char> active(M, 0);
@@ -363,6 +375,9 @@ struct Engine : EngineBase {
     cidx = dalloc<int>(nc);
     cpos = dalloc<int>(n);
     t1 = dalloc<int2>((Md + kTile - 1) / kTile);
+    t2 = dalloc<int2>((Md + kTile - 1) / kTile);
+    gmask = dalloc<int>((nc + kCGroup - 1) / kCGroup + 1);
+    lrange = dalloc<int4>(NL);
     Ixic = dalloc<double2>(mc);
     Pic = dalloc<double2>((size_t)NK * kPiParts * n * nc);
     xic = dalloc<cs>(mc);
@@ -402,15 +417,22 @@ struct Engine : EngineBase {
               "coupling structure changed; create a new problem");
     }
     std::vector<int> cp(n, -1);
-    for (int c = 0; c < nc; ++c) cp[cidx_h[c]] = c;
-    // per lead: range of its orbitals in the coupled order (term1 K range)
-    std::vector<int> llo(NL, nc), lhi(NL, 0);
+    for (int c = 0; c < nc; ++c)
+      if (cidx_h[c] >= 0) cp[cidx_h[c]] = c;
+    // per lead: range of its orbitals in the coupled order (term1 / term2 K range), and
+    // per coupled column the leads that couple it
+    std::vector<int> llo(NL, nc), lhi(NL, 0), cmask(nc, 0);
     for (int b = 0; b < NK; ++b)
       for (int m = 0; m < M; ++m) {
         const int a = m / (NX * NOT);
         for (int c = 0; c < nc; ++c) {
+          if (cidx_h[c] < 0) continue;
           const long off = ((long)b * M + m) * n + cidx_h[c];
-          if (xi[off] != 0.0 || Ixi[off] != 0.0) { llo[a] = std::min(llo[a], c); lhi[a] = std::max(lhi[a], c + 1); }
+          if (xi[off] != 0.0 || Ixi[off] != 0.0) {
+            llo[a] = std::min(llo[a], c);
+            lhi[a] = std::max(lhi[a], c + 1);
+            cmask[c] |= 1 << a;
+          }
         }
       }
     const int ntile = (Md + kTile - 1) / kTile;
@@ -429,6 +451,37 @@ struct Engine : EngineBase {
         hi = std::max(hi, lhi[ld[d]]);
       }
       tr[t] = make_int2(lo < hi ? lo : 0, lo < hi ? hi : 0);
+    }
+    // per row tile: term2 K range (ΔP ξ_c vanishes outside the row leads' orbitals)
+    std::vector<int2> tr2(ntile);
+    for (int t = 0; t < ntile; ++t) {
+      int lo = nc, hi = 0;
+      for (int d = t * kTile; d < std::min(Md, (t + 1) * kTile); ++d) {
+        const int m = perm[d];
+        if (m < 0) continue;
+        bool nzp = false;
+        for (int b = 0; b < NK; ++b) nzp = nzp || dgp[(long)b * M + m] != 0.0;
+        if (!nzp) continue;  // term2 vanishes on this row
+        lo = std::min(lo, llo[ld[d]]);
+        hi = std::max(hi, lhi[ld[d]]);
+      }
+      tr2[t] = make_int2(lo < hi ? lo : 0, lo < hi ? hi : 0);
+    }
+    // per 16-column group of the coupled order: mask of the leads that couple it; per
+    // lead: its device rows in the active block [x, y) and in the passive block [z, w)
+    const int ngroup = (nc + kCGroup - 1) / kCGroup;
+    std::vector<int> gm(std::max(ngroup, 1), 0);
+    for (int c = 0; c < nc; ++c) gm[c / kCGroup] |= cmask[c];
+    std::vector<int4> lr(NL);
+    for (int a = 0; a < NL; ++a) {
+      int alo = Md, ahi = 0, plo = Md, phi = 0;
+      for (int d = 0; d < Md; ++d) {
+        if (perm[d] < 0 || ld[d] != a) continue;
+        if (d < ML) { alo = std::min(alo, d); ahi = std::max(ahi, d + 1); }
+        else { plo = std::min(plo, d); phi = std::max(phi, d + 1); }
+      }
+      lr[a] = make_int4(alo < ahi ? alo : 0, alo < ahi ? ahi : 0, plo < phi ? plo : 0,
+                        plo < phi ? phi : 0);
     }
     const size_t mc = (size_t)NK * Md * nc;
     std::vector<double2> ixic(mc), g1((size_t)NK * Md), x1((size_t)NK * Md), x2((size_t)NK * Md);
@@ -450,7 +503,9 @@ struct Engine : EngineBase {
         x1[bd] = make_double2(xpp[bm].real(), xpp[bm].imag());
         x2[bd] = make_double2(xpm[bm].real(), xpm[bm].imag());
         for (int c = 0; c < nc; ++c) {
-          const cd x = xi[bm * n + cidx_h[c]], ix = Ixi[bm * n + cidx_h[c]];
+          const bool pad = cidx_h[c] < 0;
+          const cd x = pad ? cd(0.0) : xi[bm * n + cidx_h[c]];
+          const cd ix = pad ? cd(0.0) : Ixi[bm * n + cidx_h[c]];
           ixic[bd * nc + c] = make_double2(ix.real(), ix.imag());
           xc[bd * nc + c] = from_d2<cs>(make_double2(x.real(), x.imag()));
           const cd px = dgp[bm] * x, mx = dgm[bm] * ix;
@@ -475,6 +530,10 @@ struct Engine : EngineBase {
     }
     HIPCHK(hipMemcpyAsync(cpos, cp.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(t1, tr.data(), sizeof(int2) * ntile, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(t2, tr2.data(), sizeof(int2) * ntile, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(gmask, gm.data(), sizeof(int) * gm.size(), hipMemcpyHostToDevice,
+                          stream));
+    HIPCHK(hipMemcpyAsync(lrange, lr.data(), sizeof(int4) * NL, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(lead, ld.data(), sizeof(int) * Md, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(glp, g1.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(chip, x1.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
@@ -555,6 +614,13 @@ struct Engine : EngineBase {
   }
 
   Ell ell() const { return Ell{ell_col, ell_val, ell_R, n}; }
+
+  // The [ΔPξ_c][ħ σ_cᵀ] part of the Ψ right-hand side: rows from g0_rows() on have
+  // ΔP = 0 (passive block of the active-first order), so their tiles are skipped.
+  int g0_rows() const { return pruned ? ML : Md; }
+  KSegRowLead kseg_g0() const {
+    return KSegRowLead{t2, ML, pruned ? 1 : 0, (Md + kTile - 1) / kTile};
+  }
 
   void set_hubbard(double U_, const double* n0_) override {
     require(U_ == 0.0 || NK == 2, "the Hubbard model couples two spin blocks: NK must be 2");
@@ -738,7 +804,7 @@ struct Engine : EngineBase {
     layout_req = mode;
   }
   int layout_info(int* out) override {
-    out[0] = pruned ? 1 : 0; out[1] = ff ? 1 : 0; out[2] = Md; out[3] = ML; out[4] = nc;
+    out[0] = pruned ? 1 : 0; out[1] = ff ? 1 : 0; out[2] = Md; out[3] = ML; out[4] = nc_real;
     return 5;
   }
   void set_sparse(int mode) override {
@@ -853,7 +919,8 @@ struct Engine : EngineBase {
         GemmArgs<R, double> g{n, nc, Md, Ap, n + nc, (long)Md * (n + nc), Ixic, nc,
                               (long)Md * nc, kPiParts};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
-        HIPCHK((launch_sel<double, R, double, true, false, 0>(tsel[TS_PI], g, NK, e, st, KFull(),
+        KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, 0};
+        HIPCHK((launch_sel<double, R, double, true, false, 0>(tsel[TS_PI], g, NK, e, st, ks,
                                                               CandPi())));
       });
     }
@@ -905,7 +972,7 @@ struct Engine : EngineBase {
         const cs* X = (s == 0) ? os.y0 : omg[s - 1];
         GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
         EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
-        KSegOmgXi ks{ML, kTile, ff ? 1 : 0};
+        KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
         HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_OMGXI], g, NK, e, st, ks,
                                                      typename Cands<R>::omgxi())));
         WSumArgs<cs> w{};
@@ -926,11 +993,11 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
-          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, KFull(),
+          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, kseg_g0(),
                                                        typename Cands<R>::psi())));
         }
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
-                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1};
+                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
         const size_t lds = (size_t)hpsi_rb * n * sizeof(cs);
         const long sAp = (long)Md * (n + nc);
         const dim3 grid((Md + hpsi_rb - 1) / hpsi_rb, NK);
@@ -950,13 +1017,13 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
-          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, KFull(),
+          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, kseg_g0(),
                                                        typename Cands<R>::psi())));
         }
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
-                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1};
+                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
         HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psiR())));
       } else {
@@ -964,7 +1031,7 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
         EpiPsi<cs> e{psi[kslot], nullptr, Ap, n + nc, xic, Wst, glp, chip, dvec, lead, cpos, sps,
-                     Md, n, nc, ih, ih * ih, 1};
+                     Md, n, nc, ih, ih * ih, 1, Md};
         HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
       }
@@ -977,7 +1044,7 @@ struct Engine : EngineBase {
       // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), with the stage input
       // formed, the outer-subtraction term applied and (last stage) the error norm
       // accumulated in the epilogue
-      KSegOmega ks{t1, nc, ML, kTile, kTile, ff ? 1 : 0};  // every omega candidate is 64 x 64
+      KSegOmega ks{t1, t2, nc, ML, ff ? 1 : 0, (Md + kTile - 1) / kTile};
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
       g.xcd_order = 0;  // the skipped passive block would unbalance contiguous XCD bands

@@ -74,17 +74,17 @@ struct GemmArgs {
 
 constexpr int kThreads = 256;
 
-// K-range selection per output tile (tm = row tile, tn = column tile): up to two
-// half-open segments [lo0, hi0), [lo1, hi1) of the K axis (nseg of them); tiles
-// whose operands are known to vanish are skipped, and a tile with no segment gets
-// acc = 0 (its epilogue still runs).  Returned by value and built with selects, so
-// that the ranges stay in registers.
+// K-range selection per output tile (first row m0, first column n0, bm x bn): up
+// to two half-open segments [lo0, hi0), [lo1, hi1) of the K axis (nseg of them);
+// tiles whose operands are known to vanish are skipped, and a tile with no segment
+// gets acc = 0 (its epilogue still runs).  Returned by value and built with
+// selects, so that the ranges stay in registers.
 struct KRange {
   int lo0, hi0, lo1, hi1, nseg;
 };
 struct KFull {
-  __device__ inline KRange get(int, int, int K) const { return KRange{0, K, 0, 0, 1}; }
-  __device__ inline bool skip(int, int) const { return false; }
+  __device__ inline KRange get(int, int, int, int, int K) const { return KRange{0, K, 0, 0, 1}; }
+  __device__ inline bool skip(int, int, int, int) const { return false; }
 };
 
 // Reducing epilogues (static constexpr bool kReduce = true) accumulate one double
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
   }
   constexpr bool RED = HasReduce<Epi>::value;
   const int blin = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  if (ks.skip(ty, tx)) {  // uniform over the workgroup
+  if (ks.skip(ty * BM, tx * BN, BM, BN)) {  // uniform over the workgroup
     if constexpr (RED) {
       if (threadIdx.x == 0) epi.finish(blin, 0.0);
     }
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
   };
 
   // K tiles of this tile's segments, split evenly over the splitk parts
-  const KRange kr = ks.get(ty, tx, g.K);
+  const KRange kr = ks.get(m0, n0, BM, BN, g.K);
   const int lo0 = kr.lo0, hi0 = kr.hi0, lo1 = kr.lo1, hi1 = kr.hi1, nseg = kr.nseg;
   const int nt0 = nseg > 0 ? (hi0 - lo0 + BK - 1) / BK : 0;
   const int nt1 = nseg > 1 ? (hi1 - lo1 + BK - 1) / BK : 0;

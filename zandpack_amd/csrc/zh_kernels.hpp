@@ -170,7 +170,8 @@ __global__ void k_gather(GatherArgs<C> g) {
       const long bm = idx / g.nc;                   // b*M + m
       const int b = (int)(bm / g.M);
       const long m = bm - (long)b * g.M;
-      const double2 v = to_d2(g.Ap[b * g.sAp + m * g.ldAp + g.cidx[c]]);
+      const int o = g.cidx[c];  // -1: padding column of a lead group
+      const double2 v = o < 0 ? make_double2(0.0, 0.0) : to_d2(g.Ap[b * g.sAp + m * g.ldAp + o]);
       g.Aomg[bm * 2 * g.nc + c] = from_d2<C>(make_double2(h * v.x, h * v.y));
       g.Bomg[bm * 2 * g.nc + g.nc + c] = from_d2<C>(make_double2(h * v.x, -h * v.y));
     }
@@ -190,7 +191,8 @@ __global__ void k_gather(GatherArgs<C> g) {
       const long bc = idx / g.n;
       const int b = (int)(bc / g.nc);
       const int c = (int)(bc - (long)b * g.nc);
-      double2 s = g.sig[((long)b * g.n + o) * g.n + g.cidx[c]];
+      const int oc = g.cidx[c];
+      double2 s = oc < 0 ? make_double2(0.0, 0.0) : g.sig[((long)b * g.n + o) * g.n + oc];
       s.x *= g.hbar; s.y *= g.hbar;
       g.Bp[((long)b * (g.n + g.nc) + g.n + c) * g.n + o] = from_d2<C>(s);
     }
@@ -279,6 +281,7 @@ __global__ void k_currents(int NK, int M, int nc, const int* lead, const C* Ap, 
     const long m = idx / nc;
     if (lead[m] != a) continue;
     const int c = (int)(idx % nc);
+    if (cidx[c] < 0) continue;
     const double2 p = to_d2(Ap[b * sAp + m * ldAp + cidx[c]]);
     const double2 x = Ixic[((long)b * M + m) * nc + c];
     acc += p.x * x.x - p.y * x.y;
@@ -404,8 +407,9 @@ struct EpiSigma {
 
 // kΨ = -i h [ (acc - (χ⁺+Δ)ψ + d_col ψ)/ħ + (GL_P ξ + W/ħ²) on coupled columns ]
 // with acc = ψ H0ᵀ + ħ (ΔP ξ)_c σ_cᵀ  (Zandpack/mpi/zand:746-778).  With acc0 the
-// GEMM computed only part of acc; the rest is read from acc0 (same layout as out,
-// the element is read and then overwritten by the same thread).
+// GEMM computed only part of acc; the rest is read from acc0 for rows below
+// acc0_rows (same layout as out, the element is read and then overwritten by the
+// same thread; rows from acc0_rows on have ΔP = 0 and no such part).
 template <typename C>
 struct EpiPsi {
   C* out;                       // (NK, M, n)
@@ -418,9 +422,10 @@ struct EpiPsi {
   const StageParams* sp;
   int M, n, nc; double inv_hbar, inv_hbar2;
   int w_parts;  // split-K partials of W, layout (NK, w_parts, M, nc)
+  int acc0_rows;
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
     const long bm = (long)b * M + row;
-    if (acc0) {
+    if (acc0 && row < acc0_rows) {
       const double2 a0 = to_d2(acc0[bm * n + col]);
       re += a0.x;
       im += a0.y;
@@ -567,27 +572,69 @@ __global__ __launch_bounds__(256) void k_wsum(WSumArgs<C> g) {
 //   term2 (K = [nc, 2nc)) vanishes on passive rows (ΔP_i = 0);
 // the passive x passive block of Ω stays exactly zero when it starts at zero
 // (ff = 1) and is skipped.
-struct KSegOmega {
-  const int2* t1;  // per column tile: term1 K range [x, y)
-  int nc, ML, BM, BN, ff;
-  __device__ inline KRange get(int tm, int tn, int) const {
-    const int2 r = t1[tn];
-    const bool s1 = r.y > r.x, s2 = tm * BM < ML;
-    // term1 first when present, then term2; term2 alone otherwise
-    return KRange{s1 ? r.x : nc, s1 ? r.y : 2 * nc, nc, 2 * nc, (int)s1 + (int)s2};
+// Hull of the non-empty ranges of 64-wide tiles a..b of a per-tile range table.
+__device__ inline int2 tile_hull(const int2* t, int a, int b) {
+  int lo = 1 << 30, hi = 0;
+  for (int i = a; i <= b; ++i) {
+    const int2 q = t[i];
+    if (q.y > q.x) { lo = min(lo, q.x); hi = max(hi, q.y); }
   }
-  __device__ inline bool skip(int tm, int tn) const {
-    return ff && tm * BM >= ML && tn * BN >= ML;
+  return hi > lo ? make_int2(lo, hi) : make_int2(0, 0);
+}
+
+struct KSegOmega {
+  const int2* t1;  // per 64 columns: term1 K range [x, y)
+  const int2* t2;  // per 64 rows: term2 K range [nc + x, nc + y) (the row leads' orbitals)
+  int nc, ML, ff, nt;  // nt: 64-wide tiles in the table
+  __device__ inline KRange get(int m0, int n0, int bm, int bn, int) const {
+    const int2 r = tile_hull(t1, n0 / 64, min(nt - 1, (n0 + bn - 1) / 64));
+    const int2 q = tile_hull(t2, m0 / 64, min(nt - 1, (m0 + bm - 1) / 64));
+    const bool s1 = r.y > r.x, s2 = q.y > q.x;
+    // term1 first when present, then term2; term2 alone otherwise
+    return KRange{s1 ? r.x : nc + q.x, s1 ? r.y : nc + q.y, nc + q.x, nc + q.y,
+                  (int)s1 + (int)s2};
+  }
+  __device__ inline bool skip(int m0, int n0, int, int) const {
+    return ff && m0 >= ML && n0 >= ML;
   }
 };
 
-// W = Ω ξ_c: passive rows see only active columns when the passive block is zero.
-struct KSegOmgXi {
-  int ML, BM, ff;
-  __device__ inline KRange get(int tm, int, int K) const {
-    return KRange{0, (ff && tm * BM >= ML) ? ML : K, 0, 0, 1};
+// Products over the modes against a coupled-orbital operand (X ξ_c, ψᵀ Ixi_c): the
+// operand vanishes unless the mode's lead couples the column, so a column tile
+// whose 16-column groups all belong to one lead runs over that lead's modes only
+// (its rows in the active block, then in the passive block).  With ff the passive
+// rows of X (Ω-shaped) see only the active block.
+struct KSegLead {
+  const int* gmask;    // per 16-column group: mask of the coupling leads
+  const int4* lrange;  // per lead: active rows [x, y), passive rows [z, w)
+  int ngroup, ML, ff;
+  __device__ inline KRange get(int m0, int n0, int, int bn, int K) const {
+    const int g0 = n0 / 16, g1 = min(ngroup, (n0 + bn + 15) / 16);
+    int mask = 0;
+    for (int g = g0; g < g1; ++g) mask |= gmask[g];
+    const bool act_only = ff && m0 >= ML;
+    if (mask == 0) return KRange{0, 0, 0, 0, 0};
+    if (mask & (mask - 1)) return KRange{0, act_only ? ML : K, 0, 0, 1};
+    const int4 r = lrange[__ffs(mask) - 1];
+    const bool sa = r.y > r.x, sp = !act_only && r.w > r.z;
+    return KRange{sa ? r.x : r.z, sa ? r.y : r.w, r.z, r.w, (int)sa + (int)sp};
   }
-  __device__ inline bool skip(int, int) const { return false; }
+  __device__ inline bool skip(int, int, int, int) const { return false; }
+};
+
+// [ΔP ξ_c][ħ σ_cᵀ] (K = coupled orbitals): a row tile's K range is its leads'
+// orbitals; tiles past ML (passive rows, ΔP = 0) are skipped when skip_passive
+// (their consumer then reads no partial product for those rows).
+struct KSegRowLead {
+  const int2* t2;  // per 64 rows: K range
+  int ML, skip_passive, nt;
+  __device__ inline KRange get(int m0, int, int bm, int, int) const {
+    const int2 q = tile_hull(t2, m0 / 64, min(nt - 1, (m0 + bm - 1) / 64));
+    return q.y > q.x ? KRange{q.x, q.y, 0, 0, 1} : KRange{0, 0, 0, 0, 0};
+  }
+  __device__ inline bool skip(int m0, int, int, int) const {
+    return skip_passive && m0 >= ML;
+  }
 };
 
 // ------------------------------------------------------------ sparse H
