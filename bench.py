@@ -27,16 +27,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "RK45 steps/s @n=1000,M=4000: FP64 vs mixed at ≤1e-6 J err; p50/p99 latency"
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X dense FP64 / FP32 matrix
+HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
 
 
 def algorithmic_flops(p):
     """Real flops per launch of each GEMM class for the work the path performs.
     Dense complex count per k: n³ (Hσ) + M n² + M n n_c (ψHᵀ, ξσᵀ) + Σ_α n n_α M_α
     (Π) + M² n_c (Ωξ) + Σ_α M M_α n_α + M² n_c (Ω sources), 8 flops per complex MAC
-    and 4 per complex x real MAC.  A real Hamiltonian halves ψHᵀ and Hσ, a sparse
-    one reduces them to its non-zeros; Fermi-block pruning keeps only the
-    non-vanishing Ω source terms and skips the passive x passive block of Ωξ (the
-    terms the kernels execute)."""
+    and 4 per complex x real MAC.  What the kernels execute is less: a real
+    Hamiltonian halves ψHᵀ and Hσ and a sparse one reduces them to its non-zeros;
+    the coupling vectors of a mode live on its own lead's orbitals (n_α of them), so
+    ξσᵀ, Ωξ, Π and both Ω sources contract over those only; Fermi-block pruning drops
+    the Ω sources on passive rows/columns and the passive x passive block of Ωξ."""
     NK, n, NL, NX, NOT = p.dims
     M = p.M
     Ma = NX * NOT
@@ -51,17 +53,39 @@ def algorithmic_flops(p):
     sparse = nnz_row.max() <= 64 and 8 * nnz_row.max() <= n
     nnz = float(nnz_row.sum(axis=1).max())
     act = ((p.dGP != 0) | (p.dGM != 0)).any(axis=0).reshape(NL, Ma)  # per lead, per mode
-    colM = (p.dGM != 0).any(axis=0).reshape(NL, Ma)                  # term1 columns
-    rowP = int(((p.dGP != 0).any(axis=0)).sum())                    # term2 rows
+    colM = (p.dGM != 0).any(axis=0).reshape(NL, Ma).sum(axis=1)     # term1 columns per lead
+    rowP = (p.dGP != 0).any(axis=0).reshape(NL, Ma).sum(axis=1)     # term2 rows per lead
     ML = int(act.sum())
+    act_l = act.sum(axis=1)
     c = {
         "gemm_hsig": hmac * (nnz * n if sparse else n ** 3),
-        "gemm_psi": hmac * M * (nnz if sparse else n * n) + 8.0 * M * n * nc,
-        "gemm_pi": 8.0 * sum(n * a * Ma for a in na),
-        "gemm_omgxi": 8.0 * (M * M - (M - ML) ** 2) * nc,
-        "gemm_omega": 8.0 * (sum(M * int(colM[a].sum()) * na[a] for a in range(NL)) + rowP * M * nc),
+        "gemm_psi": hmac * M * (nnz if sparse else n * n)
+        + 8.0 * sum(int(rowP[a]) * n * na[a] for a in range(NL)),
+        "gemm_pi": 8.0 * sum(n * na[a] * Ma for a in range(NL)),
+        "gemm_omgxi": 8.0 * sum(na[a] * (ML * Ma + (M - ML) * int(act_l[a])) for a in range(NL)),
+        "gemm_omega": 8.0 * sum(M * int(colM[a]) * na[a] + int(rowP[a]) * M * na[a]
+                                for a in range(NL)),
     }
     return {k: NK * v for k, v in c.items()}, na, nc
+
+
+def algorithmic_bytes(p, precision):
+    """HBM bytes per launch of the Ω kernel (average over the seven stages): stage s
+    reads y0 and the s-1 earlier stage derivatives with a non-zero coefficient and
+    writes k_s (the last stage writes the new state instead), over the Ω elements
+    outside the passive x passive block; plus its two coupled-orbital operands."""
+    NK, n, NL, NX, NOT = p.dims
+    M = p.M
+    Ma = NX * NOT
+    act = ((p.dGP != 0) | (p.dGM != 0)).any(axis=0)
+    ML = int(act.sum())
+    nc = int(np.count_nonzero(np.any(p.xi.reshape(-1, n) != 0, axis=0)))
+    elem = 16 if precision == "fp64" else 8
+    slot = NK * (M * M - (M - ML) ** 2) * elem
+    reads = [1, 2, 3, 4, 5, 6, 6]   # y0 + k_j with a_sj != 0 (a_72 = 0)
+    writes = [1] * 7
+    per = slot * (sum(reads) + sum(writes)) / 7.0
+    return {"gemm_omega": per + 2 * NK * M * 2 * nc * elem}
 
 
 def measured_traffic(kernel, precision):
@@ -264,12 +288,22 @@ def main():
         tm = profile_leg(p, prec, 3, args.h0)
         gemms = {k: v for k, v in tm.items() if k in flops}
         dom = max(gemms, key=lambda k: gemms[k])
-        # kernel-class time is summed over NK-batched launches of one stage
-        ach = flops[dom] / (gemms[dom] * 1e-3) / 1e12
-        peak = PEAK[prec] if dom not in ("gemm_hsig", "gemm_pi") else PEAK["fp64"]
-        roof[prec] = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": peak,
-                      "unit": "TFLOP/s", "frac": ach / peak,
+        # kernel-class time is summed over NK-batched launches of one stage; the bound
+        # is the roof (matrix-core flops or HBM bytes) the kernel's work takes longest on
+        sec = gemms[dom] * 1e-3
+        peak_f = PEAK[prec] if dom not in ("gemm_hsig", "gemm_pi") else PEAK["fp64"]
+        nbytes = algorithmic_bytes(p, prec).get(dom)
+        t_f = flops[dom] / (peak_f * 1e12)
+        t_b = nbytes / (HBM_PEAK * 1e9) if nbytes else 0.0
+        if t_b > t_f:
+            ach, peak, unit, bound = nbytes / sec / 1e9, HBM_PEAK, "GB/s", "hbm"
+        else:
+            ach, peak, unit, bound = flops[dom] / sec / 1e12, peak_f, "TFLOP/s", "mfma"
+        roof[prec] = {"bound": bound, "kernel": dom, "achieved": ach, "peak": peak,
+                      "unit": unit, "frac": ach / peak,
                       "traffic": measured_traffic(dom, prec),
+                      "algorithmic_bytes": nbytes, "algorithmic_flops": flops[dom],
+                      "mfma_tflops": flops[dom] / sec / 1e12,
                       "avg_ms": gemms[dom], "kernel_ms": tm}
 
     value = legs["mixed"]["steps_per_s"]

@@ -125,16 +125,15 @@ template <> struct Cands<double> {
                    Tile<64, 32, 16, 2, false, 4>> psi;
   typedef TileList<Tile<64, 64, 8, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
                    Tile<64, 64, 16, 2, false, 2>> psiR;
-  typedef TileList<Tile<64, 64, 8, 2, false, 3>, Tile<64, 64, 16, 2, false, 3>,
-                   Tile<64, 64, 8, 2, false, 4>> omega;
+  typedef TileList<Tile<64, 64, 8, 2, false, 3>, Tile<64, 32, 8, 2, false, 4>,
+                   Tile<32, 32, 8, 2, false, 4>> omega;
   typedef TileList<Tile<64, 48, 16, 4, false, 3>, Tile<64, 32, 16, 2, false, 3>,
                    Tile<64, 32, 32, 2, false, 3>> omgxi;
 };
 template <> struct Cands<float> {
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 64, 16, 2>> psi;
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 128, 16, 2>> psiR;
-  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 64, 16, 2>, Tile<64, 64, 8, 2, false, 5>>
-      omega;
+  typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 32, 8, 2>, Tile<32, 32, 8, 2>> omega;
   typedef TileList<Tile<64, 48, 16, 4>, Tile<64, 48, 32, 4>, Tile<64, 32, 16, 2>> omgxi;
 };
 // σ-path products are FP64 in both precision modes
@@ -220,7 +219,12 @@ struct Engine : EngineBase {
   cs *Bp = nullptr, *Ap = nullptr, *Aomg = nullptr, *Bomg = nullptr, *xic = nullptr, *W = nullptr;
   cs* Pred[7] = {nullptr};  // y0 ξ_c and k_j ξ_c of the current attempt (Ω_s ξ_c by linearity)
   cs* Wst = nullptr;        // Ω_s ξ_c of the current stage
-  int te_omega_blocks = 0;  // error-norm partials written by the last stage's Ω kernel
+  // error-norm partials written by the last stage's Ω kernel: one per workgroup
+  int te_omega_blocks() const {
+    int bm = 0, bn = 0;
+    tile_dims(tsel[TS_OMEGA], typename Cands<R>::omega(), bm, bn);
+    return ((Md + bm - 1) / bm) * ((Md + bn - 1) / bn) * NK;
+  }
   double2* sig[10] = {nullptr};
   cs* psi[10] = {nullptr};
   cs* omg[10] = {nullptr};
@@ -384,9 +388,9 @@ struct Engine : EngineBase {
     W = dalloc<cs>(mc * kWParts);
     for (auto& p : Pred) p = dalloc<cs>(mc);
     Wst = dalloc<cs>(mc);
-    const int tiles = (Md + kTile - 1) / kTile;  // every Ω tile candidate is kTile x kTile
-    te_omega_blocks = tiles * tiles * NK;
-    te_part = dalloc<double>((size_t)2 * kTEBlocks + te_omega_blocks);
+    // room for the error-norm partials of the finest Ω tile candidate (32 x 32)
+    const long t32 = (Md + 31) / 32;
+    te_part = dalloc<double>((size_t)2 * kTEBlocks + t32 * t32 * NK);
     Ap = dalloc<cs>((size_t)NK * Md * (n + nc));
     Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
     Aomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
@@ -1096,7 +1100,7 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
     timed(KC_CTRL, [&] {
       hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, stream, (const double*)te_part,
-                         2 * kTEBlocks + te_omega_blocks, ctrl);
+                         2 * kTEBlocks + te_omega_blocks(), ctrl);
     });
     HIPCHK(hipGetLastError());
   }

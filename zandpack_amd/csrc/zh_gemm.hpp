@@ -341,15 +341,13 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
         }
   }
   if constexpr (RED) {
-    __shared__ double sred[kThreads];
-    sred[tid] = red;
-    __syncthreads();
+    // fixed-order reduction: butterfly within each wave, then the four wave sums
+    __shared__ double sred[kThreads / 64];
 #pragma unroll
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-      if (tid < s) sred[tid] += sred[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) epi.finish(blin, sred[0]);
+    for (int off = 32; off > 0; off >>= 1) red += __shfl_xor(red, off, 64);
+    if (lane == 0) sred[wave] = red;
+    __syncthreads();
+    if (tid == 0) epi.finish(blin, (sred[0] + sred[1]) + (sred[2] + sred[3]));
   }
 }
 
@@ -377,6 +375,15 @@ inline hipError_t launch_zgemm(const GemmArgs<RA, RB>& g, int batch, const Epi& 
 // Candidate tile configurations of one GEMM; the engine's autotuner picks one by
 // index at run time (launch_sel).
 template <typename... TLs> struct TileList {};
+
+// Output tile (BM, BN) of candidate idx of a list.
+template <typename TL0>
+inline void tile_dims(int, TileList<TL0>, int& bm, int& bn) { bm = TL0::BM; bn = TL0::BN; }
+template <typename TL0, typename TL1, typename... TLs>
+inline void tile_dims(int idx, TileList<TL0, TL1, TLs...>, int& bm, int& bn) {
+  if (idx <= 0) { bm = TL0::BM; bn = TL0::BN; return; }
+  tile_dims(idx - 1, TileList<TL1, TLs...>(), bm, bn);
+}
 
 template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B, int RMODE,
           typename Epi, typename KSeg, typename TL0>
