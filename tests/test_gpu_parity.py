@@ -193,7 +193,7 @@ def test_sparse_dense_propagation_agree(native):
         prop.close()
     (a, sa), (b, sb) = outs
     assert a["n_steps"] == b["n_steps"] and a["n_rejected"] == b["n_rejected"]
-    np.testing.assert_allclose(a["times"], b["times"], rtol=1e-12)
+    np.testing.assert_allclose(a["times"], b["times"], rtol=1e-10)
     assert rel(a["J"], b["J"]) < 1e-10
     for x, y in zip(sa, sb):
         assert rel(x, y) < 1e-10

@@ -549,7 +549,7 @@ struct Engine : EngineBase {
     have_consts = true;
   }
 
-  // Mode rows per workgroup of k_hpsi_ell: those rows live in LDS (at most 32 KiB,
+  // Mode rows per workgroup of k_psi_ell: those rows live in LDS (at most 32 KiB,
   // or one row of up to 64 KiB); 0 when a row does not fit.
   int hpsi_rows() const {
     const size_t row = (size_t)n * sizeof(cs);
@@ -1006,14 +1006,11 @@ struct Engine : EngineBase {
         const long sAp = (long)Md * (n + nc);
         const dim3 grid((Md + hpsi_rb - 1) / hpsi_rb, NK);
         if (hpsi_rb == 4)
-          hipLaunchKernelGGL((k_hpsi_ell<cs, 4, EpiPsi<cs>>), grid, dim3(256), lds, st, Md,
-                             (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
+          hipLaunchKernelGGL((k_psi_ell<cs, 4>), grid, dim3(256), lds, st, e, sAp, ell());
         else if (hpsi_rb == 2)
-          hipLaunchKernelGGL((k_hpsi_ell<cs, 2, EpiPsi<cs>>), grid, dim3(256), lds, st, Md,
-                             (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
+          hipLaunchKernelGGL((k_psi_ell<cs, 2>), grid, dim3(256), lds, st, e, sAp, ell());
         else
-          hipLaunchKernelGGL((k_hpsi_ell<cs, 1, EpiPsi<cs>>), grid, dim3(256), lds, st, Md,
-                             (const cs*)Ap, (long)(n + nc), sAp, ell(), e);
+          hipLaunchKernelGGL((k_psi_ell<cs, 1>), grid, dim3(256), lds, st, e, sAp, ell());
         HIPCHK(hipGetLastError());
       } else if (h_real) {
         // [ΔPξ_c] [ħ σ_cᵀ] into the k slot, then ψ Re(H)ᵀ with the epilogue adding it
@@ -1099,7 +1096,7 @@ struct Engine : EngineBase {
     });
     HIPCHK(hipGetLastError());
     timed(KC_CTRL, [&] {
-      hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, stream, (const double*)te_part,
+      hipLaunchKernelGGL(k_controller, dim3(1), dim3(kCtrlThreads), 0, stream, (const double*)te_part,
                          2 * kTEBlocks + te_omega_blocks(), ctrl);
     });
     HIPCHK(hipGetLastError());

@@ -347,13 +347,15 @@ __global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial
 
 // TE = sqrt(sum), accept iff TE < eps, h_next = 0.9 h (eps/TE)^0.2
 // (Zandpack/mpi/zand:396-407, Zandpack/mpi_RK4pars_dev.py:148-149).
-__global__ void k_controller(const double* partial, int np, Ctrl* ctrl) {
-  __shared__ double red[256];
+constexpr int kCtrlThreads = 1024;
+__global__ __launch_bounds__(kCtrlThreads) void k_controller(const double* partial, int np,
+                                                             Ctrl* ctrl) {
+  __shared__ double red[kCtrlThreads];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
   red[threadIdx.x] = acc;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kCtrlThreads / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
@@ -650,41 +652,108 @@ struct Ell {
   int R, n;
 };
 
-// acc[i, o] = (ψ Hᵀ)[i, o] = Σ_r ψ[i, col_r(o)] H_r(o), for RB mode rows per
-// workgroup with those rows staged in LDS; acc goes to epi (the Ψ right-hand side).
-template <typename C, int RB, typename Epi>
-__global__ __launch_bounds__(256) void k_hpsi_ell(int Md, const C* Ap, long ldAp, long sAp,
-                                                  Ell e, Epi epi) {
+// The whole Ψ right-hand side (EpiPsi's formula) with the row-compressed H, for RB
+// mode rows per workgroup held in LDS: per-row constants are read once per
+// workgroup, and for its two columns a thread loads every operand (the partial
+// product acc0, which shares the output buffer, ξ_c and W on coupled columns)
+// before its first store, so the loads of a column pair are in flight together.
+template <typename C, int RB>
+__global__ __launch_bounds__(256) void k_psi_ell(EpiPsi<C> e, long sAp, Ell h) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C* rows = reinterpret_cast<C*>(lds_raw);  // RB x n
+  __shared__ double2 rchi[RB], rglp[RB];
   const int b = blockIdx.y;
   const int i0 = blockIdx.x * RB;
-  const int n = e.n;
+  const int n = h.n, M = e.M;
   for (int q = 0; q < RB; ++q) {
     const int i = i0 + q;
     for (int o = threadIdx.x; o < n; o += blockDim.x)
-      rows[q * n + o] = (i < Md) ? Ap[b * sAp + (long)i * ldAp + o] : from_d2<C>(make_double2(0, 0));
+      rows[q * n + o] =
+          (i < M) ? e.Ap[b * sAp + (long)i * e.ldAp + o] : from_d2<C>(make_double2(0, 0));
+  }
+  if (threadIdx.x < RB) {
+    const int i = i0 + threadIdx.x;
+    double2 chi = make_double2(0.0, 0.0), g = make_double2(0.0, 0.0);
+    if (i < M) {
+      const long bm = (long)b * M + i;
+      chi = e.chip[bm];
+      chi.x += e.sp->delta[e.lead[i]];
+      g = e.glp[bm];
+    }
+    rchi[threadIdx.x] = chi;
+    rglp[threadIdx.x] = g;
   }
   __syncthreads();
-  const int* col = e.col + (long)b * e.R * n;
-  const double2* val = e.val + (long)b * e.R * n;
-  for (int o = threadIdx.x; o < n; o += blockDim.x) {
-    double ar[RB], ai[RB];
+  const double hs = e.sp->h;
+  const int* col = h.col + (long)b * h.R * n;
+  const double2* val = h.val + (long)b * h.R * n;
+  constexpr int CPT = 2;  // columns per thread per pass
+  for (int ob = threadIdx.x; ob < n; ob += CPT * blockDim.x) {
+    int oc[CPT], cp[CPT];
+    double2 dv[CPT];
+    C a0[CPT][RB], xv[CPT][RB], wv[CPT][RB];
 #pragma unroll
-    for (int q = 0; q < RB; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
-    for (int r = 0; r < e.R; ++r) {
-      const int c = col[(long)r * n + o];
-      const double2 v = val[(long)r * n + o];
+    for (int u = 0; u < CPT; ++u) {
+      const int o = ob + u * blockDim.x;
+      oc[u] = o;
+      const bool ok = o < n;
+      dv[u] = ok ? e.dvec[(long)b * n + o] : make_double2(0.0, 0.0);
+      cp[u] = ok ? e.cpos[o] : -1;
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
-        const double2 p = to_d2(rows[q * n + c]);
-        ar[q] += p.x * v.x - p.y * v.y;
-        ai[q] += p.x * v.y + p.y * v.x;
+        const int i = i0 + q;
+        const long bm = (long)b * M + i;
+        const bool rok = ok && i < M;
+        a0[u][q] = (rok && e.acc0 && i < e.acc0_rows) ? e.acc0[bm * n + o]
+                                                      : from_d2<C>(make_double2(0, 0));
+        const bool cc = rok && cp[u] >= 0;
+        xv[u][q] = cc ? e.xic[bm * e.nc + cp[u]] : from_d2<C>(make_double2(0, 0));
+        double wr = 0.0, wi = 0.0;
+        if (cc)
+          for (int s = 0; s < e.w_parts; ++s) {
+            const double2 w = to_d2(e.W[(((long)b * e.w_parts + s) * M + i) * e.nc + cp[u]]);
+            wr += w.x;
+            wi += w.y;
+          }
+        wv[u][q] = from_d2<C>(make_double2(wr, wi));
       }
     }
 #pragma unroll
-    for (int q = 0; q < RB; ++q)
-      if (i0 + q < Md) epi(b, i0 + q, o, ar[q], ai[q]);
+    for (int u = 0; u < CPT; ++u) {
+      const int o = oc[u];
+      if (o >= n) continue;
+      double ar[RB], ai[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
+      for (int r = 0; r < h.R; ++r) {
+        const int c = col[(long)r * n + o];
+        const double2 v = val[(long)r * n + o];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+          const double2 p = to_d2(rows[q * n + c]);
+          ar[q] += p.x * v.x - p.y * v.y;
+          ai[q] += p.x * v.y + p.y * v.x;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const int i = i0 + q;
+        if (i >= M) continue;
+        const double2 a = to_d2(a0[u][q]);
+        const double re = ar[q] + a.x, im = ai[q] + a.y;
+        const double2 p = to_d2(rows[q * n + o]);
+        const double2 cpx = cmul(rchi[q], p), dp = cmul(dv[u], p);
+        double vr = (re - cpx.x + dp.x) * e.inv_hbar;
+        double vi = (im - cpx.y + dp.y) * e.inv_hbar;
+        if (cp[u] >= 0) {
+          const double2 gx = cmul(rglp[q], to_d2(xv[u][q]));
+          const double2 w = to_d2(wv[u][q]);
+          vr += gx.x + w.x * e.inv_hbar2;
+          vi += gx.y + w.y * e.inv_hbar2;
+        }
+        e.out[((long)b * M + i) * n + o] = from_d2<C>(make_double2(hs * vi, -hs * vr));
+      }
+    }
   }
 }
 
