@@ -27,6 +27,8 @@ enum { ZH_OK = 0, ZH_EINVAL = -1, ZH_EHIP = -2, ZH_ENOMEM = -3, ZH_ESTATE = -4, 
 /* bias waveform kinds (Zandpack/Pulses.py:52-94 shapes) */
 enum { ZH_PULSE_NONE = 0, ZH_PULSE_STEP = 1, ZH_PULSE_GAUSS = 2, ZH_PULSE_RAMP = 3, ZH_PULSE_SIN = 4,
        ZH_PULSE_BOX = 5 };
+/* dissipator rate kinds */
+enum { ZH_DISS_NONE = 0, ZH_DISS_CONST = 1, ZH_DISS_STEP = 2 };
 /* time bookkeeping after an accepted step */
 enum { ZH_TIME_REFERENCE = 0, /* t += h_next, as Zandpack/mpi/zand:406-407,427 */
        ZH_TIME_CORRECT = 1 }; /* t += h of the step just taken */
@@ -97,6 +99,17 @@ int zh_set_hubbard(zh_problem* p, double U, const double* n0);
  *   STEP amp for t >= p1; GAUSS amp exp(-((t-p1)/p2)^2); RAMP amp (1 - exp(-t/p1));
  *   SIN amp sin(p1 t / hbar); BOX amp (tanh(t/p2) - tanh((t-p1)/p2))/2. */
 int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, const double* weights);
+
+/* Device-evaluated dissipator of the sigma equation, dissipator(t, sigma) =
+ * eta(t) (sigma - sigma_eq), subtracted as D -= hbar * dissipator(t, sigma)
+ * (Zandpack/mpi/zand:343, the Bias.dissipator hook; the form of
+ * examples/Hubbard/Calculation/Bias.py:29-39).  kind: ZH_DISS_NONE (default),
+ * ZH_DISS_CONST eta = eta0, ZH_DISS_STEP eta = eta0 / (1 + exp((t - t0) / width)).
+ * sig_eq (NK, n, n) complex128 is required unless kind is ZH_DISS_NONE.  It enters
+ * the attempts (zh_attempt*, zh_propagate); zh_rhs evaluates the in-process closure
+ * f of the reference, which has no dissipator. */
+int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double width,
+                      const double* sig_eq);
 
 /* Resident state y0 = (sigma, Psi, Omega), reference layout, complex128. */
 int zh_state_put(zh_problem* p, const double* sig, const double* psi, const double* omg);

@@ -168,10 +168,13 @@ def coupled_orbitals(consts):
     return np.where(nz.reshape(-1, n).any(axis=0))[0]
 
 
-def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, prune=False):
+def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, prune=False,
+        diss=None):
     """One stage of zand: returns (h*dσ/dt, h*dΨ/dt, h*dΩ/dt).
 
     σ: mpi/zand:340-387; Ψ: mpi/zand:746-778; Ω: mpi/zand:784-809.
+    ``diss`` is the value of Bias.dissipator(ts, σ) at the stage, subtracted as
+    ``D -= ħ·dissipator`` (mpi/zand:343); None means 0.
     ``Ht`` is the full Hamiltonian at the stage (H0 + dH(ts, σ)); ``delta`` is
     Δ_α(ts) per lead.  ``prune=False`` is dense in the orbital index;
     ``prune=True`` restricts every ξ/Ixi contraction to the coupled orbitals as
@@ -196,6 +199,8 @@ def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, pru
     # σ: D = Hσ; D -= D†; D *= -i  (mpi/zand:340-342)
     D = Ht @ sig
     D = -1j * (D - D.conj().transpose(0, 2, 1))
+    if diss is not None:
+        D = D - hbar * diss
     # Q_α = (Ψ_αᵀ Ixi_α + h.c.)/ħ  (mpi/zand:91-110)
     Q = np.zeros((NK, NL, n, n), complex)
     for a in range(NL):
@@ -326,7 +331,15 @@ def zero_dH(t, sig):
 # ---------------------------------------------------------------------------
 # DOP54 attempt / propagation with zand's control semantics
 # ---------------------------------------------------------------------------
-def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None):
+def relaxation_dissipator(eta, sig_eq):
+    """dissipator(t, σ) = η(t)(σ - σ_eq), the form of
+    examples/Hubbard/Calculation/Bias.py:29-39 (``eta`` a callable of t)."""
+    def diss(t, sig):
+        return eta(t) * (sig - sig_eq)
+    return diss
+
+
+def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, dissipator=None):
     """One 7-stage attempt (mpi/zand:301-396).  Returns (ks, TE, J_stage0, y_new).
 
     ``y0`` = (σ, Ψ, Ω); ``ks[i]`` are the h-scaled stage derivatives.  J is taken
@@ -347,7 +360,8 @@ def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None):
             yi.append(v)
         Ht = H0 + dH(ts, yi[0])
         delta = np.array([bias(ts, a) for a in range(NL)], float)
-        k = rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar)
+        dv = None if dissipator is None else dissipator(ts, yi[0])
+        k = rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar, diss=dv)
         if i == 0:
             J0 = currents(consts, yi[1], hbar=hbar)
         ks.append(k)
@@ -362,7 +376,7 @@ def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None):
 
 
 def propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
-              time_update="reference", max_attempts=100000, fixed_h=False):
+              time_update="reference", max_attempts=100000, fixed_h=False, dissipator=None):
     """zand's outer loop (mpi/zand:297-431).
 
     Accept iff TE < eps; h <- 0.9 h (eps/TE)^0.2 after every attempt; on accept
@@ -384,7 +398,8 @@ def propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
             attempts += 1
             if attempts > max_attempts:
                 raise RuntimeError("attempt limit reached")
-            ks, TE, J0, y_new = dop54_attempt(consts, y, t, h, H0, dH, bias, NL, hbar, tab)
+            ks, TE, J0, y_new = dop54_attempt(consts, y, t, h, H0, dH, bias, NL, hbar, tab,
+                                              dissipator=dissipator)
             if not np.isfinite(TE):
                 raise FloatingPointError("non-finite error estimate")
             accept = fixed_h or (TE < eps)

@@ -199,6 +199,35 @@ def test_sparse_dense_propagation_agree(native):
         assert rel(x, y) < 1e-10
 
 
+@pytest.mark.parametrize("case", ["small", "small_spin_hubbard"])
+@pytest.mark.parametrize("kind", ["const", "step"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_attempt_dissipator(native, case, kind, precision):
+    """One attempt with dissipator(t, σ) = η(t)(σ - σ_eq) (mpi/zand:343)."""
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    rng = np.random.default_rng(21)
+    NK, n = p.dims[0], p.dims[1]
+    A = rng.normal(size=(NK, n, n)) + 1j * rng.normal(size=(NK, n, n))
+    p.dissipator = builder.Dissipator(kind, eta0=2.5, t0=0.31, width=0.02,
+                                      sig_eq=p.sigma0 + 0.1 * (A + A.conj().transpose(0, 2, 1)))
+    y0 = _rand_state(p, 14, scale=0.02)
+    say("attempt dissipator", case, kind, precision)
+    prop = Propagator(p, precision, state=y0)
+    t0, h, eps = 0.3, 0.02, 1e-3
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2,
+                                        dissipator=p.dissipator)
+    r = prop.attempt(t0, h, eps, force_accept=True)
+    tol = 1e-10 if precision == "fp64" else 1e-4
+    assert abs(r["te"] - TE) <= tol * TE
+    s = prop.sigma()
+    assert rel(s, y_new[0]) < (1e-12 if precision == "fp64" else 1e-6)
+    # without the dissipator the step differs by far more than that
+    ks0, TE0, _, y_nd = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2)
+    assert rel(y_nd[0], y_new[0]) > 1e-4
+    prop.close()
+
+
 def test_attempt_reject_keeps_state(native):
     from zandpack_amd.engine import Propagator
     p = CASES["small"]()

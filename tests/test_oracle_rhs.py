@@ -94,6 +94,36 @@ def test_von_neumann_closed_form():
     assert out["n_reject"] >= 0 and len(out["times"]) > 5
 
 
+@pytest.mark.parametrize("kind", ["const", "step"])
+def test_dissipator_relaxation_closed_form(kind):
+    """D -= ħ·dissipator with dissipator = η(t)(σ - σ_eq) (mpi/zand:343) on an
+    isolated device with [H, σ_eq] = 0: σ(t) = σ_eq + U(σ0 - σ_eq)U† exp(-∫η)."""
+    p = builder.config_small(seed=12)
+    p.xi[:] = 0
+    p.Ixi[:] = 0
+    rng = np.random.default_rng(5)
+    sig0, psi, omg = _rand_state(p, rng)
+    psi[:] = 0
+    omg[:] = 0
+    e, v = np.linalg.eigh(p.H0[0])
+    sig_eq = ((v * (e < 0)) @ v.conj().T)[None]          # a function of H: commutes with it
+    d = builder.Dissipator(kind, eta0=0.7, t0=0.4, width=0.15, sig_eq=sig_eq)
+    T = 0.9
+    out = O.propagate(p.consts(), (sig0, psi, omg), 0.0, T, 0.01, 1e-11, p.H0,
+                      O.zero_dH, lambda t, a: 0.0, 2, time_update="correct", dissipator=d)
+    t_end = out["times"][-1]
+    if kind == "const":
+        decay = np.exp(-0.7 * t_end)
+    else:  # ∫ η0 / (1 + e^{(t - t0)/w}) dt = η0 (t - w ln(1 + e^{(t - t0)/w})) + const
+        F = lambda t: 0.7 * (t - 0.15 * np.log1p(np.exp((t - 0.4) / 0.15)))
+        decay = np.exp(-(F(t_end) - F(0.0)))
+    U = (v * np.exp(-1j * e * t_end / O.HBAR)) @ v.conj().T
+    exact = sig_eq[0] + U @ (sig0[0] - sig_eq[0]) @ U.conj().T * decay
+    assert np.abs(out["sigma"][-1][0] - exact).max() < 1e-8
+    # and the rate really entered: without it the answer is off by the decay
+    assert np.abs(out["sigma"][-1][0] - exact).max() < 1e-3 * np.abs(1 - decay)
+
+
 def test_time_update_quirk():
     """Reference bookkeeping advances t by the *new* h (mpi/zand:406-407,427)."""
     p = builder.config_small(seed=12)

@@ -215,6 +215,35 @@ class Pulse:
 
 
 @dataclass
+class Dissipator:
+    """dissipator(t, σ) = η(t)(σ - σ_eq), subtracted in the σ equation as
+    D -= ħ·dissipator (Zandpack/mpi/zand:343; the form of
+    examples/Hubbard/Calculation/Bias.py:29-39).  kinds: 'none', 'const'
+    (η = eta0), 'step' (η = eta0 / (1 + exp((t - t0)/width)))."""
+    kind: str = "none"
+    eta0: float = 0.0
+    t0: float = 0.0
+    width: float = 1.0
+    sig_eq: np.ndarray | None = None  # (NK, n, n)
+
+    KINDS = {"none": 0, "const": 1, "step": 2}
+
+    def eta(self, t):
+        if self.kind == "none":
+            return 0.0
+        if self.kind == "const":
+            return self.eta0
+        if self.kind == "step":
+            return self.eta0 / (1.0 + np.exp((t - self.t0) / self.width))
+        raise ValueError(self.kind)
+
+    def __call__(self, t, sig):
+        if self.kind == "none":
+            return np.zeros_like(sig)
+        return self.eta(t) * (sig - self.sig_eq)
+
+
+@dataclass
 class AMEProblem:
     """Everything one propagation needs, in the reference's array layout."""
     H0: np.ndarray          # (NK, n, n)
@@ -230,6 +259,7 @@ class AMEProblem:
     pulse: Pulse = field(default_factory=Pulse)
     hubbard_U: float = 0.0
     hubbard_n0: np.ndarray | None = None  # (NK, n) reference densities
+    dissipator: Dissipator = field(default_factory=Dissipator)
     hbar: float = HBAR
     name: str = "problem"
     t_end: float = 10.0

@@ -88,6 +88,7 @@ struct EngineBase {
                              const double* xpp, const double* xpm) = 0;
   virtual void set_hubbard(double U, const double* n0) = 0;
   virtual void set_pulse(const PulseDesc& p) = 0;
+  virtual void set_dissipator(const DissDesc& d, const double* sig_eq) = 0;
   virtual void state_put(const double* sig, const double* psi, const double* omg) = 0;
   virtual void state_get(double* sig, double* psi, double* omg) = 0;
   virtual void sigma_get(double* sig) = 0;
@@ -184,6 +185,8 @@ struct Engine : EngineBase {
   std::vector<void*> allocs;
   Tableau tab = dop54();
   PulseDesc pulse;
+  DissDesc diss{ZH_DISS_NONE, 0.0, 0.0, 1.0};
+  double2* sig_eq = nullptr;  // σ_eq of the dissipator (allocated on first use)
   double U = 0.0;
   bool have_consts = false, have_state = false, h_dirty = false;
   bool h_real = false;     // the Hamiltonian in Hcur / Bp has no imaginary part
@@ -637,6 +640,19 @@ struct Engine : EngineBase {
     }
   }
 
+  void set_dissipator(const DissDesc& d, const double* seq) override {
+    require(have_consts, "set the constants first", ZH_ESTATE);
+    HIPCHK(hipSetDevice(device));
+    drop_graphs();
+    if (d.kind != ZH_DISS_NONE) {
+      if (!sig_eq) sig_eq = dalloc<double2>((size_t)NK * n * n);
+      HIPCHK(hipMemcpyAsync(sig_eq, seq, sizeof(double2) * NK * n * n, hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    diss = d;
+  }
+
   void set_pulse(const PulseDesc& p) override {
     drop_graphs();
     pulse = p;
@@ -959,7 +975,8 @@ struct Engine : EngineBase {
     timed(KC_SIGSYM, [&] {
       dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
       hipLaunchKernelGGL(k_sigma_sym, grid, dim3(32, 8), 0, st, n, (const double2*)G,
-                         sig[kslot], sps, ih);
+                         sig[kslot], sps, ih, sig_in,
+                         (const double2*)(diss.kind != ZH_DISS_NONE ? sig_eq : nullptr));
     });
     HIPCHK(hipGetLastError());
   }
@@ -1106,7 +1123,7 @@ struct Engine : EngineBase {
   void attempt_body() {
     timed(KC_PARAMS, [&] {
       hipLaunchKernelGGL(k_stage_params, dim3(1), dim3(1), 0, stream, (const Ctrl*)ctrl, pulse,
-                         tab, NL, sp);
+                         diss, tab, NL, sp);
     });
     HIPCHK(hipGetLastError());
     for (int s = 0; s < 7; ++s) {
@@ -1408,6 +1425,18 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
     d.kind = kind; d.amp = amp; d.p1 = p1; d.p2 = p2;
     for (int a = 0; a < p->dims.NL; ++a) d.w[a] = w[a];
     p->eng->set_pulse(d);
+  });
+}
+
+int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double width,
+                      const double* sig_eq) {
+  return zh_try([&] {
+    require(p, "null argument");
+    require(kind >= ZH_DISS_NONE && kind <= ZH_DISS_STEP, "unknown dissipator kind");
+    require(kind == ZH_DISS_NONE || sig_eq, "the dissipator needs sigma_eq");
+    require(kind != ZH_DISS_STEP || width != 0.0, "zero dissipator step width");
+    DissDesc d{kind, eta0, t0, width == 0.0 ? 1.0 : width};
+    p->eng->set_dissipator(d, sig_eq);
   });
 }
 

@@ -12,7 +12,22 @@ namespace zh {
 struct StageParams {
   double ts, h;
   double delta[ZH_MAX_LEADS];
+  double eta;  // dissipator rate at ts
 };
+
+// dissipator(t, σ) = η(t)(σ - σ_eq) (Zandpack/mpi/zand:343 hook; the form of
+// examples/Hubbard/Calculation/Bias.py:29-39).
+struct DissDesc {
+  int kind;
+  double eta0, t0, width;
+};
+__device__ inline double diss_rate(const DissDesc& d, double t) {
+  switch (d.kind) {
+    case ZH_DISS_CONST: return d.eta0;
+    case ZH_DISS_STEP: return d.eta0 / (1.0 + exp((t - d.t0) / d.width));
+    default: return 0.0;
+  }
+}
 
 struct Ctrl {
   double t0, h, eps;
@@ -59,8 +74,8 @@ __device__ inline double pulse_value(const PulseDesc& p, double t) {
 }
 
 // Stage times and biases of one attempt from the device control block.
-__global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, Tableau tab, int NL,
-                               StageParams* sp) {
+__global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, DissDesc diss, Tableau tab,
+                               int NL, StageParams* sp) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const double t0 = ctrl->t0, h = ctrl->h;
   for (int s = 0; s < 7; ++s) {
@@ -69,6 +84,7 @@ __global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, Tableau tab, i
     sp[s].h = h;
     const double v = pulse_value(pulse, ts);
     for (int a = 0; a < NL; ++a) sp[s].delta[a] = pulse.w[a] * v;
+    sp[s].eta = diss_rate(diss, ts);
   }
 }
 
@@ -240,8 +256,9 @@ __global__ void k_transpose_h(int n, long ldBp_batch, const double2* H, C* Bp, d
 }
 
 // dsig = (h/hbar) (G + G^H)   (Zandpack/mpi/zand:340-342, 384-387 with G = -iHσ + Π/ħ)
+// With sig_eq, also the dissipator term -h η(ts)(σ - σ_eq) of the stage input σ.
 __global__ void k_sigma_sym(int n, const double2* G, double2* out, const StageParams* sp,
-                            double inv_hbar) {
+                            double inv_hbar, const double2* sig_in, const double2* sig_eq) {
   __shared__ double2 tile[32][33];
   const int b = blockIdx.z;
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
@@ -259,7 +276,15 @@ __global__ void k_sigma_sym(int n, const double2* G, double2* out, const StagePa
     if (i < n && j < n) {
       const double2 g = Gb[(long)i * n + j];
       const double2 gt = tile[tx][r];  // G[j][i]
-      out[((long)b * n + i) * n + j] = make_double2(f * (g.x + gt.x), f * (g.y - gt.y));
+      const long off = ((long)b * n + i) * n + j;
+      double2 v = make_double2(f * (g.x + gt.x), f * (g.y - gt.y));
+      if (sig_eq) {
+        const double e = sp->h * sp->eta;
+        const double2 s = sig_in[off], q = sig_eq[off];
+        v.x -= e * (s.x - q.x);
+        v.y -= e * (s.y - q.y);
+      }
+      out[off] = v;
     }
   }
 }

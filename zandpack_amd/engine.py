@@ -43,6 +43,8 @@ class Propagator:
             n0 = np.ascontiguousarray(p.hubbard_n0, dtype=np.float64)
             N.check(self.lib.zh_set_hubbard(self.h, float(p.hubbard_U), N.ptr(n0)))
         self.set_pulse(p.pulse)
+        if p.dissipator.kind != "none":
+            self.set_dissipator(p.dissipator)
         N.check(self.lib.zh_set_graph(self.h, 1 if graph else 0))
         self.put_state(*(state if state is not None else p.zero_state()))
         self.tiles = self.autotune() if autotune else None
@@ -54,6 +56,17 @@ class Propagator:
         w = np.ascontiguousarray(w[: self.NL])
         N.check(self.lib.zh_set_pulse(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
                                       float(pulse.p1), float(pulse.p2), N.ptr(w)))
+
+    def set_dissipator(self, d):
+        """dissipator(t, σ) = η(t)(σ - σ_eq) of the attempts (builder.Dissipator)."""
+        kind = d.KINDS[d.kind]
+        seq = None
+        if kind:
+            seq = N.c128(d.sig_eq)
+            assert seq.shape == (self.NK, self.n, self.n)
+        N.check(self.lib.zh_set_dissipator(self.h, kind, float(d.eta0), float(d.t0),
+                                           float(d.width), N.ptr(seq) if kind else None))
+        self._diss = seq
 
     def layout_info(self):
         out = (C.c_int * 5)()
