@@ -127,7 +127,8 @@ def run_leg(p, precision, K, W, eps, h0):
     el = time.perf_counter() - t_start
     dev = prop.timing()
     lay = prop.layout_info()
-    out = dict(layout=lay, tiles=prop.tiles, steps_per_s=acc / el, attempts_per_s=K / el, ms_per_attempt=1e3 * el / K,
+    out = dict(layout=lay, tiles=prop.tiles, steps_per_s=acc / el, attempts_per_s=K / el,
+               rhs_evals_per_s=7 * K / el, reject_ratio=rej / K, ms_per_attempt=1e3 * el / K,
                device_ms_per_attempt=dev["attempt"][0], accepted=acc, rejected=rej,
                h_last=h, t_reached=t)
     prop.close()
@@ -172,6 +173,29 @@ def replay_error(p, eps, h0, n_steps):
     jerr = float(np.abs(Jm - J64).max() / max(np.abs(J64).max(), 1e-300))
     serr = float(np.linalg.norm(sm - s64) / np.linalg.norm(s64))
     return jerr, serr, float(np.abs(J64).max())
+
+
+def freerun_h_diff(p, eps, h0, n_steps):
+    """Free-running mode: both precisions with their own step control for n_steps
+    accepted steps; the relative difference of the accepted-h sequences."""
+    from zandpack_amd.engine import Propagator
+    seqs = {}
+    for prec in ("fp64", "mixed"):
+        a = Propagator(p, prec, autotune=True)
+        t, h, hs, rej = 0.0, h0, [], 0
+        while len(hs) < n_steps:
+            r = a.attempt(t, h, eps)
+            h_old, h = h, r["h_next"]
+            if r["accepted"]:
+                hs.append(h_old)
+                t += h
+            else:
+                rej += 1
+        a.close()
+        seqs[prec] = (np.array(hs), rej)
+    h64, hm = seqs["fp64"][0], seqs["mixed"][0]
+    return dict(steps=n_steps, h_rel_diff=float(np.abs(hm - h64).max() / np.abs(h64).max()),
+                rejected_fp64=seqs["fp64"][1], rejected_mixed=seqs["mixed"][1])
 
 
 def cpu_baseline(p):
@@ -280,6 +304,7 @@ def main():
         legs[prec] = run_leg(p, prec, args.steps, args.warmup, args.eps, args.h0)
     print("# replay error", file=sys.stderr, flush=True)
     jerr, serr, jmax = replay_error(p, args.eps, args.h0, args.replay_steps)
+    free = freerun_h_diff(p, args.eps, args.h0, args.replay_steps)
 
     flops, na, nc = algorithmic_flops(p)
     roof = {}
@@ -328,7 +353,7 @@ def main():
         "mixed": legs["mixed"],
         "speedup_mixed_vs_fp64": value / legs["fp64"]["steps_per_s"],
         "replay": {"steps": args.replay_steps, "J_rel_err_mixed": jerr,
-                   "sigma_rel_err_mixed": serr, "J_max": jmax},
+                   "sigma_rel_err_mixed": serr, "J_max": jmax, "free_running": free},
         "roofline": {k: v for k, v in roof["mixed"].items() if k != "kernel_ms"},
         "roofline_fp64": {k: v for k, v in roof["fp64"].items() if k != "kernel_ms"},
         "kernel_ms": {"fp64": roof["fp64"]["kernel_ms"], "mixed": roof["mixed"]["kernel_ms"]},

@@ -5,7 +5,10 @@
     rocprofv3 --pmc FETCH_SIZE -d out -o fetch -- python3 -m zandpack_amd.profile_c3 --copy-mib 256
 
 ``--copy-mib`` also runs one device copy of that many MiB (torch), whose counters
-calibrate FETCH_SIZE / WRITE_SIZE against a known byte count.
+calibrate FETCH_SIZE / WRITE_SIZE against a known byte count.  The tiles are
+autotuned first (those timing attempts show up in a trace as well); the last
+``--attempts`` attempts run with the chosen tiles.  ``--serial`` runs them the way
+bench.py's per-class profile does (one stream, HIP events around each class).
 """
 import argparse
 import sys
@@ -17,6 +20,8 @@ def main():
     ap.add_argument("--attempts", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="per-class timing mode: one stream, no graph (as bench.py's kernel profile)")
     a = ap.parse_args()
     from zandpack_amd import builder
     from zandpack_amd.engine import Propagator
@@ -28,6 +33,8 @@ def main():
         del x, y
     p = builder.config_c3(seed=0)
     prop = Propagator(p, a.precision, graph=not a.no_graph, autotune=True)
+    if a.serial:
+        prop.set_profile(True)
     t, h = 0.0, 0.002
     for _ in range(a.attempts):
         r = prop.attempt(t, h, 1.0, force_accept=True)
