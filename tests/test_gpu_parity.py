@@ -71,6 +71,8 @@ CASES = {
     "wide": lambda: builder.config_small(n_w=6, n_l=7, N_L=5, slots=6, NOT=3, seed=4),
     # 72 coupled orbitals: the 80-column tiles of the Ωξ and Π products
     "wide_edges": lambda: builder.config_small(n_w=36, n_l=3, N_L=2, slots=2, NOT=4, seed=9),
+    # a one-column ribbon: both leads couple the same orbitals (no lead-aligned tiles)
+    "shared_orbitals": lambda: builder.config_small(n_w=4, n_l=1, seed=13),
 }
 
 
@@ -98,7 +100,7 @@ def test_rhs_parity(native, case, precision, layout, hproducts):
 
 
 @pytest.mark.parametrize("case", ["small", "complex_H", "small_spin_hubbard", "chain20",
-                                  "tile_aligned"])
+                                  "tile_aligned", "shared_orbitals"])
 @pytest.mark.parametrize("layout", ["reference", "pruned"])
 @pytest.mark.parametrize("zero_ff", [False, True])
 def test_attempt_parity(native, case, layout, zero_ff):

@@ -362,7 +362,7 @@ struct Engine : EngineBase {
     for (int s = 0; s < 10; ++s) {
       sig[s] = dalloc<double2>(nn);
       if (s != 9) psi[s] = dalloc<cs>(mn);
-      omg[s] = dalloc<cs>(mm);
+      if (s != 6) omg[s] = dalloc<cs>(mm);  // k_7 of Ω is consumed in place (error norm)
     }
     lead = dalloc<int>(Md);
     Hcur = dalloc<double2>(nn);
@@ -767,7 +767,7 @@ struct Engine : EngineBase {
   void clear_ff() {
     if (ML >= Md) return;
     for (int s = 0; s < 10; ++s)
-      for (int b = 0; b < NK; ++b)
+      for (int b = 0; b < NK && omg[s]; ++b)
         HIPCHK(hipMemset2DAsync(omg[s] + ((size_t)b * Md + ML) * Md + ML, sizeof(cs) * Md, 0,
                                 sizeof(cs) * (Md - ML), Md - ML, stream));
     HIPCHK(hipStreamSynchronize(stream));

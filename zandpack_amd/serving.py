@@ -33,10 +33,17 @@ def footprint_bytes(problem: AMEProblem, precision: str = "mixed") -> int:
     Md = M + 64  # active-block padding bound
     cs = 16 if precision == "fp64" else 8
     nc = int(np.count_nonzero(np.any(problem.xi.reshape(-1, n) != 0, axis=0)))
-    slots = 10 * NK * (Md * Md * cs + n * n * 16) + 9 * NK * Md * n * cs
-    ops = NK * (Md * (n + nc) * cs + (n + nc) * n * cs + 4 * Md * nc * cs + 3 * Md * nc * cs
-                + Md * nc * 16 + 8 * n * nc * 16 + 3 * n * n * 16 + 3 * Md * 16)
-    return int(slots + ops + (1 << 20))
+    ncp = nc + 16 * NL  # lead groups padded to 16 columns
+    slots = NK * (9 * Md * Md * cs + 10 * n * n * 16 + 9 * Md * n * cs)
+    ops = NK * (Md * (n + ncp) * cs + (n + ncp) * n * cs       # Ap, Bp
+                + 4 * Md * ncp * cs                              # Aomg, Bomg
+                + (1 + 4 + 7 + 1) * Md * ncp * cs                # ξ_c, Ωξ parts, products, W
+                + Md * ncp * 16 + 8 * n * ncp * 16               # Ixi_c, Π parts
+                + 5 * n * n * 16                                 # H, its real copies, G, σ_eq
+                + 64 * n * 20                                    # row-compressed H
+                + 8 * Md * 16)                                   # per-mode constants
+    te = 8 * (2 * 1024 + ((Md + 15) // 16) ** 2 * NK)
+    return int(slots + ops + te + (1 << 20))
 
 
 @dataclass
