@@ -111,6 +111,18 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
 int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double width,
                       const double* sig_eq);
 
+/* Steady state of Zandpack/cmdtools/psinought (:180-449): Psi0 solving psinought's
+ * real-linear system (Av3 / B3, :199-239) for the resident sigma taken as sigma0, the
+ * bias at t0 and H(t0, sigma0) (Hubbard term included), and the closed-form Omega0
+ * (:447-449); (sigma0, Psi0, Omega0) becomes the resident state.  BiCGSTAB over the
+ * real inner product to ||b - A Psi|| <= tol ||b||, at most maxiter iterations; with
+ * the eigenpairs of H(t0, sigma0) (evecs (NK,n,n) complex128, column k = vector k;
+ * evals (NK,n) real; both NULL for none) it is right-preconditioned with
+ * (chi+_i + Delta - H)^-1.  Returns ZH_ELIMIT when not converged (the state then
+ * holds the last iterate); iters and rel_resid may be NULL. */
+int zh_steady_state(zh_problem* p, double t0, double tol, int maxiter, const double* evecs,
+                    const double* evals, int* iters, double* rel_resid);
+
 /* Resident state y0 = (sigma, Psi, Omega), reference layout, complex128. */
 int zh_state_put(zh_problem* p, const double* sig, const double* psi, const double* omg);
 int zh_state_get(zh_problem* p, double* sig, double* psi, double* omg);

@@ -438,3 +438,51 @@ def stationary_omega(consts, psi, delta, hbar=HBAR):
            + (consts["dGP"].reshape(NK, M)[..., None] * xi) @ p.conj().transpose(0, 2, 1))
     om = -src * 1j * hbar / (xpm[:, None, :] - xpp[:, :, None])
     return om.reshape(NK, NL, NX, NOT, NL, NX, NOT)
+
+
+# ---------------------------------------------------------------------------
+# Steady state (cmdtools/psinought)
+# ---------------------------------------------------------------------------
+def steady_state_operator(consts, psi, H, delta, hbar=HBAR):
+    """A(ψ) of cmdtools/psinought:199-226 (``Av3``): with Ω₀(ψ) the closed form of
+    ``stationary_omega`` (psinought:447-449),
+        A(ψ)_i = (χ⁺_i + Δ_α(i) - H) ψ_i - (Ω₀(ψ) ξ)_i / ħ
+    (psinought writes the Ω₀ term as Σ_j ξ_j i/(χ⁻_j-χ⁺_i) (ΔM_j Ixi_j·ψ_i + ΔP_i ψ_j*·ξ_i)).
+    Real-linear in ψ (ψ* enters)."""
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    lead = lead_of_mode(NL, NX, NOT)
+    xpp = consts["Xpp"].reshape(NK, M) + np.asarray(delta, float)[lead]
+    p = psi.reshape(NK, M, n)
+    om = stationary_omega(consts, psi, delta, hbar).reshape(NK, M, M)
+    W = om @ consts["xi"].reshape(NK, M, n)
+    out = xpp[..., None] * p - p @ H.transpose(0, 2, 1) - W / hbar
+    return out.reshape(psi.shape)
+
+
+def steady_state_rhs(consts, sig0, hbar=HBAR):
+    """b of cmdtools/psinought:228-239 (``B3``): ħ Λ^{<+}_i ξ_i + ħ ΔP_i σ₀ ξ_i."""
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    xi = consts["xi"].reshape(NK, M, n)
+    b = hbar * consts["GL_P"].reshape(NK, M)[..., None] * xi
+    b = b + hbar * consts["dGP"].reshape(NK, M)[..., None] * (xi @ sig0.transpose(0, 2, 1))
+    return b.reshape(consts["xi"].shape)
+
+
+def steady_state_dense(consts, H, sig0, delta, hbar=HBAR):
+    """Ψ₀ solving A(ψ) = b as the real 2N x 2N system psinought hands to its Krylov
+    solver (psinought:203-209, 418-419), here by a dense solve (tiny systems only).
+    Returns (Ψ₀, Ω₀)."""
+    shape = consts["xi"].shape
+    N = int(np.prod(shape))
+    b = steady_state_rhs(consts, sig0, hbar).reshape(N)
+    A = np.zeros((2 * N, 2 * N))
+    e = np.zeros(N, complex)
+    for k in range(2 * N):
+        e[:] = 0
+        e[k % N] = 1.0 if k < N else 1j
+        col = steady_state_operator(consts, e.reshape(shape), H, delta, hbar).reshape(N)
+        A[:N, k] = col.real
+        A[N:, k] = col.imag
+    x = np.linalg.solve(A, np.concatenate([b.real, b.imag]))
+    psi0 = (x[:N] + 1j * x[N:]).reshape(shape)
+    return psi0, stationary_omega(consts, psi0, delta, hbar)

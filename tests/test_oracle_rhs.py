@@ -124,6 +124,28 @@ def test_dissipator_relaxation_closed_form(kind):
     assert np.abs(out["sigma"][-1][0] - exact).max() < 1e-3 * np.abs(1 - decay)
 
 
+@pytest.mark.parametrize("NK,U", [(1, 0.0), (2, 1.5)])
+def test_steady_state_makes_rhs_vanish(NK, U):
+    """Ψ₀ from psinought's linear system (dense solve) and its closed-form Ω₀ are a
+    fixed point of the equations of motion: dΨ = dΩ = 0 (psinought's own check,
+    cmdtools/psinought:455-462, made exact)."""
+    p = builder.config_small(NK=NK, hubbard_U=U, seed=17)
+    sig0 = p.sigma0
+    H = p.H0 + p.dH(0.0, sig0)
+    delta = np.array([0.13, -0.08])
+    psi0, om0 = O.steady_state_dense(p.consts(), H, sig0, delta)
+    ds, dp, do = O.rhs(p.consts(), sig0, psi0, om0, H, delta)
+    assert np.abs(dp).max() < 1e-10 * max(1.0, np.abs(psi0).max())
+    assert np.abs(do).max() < 1e-10 * max(1.0, np.abs(om0).max())
+    assert np.abs(psi0).max() > 1e-3  # a non-trivial solution
+    # the real-linear operator: A(c ψ) = c A(ψ) only for real c
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=psi0.shape) + 1j * rng.normal(size=psi0.shape)
+    Ax = O.steady_state_operator(p.consts(), x, H, delta)
+    assert np.allclose(O.steady_state_operator(p.consts(), 2.5 * x, H, delta), 2.5 * Ax)
+    assert not np.allclose(O.steady_state_operator(p.consts(), 1j * x, H, delta), 1j * Ax)
+
+
 def test_time_update_quirk():
     """Reference bookkeeping advances t by the *new* h (mpi/zand:406-407,427)."""
     p = builder.config_small(seed=12)

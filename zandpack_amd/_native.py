@@ -63,6 +63,8 @@ SIGNATURES = {
     "zh_set_layout": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_layout_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_sparse": (C.c_int, [C.c_void_p, C.c_int]),
+    "zh_steady_state": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "zh_set_dissipator": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                     _dp]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

@@ -92,6 +92,8 @@ struct EngineBase {
   virtual void state_put(const double* sig, const double* psi, const double* omg) = 0;
   virtual void state_get(double* sig, double* psi, double* omg) = 0;
   virtual void sigma_get(double* sig) = 0;
+  virtual void steady_state(double t0, double tol, int maxiter, const double* evecs,
+                            const double* evals, int* iters, double* resid) = 0;
   virtual void rhs(const double* Ht, const double* delta, const double* sig, const double* psi,
                    const double* omg, double* dsig, double* dpsi, double* domg, double* J) = 0;
   virtual void attempt(zh_attempt_info* info, double* J) = 0;
@@ -158,6 +160,21 @@ static int pow2_bucket(int v) {
   while (b < v) b <<= 1;
   return b;
 }
+
+// Scratch device buffers of one call, freed when it returns (also on error).
+struct DevBuf {
+  std::vector<void*> p;
+  template <typename T> T* get(size_t count) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+      throw Error{ZH_ENOMEM, "hipMalloc failed for a scratch buffer"};
+    p.push_back(q);
+    return (T*)q;
+  }
+  ~DevBuf() {
+    for (void* q : p) (void)hipFree(q);
+  }
+};
 
 template <typename R>  // storage real type of Psi / Omega and compute type of their GEMMs
 struct Engine : EngineBase {
@@ -879,6 +896,19 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
   }
 
+  // Stage inputs on the coupled orbitals, the Hubbard diagonal and the per-mode
+  // shifts of the Ω outer term (k_gather), on the main stream.
+  void gather(const StageParams* sps, const double2* sig_in, bool hub) {
+    GatherArgs<cs> g;
+    g.NK = NK; g.n = n; g.M = Md; g.nc = nc;
+    g.Ap = Ap; g.ldAp = n + nc; g.sAp = (long)Md * (n + nc);
+    g.Aomg = Aomg; g.Bomg = Bomg; g.Bp = Bp; g.sig = sig_in; g.cidx = cidx;
+    g.dvec = dvec; g.n0 = n0; g.U = U; g.hbar = hbar; g.hubbard = hub && U != 0.0;
+    g.sp = sps; g.chip = chip; g.chim = chim; g.lead = lead; g.u = uvec; g.v = vvec;
+    hipLaunchKernelGGL(k_gather<cs>, dim3(512, 4), dim3(256), 0, stream, g);
+    HIPCHK(hipGetLastError());
+  }
+
   // Ω side of one stage: stage input y0 + Σ a_j k_j, where Ω_s goes (ynew, the
   // last stage), whether k_s is stored and whether the error norm is accumulated.
   struct OmegaStage {
@@ -897,16 +927,7 @@ struct Engine : EngineBase {
   void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, bool currents,
                  bool hub) {
     const StageParams* sps = sp + s;
-    timed(KC_GATHER, [&] {
-      GatherArgs<cs> g;
-      g.NK = NK; g.n = n; g.M = Md; g.nc = nc;
-      g.Ap = Ap; g.ldAp = n + nc; g.sAp = (long)Md * (n + nc);
-      g.Aomg = Aomg; g.Bomg = Bomg; g.Bp = Bp; g.sig = sig_in; g.cidx = cidx;
-      g.dvec = dvec; g.n0 = n0; g.U = U; g.hbar = hbar; g.hubbard = hub && U != 0.0;
-      g.sp = sps; g.chip = chip; g.chim = chim; g.lead = lead; g.u = uvec; g.v = vvec;
-      hipLaunchKernelGGL(k_gather<cs>, dim3(512, 4), dim3(256), 0, stream, g);
-    });
-    HIPCHK(hipGetLastError());
+    timed(KC_GATHER, [&] { gather(sps, sig_in, hub); });
     // the three branches (serial on the main stream when profiling, so that the
     // per-class event timings stay meaningful)
     const bool conc = !profile;
@@ -1287,6 +1308,223 @@ struct Engine : EngineBase {
     collect(info, J);
   }
 
+  // ---------------------------------------------------------------- steady state
+  // Ψ₀ and Ω₀ of cmdtools/psinought (:180-449) for the resident σ (taken as σ₀), the
+  // bias at t0 and H(t0, σ₀): BiCGSTAB over the real inner product (the operator is
+  // real-linear, ψ* enters), right-preconditioned with (χ⁺_i + Δ - H)^{-1} from the
+  // eigenpairs of H(t0, σ₀) when given (psinought's block preconditioner without
+  // its coupling terms).  Writes (σ₀, Ψ₀, Ω₀) as the resident state.
+  void steady_state(double t0, double tol, int maxiter, const double* evecs, const double* evals,
+                    int* iters, double* resid) override {
+    require(have_consts && have_state, "steady state needs constants and a resident sigma",
+            ZH_ESTATE);
+    require(tol > 0.0 && maxiter > 0, "tolerance and iteration limit must be positive");
+    HIPCHK(hipSetDevice(device));
+    ensure_h0();
+    drop_graphs();
+    struct Restore {  // profile mode back as it was, also on error
+      bool& f;
+      bool v;
+      ~Restore() { f = v; }
+    } restore{profile, profile};
+    profile = false;
+    // stage parameters at t0 with h = 1 (sp[0]: Δ(t0), the dissipator rate unused)
+    ctrl_h->t0 = t0; ctrl_h->h = 1.0; ctrl_h->eps = 1.0; ctrl_h->force = 1;
+    HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_stage_params, dim3(1), dim3(1), 0, stream, (const Ctrl*)ctrl, pulse,
+                       diss, tab, NL, sp);
+    HIPCHK(hipGetLastError());
+    const StageParams* sp0 = sp;
+    const long N = (long)NK * Md * n;
+    const long mc = (long)NK * Md * nc;
+    const double2* sig0 = sig[iy0];
+    DevBuf buf;
+    double2 *x = buf.get<double2>(N), *r = buf.get<double2>(N), *rh = buf.get<double2>(N),
+            *pv = buf.get<double2>(N), *v = buf.get<double2>(N), *sv = buf.get<double2>(N),
+            *tv = buf.get<double2>(N), *bv = buf.get<double2>(N);
+    cs* hpsi = buf.get<cs>(N);
+    double* part = buf.get<double>(kDotBlocks);
+    const bool prec = evecs && evals;
+    double2 *ph = pv, *sh = sv;
+    cs *Vc = nullptr, *Vt = nullptr, *y1 = nullptr, *y2 = nullptr;
+    double* ev = nullptr;
+    if (prec) {
+      ph = buf.get<double2>(N);
+      sh = buf.get<double2>(N);
+      y1 = buf.get<cs>(N);
+      y2 = buf.get<cs>(N);
+      Vc = buf.get<cs>((size_t)NK * n * n);
+      Vt = buf.get<cs>((size_t)NK * n * n);
+      ev = buf.get<double>((size_t)NK * n);
+      std::vector<cs> hc((size_t)NK * n * n), ht((size_t)NK * n * n);
+      for (int b = 0; b < NK; ++b)
+        for (int o = 0; o < n; ++o)
+          for (int k = 0; k < n; ++k) {
+            const size_t at = ((size_t)b * n + o) * n + k;
+            hc[at] = from_d2<cs>(make_double2(evecs[2 * at], -evecs[2 * at + 1]));
+            ht[((size_t)b * n + k) * n + o] = from_d2<cs>(make_double2(evecs[2 * at], evecs[2 * at + 1]));
+          }
+      HIPCHK(hipMemcpyAsync(Vc, hc.data(), sizeof(cs) * hc.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(Vt, ht.data(), sizeof(cs) * ht.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(ev, evals, sizeof(double) * NK * n, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    const dim3 eg(2048), eb(256);
+    auto lincomb = [&](double a, const double2* xa, double b, const double2* yb, double c,
+                       const double2* zc, double2* out) {
+      hipLaunchKernelGGL(k_lincomb, eg, eb, 0, stream, N, a, xa, b, yb, c, zc, out);
+      HIPCHK(hipGetLastError());
+    };
+    std::vector<double> ph_(kDotBlocks);
+    auto dot = [&](const double2* a, const double2* b) {
+      hipLaunchKernelGGL(k_dot, dim3(kDotBlocks), dim3(256), 0, stream, N, a, b, part);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(ph_.data(), part, sizeof(double) * kDotBlocks, hipMemcpyDeviceToHost,
+                            stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      double acc = 0.0;
+      for (double q : ph_) acc += q;
+      return acc;
+    };
+    // Ω₀(ψ) of the ψ block of Ap into om (FF tiles skipped when the block is zero)
+    auto omega0 = [&](cs* om) {
+      KSegOmega ks{t1, t2, nc, ML, ff ? 1 : 0, (Md + kTile - 1) / kTile};
+      GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
+                       (long)Md * 2 * nc};
+      g.xcd_order = 0;
+      EpiOmegaSteady<cs> e{om, uvec, vvec, Md};
+      HIPCHK((launch_sel<R, R, R, false, true, 0>(tsel[TS_OMEGA], g, NK, e, stream, ks,
+                                                  typename Cands<R>::omega())));
+    };
+    auto to_ap = [&](const double2* xin) {
+      hipLaunchKernelGGL(k_to_ap<cs>, eg, eb, 0, stream, N, n, xin, Ap, (long)(n + nc));
+      HIPCHK(hipGetLastError());
+    };
+    // y = A(x)
+    auto apply_A = [&](const double2* xin, double2* yout) {
+      to_ap(xin);
+      gather(sp0, sig0, true);
+      omega0(omg[0]);
+      GemmArgs<R, R> g{Md, nc, Md, omg[0], Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
+      EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
+      KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
+      HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_OMGXI], g, NK, e, stream, ks,
+                                                   typename Cands<R>::omgxi())));
+      WSumArgs<cs> w{};
+      w.part = W;
+      for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
+      w.W = Wst;
+      w.s = 0; w.parts = kWParts; w.NK = NK; w.M = Md; w.nc = nc;
+      hipLaunchKernelGGL(k_wsum<cs>, dim3(1024), dim3(256), 0, stream, w);
+      HIPCHK(hipGetLastError());
+      if (h_sparse) {
+        hipLaunchKernelGGL(k_ell_rows<cs>, eg, eb, 0, stream, NK, Md, (const cs*)Ap,
+                           (long)(n + nc), ell(), hpsi);
+        HIPCHK(hipGetLastError());
+      } else {
+        GemmArgs<R, R> gh{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
+        EpiStore<cs> eh{hpsi, n, (long)Md * n, 1.0, 0.0};
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], gh, NK, eh, stream, KFull(),
+                                                     typename Cands<R>::psi())));
+      }
+      SteadyApplyArgs<cs> a{NK, Md, n, nc, xin, hpsi, Wst, chip, dvec, lead, cpos, sp0,
+                            1.0 / hbar, yout};
+      hipLaunchKernelGGL(k_steady_apply<cs>, eg, eb, 0, stream, a);
+      HIPCHK(hipGetLastError());
+    };
+    // z = M^{-1} r (identity without eigenpairs)
+    auto apply_M = [&](const double2* rin, double2* zout) {
+      hipLaunchKernelGGL(k_to_ap<cs>, eg, eb, 0, stream, N, n, rin, y1, (long)n);
+      HIPCHK(hipGetLastError());
+      GemmArgs<R, R> g1{Md, n, n, y1, n, (long)Md * n, Vc, n, (long)n * n};
+      EpiPrecScale<cs> e1{y2, ev, chip, lead, sp0, Md, n};
+      HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g1, NK, e1, stream, KFull(),
+                                                   typename Cands<R>::psi())));
+      GemmArgs<R, R> g2{Md, n, n, y2, n, (long)Md * n, Vt, n, (long)n * n};
+      EpiStore<double2> e2{zout, n, (long)Md * n, 1.0, 0.0};
+      HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g2, NK, e2, stream, KFull(),
+                                                   typename Cands<R>::psi())));
+    };
+    // b: gather with ψ = 0 (σ₀ columns, Hubbard diagonal, shifts), then the ξσ₀ product
+    lincomb(0.0, nullptr, 0.0, nullptr, 0.0, nullptr, x);
+    to_ap(x);
+    gather(sp0, sig0, true);
+    if (nc > 0) {
+      EpiStore<cs> e0{hpsi, n, (long)Md * n, 1.0, 0.0};
+      GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
+                        (long)(n + nc) * n};
+      HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, stream, kseg_g0(),
+                                                   typename Cands<R>::psi())));
+    }
+    SteadyRhsArgs<cs> ra{NK, Md, n, nc, nc > 0 ? g0_rows() : 0, hpsi, xic, glp, cidx, hbar, bv};
+    hipLaunchKernelGGL(k_steady_rhs<cs>, eg, eb, 0, stream, ra);
+    hipLaunchKernelGGL(k_steady_rhs_xi<cs>, eg, eb, 0, stream, ra);
+    HIPCHK(hipGetLastError());
+    const double bnorm = std::sqrt(dot(bv, bv));
+    int it = 0;
+    double res = 0.0;
+    bool ok = bnorm == 0.0;
+    if (!ok) {
+      // x0 = M^{-1} b (psinought starts from preconditioner @ b), else 0
+      if (prec) apply_M(bv, x);
+      apply_A(x, r);
+      lincomb(1.0, bv, -1.0, r, 0.0, nullptr, r);
+      lincomb(1.0, r, 0.0, nullptr, 0.0, nullptr, rh);
+      lincomb(0.0, nullptr, 0.0, nullptr, 0.0, nullptr, pv);
+      lincomb(0.0, nullptr, 0.0, nullptr, 0.0, nullptr, v);
+      double rho = 1.0, alpha = 1.0, omega = 1.0;
+      res = std::sqrt(dot(r, r)) / bnorm;
+      ok = res < tol;
+      while (!ok && it < maxiter) {
+        ++it;
+        const double rho1 = dot(rh, r);
+        require(rho1 != 0.0 && std::isfinite(rho1), "steady state: Krylov breakdown (rho = 0)",
+                ZH_ENONFINITE);
+        const double beta = (rho1 / rho) * (alpha / omega);
+        lincomb(1.0, r, beta, pv, -beta * omega, v, pv);
+        if (prec) apply_M(pv, ph);
+        apply_A(ph, v);
+        const double rv = dot(rh, v);
+        require(rv != 0.0 && std::isfinite(rv), "steady state: Krylov breakdown (r·v = 0)",
+                ZH_ENONFINITE);
+        alpha = rho1 / rv;
+        lincomb(1.0, r, -alpha, v, 0.0, nullptr, sv);
+        const double snorm = std::sqrt(dot(sv, sv)) / bnorm;
+        if (snorm < tol) {
+          lincomb(1.0, x, alpha, ph, 0.0, nullptr, x);
+          lincomb(1.0, sv, 0.0, nullptr, 0.0, nullptr, r);
+          res = snorm;
+          ok = true;
+          break;
+        }
+        if (prec) apply_M(sv, sh);
+        apply_A(sh, tv);
+        const double tt = dot(tv, tv);
+        require(tt != 0.0 && std::isfinite(tt), "steady state: Krylov breakdown (t = 0)",
+                ZH_ENONFINITE);
+        omega = dot(tv, sv) / tt;
+        lincomb(1.0, x, alpha, ph, omega, sh, x);
+        lincomb(1.0, sv, -omega, tv, 0.0, nullptr, r);
+        rho = rho1;
+        res = std::sqrt(dot(r, r)) / bnorm;
+        require(std::isfinite(res), "steady state: non-finite residual", ZH_ENONFINITE);
+        ok = res < tol;
+      }
+    }
+    // resident state (σ₀, Ψ₀, Ω₀(Ψ₀)); the passive block of Ω₀ is zero
+    hipLaunchKernelGGL(k_to_ap<cs>, eg, eb, 0, stream, N, n, (const double2*)x, psi[iy0], (long)n);
+    HIPCHK(hipGetLastError());
+    to_ap(x);
+    gather(sp0, sig0, true);
+    omega0(omg[iy0]);
+    HIPCHK(hipStreamSynchronize(stream));
+    set_ff(true);
+    if (iters) *iters = it;
+    if (resid) *resid = res;
+    require(ok, "steady state: not converged to the tolerance within the iteration limit "
+                "(the state holds the last iterate)", ZH_ELIMIT);
+  }
+
   void rhs(const double* Ht, const double* delta, const double* s, const double* p,
            const double* o, double* ds, double* dp, double* dom, double* J) override {
     require(have_consts, "constants not set", ZH_ESTATE);
@@ -1425,6 +1663,14 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
     d.kind = kind; d.amp = amp; d.p1 = p1; d.p2 = p2;
     for (int a = 0; a < p->dims.NL; ++a) d.w[a] = w[a];
     p->eng->set_pulse(d);
+  });
+}
+
+int zh_steady_state(zh_problem* p, double t0, double tol, int maxiter, const double* evecs,
+                    const double* evals, int* iters, double* rel_resid) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->steady_state(t0, tol, maxiter, evecs, evals, iters, rel_resid);
   });
 }
 

@@ -809,4 +809,191 @@ __global__ __launch_bounds__(256) void k_hsig_ell(const double2* sig, Ell e, Epi
   }
 }
 
+// ------------------------------------------------------------ steady state
+// (cmdtools/psinought): Ψ₀ solves the real-linear system A(ψ) = b with
+//   A(ψ)_i = (χ⁺_i + Δ_α(i) - H) ψ_i - (Ω₀(ψ) ξ)_i / ħ,   b_i = ħ Λ^{<+}_i ξ_i + ħ ΔP_i σ₀ ξ_i,
+// Ω₀(ψ)_ij = -(ΔM_j Ixi_j·ψ_i + ΔP_i ψ_j*·ξ_i) iħ / (χ⁻_j - χ⁺_i)  (psinought:199-239, 447-449).
+
+// Ω₀ from the Ω sources acc (gathered with h = 1, so u - v = (χ⁺_i - χ⁻_j)/ħ):
+// Ω₀ = i acc / (u_i - v_j); inert padding modes (u = v = 0, acc = 0) give 0.
+template <typename C>
+struct EpiOmegaSteady {
+  C* out;
+  const double2* u; const double2* v;
+  int M;
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    const double2 a = u[(long)b * M + row], c = v[(long)b * M + col];
+    const double dr = a.x - c.x, di = a.y - c.y;
+    const double den = dr * dr + di * di;
+    double2 o = make_double2(0.0, 0.0);
+    if (den > 0.0) {
+      // i (re + i im) / (dr + i di) = i (re + i im)(dr - i di) / den
+      const double qr = (re * dr + im * di) / den, qi = (im * dr - re * di) / den;
+      o = make_double2(-qi, qr);
+    }
+    out[((long)b * M + row) * M + col] = from_d2<C>(o);
+  }
+};
+
+// Krylov vector (FP64, (NK, M, n)) -> the ψ block of Ap (storage type).
+template <typename C>
+__global__ void k_to_ap(long total, int n, const double2* x, C* Ap, long ldAp) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long row = idx / n;
+    const int o = (int)(idx - row * n);
+    Ap[row * ldAp + o] = from_d2<C>(x[idx]);
+  }
+}
+
+// HPsi[i, o] = (ψ Hᵀ)[i, o] over the row-compressed H (ψ read from Ap).
+template <typename C>
+__global__ void k_ell_rows(int NK, int M, const C* Ap, long ldAp, Ell h, C* out) {
+  const long total = (long)NK * M * h.n;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long row = idx / h.n;  // b*M + i
+    const int o = (int)(idx - row * h.n);
+    const int b = (int)(row / M);
+    const int* col = h.col + (long)b * h.R * h.n;
+    const double2* val = h.val + (long)b * h.R * h.n;
+    double ar = 0.0, ai = 0.0;
+    for (int r = 0; r < h.R; ++r) {
+      const double2 p = to_d2(Ap[row * ldAp + col[(long)r * h.n + o]]);
+      const double2 v = val[(long)r * h.n + o];
+      ar += p.x * v.x - p.y * v.y;
+      ai += p.x * v.y + p.y * v.x;
+    }
+    out[idx] = from_d2<C>(make_double2(ar, ai));
+  }
+}
+
+// y = (χ⁺_i + Δ_α(i) - d_o) x - HPsi - W[:, c(o)]/ħ   (A(ψ) with HPsi = ψ H0ᵀ, d the
+// Hubbard diagonal, W = Ω₀ ξ_c).
+template <typename C>
+struct SteadyApplyArgs {
+  int NK, M, n, nc;
+  const double2* x; const C* hpsi; const C* W;
+  const double2* chip; const double2* dvec;
+  const int* lead; const int* cpos;
+  const StageParams* sp;
+  double inv_hbar;
+  double2* y;
+};
+template <typename C>
+__global__ void k_steady_apply(SteadyApplyArgs<C> g) {
+  const long total = (long)g.NK * g.M * g.n;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long bm = idx / g.n;
+    const int o = (int)(idx - bm * g.n);
+    const int b = (int)(bm / g.M);
+    const int i = (int)(bm - (long)b * g.M);
+    double2 chi = g.chip[bm];
+    chi.x += g.sp->delta[g.lead[i]];
+    const double2 d = g.dvec[(long)b * g.n + o];
+    chi.x -= d.x;
+    chi.y -= d.y;
+    const double2 xv = g.x[idx];
+    const double2 hp = to_d2(g.hpsi[idx]);
+    double2 r = cmul(chi, xv);
+    r.x -= hp.x;
+    r.y -= hp.y;
+    const int c = g.cpos[o];
+    if (c >= 0) {
+      const double2 w = to_d2(g.W[bm * g.nc + c]);
+      r.x -= w.x * g.inv_hbar;
+      r.y -= w.y * g.inv_hbar;
+    }
+    g.y[idx] = r;
+  }
+}
+
+// b = ħ ΔP_i σ₀ ξ_i (from its GEMM, or 0 on rows it skipped) + ħ Λ^{<+}_i ξ_i.
+template <typename C>
+struct SteadyRhsArgs {
+  int NK, M, n, nc, g0_rows;
+  const C* g0;                 // (NK, M, n): ħ ΔP ξ_c σ_cᵀ
+  const C* xic; const double2* glp;
+  const int* cidx;
+  double hbar;
+  double2* bvec;
+};
+template <typename C>
+__global__ void k_steady_rhs(SteadyRhsArgs<C> g) {
+  const long total = (long)g.NK * g.M * g.n;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long bm = idx / g.n;
+    const int i = (int)(bm % g.M);
+    g.bvec[idx] = i < g.g0_rows ? to_d2(g.g0[idx]) : make_double2(0.0, 0.0);
+  }
+}
+template <typename C>
+__global__ void k_steady_rhs_xi(SteadyRhsArgs<C> g) {
+  // one thread per (mode row, coupled column): columns are distinct orbitals
+  const long total = (long)g.NK * g.M * g.nc;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long bm = idx / g.nc;
+    const int c = (int)(idx - bm * g.nc);
+    const int o = g.cidx[c];
+    if (o < 0) continue;
+    const double2 gx = cmul(g.glp[bm], to_d2(g.xic[bm * g.nc + c]));
+    double2& t = g.bvec[bm * g.n + o];
+    t.x += g.hbar * gx.x;
+    t.y += g.hbar * gx.y;
+  }
+}
+
+// out = a x + b y + c z over FP64 complex vectors (real coefficients; null skips).
+__global__ void k_lincomb(long total, double a, const double2* x, double b, const double2* y,
+                          double c, const double2* z, double2* out) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    double2 r = make_double2(0.0, 0.0);
+    if (x) { const double2 v = x[idx]; r.x += a * v.x; r.y += a * v.y; }
+    if (y) { const double2 v = y[idx]; r.x += b * v.x; r.y += b * v.y; }
+    if (z) { const double2 v = z[idx]; r.x += c * v.x; r.y += c * v.y; }
+    out[idx] = r;
+  }
+}
+
+// Real inner product Σ (x.re y.re + x.im y.im) of FP64 complex vectors: one partial
+// per block (fixed-order tree), summed by the host in block order.
+constexpr int kDotBlocks = 512;
+__global__ __launch_bounds__(256) void k_dot(long total, const double2* x, const double2* y,
+                                             double* part) {
+  double acc = 0.0;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const double2 a = x[idx], b = y[idx];
+    acc += a.x * b.x + a.y * b.y;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Preconditioner (χ⁺_i + Δ - H)^{-1} = V diag(1/(χ⁺_i + Δ - e_k)) V†, applied as
+// Y = R conj(V) (GEMM), this scaling, Z = Y Vᵀ (GEMM).
+template <typename C>
+struct EpiPrecScale {
+  C* out; const double* eval; const double2* chip; const int* lead; const StageParams* sp;
+  int M, n;
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    double2 chi = chip[(long)b * M + row];
+    chi.x += sp->delta[lead[row]] - eval[(long)b * n + col];
+    const double den = chi.x * chi.x + chi.y * chi.y;
+    double2 o = make_double2(0.0, 0.0);
+    if (den > 0.0) o = make_double2((re * chi.x + im * chi.y) / den, (im * chi.x - re * chi.y) / den);
+    out[((long)b * M + row) * n + col] = from_d2<C>(o);
+  }
+};
+
 }  // namespace zh
