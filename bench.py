@@ -230,6 +230,26 @@ def replay_fp64_error(req):
     return float(np.abs(Jm - J).max() / max(np.abs(J).max(), 1e-300))
 
 
+def steady_leg(p, tol=1e-8, maxiter=1500):
+    """Steady state (cmdtools/psinought equivalent) of the same workload in FP64:
+    wall time including the host eigendecomposition of H(σ₀), iterations, relative
+    residual, and the fixed-point check max|dΨ| / max|b|/ħ from the device RHS."""
+    from oracle import zand_oracle as O  # only for the b scale of the check
+    from zandpack_amd.engine import Propagator
+    prop = Propagator(p, "fp64")
+    t0 = time.perf_counter()
+    its, res = prop.steady_state(0.0, tol=tol, maxiter=maxiter)
+    el = time.perf_counter() - t0
+    s, ps, om = prop.get_state()
+    Ht = p.H0 + p.dH(0.0, s)
+    delta = np.array([p.bias(0.0, a) for a in range(p.dims[2])])
+    _, dp, _, _ = prop.rhs(Ht, delta, s, ps, om, want_omega=False)
+    scale = np.abs(O.steady_state_rhs(p.consts(), s)).max() / O.HBAR
+    prop.close()
+    return {"precision": "fp64", "seconds": el, "iterations": its, "rel_residual": res,
+            "tol": tol, "max_dpsi_rel": float(np.abs(dp).max() / scale)}
+
+
 def batched_leg(n_inst, steps, seed=0):
     """Config 4: n_inst uneven propagations (n in {100..800}, M = 4n) advanced
     together, each for ``steps`` accepted steps; aggregate steps/s."""
@@ -360,6 +380,8 @@ def main():
     }
     if not args.no_serving:
         print("# batched (config 4)", file=sys.stderr, flush=True)
+        print("# steady state", file=sys.stderr, flush=True)
+        line["steady_state"] = steady_leg(p)
         line["batched"] = batched_leg(args.batched_instances, args.batched_steps)
         print("# serving (config 5)", file=sys.stderr, flush=True)
         line["serving"] = serving_leg(args.serving_requests, args.serving_rate,

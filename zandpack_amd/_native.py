@@ -14,6 +14,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzandhip
 
 ZH_FP64, ZH_MIXED = 0, 1
 ZH_TIME_REFERENCE, ZH_TIME_CORRECT = 0, 1
+ZH_ELIMIT = -6
 PULSE_KINDS = {"none": 0, "step": 1, "gauss": 2, "ramp": 3, "sin": 4, "box": 5}
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE", -6: "ELIMIT"}
 

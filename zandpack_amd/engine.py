@@ -57,11 +57,13 @@ class Propagator:
         N.check(self.lib.zh_set_pulse(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
                                       float(pulse.p1), float(pulse.p2), N.ptr(w)))
 
-    def steady_state(self, t0=0.0, tol=1e-10, maxiter=500, precondition=True):
+    def steady_state(self, t0=0.0, tol=1e-10, maxiter=500, precondition=True,
+                     raise_on_limit=True):
         """Ψ₀, Ω₀ of cmdtools/psinought for the resident σ (as σ₀), the bias at t0 and
         H(t0, σ₀); they become the resident state.  Returns (iterations, relative
         residual).  The preconditioner's eigenpairs of H(t0, σ₀) = H0 + dH(t0, σ₀)
-        come from a host eigendecomposition (LAPACK)."""
+        come from a host eigendecomposition (LAPACK).  With raise_on_limit=False an
+        unconverged solve returns its counts instead of raising."""
         ev = vec = None
         if precondition:
             s0 = self.sigma()
@@ -69,8 +71,10 @@ class Propagator:
             e, v = np.linalg.eigh(Hs)
             ev, vec = np.ascontiguousarray(e, np.float64), np.ascontiguousarray(v, np.complex128)
         its, res = C.c_int(0), C.c_double(0.0)
-        N.check(self.lib.zh_steady_state(self.h, float(t0), float(tol), int(maxiter),
-                                         N.ptr(vec), N.ptr(ev), C.byref(its), C.byref(res)))
+        code = self.lib.zh_steady_state(self.h, float(t0), float(tol), int(maxiter),
+                                        N.ptr(vec), N.ptr(ev), C.byref(its), C.byref(res))
+        if code != N.ZH_ELIMIT or raise_on_limit:
+            N.check(code)
         return its.value, res.value
 
     def set_dissipator(self, d):
