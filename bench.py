@@ -230,7 +230,7 @@ def replay_fp64_error(req):
     return float(np.abs(Jm - J).max() / max(np.abs(J).max(), 1e-300))
 
 
-def steady_leg(p, tol=1e-8, maxiter=1500):
+def steady_leg(p, tol=1e-10, maxiter=200):
     """Steady state (cmdtools/psinought equivalent) of the same workload in FP64:
     wall time including the host eigendecomposition of H(σ₀), iterations, relative
     residual, and the fixed-point check max|dΨ| / max|b|/ħ from the device RHS."""

@@ -111,17 +111,36 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
 int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double width,
                       const double* sig_eq);
 
+/* Preconditioner of zh_steady_state: (chi+_i + Delta - H + B_i)^-1 per mode i, with
+ * H = H(t0, sigma0) through its eigenpairs (evecs (NK,n,n) complex128, column k =
+ * vector k; evals (NK,n) real) and, optionally, the lead self-energy term
+ * B_i = sum_j xi_j Ixi_j^T i dGM_j / (chi-_j - chi+_i) of psinought's block
+ * preconditioner (Zandpack/cmdtools/psinought:256-301) by the Woodbury identity on
+ * the ncr orbitals corb: per group g of modes sharing chi+ (group: (NK, NL, NX, NOT),
+ * reference mode order, ids < ngroups), K (ngroups, ncr, ncr) complex128 =
+ * C_g (I + G_g C_g)^-1 with C_g the corb block of B and G_g that of
+ * (chi+_g - H)^-1.  ncr = 0 leaves the self-energy out. */
+typedef struct zh_precond {
+  const double* evecs;
+  const double* evals;
+  int ncr;
+  const int* corb;
+  int ngroups;
+  const int* group;
+  const double* K;
+} zh_precond;
+
 /* Steady state of Zandpack/cmdtools/psinought (:180-449): Psi0 solving psinought's
  * real-linear system (Av3 / B3, :199-239) for the resident sigma taken as sigma0, the
  * bias at t0 and H(t0, sigma0) (Hubbard term included), and the closed-form Omega0
  * (:447-449); (sigma0, Psi0, Omega0) becomes the resident state.  BiCGSTAB over the
- * real inner product to ||b - A Psi|| <= tol ||b||, at most maxiter iterations; with
- * the eigenpairs of H(t0, sigma0) (evecs (NK,n,n) complex128, column k = vector k;
- * evals (NK,n) real; both NULL for none) it is right-preconditioned with
- * (chi+_i + Delta - H)^-1.  Returns ZH_ELIMIT when not converged (the state then
- * holds the last iterate); iters and rel_resid may be NULL. */
-int zh_steady_state(zh_problem* p, double t0, double tol, int maxiter, const double* evecs,
-                    const double* evals, int* iters, double* rel_resid);
+ * real inner product to ||b - A Psi|| <= tol ||b||, at most maxiter iterations,
+ * right-preconditioned with pc (NULL: none; the bare operator's real spectrum
+ * surrounds zero and rarely converges).  Returns ZH_ELIMIT when not converged (the
+ * state then holds the last iterate); iters and rel_resid may be NULL and are set
+ * in both cases. */
+int zh_steady_state(zh_problem* p, double t0, double tol, int maxiter, const zh_precond* pc,
+                    int* iters, double* rel_resid);
 
 /* Resident state y0 = (sigma, Psi, Omega), reference layout, complex128. */
 int zh_state_put(zh_problem* p, const double* sig, const double* psi, const double* omg);

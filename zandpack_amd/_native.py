@@ -31,6 +31,12 @@ class zh_dims(C.Structure):
                 ("device", C.c_int)]
 
 
+class zh_precond(C.Structure):
+    _fields_ = [("evecs", C.POINTER(C.c_double)), ("evals", C.POINTER(C.c_double)),
+                ("ncr", C.c_int), ("corb", C.POINTER(C.c_int)), ("ngroups", C.c_int),
+                ("group", C.POINTER(C.c_int)), ("K", C.POINTER(C.c_double))]
+
+
 class zh_attempt_info(C.Structure):
     _fields_ = [("t0", C.c_double), ("h", C.c_double), ("eps", C.c_double),
                 ("force_accept", C.c_int), ("te", C.c_double), ("h_next", C.c_double),
@@ -64,8 +70,8 @@ SIGNATURES = {
     "zh_set_layout": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_layout_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_sparse": (C.c_int, [C.c_void_p, C.c_int]),
-    "zh_steady_state": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp,
-                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "zh_steady_state": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                  C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "zh_set_dissipator": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                     _dp]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

@@ -92,8 +92,8 @@ struct EngineBase {
   virtual void state_put(const double* sig, const double* psi, const double* omg) = 0;
   virtual void state_get(double* sig, double* psi, double* omg) = 0;
   virtual void sigma_get(double* sig) = 0;
-  virtual void steady_state(double t0, double tol, int maxiter, const double* evecs,
-                            const double* evals, int* iters, double* resid) = 0;
+  virtual void steady_state(double t0, double tol, int maxiter, const zh_precond* pc,
+                            int* iters, double* resid) = 0;
   virtual void rhs(const double* Ht, const double* delta, const double* sig, const double* psi,
                    const double* omg, double* dsig, double* dpsi, double* domg, double* J) = 0;
   virtual void attempt(zh_attempt_info* info, double* J) = 0;
@@ -1314,8 +1314,8 @@ struct Engine : EngineBase {
   // real-linear, ψ* enters), right-preconditioned with (χ⁺_i + Δ - H)^{-1} from the
   // eigenpairs of H(t0, σ₀) when given (psinought's block preconditioner without
   // its coupling terms).  Writes (σ₀, Ψ₀, Ω₀) as the resident state.
-  void steady_state(double t0, double tol, int maxiter, const double* evecs, const double* evals,
-                    int* iters, double* resid) override {
+  void steady_state(double t0, double tol, int maxiter, const zh_precond* pc, int* iters,
+                    double* resid) override {
     require(have_consts && have_state, "steady state needs constants and a resident sigma",
             ZH_ESTATE);
     require(tol > 0.0 && maxiter > 0, "tolerance and iteration limit must be positive");
@@ -1344,7 +1344,18 @@ struct Engine : EngineBase {
             *tv = buf.get<double2>(N), *bv = buf.get<double2>(N);
     cs* hpsi = buf.get<cs>(N);
     double* part = buf.get<double>(kDotBlocks);
-    const bool prec = evecs && evals;
+    const bool prec = pc && pc->evecs && pc->evals;
+    const double* evecs = prec ? pc->evecs : nullptr;
+    const double* evals = prec ? pc->evals : nullptr;
+    const int ncr = prec && pc->ncr > 0 ? pc->ncr : 0;
+    if (ncr) {
+      require(pc->corb && pc->group && pc->K && pc->ngroups > 0, "incomplete preconditioner");
+      for (int c = 0; c < ncr; ++c)
+        require(pc->corb[c] >= 0 && pc->corb[c] < n, "preconditioner orbital out of range");
+      for (long m = 0; m < (long)NK * M; ++m)
+        require(pc->group[m] >= 0 && pc->group[m] < pc->ngroups,
+                "preconditioner group out of range");
+    }
     double2 *ph = pv, *sh = sv;
     cs *Vc = nullptr, *Vt = nullptr, *y1 = nullptr, *y2 = nullptr;
     double* ev = nullptr;
@@ -1367,6 +1378,40 @@ struct Engine : EngineBase {
       HIPCHK(hipMemcpyAsync(Vc, hc.data(), sizeof(cs) * hc.size(), hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(Vt, ht.data(), sizeof(cs) * ht.size(), hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(ev, evals, sizeof(double) * NK * n, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    // self-energy correction: V_c (n x ncr, V[corb_c][k] at (k, c)), conj V_c (ncr x n),
+    // K per group, group per device mode
+    cs *VcT = nullptr, *Vcc = nullptr, *Zc = nullptr, *Wc = nullptr;
+    double2* Kd = nullptr;
+    int* gdev = nullptr;
+    if (ncr) {
+      VcT = buf.get<cs>((size_t)NK * n * ncr);
+      Vcc = buf.get<cs>((size_t)NK * ncr * n);
+      Zc = buf.get<cs>((size_t)NK * Md * ncr);
+      Wc = buf.get<cs>((size_t)NK * Md * ncr);
+      Kd = buf.get<double2>((size_t)pc->ngroups * ncr * ncr);
+      gdev = buf.get<int>((size_t)NK * Md);
+      std::vector<cs> a((size_t)NK * n * ncr), c((size_t)NK * ncr * n);
+      for (int b = 0; b < NK; ++b)
+        for (int q = 0; q < ncr; ++q)
+          for (int k = 0; k < n; ++k) {
+            const size_t at = ((size_t)b * n + pc->corb[q]) * n + k;
+            a[((size_t)b * n + k) * ncr + q] =
+                from_d2<cs>(make_double2(evecs[2 * at], evecs[2 * at + 1]));
+            c[((size_t)b * ncr + q) * n + k] =
+                from_d2<cs>(make_double2(evecs[2 * at], -evecs[2 * at + 1]));
+          }
+      std::vector<int> gd((size_t)NK * Md, -1);
+      for (int b = 0; b < NK; ++b)
+        for (int d = 0; d < Md; ++d)
+          if (perm[d] >= 0) gd[(size_t)b * Md + d] = pc->group[(size_t)b * M + perm[d]];
+      HIPCHK(hipMemcpyAsync(VcT, a.data(), sizeof(cs) * a.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(Vcc, c.data(), sizeof(cs) * c.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(Kd, pc->K, sizeof(double2) * pc->ngroups * ncr * ncr,
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(gdev, gd.data(), sizeof(int) * gd.size(), hipMemcpyHostToDevice,
+                            stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
     const dim3 eg(2048), eb(256);
@@ -1440,6 +1485,20 @@ struct Engine : EngineBase {
       EpiPrecScale<cs> e1{y2, ev, chip, lead, sp0, Md, n};
       HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g1, NK, e1, stream, KFull(),
                                                    typename Cands<R>::psi())));
+      if (ncr) {
+        // Woodbury: y2 -= D V† U K Uᵀ V y2 (U: the corb orbitals)
+        GemmArgs<R, R> ga{Md, ncr, n, y2, n, (long)Md * n, VcT, ncr, (long)n * ncr};
+        EpiStore<cs> ea{Zc, ncr, (long)Md * ncr, 1.0, 0.0};
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], ga, NK, ea, stream, KFull(),
+                                                     typename Cands<R>::psi())));
+        hipLaunchKernelGGL(k_group_matvec<cs>, eg, eb, 0, stream, NK, Md, ncr, (const int*)gdev,
+                           (const double2*)Kd, (const cs*)Zc, Wc);
+        HIPCHK(hipGetLastError());
+        GemmArgs<R, R> gb{Md, n, ncr, Wc, ncr, (long)Md * ncr, Vcc, n, (long)ncr * n};
+        EpiPrecCorr<cs> eb2{y2, ev, chip, lead, sp0, Md, n};
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], gb, NK, eb2, stream, KFull(),
+                                                     typename Cands<R>::psi())));
+      }
       GemmArgs<R, R> g2{Md, n, n, y2, n, (long)Md * n, Vt, n, (long)n * n};
       EpiStore<double2> e2{zout, n, (long)Md * n, 1.0, 0.0};
       HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g2, NK, e2, stream, KFull(),
@@ -1666,11 +1725,11 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
   });
 }
 
-int zh_steady_state(zh_problem* p, double t0, double tol, int maxiter, const double* evecs,
-                    const double* evals, int* iters, double* rel_resid) {
+int zh_steady_state(zh_problem* p, double t0, double tol, int maxiter, const zh_precond* pc,
+                    int* iters, double* rel_resid) {
   return zh_try([&] {
     require(p, "null argument");
-    p->eng->steady_state(t0, tol, maxiter, evecs, evals, iters, rel_resid);
+    p->eng->steady_state(t0, tol, maxiter, pc, iters, rel_resid);
   });
 }
 

@@ -980,6 +980,49 @@ __global__ __launch_bounds__(256) void k_dot(long total, const double2* x, const
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Self-energy correction of the preconditioner (Woodbury on the corb orbitals):
+// Wc_i = K_{g(i)} Zc_i for every mode row i (g < 0: padding mode, Wc_i = 0).
+template <typename C>
+__global__ void k_group_matvec(int NK, int M, int ncr, const int* group, const double2* K,
+                               const C* Zc, C* Wc) {
+  const long total = (long)NK * M * ncr;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long bm = idx / ncr;
+    const int c = (int)(idx - bm * ncr);
+    const int g = group[bm];
+    double wr = 0.0, wi = 0.0;
+    if (g >= 0) {
+      const double2* Kr = K + ((long)g * ncr + c) * ncr;
+      const C* z = Zc + bm * ncr;
+      for (int q = 0; q < ncr; ++q) {
+        const double2 k = Kr[q], v = to_d2(z[q]);
+        wr += k.x * v.x - k.y * v.y;
+        wi += k.x * v.y + k.y * v.x;
+      }
+    }
+    Wc[idx] = from_d2<C>(make_double2(wr, wi));
+  }
+}
+
+// y[i, k] -= T[i, k] / (χ⁺_i + Δ - e_k)   (T from the GEMM; y read and written here)
+template <typename C>
+struct EpiPrecCorr {
+  C* y; const double* eval; const double2* chip; const int* lead; const StageParams* sp;
+  int M, n;
+  __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    double2 chi = chip[(long)b * M + row];
+    chi.x += sp->delta[lead[row]] - eval[(long)b * n + col];
+    const double den = chi.x * chi.x + chi.y * chi.y;
+    const long off = ((long)b * M + row) * n + col;
+    if (den > 0.0) {
+      const double2 yv = to_d2(y[off]);
+      const double qr = (re * chi.x + im * chi.y) / den, qi = (im * chi.x - re * chi.y) / den;
+      y[off] = from_d2<C>(make_double2(yv.x - qr, yv.y - qi));
+    }
+  }
+};
+
 // Preconditioner (χ⁺_i + Δ - H)^{-1} = V diag(1/(χ⁺_i + Δ - e_k)) V†, applied as
 // Y = R conj(V) (GEMM), this scaling, Z = Y Vᵀ (GEMM).
 template <typename C>

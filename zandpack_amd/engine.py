@@ -58,21 +58,32 @@ class Propagator:
                                       float(pulse.p1), float(pulse.p2), N.ptr(w)))
 
     def steady_state(self, t0=0.0, tol=1e-10, maxiter=500, precondition=True,
-                     raise_on_limit=True):
+                     self_energy=True, raise_on_limit=True):
         """Ψ₀, Ω₀ of cmdtools/psinought for the resident σ (as σ₀), the bias at t0 and
         H(t0, σ₀); they become the resident state.  Returns (iterations, relative
-        residual).  The preconditioner's eigenpairs of H(t0, σ₀) = H0 + dH(t0, σ₀)
-        come from a host eigendecomposition (LAPACK).  With raise_on_limit=False an
-        unconverged solve returns its counts instead of raising."""
-        ev = vec = None
+        residual).  The preconditioner (χ⁺_i + Δ - H + B_i)^{-1} is built on the host:
+        eigenpairs of H(t0, σ₀) = H0 + dH(t0, σ₀) (LAPACK) and, with self_energy, the
+        Woodbury factors of the lead self-energy B_i (``steady_preconditioner``).  With
+        raise_on_limit=False an unconverged solve returns its counts instead of raising."""
+        keep = []
+        pc = None
         if precondition:
             s0 = self.sigma()
             Hs = np.asarray(self.problem.H0 + self.problem.dH(t0, s0), complex)
-            e, v = np.linalg.eigh(Hs)
-            ev, vec = np.ascontiguousarray(e, np.float64), np.ascontiguousarray(v, np.complex128)
+            pre = steady_preconditioner(self.problem, Hs, t0, self_energy)
+            keep = list(pre.values())
+            pc = N.zh_precond(N.ptr(pre["evecs"]), N.ptr(pre["evals"]), 0, None, 0, None, None)
+            if self_energy and pre["corb"].size:
+                pc.ncr = int(pre["corb"].size)
+                pc.corb = pre["corb"].ctypes.data_as(C.POINTER(C.c_int))
+                pc.ngroups = int(pre["K"].shape[0])
+                pc.group = pre["group"].ctypes.data_as(C.POINTER(C.c_int))
+                pc.K = N.ptr(pre["K"])
         its, res = C.c_int(0), C.c_double(0.0)
         code = self.lib.zh_steady_state(self.h, float(t0), float(tol), int(maxiter),
-                                        N.ptr(vec), N.ptr(ev), C.byref(its), C.byref(res))
+                                        C.byref(pc) if pc is not None else None,
+                                        C.byref(its), C.byref(res))
+        del keep
         if code != N.ZH_ELIMIT or raise_on_limit:
             N.check(code)
         return its.value, res.value
@@ -219,6 +230,51 @@ class Propagator:
             self.close()
         except Exception:
             pass
+
+
+def steady_preconditioner(problem, H, t0, self_energy=True):
+    """Host factors of the steady-state preconditioner (χ⁺_i + Δ - H + B_i)^{-1}
+    (psinought's block preconditioner, Zandpack/cmdtools/psinought:256-301):
+    eigenpairs of H per k, and for each group g of modes with one χ⁺ the Woodbury
+    matrix K_g = C_g (I + G_g C_g)^{-1}, where on the coupled orbitals corb
+      C_g = Σ_j ξ_j Ixi_jᵀ · i ΔM_j / (χ⁻_j - χ⁺_g)   (the T1 part of the operator)
+      G_g = [(χ⁺_g - H)^{-1}]_corb,corb.
+    χ± include the bias at t0."""
+    NK, n, NL, NX, NOT = problem.dims
+    M = problem.M
+    e, v = np.linalg.eigh(np.asarray(H, complex))
+    out = dict(evecs=np.ascontiguousarray(v, np.complex128),
+               evals=np.ascontiguousarray(e, np.float64),
+               corb=np.zeros(0, np.int32))
+    if not self_energy:
+        return out
+    xi = problem.xi.reshape(NK, M, n)
+    Ixi = problem.Ixi.reshape(NK, M, n)
+    corb = np.nonzero(np.any(xi.reshape(-1, n) != 0, axis=0) |
+                      np.any(Ixi.reshape(-1, n) != 0, axis=0))[0].astype(np.int32)
+    if corb.size == 0:
+        return out
+    lead = np.repeat(np.arange(NL), NX * NOT)
+    dl = np.array([problem.bias(t0, a) for a in range(NL)])[lead]
+    xpp = problem.Xpp.reshape(NK, M) + dl
+    xpm = problem.Xpm.reshape(NK, M) + dl
+    dgm = problem.dGM.reshape(NK, M)
+    group = np.zeros((NK, M), np.int32)
+    Ks = []
+    for k in range(NK):
+        vals, inv = np.unique(xpp[k], return_inverse=True)
+        group[k] = inv + len(Ks)
+        Xc = xi[k][:, corb].T            # (ncr, M)
+        Ic = Ixi[k][:, corb].T
+        Vc = v[k][corb]                  # (ncr, n)
+        for chi in vals:
+            f = 1j * dgm[k] / (xpm[k] - chi)
+            Cg = (Xc * f) @ Ic.T
+            Gg = (Vc / (chi - e[k])) @ Vc.conj().T
+            Ks.append(Cg @ np.linalg.inv(np.eye(corb.size) + Gg @ Cg))
+    out.update(corb=np.ascontiguousarray(corb), group=np.ascontiguousarray(group),
+               K=np.ascontiguousarray(np.array(Ks), np.complex128))
+    return out
 
 
 def selftest_gemm(A, B, transA=False, transB=False, precision="fp64"):
