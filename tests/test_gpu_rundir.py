@@ -1,0 +1,97 @@
+"""zand run directories on the device engine: the _save/ files carry the same
+times and currents as the engine's own adaptive loop, a checkpoint resumes the
+trajectory, and the command line runs an Arrays/ directory end to end."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from zandpack_amd import builder, rundir
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _problem():
+    p = builder.config_small(NK=2, seed=3)
+    p.pulse = builder.Pulse("step", amp=0.2, p1=0.0, weights=(0.5, -0.5))
+    return p
+
+
+T1, H0, EPS = 0.6, 0.01, 1e-8
+
+
+def _reference_run(p):
+    from zandpack_amd.engine import Propagator
+    prop = Propagator(p, "fp64")
+    r = prop.propagate(0.0, T1, H0, EPS)
+    prop.close()
+    return r
+
+
+def test_save_dir_matches_engine_loop(native, tmp_path):
+    from zandpack_amd.engine import Propagator
+    p = _problem()
+    say("rundir save", p.dims)
+    ref = _reference_run(p)
+    prop = Propagator(p, "fp64")
+    d = str(tmp_path / "x_save")
+    w = rundir.SaveWriter(d, p.dims[2], saveevery=5, n_dm_compress=1, t0=0.0, sig0=prop.sigma())
+    t, h, acc, rej = rundir.propagate_to_save(prop, w, 0.0, T1, H0, EPS)
+    sig_end = prop.sigma()
+    prop.close()
+    assert acc == ref["n_steps"] and rej == ref["n_rejected"]
+    assert abs(h - ref["h_final"]) <= 1e-10 * abs(ref["h_final"])
+    T, C = rundir.combine_currents([d], p.dims[2])
+    k = len(T)
+    assert k == (acc // 5) * 5 and k > 0
+    np.testing.assert_allclose(T, ref["times"][:k], rtol=1e-10, atol=0)
+    for a in range(p.dims[2]):
+        np.testing.assert_allclose(C[a], ref["J"][:k, a], rtol=1e-10, atol=1e-14)
+    last = max(int(f[2:-4]) for f in os.listdir(d) if f.startswith("DM") and not f.startswith("DMt"))
+    dm = np.load(os.path.join(d, f"DM{last}.npy"))
+    assert dm.dtype == np.complex64 and dm.shape[1:] == sig_end.shape
+    if last == acc:
+        np.testing.assert_allclose(dm[-1], sig_end, atol=1e-6)
+
+
+def test_checkpoint_resume_repeats_trajectory(native, tmp_path):
+    from zandpack_amd.engine import Propagator
+    p = _problem()
+    say("rundir checkpoint", p.dims)
+    ref = _reference_run(p)
+    d = str(tmp_path / "x_save")
+    prop = Propagator(p, "fp64")
+    w = rundir.SaveWriter(d, p.dims[2], saveevery=1000, t0=0.0, sig0=prop.sigma())
+    rundir.propagate_to_save(prop, w, 0.0, T1, H0, EPS, checkpoints=(0.3,))
+    prop.close()
+    t_ck, h_ck, s, ps, om = rundir.load_checkpoint(d)
+    i = int(np.argmin(np.abs(ref["times"] - t_ck)))
+    assert abs(ref["times"][i] - t_ck) <= 1e-12 and t_ck > 0.3 and ref["times"][i - 1] <= 0.3
+    prop = Propagator(p, "fp64", state=(s, ps, om))
+    r = prop.propagate(t_ck, T1, h_ck, EPS)
+    prop.close()
+    n = len(ref["times"]) - i
+    assert r["n_steps"] == n - 1
+    np.testing.assert_allclose(r["times"], ref["times"][i:], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(r["J"], ref["J"][i:], rtol=1e-9, atol=1e-13)
+
+
+def test_command_line_on_arrays_dir(native, tmp_path):
+    p = _problem()
+    say("rundir cli", p.dims)
+    ref = _reference_run(p)
+    rundir.write_arrays(p, str(tmp_path / "x" / "Arrays"))
+    rc = rundir.main([f"Dir={tmp_path}", "name=x", f"t1={T1}", f"stepsize={H0}", f"eps={EPS}",
+                      "saveevery=4", "pulse=step", "amp=0.2", "p1=0", "weights=0.5,-0.5"])
+    assert rc == 0
+    T, C = rundir.combine_currents([str(tmp_path / "x_save")], p.dims[2])
+    k = len(T)
+    assert k > 0
+    np.testing.assert_allclose(T, ref["times"][:k], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(C[0], ref["J"][:k, 0], rtol=1e-9, atol=1e-13)

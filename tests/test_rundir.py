@@ -1,0 +1,57 @@
+"""zand run-directory formats: Arrays/ in, <name>_save/ out (CPU side)."""
+import os
+
+import numpy as np
+import pytest
+
+from zandpack_amd import builder, rundir
+
+
+def test_arrays_round_trip_with_channel_cut(tmp_path):
+    p = builder.config_small(NK=2, seed=5)
+    NK, n, NL, NX, NOT = p.dims
+    # a dead trailing channel (GG_P = GL_P = 0) that zand's NOT cut removes
+    pad = lambda a: np.concatenate([a, np.zeros(a.shape[:3] + (1,) + a.shape[4:], a.dtype)], axis=3)
+    q = builder.AMEProblem(H0=p.H0, sigma0=p.sigma0, xi=pad(p.xi), Ixi=pad(p.Ixi),
+                           GG_P=pad(p.GG_P), GL_P=pad(p.GL_P), GG_M=pad(p.GG_M), GL_M=pad(p.GL_M),
+                           Xpp=pad(p.Xpp), Xpm=pad(p.Xpm))
+    rundir.write_arrays(q, str(tmp_path / "Arrays"), zero_tol=1e-12)
+    assert (tmp_path / "Arrays" / "xi.npz").exists() and (tmp_path / "Arrays" / "H_Ortho.npy").exists()
+    r = rundir.load_arrays(str(tmp_path / "Arrays"))
+    assert r.dims == p.dims
+    for k in ("H0", "sigma0", "xi", "Ixi", "GG_P", "GL_P", "GG_M", "GL_M", "Xpp", "Xpm"):
+        assert np.array_equal(getattr(r, k), getattr(p, k)), k
+
+
+def test_not_per_lead_known_answer():
+    GG = np.zeros((1, 2, 3, 5))
+    GL = np.zeros((1, 2, 3, 5))
+    GG[0, 0, 1, 2] = 1.0      # lead 0: channel 2 live -> 3
+    GL[0, 1, 2, 4] = 1e-3     # lead 1: channel 4 live -> 5
+    GG[0, 1, 0, 0] = 1e-20    # below the tolerance
+    assert list(rundir.not_per_lead(GG, GL, 1e-12)) == [3, 5]
+
+
+def test_save_writer_bookkeeping_and_reader(tmp_path):
+    NL, NK, n = 2, 1, 3
+    d = str(tmp_path / "x_save")
+    rng = np.random.default_rng(1)
+    sig0 = rng.normal(size=(NK, n, n)) + 0j
+    w = rundir.SaveWriter(d, NL, saveevery=4, n_dm_compress=2, t0=0.0, sig0=sig0)
+    Js, ts = [], [0.0]
+    for s in range(10):
+        t = 0.1 * (s + 1)
+        J = rng.normal(size=(NL, NK))
+        Js.append(J)
+        ts.append(t)
+        w.step(t, sig0 + s, J)
+    files = sorted(os.listdir(d))
+    assert "DM4.npy" in files and "DM8.npy" in files and "DM12.npy" not in files
+    assert np.load(os.path.join(d, "DM4.npy")).dtype == np.complex64
+    assert np.load(os.path.join(d, "DM4.npy")).shape[0] == 3      # σ0 + 4 steps, every 2nd
+    assert np.load(os.path.join(d, "DM8.npy")).shape[0] == 2      # 4 steps, every 2nd
+    T, C = rundir.combine_currents([d], NL)
+    # zand's reader: one time per saved DM, minus the trailing one
+    assert np.allclose(T, ts[:8])
+    for a in range(NL):
+        assert np.allclose(C[a], np.array([J[a] for J in Js[:8]]))

@@ -1,0 +1,247 @@
+"""Zand run directories on the GPU engine.
+
+Reads the ``<name>/Arrays/`` inputs of ``Zandpack/mpi/zand`` and writes its
+``<name>_save/`` outputs with the same file names, dtypes and bookkeeping, so a
+directory prepared for ``mpirun -np P zand Dir=<dir>`` runs here unchanged and
+the reference's post-processing (``mpi_tools.combine_currents``, ``td_info``)
+reads the results:
+
+    python -m zandpack_amd.rundir Dir=<dir> name=<name> t1=50 eps=1e-8 stepsize=0.01 \\
+        [t0=0 saveevery=100 n_dm_compress=1 usesave=False checkpoints=10,20 \\
+         precision=fp64 hbar=0.6582119569 pulse=step amp=0.1 p1=0 p2=0 weights=0.5,-0.5 \\
+         hubbard_U=0 hubbard_n0=<file.npy> steady_state=False]
+
+Keywords follow ``Initial.py`` (Zandpack/mpi/zand:57-71).  The ``Bias.py`` hooks
+(arbitrary Python) are replaced by the device models (``pulse*``, ``hubbard_*``);
+a run that needs an arbitrary ``dH``/``bias`` uses the in-process drivers of
+``zandpack_amd.drop_in`` instead.
+"""
+from __future__ import annotations
+
+import os
+import re
+import sys
+
+import numpy as np
+
+from .builder import AMEProblem, HBAR, Pulse
+
+ARRAY_KEYS = ("H_Ortho", "DM_Ortho", "psi_shape", "GG_P", "GL_P", "GG_M", "GL_M", "Xpp", "Xpm",
+              "xi", "Ixi", "zero_tol")
+
+
+def flexload(path):
+    """Array at ``path`` (.npy, an .npz's first array, or the .npz twin of a missing
+    .npy), as Zandpack/Loader.py:12-32 resolves it.  Loads never unpickle."""
+    for p in (path, path.replace(".npy", ".npz")):
+        if not os.path.exists(p):
+            continue
+        f = np.load(p, allow_pickle=False)
+        if isinstance(f, np.ndarray):
+            return f
+        with f:
+            return f[f.files[0]]
+    raise FileNotFoundError(path)
+
+
+def not_per_lead(GG_P, GL_P, zero_tol):
+    """Channels kept per lead: one past the last channel index whose GG_P or GL_P
+    exceeds zero_tol for some (k, pole) (Zandpack/mpi_splitter.py:555-580)."""
+    nk, nl, nx, nc = GG_P.shape
+    live = (np.abs(GG_P).max(axis=0) >= zero_tol) | (np.abs(GL_P).max(axis=0) >= zero_tol)
+    out = []
+    for a in range(nl):
+        idx = np.nonzero(live[a].any(axis=0))[0]
+        out.append(int(idx.max()) + 1 if idx.size else 0)
+    return np.array(out)
+
+
+def load_arrays(arrays_dir, hbar=HBAR, name="problem"):
+    """AMEProblem from a zand ``Arrays/`` directory, channels cut to NOT = max over
+    leads of ``not_per_lead`` (Zandpack/mpi/zand:113-124, 218, 568-576)."""
+    d = {k: flexload(os.path.join(arrays_dir, k + ".npy")) for k in ARRAY_KEYS}
+    zt = float(np.asarray(d["zero_tol"]))
+    NOT = int(not_per_lead(d["GG_P"], d["GL_P"], zt).max())
+    cut = lambda a: np.ascontiguousarray(np.asarray(a)[:, :, :, :NOT], dtype=complex)
+    return AMEProblem(H0=np.asarray(d["H_Ortho"], complex), sigma0=np.asarray(d["DM_Ortho"], complex),
+                      xi=cut(d["xi"]), Ixi=cut(d["Ixi"]), GG_P=cut(d["GG_P"]), GL_P=cut(d["GL_P"]),
+                      GG_M=cut(d["GG_M"]), GL_M=cut(d["GL_M"]), Xpp=cut(d["Xpp"]),
+                      Xpm=cut(d["Xpm"]), hbar=hbar, name=name)
+
+
+def write_arrays(problem, arrays_dir, zero_tol=1e-12):
+    """The inputs zand reads, under the names Zandpack/Writer.py:61-99 writes
+    (xi and Ixi compressed, as ``arr_0``), for a problem built here."""
+    os.makedirs(arrays_dir, exist_ok=True)
+    j = lambda k: os.path.join(arrays_dir, k)
+    NK, n, NL, NX, NOT = problem.dims
+    np.save(j("H_Ortho"), problem.H0)
+    np.save(j("DM_Ortho"), problem.sigma0)
+    np.save(j("psi_shape"), np.array([NK, NL, NX, NOT, n]))
+    np.save(j("omg_shape"), np.array([NK, NL, NX, NOT, NL, NX, NOT]))
+    for k in ("Xpp", "Xpm", "GG_P", "GL_P", "GG_M", "GL_M"):
+        np.save(j(k), getattr(problem, k))
+    np.save(j("diff_GGP_GLP"), problem.dGP)
+    np.save(j("diff_GGM_GLM"), problem.dGM)
+    np.save(j("num_leads"), NL)
+    np.save(j("noT"), NOT)
+    np.save(j("zero_tol"), zero_tol)
+    np.savez_compressed(j("xi"), problem.xi)
+    np.savez_compressed(j("Ixi"), problem.Ixi)
+
+
+class SaveWriter:
+    """``<name>_save/`` as zand writes it (Zandpack/mpi/zand:256-265, 412-460):
+    every ``saveevery`` accepted steps DM<step> (complex64, every n_dm_compress-th),
+    DMt<step>, times<step>, current_<a>_<step> ((steps, NK) per lead); checkpoints
+    last_sig, last_time, last_stepsize and the full last_psi / last_omg that
+    ``LoadFromFull`` resumes from (:535-542)."""
+
+    def __init__(self, save_dir, NL, saveevery=100, n_dm_compress=1, t0=0.0, sig0=None):
+        self.dir = save_dir
+        self.NL = NL
+        self.saveevery = int(saveevery)
+        self.n = int(n_dm_compress)
+        os.makedirs(save_dir, exist_ok=True)
+        for f in os.listdir(save_dir):  # a new run drops earlier DM files (:262-265)
+            if f.startswith("DM"):
+                os.remove(os.path.join(save_dir, f))
+        self.steps = 0
+        self.currents = [[] for _ in range(NL)]
+        self.DM = [] if sig0 is None else [np.array(sig0)]
+        self.times = [float(t0)]
+
+    def _p(self, f):
+        return os.path.join(self.dir, f)
+
+    def step(self, t, sig, J):
+        """One accepted step: the new time, the new σ and the stage-0 currents J (NL, NK)."""
+        for a in range(self.NL):
+            self.currents[a].append(np.array(J[a], float))
+        self.steps += 1
+        self.DM.append(np.array(sig))
+        self.times.append(float(t))
+        if self.steps % self.saveevery == 0 and self.steps >= self.saveevery:
+            self.flush()
+
+    def flush(self):
+        s, L = self.steps, len(self.DM)
+        np.save(self._p(f"DM{s}"), np.array(self.DM[::self.n]).astype(np.complex64))
+        dm_t = np.array(self.times[-L:])
+        if len(dm_t) > L:
+            dm_t = dm_t[1:]
+        np.save(self._p(f"DMt{s}"), dm_t[::self.n])
+        np.save(self._p(f"times{s}"), self.times[-L:])
+        for a, c in enumerate(self.currents):
+            np.save(self._p(f"current_{a}_{s}"), np.array(c[-L:]))
+        self.DM = []
+
+    def checkpoint(self, t, h, sig, psi, omg):
+        np.save(self._p("last_sig"), sig)
+        np.save(self._p("last_time"), float(t))
+        np.save(self._p("last_stepsize"), float(h))
+        np.save(self._p("last_psi"), psi)
+        np.save(self._p("last_omg"), omg)
+
+
+def load_checkpoint(save_dir):
+    """(t, h, σ, Ψ, Ω) written by ``SaveWriter.checkpoint`` (or zand's full-state files)."""
+    p = lambda f: os.path.join(save_dir, f)
+    return (float(np.load(p("last_time.npy"))), float(np.load(p("last_stepsize.npy"))),
+            np.load(p("last_sig.npy")), np.load(p("last_psi.npy")), np.load(p("last_omg.npy")))
+
+
+def propagate_to_save(prop, writer, t0, t1, h, eps, checkpoints=(), time_update="reference",
+                      max_attempts=10 ** 7):
+    """zand's outer loop (Zandpack/mpi/zand:297-431) on the device engine with zand's
+    outputs: currents from stage 0 of each accepted attempt, t += h_next after
+    acceptance (``time_update='correct'``: t += h).  Once t has passed a checkpoint
+    time, the state at the start of the next accepted step, that t and the step size
+    taken from it are saved (:412-420), so resuming repeats that step.
+    Returns (t, h, accepted, rejected)."""
+    ck = np.array(sorted(c for c in checkpoints if t0 < c < t1), float)
+    done = np.zeros(len(ck), bool)
+    t, acc, rej, attempts = float(t0), 0, 0, 0
+    while t < t1:
+        due = (t > ck) & ~done
+        snap = prop.get_state() if due.any() else None
+        while True:
+            attempts += 1
+            if attempts > max_attempts:
+                raise RuntimeError("attempt limit reached")
+            r = prop.attempt(t, h, eps)
+            if not np.isfinite(r["te"]):
+                raise FloatingPointError("non-finite error estimate")
+            h_old, h = h, r["h_next"]
+            if r["accepted"]:
+                break
+            rej += 1
+        acc += 1
+        if snap is not None:
+            writer.checkpoint(t, h_old, *snap)
+            done[np.nonzero(due)[0][0]] = True
+        t = t + (h if time_update == "reference" else h_old)
+        writer.step(t, prop.sigma(), r["J"])
+    return t, h, acc, rej
+
+
+def _parse(argv):
+    kw = {}
+    for a in argv:
+        k, _, v = a.replace(" ", "").partition("=")
+        kw[k] = v
+    return kw
+
+
+def main(argv=None):
+    from .engine import Propagator
+    kw = _parse(sys.argv[1:] if argv is None else argv)
+    run_dir = kw.pop("Dir", ".")
+    name = kw.pop("name")
+    f = lambda k, d: float(kw.get(k, d))
+    b = lambda k, d: str(kw.get(k, d)).lower() in ("1", "true", "yes")
+    hbar = f("hbar", HBAR)
+    p = load_arrays(os.path.join(run_dir, name, "Arrays"), hbar=hbar, name=name)
+    p.pulse = Pulse(kw.get("pulse", "none"), f("amp", 0.0), f("p1", 0.0), f("p2", 0.0),
+                    tuple(float(x) for x in kw.get("weights", "0.5,-0.5").split(",")))
+    if f("hubbard_U", 0.0):
+        p.hubbard_U = f("hubbard_U", 0.0)
+        p.hubbard_n0 = np.load(kw["hubbard_n0"], allow_pickle=False)
+    save_dir = os.path.join(run_dir, name + "_save")
+    t0, t1, h = f("t0", 0.0), f("t1", 10.0), f("stepsize", 0.01)
+    state = None
+    if b("usesave", False):
+        t0, h, s, ps, om = load_checkpoint(save_dir)
+        state = (s, ps, om)
+    prop = Propagator(p, kw.get("precision", "fp64"), state=state, autotune=True)
+    if state is None and b("steady_state", False):
+        its, res = prop.steady_state(t0)
+        print(f"steady state: {its} iterations, relative residual {res:.2e}", flush=True)
+    ck = [float(x) for x in kw.get("checkpoints", "").split(",") if x]
+    w = SaveWriter(save_dir, p.dims[2], int(f("saveevery", 100)), int(f("n_dm_compress", 1)),
+                   t0, prop.sigma())
+    t, h, acc, rej = propagate_to_save(prop, w, t0, t1, h, f("eps", 1e-8), ck,
+                                       kw.get("time_update", "reference"))
+    print(f"t={t:.6g} accepted={acc} rejected={rej} h={h:.6g}", flush=True)
+    prop.close()
+    return 0
+
+
+def combine_currents(save_dirs, NL=2):
+    """The reading side of the format, as Zandpack/mpi_tools.py:19-49 reads it: the
+    current files of each directory in natural step order, stacked; the times files
+    concatenated with one trailing time dropped per directory."""
+    nat = lambda s: [int(c) if c.isdigit() else c for c in re.split(r"(\d+)", s)]
+    C = [[] for _ in range(NL)]
+    T = []
+    for d in save_dirs:
+        fs = sorted((v for v in os.listdir(d) if v.startswith("current_")), key=nat)
+        for ff in fs:
+            C[int(ff.split("_")[1])].append(np.load(os.path.join(d, ff)))
+        for ff in sorted((v for v in os.listdir(d) if v.startswith("times")), key=nat):
+            T.append(np.load(os.path.join(d, ff)))
+    return np.hstack(T)[:-len(save_dirs)], [np.vstack(c) for c in C]
+
+
+if __name__ == "__main__":
+    sys.exit(main())
