@@ -111,6 +111,17 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
 int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double width,
                       const double* sig_eq);
 
+/* Non-orthogonal basis (Zandpack/mpi/nozand): Sinv (NK, n, n) complex128 is S^-1
+ * (nozand's Sm1 = S^(-0.5) S^(-0.5), :259-265); H0, xi, Ixi, sigma and psi are then
+ * in the non-orthogonal basis.  S^-1 enters as nozand applies it: S^-1 H sigma in
+ * the commutator (:456-461), Pi_a S^-1 in the sigma equation (:489-497), S^-1 H on
+ * psi (:1075-1085), S^-1 on the GL_P xi constant (:1003) and on Omega xi
+ * (:1208-1210); Omega, the currents and the error norm are unchanged in form.  An
+ * explicit Ht (zh_rhs) is multiplied by S^-1 on the device.  Call after
+ * zh_set_constants; NULL returns to the orthogonal form.  Not combined with the
+ * device Hubbard term or zh_steady_state (ZH_EINVAL / ZH_ESTATE). */
+int zh_set_overlap(zh_problem* p, const double* Sinv);
+
 /* Preconditioner of zh_steady_state: (chi+_i + Delta - H + B_i)^-1 per mode i, with
  * H = H(t0, sigma0) through its eigenpairs (evecs (NK,n,n) complex128, column k =
  * vector k; evals (NK,n) real) and, optionally, the lead self-energy term

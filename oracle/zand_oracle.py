@@ -14,6 +14,7 @@ its orthogonal MPI propagator, written as one process:
 * DOP54 tables, controller -- Zandpack/mpi_RK4pars_dev.py:118-149
 * loop form of the RHS     -- Zandpack/TimedependentTransport.py:2798-2873
 * Hu poles / Fermi expansion -- Zandpack/PadeDecomp.py:36-42, 84-152
+* non-orthogonal form      -- Zandpack/mpi/nozand:456-497, 1003, 1075-1085, 1208-1210
 
 Array layout is the reference's (per k):
 ``sigma (NK,n,n)``, ``psi (NK,NL,NX,NOT,n)``, ``omega (NK,NL,NX,NOT,NL,NX,NOT)``,
@@ -29,6 +30,9 @@ Pinning (what makes this oracle trustworthy; see tests/test_oracle_*.py):
                          block of dΩ vanishes, von-Neumann closed form when ξ=0,
                          closed-form stationary Ω makes dΩ=0.
 * ``propagate``       -- von-Neumann closed form σ(t)=U σ0 U† to the DOP54 error level.
+* ``rhs(Sinv=...)``   -- the nozand form equals the orthogonal form carried to the
+                         non-orthogonal basis (dσ_n = S^-½ dσ_o S^-½, dΨ_n = S^-½ dΨ_o,
+                         dΩ and J unchanged); S = I reduces it to zand.
 """
 from __future__ import annotations
 
@@ -169,7 +173,7 @@ def coupled_orbitals(consts):
 
 
 def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, prune=False,
-        diss=None):
+        diss=None, Sinv=None):
     """One stage of zand: returns (h*dσ/dt, h*dΨ/dt, h*dΩ/dt).
 
     σ: mpi/zand:340-387; Ψ: mpi/zand:746-778; Ω: mpi/zand:784-809.
@@ -180,6 +184,12 @@ def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, pru
     ``prune=True`` restricts every ξ/Ixi contraction to the coupled orbitals as
     zand does (mpi/zand:589-602, 641-658) -- the same sums, since ξ vanishes
     elsewhere, at the reference's cost.
+
+    ``Sinv`` (NK, n, n) selects the non-orthogonal form of ``mpi/nozand``, where
+    ξ, Ixi, H, σ are in the non-orthogonal basis and S⁻¹ enters four terms:
+    S⁻¹Hσ in the commutator (nozand:456-461), Π_α S⁻¹ in Q (:489-497), S⁻¹H in
+    the Ψ propagator term (:1075-1085), and S⁻¹ on the constant and Ωξ terms of
+    dΨ (:1003, 1208-1210).  The ΔP σξ term, Ω and the currents carry no S⁻¹.
     """
     NK, NL, NX, NOT, n, M = _flat(consts)
     xi = consts["xi"].reshape(NK, M, n)
@@ -196,8 +206,10 @@ def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, pru
     cidx = coupled_orbitals(consts) if prune else slice(None)
     xc, ixc, pc = xi[:, :, cidx], Ixi[:, :, cidx], p[:, :, cidx]
 
-    # σ: D = Hσ; D -= D†; D *= -i  (mpi/zand:340-342)
+    # σ: D = Hσ; D -= D†; D *= -i  (mpi/zand:340-342); nozand: D = S⁻¹(Hσ)
     D = Ht @ sig
+    if Sinv is not None:
+        D = Sinv @ D
     D = -1j * (D - D.conj().transpose(0, 2, 1))
     if diss is not None:
         D = D - hbar * diss
@@ -207,15 +219,24 @@ def rhs(consts, sig, psi, omg, Ht, delta, h=1.0, hbar=HBAR, return_pi=False, pru
         sl = slice(a * NX * NOT, (a + 1) * NX * NOT)
         r = np.zeros((NK, n, n), complex)
         r[:, :, cidx] = p[:, sl, :].transpose(0, 2, 1) @ ixc[:, sl, :]
+        if Sinv is not None:
+            r = r @ Sinv
         Q[:, a] = (r + r.conj().transpose(0, 2, 1)) / hbar
     dsig = (h / hbar) * (D + Q.sum(axis=1))
 
     # Ψ (mpi/zand:746-778)
-    dpsi = p @ (Ht / hbar).transpose(0, 2, 1)
+    HS = Ht if Sinv is None else Sinv @ Ht
+    dpsi = p @ (HS / hbar).transpose(0, 2, 1)
     dpsi -= (xpp / hbar)[..., None] * p
-    dpsi += glp[..., None] * xi
-    dpsi += dgp[..., None] * (xc @ sig[:, :, cidx].transpose(0, 2, 1))
-    dpsi[:, :, cidx] += W @ (xc / hbar ** 2)
+    if Sinv is None:
+        dpsi += glp[..., None] * xi
+        dpsi += dgp[..., None] * (xc @ sig[:, :, cidx].transpose(0, 2, 1))
+        dpsi[:, :, cidx] += W @ (xc / hbar ** 2)
+    else:
+        dpsi += dgp[..., None] * (xc @ sig[:, :, cidx].transpose(0, 2, 1))
+        v = glp[..., None] * xi
+        v[:, :, cidx] += W @ (xc / hbar ** 2)
+        dpsi += v @ Sinv.transpose(0, 2, 1)
     dpsi *= -1j * h
 
     # Ω (mpi/zand:784-809)
@@ -339,7 +360,8 @@ def relaxation_dissipator(eta, sig_eq):
     return diss
 
 
-def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, dissipator=None):
+def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, dissipator=None,
+                  Sinv=None):
     """One 7-stage attempt (mpi/zand:301-396).  Returns (ks, TE, J_stage0, y_new).
 
     ``y0`` = (σ, Ψ, Ω); ``ks[i]`` are the h-scaled stage derivatives.  J is taken
@@ -361,7 +383,7 @@ def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, diss
         Ht = H0 + dH(ts, yi[0])
         delta = np.array([bias(ts, a) for a in range(NL)], float)
         dv = None if dissipator is None else dissipator(ts, yi[0])
-        k = rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar, diss=dv)
+        k = rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar, diss=dv, Sinv=Sinv)
         if i == 0:
             J0 = currents(consts, yi[1], hbar=hbar)
         ks.append(k)
@@ -376,7 +398,8 @@ def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, diss
 
 
 def propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
-              time_update="reference", max_attempts=100000, fixed_h=False, dissipator=None):
+              time_update="reference", max_attempts=100000, fixed_h=False, dissipator=None,
+              Sinv=None):
     """zand's outer loop (mpi/zand:297-431).
 
     Accept iff TE < eps; h <- 0.9 h (eps/TE)^0.2 after every attempt; on accept
@@ -399,7 +422,7 @@ def propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
             if attempts > max_attempts:
                 raise RuntimeError("attempt limit reached")
             ks, TE, J0, y_new = dop54_attempt(consts, y, t, h, H0, dH, bias, NL, hbar, tab,
-                                              dissipator=dissipator)
+                                              dissipator=dissipator, Sinv=Sinv)
             if not np.isfinite(TE):
                 raise FloatingPointError("non-finite error estimate")
             accept = fixed_h or (TE < eps)

@@ -1,0 +1,170 @@
+"""GPU parity of the non-orthogonal (nozand) form against the oracle's, on
+problems carried to a non-orthogonal basis by a banded overlap.  FP64 bar 1e-11
+relative for one evaluation (S⁻¹ adds one product to each term), 1e-10 for
+propagated currents; mixed precision 2e-5 for one evaluation."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import zand_oracle as O
+from zandpack_amd import builder
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _nonortho(p, seed=1, band=2, zero_tol=1e-10):
+    NK, n = p.dims[:2]
+    Sm05 = builder.lowdin_factors(builder.random_overlap(NK, n, band=band, seed=seed))
+    q = builder.to_nonorthogonal(p, Sm05, zero_tol=zero_tol)
+    return q
+
+
+def _complex_h(seed=8):
+    p = builder.config_small(seed=seed)
+    rng = np.random.default_rng(seed)
+    n = p.H0.shape[-1]
+    A = rng.normal(size=(n, n))
+    p.H0 = p.H0 + 0.2j * (A - A.T)[None]
+    return p
+
+
+CASES = {
+    "small": lambda: _nonortho(builder.config_small(seed=1)),
+    "small_nk2": lambda: _nonortho(builder.config_small(NK=2, seed=3), seed=4),
+    "complex_H": lambda: _nonortho(_complex_h(), seed=2),
+    "chain20": lambda: _nonortho(builder.config_chain(), seed=5, band=1),
+    "tile_aligned": lambda: _nonortho(
+        builder.config_small(n_w=8, n_l=3, N_L=4, slots=4, NOT=8, seed=6), seed=6),
+    # short-range overlap: the coupled set grows by the overlap's reach only
+    "block_overlap": lambda: _nonortho(builder.config_small(n_w=6, n_l=3, seed=7), seed=7,
+                                       band=1, zero_tol=1e-3),
+}
+
+
+def _rand_state(p, seed, scale=0.1):
+    rng = np.random.default_rng(seed)
+    NK, n, NL, NX, NOT = p.dims
+    psi = scale * (rng.normal(size=p.xi.shape) + 1j * rng.normal(size=p.xi.shape))
+    shp = (NK, NL, NX, NOT, NL, NX, NOT)
+    omg = scale * (rng.normal(size=shp) + 1j * rng.normal(size=shp))
+    A = rng.normal(size=(NK, n, n)) + 1j * rng.normal(size=(NK, n, n))
+    sig = p.sigma0 + 0.05 * (A + A.conj().transpose(0, 2, 1))
+    return sig, psi, omg
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("layout", ["reference", "pruned"])
+def test_rhs_parity_nonortho(native, case, precision, layout):
+    from zandpack_amd.engine import Propagator
+    q = CASES[case]()
+    sig, psi, omg = _rand_state(q, 7)
+    delta = np.array([0.05, -0.03])
+    ref = O.rhs(q.consts(), sig, psi, omg, q.H0, delta, Sinv=q.Sinv)
+    Jref = O.currents(q.consts(), psi)
+    say("nonortho rhs", case, precision, layout, q.dims)
+    prop = Propagator(q, precision, layout=layout)
+    ds, dp, do, J = prop.rhs(q.H0, delta, sig, psi, omg)
+    info = prop.hamiltonian_info()
+    assert not info["sparse"] and not info["real"]
+    tol = 1e-11 if precision == "fp64" else 2e-5
+    for got, want in zip((ds, dp, do), ref):
+        assert rel(got, want) < tol, (case, precision, rel(got, want))
+    assert rel(J, Jref) < (1e-12 if precision == "fp64" else 1e-5)
+    prop.close()
+
+
+@pytest.mark.parametrize("case", ["small", "small_nk2", "complex_H", "chain20", "tile_aligned"])
+@pytest.mark.parametrize("layout", ["reference", "pruned"])
+def test_attempt_parity_nonortho(native, case, layout):
+    from zandpack_amd.engine import Propagator
+    q = CASES[case]()
+    q.pulse = builder.Pulse("step", amp=0.2, p1=0.0, weights=(0.5, -0.5))
+    y0 = _rand_state(q, 11, scale=0.02)
+    say("nonortho attempt", case, layout)
+    prop = Propagator(q, "fp64", state=y0, layout=layout)
+    t0, h, eps = 0.3, 0.02, 1e-3
+    ks, TE, J0, y_new = O.dop54_attempt(q.consts(), y0, t0, h, q.H0, q.dH, q.bias, q.dims[2],
+                                        Sinv=q.Sinv)
+    r = prop.attempt(t0, h, eps)
+    assert abs(r["te"] - TE) <= 1e-10 * TE
+    assert r["accepted"] == (TE < eps)
+    assert rel(r["J"], J0) < 1e-12
+    want = y_new if r["accepted"] else y0
+    for got, w in zip(prop.get_state(), want):
+        assert rel(got, w) < 1e-11
+    prop.close()
+
+
+def test_propagate_nonortho_chain20(native):
+    """nozand's adaptive loop (error norm in the non-orthogonal basis), 0 -> 2 fs;
+    then mixed precision replaying the accepted (t, h) sequence: J within 1e-6."""
+    from zandpack_amd.engine import Propagator
+    q = CASES["chain20"]()
+    q.pulse = builder.Pulse("step", amp=0.1, p1=0.0, weights=(0.5, -0.5))
+    y0 = q.zero_state()
+    say("nonortho propagate")
+    ref = O.propagate(q.consts(), y0, 0.0, 2.0, 0.01, 1e-8, q.H0, q.dH, q.bias, 2, Sinv=q.Sinv)
+    prop = Propagator(q, "fp64")
+    out = prop.propagate(0.0, 2.0, 0.01, 1e-8)
+    assert out["n_steps"] == len(ref["times"]) - 1
+    assert out["n_rejected"] == ref["n_reject"]
+    np.testing.assert_allclose(out["times"], ref["times"], rtol=1e-10)
+    assert rel(out["J"], ref["J"]) < 1e-10
+    assert rel(prop.sigma(), ref["y"][0]) < 1e-10
+    prop.close()
+    b = Propagator(q, "mixed")
+    Jm = np.array([b.attempt(t, h, 1.0, force_accept=True)["J"]
+                   for t, h in zip(out["times"][:-1], out["h"])])
+    b.close()
+    err = np.abs(Jm - out["J"]).max() / np.abs(out["J"]).max()
+    say("nonortho mixed J rel err", err)
+    assert err < 1e-6
+
+
+def test_identity_overlap_is_orthogonal(native):
+    from zandpack_amd.engine import Propagator
+    p = builder.config_small(NK=2, seed=9)
+    NK, n = p.dims[:2]
+    sig, psi, omg = _rand_state(p, 3)
+    delta = np.array([0.02, -0.04])
+    say("nonortho identity")
+    a = Propagator(p, "fp64")
+    ra = a.rhs(p.H0, delta, sig, psi, omg)
+    a.close()
+    b = Propagator(p, "fp64")
+    b.set_overlap(np.broadcast_to(np.eye(n), (NK, n, n)).astype(complex))
+    rb = b.rhs(p.H0, delta, sig, psi, omg)
+    for x, y in zip(rb, ra):
+        assert rel(x, y) < 1e-13
+    b.set_overlap(None)  # back to the orthogonal form
+    rc = b.rhs(p.H0, delta, sig, psi, omg)
+    for x, y in zip(rc, ra):
+        assert rel(x, y) < 1e-15
+    b.close()
+
+
+def test_overlap_rejections(native):
+    from zandpack_amd.engine import Propagator
+    say("nonortho rejections")
+    q = CASES["small"]()
+    prop = Propagator(q, "fp64")
+    with pytest.raises(Exception, match="orthogonal"):
+        prop.steady_state(0.0)
+    with pytest.raises(Exception, match="overlap"):
+        prop.set_hproducts("sparse")
+    prop.close()
+    h = builder.config_small(NK=2, hubbard_U=2.0, seed=2)
+    h.Sinv = np.broadcast_to(np.eye(h.dims[1]), (2, h.dims[1], h.dims[1])).astype(complex)
+    with pytest.raises(Exception, match="orthogonal"):
+        Propagator(h, "fp64")

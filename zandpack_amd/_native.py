@@ -74,6 +74,7 @@ SIGNATURES = {
                                   C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "zh_set_dissipator": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                     _dp]),
+    "zh_set_overlap": (C.c_int, [C.c_void_p, _dp]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_set_profile": (C.c_int, [C.c_void_p, C.c_int]),

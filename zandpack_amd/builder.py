@@ -260,6 +260,7 @@ class AMEProblem:
     hubbard_U: float = 0.0
     hubbard_n0: np.ndarray | None = None  # (NK, n) reference densities
     dissipator: Dissipator = field(default_factory=Dissipator)
+    Sinv: np.ndarray | None = None  # (NK, n, n) S⁻¹: the non-orthogonal (nozand) form
     hbar: float = HBAR
     name: str = "problem"
     t_end: float = 10.0
@@ -308,6 +309,54 @@ class AMEProblem:
 
     def bias(self, t, a):
         return self.pulse.bias(t, a)
+
+
+def lowdin_factors(S):
+    """S^(-1/2) of a Hermitian positive-definite overlap (NK, n, n), the ``S^(-0.5)``
+    array a zand run directory holds (Zandpack/Writer.py:83)."""
+    e, v = np.linalg.eigh(np.asarray(S, complex))
+    return (v * (1.0 / np.sqrt(e))[:, None, :]) @ v.conj().transpose(0, 2, 1)
+
+
+def to_nonorthogonal(p, Sm05, zero_tol=1e-10):
+    """The problem in the non-orthogonal basis that ``mpi/nozand`` propagates, from
+    the orthogonalised arrays of ``p`` and S^(-1/2) (Zandpack/mpi/nozand:259-300,
+    376, 808-812): H = S^½ H_o S^½ (entries below 1e-13 dropped, Hermitian part),
+    σ₀ = S^-½ σ_o S^-½, ξ = S^½ ξ_o, Ixi = Ixi_o S^½, S⁻¹ = (S^-½)².  Per lead, the
+    orbitals where every ξ and Ixi entry is at most ``zero_tol`` are dropped, as
+    nozand's orbital index lists do (``dot_prod_zero_tol``, :242-245, 378-383, 917-928)."""
+    Sm05 = np.asarray(Sm05, complex)
+    S05 = np.linalg.inv(Sm05)
+    H = S05 @ np.asarray(p.H0, complex) @ S05
+    H[np.abs(H) < 1e-13] = 0.0
+    H = 0.5 * (H + H.conj().transpose(0, 2, 1))
+    sig = Sm05 @ np.asarray(p.sigma0, complex) @ Sm05
+    NK, n, NL, NX, NOT = p.dims
+    xi = np.einsum("kij,kaxcj->kaxci", S05, p.xi)
+    Ixi = np.einsum("kaxcj,kji->kaxci", p.Ixi, S05)
+    for a in range(NL):
+        live = (np.abs(xi[:, a]) > zero_tol).any(axis=(0, 1, 2)) | \
+               (np.abs(Ixi[:, a]) > zero_tol).any(axis=(0, 1, 2))
+        xi[:, a, ..., ~live] = 0.0
+        Ixi[:, a, ..., ~live] = 0.0
+    q = AMEProblem(H0=H, sigma0=sig, xi=xi, Ixi=Ixi, GG_P=p.GG_P, GL_P=p.GL_P, GG_M=p.GG_M,
+                   GL_M=p.GL_M, Xpp=p.Xpp, Xpm=p.Xpm, pulse=p.pulse, dissipator=p.dissipator,
+                   Sinv=Sm05 @ Sm05, hbar=p.hbar, name=p.name + "_nonortho", t_end=p.t_end,
+                   eps=p.eps, h0=p.h0)
+    return q
+
+
+def random_overlap(NK, n, strength=0.15, band=2, seed=0):
+    """A banded Hermitian positive-definite overlap I + B (|B| small, |i-j| <= band)."""
+    rng = np.random.default_rng(seed)
+    S = np.zeros((NK, n, n), complex)
+    for k in range(NK):
+        B = np.zeros((n, n), complex)
+        for d in range(1, band + 1):
+            v = strength * (rng.normal(size=n - d) + 0.3j * rng.normal(size=n - d)) / d
+            B += np.diag(v, d)
+        S[k] = np.eye(n) + B + B.conj().T
+    return S
 
 
 def assemble(H0k, leads, N_F, NOT, mu, kT, sigma0=None, **kw):

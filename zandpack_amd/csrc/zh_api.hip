@@ -89,6 +89,7 @@ struct EngineBase {
   virtual void set_hubbard(double U, const double* n0) = 0;
   virtual void set_pulse(const PulseDesc& p) = 0;
   virtual void set_dissipator(const DissDesc& d, const double* sig_eq) = 0;
+  virtual void set_overlap(const double* Sinv) = 0;
   virtual void state_put(const double* sig, const double* psi, const double* omg) = 0;
   virtual void state_get(double* sig, double* psi, double* omg) = 0;
   virtual void sigma_get(double* sig) = 0;
@@ -218,6 +219,17 @@ struct Engine : EngineBase {
   int hpsi_rb = 0;         // mode rows per workgroup of the sparse ψHᵀ kernel
   std::vector<double> H0h; // host copy of H0 (re-installed after an explicit rhs call)
   std::vector<int> perm;   // device mode -> reference mode (-1: inert padding mode)
+  // Non-orthogonal basis (the nozand form): S⁻¹ multiplies H in both H products and
+  // carries the constant and Ωξ terms of dΨ and Π of dσ from the coupled orbitals to
+  // every orbital.  H products then run dense with the complex S⁻¹H.
+  bool overlap = false;
+  double2* Sdev = nullptr;    // S⁻¹ (NK, n, n)
+  double2* Sc = nullptr;      // S⁻¹[c, :] (NK, nc, n): B operand of Π_c S⁻¹[c, :]
+  cs* Bs = nullptr;           // S⁻¹[:, c]ᵀ (NK, nc, n): B operand of the Ψ term
+  double2* Pis = nullptr;     // Π_c with its split-K partials summed (NK, n, nc)
+  double2* Qs = nullptr;      // Π_c S⁻¹[c, :] (NK, n, n)
+  cs* Yov = nullptr;          // ħ GL_P ξ_c + Ω_s ξ_c / ħ (NK, Md, nc)
+  int* cpos_none = nullptr;   // (n) all -1: the coupled-column epilogue terms off
   std::vector<int> cidx_h; // coupled orbitals in device column order
 
   // device buffers
@@ -578,9 +590,29 @@ struct Engine : EngineBase {
     return row <= 65536 ? 1 : 0;
   }
 
+  // S⁻¹H (FP64 MFMA on the device), read back for the host-side analysis below.
+  std::vector<double> apply_sinv(const double* H) {
+    const long nn = (long)n * n;
+    HIPCHK(hipMemcpyAsync(Hcur, H, sizeof(double2) * NK * nn, hipMemcpyHostToDevice, stream));
+    GemmArgs<double, double> g{n, n, n, Sdev, n, nn, Hcur, n, nn};
+    EpiStore<double2> e{Qs, n, nn, 1.0, 0.0};
+    HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, stream,
+                                                                KFull(), CandHsig())));
+    std::vector<double> out(2 * (size_t)NK * nn);
+    HIPCHK(hipMemcpyAsync(out.data(), Qs, sizeof(double2) * NK * nn, hipMemcpyDeviceToHost,
+                          stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return out;
+  }
+
   void install_h(const double* H) {
+    std::vector<double> SH;
+    if (overlap) {
+      SH = apply_sinv(H);
+      H = SH.data();
+    }
     // a real Hamiltonian halves the MFMA work of the Hσ and ψHᵀ products
-    bool real = true;
+    bool real = !overlap;
     for (size_t i = 0; i < (size_t)NK * n * n && real; ++i) real = H[2 * i + 1] == 0.0;
     // entries per row: with few, both H products run as gathers (row-compressed H)
     int rmax = 0;
@@ -591,9 +623,10 @@ struct Engine : EngineBase {
       rmax = std::max(rmax, c);
     }
     const int rb = hpsi_rows();
-    const bool fits = rmax <= kEllMax && rb > 0;
+    const bool fits = rmax <= kEllMax && rb > 0 && !overlap;
     bool sparse = false;
     if (sparse_req == 2) {
+      require(!overlap, "row-compressed H products are not available with an overlap");
       require(fits, "row-compressed H needs at most " + std::to_string(kEllMax) +
                         " non-zeros per row and a Psi row of at most 64 KiB (rows hold " +
                         std::to_string(rmax) + ")");
@@ -648,6 +681,7 @@ struct Engine : EngineBase {
 
   void set_hubbard(double U_, const double* n0_) override {
     require(U_ == 0.0 || NK == 2, "the Hubbard model couples two spin blocks: NK must be 2");
+    require(U_ == 0.0 || !overlap, "the device Hubbard term is defined for an orthogonal basis");
     require(have_consts, "set the constants first", ZH_ESTATE);
     drop_graphs();
     U = U_;
@@ -655,6 +689,56 @@ struct Engine : EngineBase {
       HIPCHK(hipMemcpyAsync(n0, n0_, sizeof(double) * NK * n, hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
+  }
+
+  // S⁻¹ of the non-orthogonal form (null: back to the orthogonal form).
+  void set_overlap(const double* S) override {
+    HIPCHK(hipSetDevice(device));
+    require(have_consts, "set the constants first", ZH_ESTATE);
+    require(!S || U == 0.0, "the device Hubbard term is defined for an orthogonal basis");
+    drop_graphs();
+    if (!S) {
+      overlap = false;
+      install_h(H0h.data());
+      h_dirty = false;
+      return;
+    }
+    const long nn = (long)n * n;
+    if (!Sdev) {
+      Sdev = dalloc<double2>((size_t)NK * nn);
+      Sc = dalloc<double2>((size_t)NK * nc * n);
+      Bs = dalloc<cs>((size_t)NK * nc * n);
+      Pis = dalloc<double2>((size_t)NK * n * nc);
+      Qs = dalloc<double2>((size_t)NK * nn);
+      Yov = dalloc<cs>((size_t)NK * Md * nc);
+      cpos_none = dalloc<int>(n);
+      std::vector<int> none(n, -1);
+      HIPCHK(hipMemcpyAsync(cpos_none, none.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    std::vector<double2> sc((size_t)NK * nc * n), bs_d((size_t)NK * nc * n);
+    const double2* Sv = (const double2*)S;
+    for (int b = 0; b < NK; ++b)
+      for (int c = 0; c < nc; ++c) {
+        const int o = cidx_h[c];
+        for (int j = 0; j < n; ++j) {
+          const size_t at = ((size_t)b * nc + c) * n + j;
+          sc[at] = o < 0 ? make_double2(0, 0) : Sv[(size_t)b * nn + (size_t)o * n + j];
+          bs_d[at] = o < 0 ? make_double2(0, 0) : Sv[(size_t)b * nn + (size_t)j * n + o];
+        }
+      }
+    std::vector<cs> bs(bs_d.size());
+    for (size_t i = 0; i < bs.size(); ++i) bs[i] = from_d2<cs>(bs_d[i]);
+    HIPCHK(hipMemcpyAsync(Sdev, S, sizeof(double2) * NK * nn, hipMemcpyHostToDevice, stream));
+    if (nc > 0) {
+      HIPCHK(hipMemcpyAsync(Sc, sc.data(), sizeof(double2) * sc.size(), hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipMemcpyAsync(Bs, bs.data(), sizeof(cs) * bs.size(), hipMemcpyHostToDevice, stream));
+    }
+    overlap = true;
+    install_h(H0h.data());  // synchronises: the host vectors above are read before return
+    h_dirty = false;
   }
 
   void set_dissipator(const DissDesc& d, const double* seq) override {
@@ -965,6 +1049,19 @@ struct Engine : EngineBase {
                                                               CandPi())));
       });
     }
+    const bool ovl = overlap && nc > 0;
+    if (ovl) {
+      timed(KC_GEMM_PI, [&] {
+        // Q = Π_c S⁻¹[c, :]  (n x nc)(nc x n), FP64  (Zandpack/mpi/nozand:489-497)
+        hipLaunchKernelGGL(k_sum_parts, dim3(512), dim3(256), 0, st, NK, kPiParts,
+                           (long)n * nc, (const double2*)Pic, Pis);
+        HIPCHK(hipGetLastError());
+        GemmArgs<double, double> g{n, n, nc, Pis, nc, (long)n * nc, Sc, n, (long)nc * n};
+        EpiStore<double2> e{Qs, n, (long)n * n, 1.0, 0.0};
+        HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, st,
+                                                                    KFull(), CandHsig())));
+      });
+    }
     if (currents) {
       // with no coupled orbitals the reductions run over empty ranges and give J = 0
       timed(KC_CURRENTS, [&] {
@@ -978,7 +1075,8 @@ struct Engine : EngineBase {
     }
     timed(KC_GEMM_HSIG, [&] {
       GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n};
-      EpiSigma e{G, sig_in, dvec, Pic, cpos, n, nc, ih, kPiParts};
+      EpiSigma e{G, sig_in, dvec, Pic, overlap ? cpos_none : cpos, n, nc, ih, kPiParts,
+                 ovl ? Qs : nullptr};
       if (h_sparse) {
         constexpr int rb = 8;
         hipLaunchKernelGGL((k_hsig_ell<rb, EpiSigma>), dim3((n + 255) / 256, (n + rb - 1) / rb, NK),
@@ -1027,6 +1125,21 @@ struct Engine : EngineBase {
         HIPCHK(hipGetLastError());
       });
     }
+    const bool ovl = overlap && nc > 0;
+    if (ovl) {
+      timed(KC_GEMM_PSI, [&] {
+        // (ħ GL_P ξ_c + Ω_s ξ_c/ħ) S⁻¹[:, c]ᵀ into the k slot  (Md x nc)(nc x n); the
+        // Ψ epilogue adds it to ψ(S⁻¹H)ᵀ + ħ ΔPξ_c σ_cᵀ  (Zandpack/mpi/nozand:1003, 1208-1210)
+        const long total = (long)NK * Md * nc;
+        hipLaunchKernelGGL(k_overlap_y<cs>, dim3(1024), dim3(256), 0, st, total, nc,
+                           (const double2*)glp, (const cs*)xic, (const cs*)Wst, Yov, hbar);
+        HIPCHK(hipGetLastError());
+        GemmArgs<R, R> g{Md, n, nc, Yov, nc, (long)Md * nc, Bs, n, (long)nc * n};
+        EpiStore<cs> e{psi[kslot], n, (long)Md * n, 1.0, 0.0};
+        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, st, KFull(),
+                                                     typename Cands<R>::psi())));
+      });
+    }
     timed(KC_GEMM_PSI, [&] {
       if (h_sparse) {
         // [ΔPξ_c] [ħ σ_cᵀ] into the k slot (MFMA, K = nc), then the gather ψHᵀ with
@@ -1066,11 +1179,12 @@ struct Engine : EngineBase {
         HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psiR())));
       } else {
-        // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (Md x (n+nc)) ((n+nc) x n)
+        // [ψ | ΔPξ_c] [H0ᵀ ; ħ σ_cᵀ]  (Md x (n+nc)) ((n+nc) x n); with an overlap
+        // H0 is S⁻¹H and the coupled-column terms come in through acc0
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
-        EpiPsi<cs> e{psi[kslot], nullptr, Ap, n + nc, xic, Wst, glp, chip, dvec, lead, cpos, sps,
-                     Md, n, nc, ih, ih * ih, 1, Md};
+        EpiPsi<cs> e{psi[kslot], ovl ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
+                     dvec, lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md};
         HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
       }
@@ -1316,6 +1430,7 @@ struct Engine : EngineBase {
   // its coupling terms).  Writes (σ₀, Ψ₀, Ω₀) as the resident state.
   void steady_state(double t0, double tol, int maxiter, const zh_precond* pc, int* iters,
                     double* resid) override {
+    require(!overlap, "the steady-state solver is defined for an orthogonal basis");
     require(have_consts && have_state, "steady state needs constants and a resident sigma",
             ZH_ESTATE);
     require(tol > 0.0 && maxiter > 0, "tolerance and iteration limit must be positive");
@@ -1742,6 +1857,13 @@ int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double wi
     require(kind != ZH_DISS_STEP || width != 0.0, "zero dissipator step width");
     DissDesc d{kind, eta0, t0, width == 0.0 ? 1.0 : width};
     p->eng->set_dissipator(d, sig_eq);
+  });
+}
+
+int zh_set_overlap(zh_problem* p, const double* Sinv) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_overlap(Sinv);
   });
 }
 

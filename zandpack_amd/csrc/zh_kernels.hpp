@@ -406,11 +406,13 @@ struct EpiStore {
   }
 };
 
-// G = -i (H0 σ) - i d_row σ + Π_c[row, cpos[col]] / ħ
+// G = -i (H0 σ) - i d_row σ + Π_c[row, cpos[col]] / ħ, or + Q[row, col] / ħ with
+// Q = Π_c S⁻¹[c, :] in the non-orthogonal form (cpos then all -1)
 struct EpiSigma {
   double2* G; const double2* sig; const double2* dvec; const double2* Pic; const int* cpos;
   int n, nc; double inv_hbar;
   int pi_parts;  // split-K partials of Π_c, layout (NK, pi_parts, n, nc)
+  const double2* Q = nullptr;  // (NK, n, n) or null
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
     const long off = ((long)b * n + row) * n + col;
     const double2 s = sig[off];
@@ -428,9 +430,45 @@ struct EpiSigma {
       gr += pr * inv_hbar;
       gi += pi * inv_hbar;
     }
+    if (Q) {
+      const double2 q = Q[off];
+      gr += q.x * inv_hbar;
+      gi += q.y * inv_hbar;
+    }
     G[off] = make_double2(gr, gi);
   }
 };
+
+// out (NK, per) = Σ_s in (NK, parts, per): split-K partials summed in a fixed order.
+__global__ void k_sum_parts(int NK, int parts, long per, const double2* in, double2* out) {
+  const long total = (long)NK * per;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const long b = i / per, r = i - b * per;
+    double x = 0.0, y = 0.0;
+    for (int s = 0; s < parts; ++s) {
+      const double2 v = in[(b * parts + s) * per + r];
+      x += v.x;
+      y += v.y;
+    }
+    out[i] = make_double2(x, y);
+  }
+}
+
+// Non-orthogonal Ψ term (Zandpack/mpi/nozand:1003, 1208-1210): the coupled-orbital
+// vector ħ GL_P ξ_c + Ω_s ξ_c / ħ that S⁻¹[:, c] carries to every orbital.
+// glp (NK, M); xic, W, Y (NK, M, nc).
+template <typename C>
+__global__ void k_overlap_y(long total, int nc, const double2* glp, const C* xic, const C* W,
+                            C* Y, double hbar) {
+  const double ih = 1.0 / hbar;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const double2 gx = cmul(glp[i / nc], to_d2(xic[i]));
+    const double2 w = to_d2(W[i]);
+    Y[i] = from_d2<C>(make_double2(hbar * gx.x + w.x * ih, hbar * gx.y + w.y * ih));
+  }
+}
 
 // kΨ = -i h [ (acc - (χ⁺+Δ)ψ + d_col ψ)/ħ + (GL_P ξ + W/ħ²) on coupled columns ]
 // with acc = ψ H0ᵀ + ħ (ΔP ξ)_c σ_cᵀ  (Zandpack/mpi/zand:746-778).  With acc0 the

@@ -45,6 +45,8 @@ class Propagator:
         self.set_pulse(p.pulse)
         if p.dissipator.kind != "none":
             self.set_dissipator(p.dissipator)
+        if p.Sinv is not None:
+            self.set_overlap(p.Sinv)
         N.check(self.lib.zh_set_graph(self.h, 1 if graph else 0))
         self.put_state(*(state if state is not None else p.zero_state()))
         self.tiles = self.autotune() if autotune else None
@@ -56,6 +58,15 @@ class Propagator:
         w = np.ascontiguousarray(w[: self.NL])
         N.check(self.lib.zh_set_pulse(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
                                       float(pulse.p1), float(pulse.p2), N.ptr(w)))
+
+    def set_overlap(self, Sinv):
+        """Non-orthogonal (nozand) form with S⁻¹ (NK, n, n); None: orthogonal form."""
+        if Sinv is None:
+            N.check(self.lib.zh_set_overlap(self.h, None))
+            return
+        self._sinv = N.c128(Sinv)
+        assert self._sinv.shape == (self.NK, self.n, self.n)
+        N.check(self.lib.zh_set_overlap(self.h, N.ptr(self._sinv)))
 
     def steady_state(self, t0=0.0, tol=1e-10, maxiter=500, precondition=True,
                      self_energy=True, raise_on_limit=True):
