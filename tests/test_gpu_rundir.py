@@ -95,3 +95,26 @@ def test_command_line_on_arrays_dir(native, tmp_path):
     assert k > 0
     np.testing.assert_allclose(T, ref["times"][:k], rtol=1e-10, atol=0)
     np.testing.assert_allclose(C[0], ref["J"][:k, 0], rtol=1e-9, atol=1e-13)
+
+
+def test_command_line_nonorthogonal(native, tmp_path):
+    """A directory with S^(-0.5) run as nozand runs it."""
+    from zandpack_amd.engine import Propagator
+    p = _problem()
+    NK, n = p.dims[:2]
+    Sm05 = builder.lowdin_factors(builder.random_overlap(NK, n, seed=8))
+    q = builder.to_nonorthogonal(p, Sm05)
+    say("rundir cli nonorthogonal", q.dims)
+    prop = Propagator(q, "fp64")
+    ref = prop.propagate(0.0, T1, H0, EPS)
+    prop.close()
+    rundir.write_arrays(p, str(tmp_path / "x" / "Arrays"), Sm05=Sm05)
+    rc = rundir.main([f"Dir={tmp_path}", "name=x", f"t1={T1}", f"stepsize={H0}", f"eps={EPS}",
+                      "saveevery=4", "pulse=step", "amp=0.2", "p1=0", "weights=0.5,-0.5",
+                      "nonorthogonal=True"])
+    assert rc == 0
+    T, C = rundir.combine_currents([str(tmp_path / "x_save")], p.dims[2])
+    k = len(T)
+    assert k > 0
+    np.testing.assert_allclose(T, ref["times"][:k], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(C[1], ref["J"][:k, 1], rtol=1e-9, atol=1e-13)

@@ -55,3 +55,30 @@ def test_save_writer_bookkeeping_and_reader(tmp_path):
     assert np.allclose(T, ts[:8])
     for a in range(NL):
         assert np.allclose(C[a], np.array([J[a] for J in Js[:8]]))
+
+
+def test_nonorthogonal_arrays(tmp_path):
+    p = builder.config_small(NK=2, seed=6)
+    NK, n = p.dims[:2]
+    Sm05 = builder.lowdin_factors(builder.random_overlap(NK, n, seed=2))
+    rundir.write_arrays(p, str(tmp_path / "Arrays"), Sm05=Sm05)
+    q = rundir.load_arrays(str(tmp_path / "Arrays"), nonorthogonal=True)
+    want = builder.to_nonorthogonal(p, Sm05)
+    for k in ("H0", "sigma0", "xi", "Ixi", "Sinv"):
+        assert np.array_equal(getattr(q, k), getattr(want, k)), k
+
+
+def test_nonorthogonal_checkpoint_psi_basis(tmp_path):
+    """last_psi is written in the orthogonal basis and read back with S^-1/2."""
+    p = builder.config_small(NK=1, seed=2)
+    NK, n = p.dims[:2]
+    Sm05 = builder.lowdin_factors(builder.random_overlap(NK, n, seed=3))
+    rng = np.random.default_rng(0)
+    psi = rng.normal(size=p.xi.shape) + 1j * rng.normal(size=p.xi.shape)
+    s, o = p.zero_state()[0], p.zero_state()[2]
+    w = rundir.SaveWriter(str(tmp_path), p.dims[2], S05=np.linalg.inv(Sm05))
+    w.checkpoint(1.5, 0.01, s, psi, o)
+    on_disk = np.load(str(tmp_path / "last_psi.npy"))
+    assert np.allclose(np.einsum("kij,kaxcj->kaxci", Sm05, on_disk), psi, atol=1e-12)
+    t, h, s2, psi2, o2 = rundir.load_checkpoint(str(tmp_path), Sm05)
+    assert (t, h) == (1.5, 0.01) and np.allclose(psi2, psi, atol=1e-12)

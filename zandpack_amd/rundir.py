@@ -9,9 +9,12 @@ reads the results:
     python -m zandpack_amd.rundir Dir=<dir> name=<name> t1=50 eps=1e-8 stepsize=0.01 \\
         [t0=0 saveevery=100 n_dm_compress=1 usesave=False checkpoints=10,20 \\
          precision=fp64 hbar=0.6582119569 pulse=step amp=0.1 p1=0 p2=0 weights=0.5,-0.5 \\
-         hubbard_U=0 hubbard_n0=<file.npy> steady_state=False]
+         hubbard_U=0 hubbard_n0=<file.npy> steady_state=False nonorthogonal=False]
 
-Keywords follow ``Initial.py`` (Zandpack/mpi/zand:57-71).  The ``Bias.py`` hooks
+Keywords follow ``Initial.py`` (Zandpack/mpi/zand:57-71).  ``nonorthogonal=True``
+runs the directory as ``mpi/nozand`` does: ``Arrays/S^(-0.5).npy`` carries the
+problem to the non-orthogonal basis (``builder.to_nonorthogonal``, with
+``dot_prod_zero_tol`` from the environment as nozand reads it, default 1e-10).  The ``Bias.py`` hooks
 (arbitrary Python) are replaced by the device models (``pulse*``, ``hubbard_*``);
 a run that needs an arbitrary ``dH``/``bias`` uses the in-process drivers of
 ``zandpack_amd.drop_in`` instead.
@@ -24,7 +27,7 @@ import sys
 
 import numpy as np
 
-from .builder import AMEProblem, HBAR, Pulse
+from .builder import AMEProblem, HBAR, Pulse, to_nonorthogonal
 
 ARRAY_KEYS = ("H_Ortho", "DM_Ortho", "psi_shape", "GG_P", "GL_P", "GG_M", "GL_M", "Xpp", "Xpm",
               "xi", "Ixi", "zero_tol")
@@ -56,22 +59,32 @@ def not_per_lead(GG_P, GL_P, zero_tol):
     return np.array(out)
 
 
-def load_arrays(arrays_dir, hbar=HBAR, name="problem"):
+def load_arrays(arrays_dir, hbar=HBAR, name="problem", nonorthogonal=False):
     """AMEProblem from a zand ``Arrays/`` directory, channels cut to NOT = max over
-    leads of ``not_per_lead`` (Zandpack/mpi/zand:113-124, 218, 568-576)."""
+    leads of ``not_per_lead`` (Zandpack/mpi/zand:113-124, 218, 568-576).  With
+    ``nonorthogonal`` the problem nozand propagates (Zandpack/mpi/nozand:259-311):
+    the arrays carried to the basis of ``S^(-0.5).npy``."""
     d = {k: flexload(os.path.join(arrays_dir, k + ".npy")) for k in ARRAY_KEYS}
     zt = float(np.asarray(d["zero_tol"]))
     NOT = int(not_per_lead(d["GG_P"], d["GL_P"], zt).max())
     cut = lambda a: np.ascontiguousarray(np.asarray(a)[:, :, :, :NOT], dtype=complex)
-    return AMEProblem(H0=np.asarray(d["H_Ortho"], complex), sigma0=np.asarray(d["DM_Ortho"], complex),
-                      xi=cut(d["xi"]), Ixi=cut(d["Ixi"]), GG_P=cut(d["GG_P"]), GL_P=cut(d["GL_P"]),
-                      GG_M=cut(d["GG_M"]), GL_M=cut(d["GL_M"]), Xpp=cut(d["Xpp"]),
-                      Xpm=cut(d["Xpm"]), hbar=hbar, name=name)
+    p = AMEProblem(H0=np.asarray(d["H_Ortho"], complex), sigma0=np.asarray(d["DM_Ortho"], complex),
+                   xi=cut(d["xi"]), Ixi=cut(d["Ixi"]), GG_P=cut(d["GG_P"]), GL_P=cut(d["GL_P"]),
+                   GG_M=cut(d["GG_M"]), GL_M=cut(d["GL_M"]), Xpp=cut(d["Xpp"]),
+                   Xpm=cut(d["Xpm"]), hbar=hbar, name=name)
+    if not nonorthogonal:
+        return p
+    Sm05 = flexload(os.path.join(arrays_dir, "S^(-0.5).npy"))
+    q = to_nonorthogonal(p, Sm05, float(os.environ.get("dot_prod_zero_tol", 1e-10)))
+    q.name = name
+    q.Sm05 = np.asarray(Sm05, complex)
+    return q
 
 
-def write_arrays(problem, arrays_dir, zero_tol=1e-12):
+def write_arrays(problem, arrays_dir, zero_tol=1e-12, Sm05=None):
     """The inputs zand reads, under the names Zandpack/Writer.py:61-99 writes
-    (xi and Ixi compressed, as ``arr_0``), for a problem built here."""
+    (xi and Ixi compressed, as ``arr_0``), for a problem built here; ``Sm05`` adds
+    the ``S^(-0.5)`` array nozand reads (Writer.py:83)."""
     os.makedirs(arrays_dir, exist_ok=True)
     j = lambda k: os.path.join(arrays_dir, k)
     NK, n, NL, NX, NOT = problem.dims
@@ -88,6 +101,8 @@ def write_arrays(problem, arrays_dir, zero_tol=1e-12):
     np.save(j("zero_tol"), zero_tol)
     np.savez_compressed(j("xi"), problem.xi)
     np.savez_compressed(j("Ixi"), problem.Ixi)
+    if Sm05 is not None:
+        np.save(j("S^(-0.5)"), np.asarray(Sm05, complex))
 
 
 class SaveWriter:
@@ -95,9 +110,13 @@ class SaveWriter:
     every ``saveevery`` accepted steps DM<step> (complex64, every n_dm_compress-th),
     DMt<step>, times<step>, current_<a>_<step> ((steps, NK) per lead); checkpoints
     last_sig, last_time, last_stepsize and the full last_psi / last_omg that
-    ``LoadFromFull`` resumes from (:535-542)."""
+    ``LoadFromFull`` resumes from (:535-542).  ``S05`` (non-orthogonal runs) writes
+    last_psi in the orthogonal basis, S^½ Ψ, which nozand's full-state loader
+    carries back with S^-½ (Zandpack/mpi/nozand:720-723); σ stays in its own basis."""
 
-    def __init__(self, save_dir, NL, saveevery=100, n_dm_compress=1, t0=0.0, sig0=None):
+    def __init__(self, save_dir, NL, saveevery=100, n_dm_compress=1, t0=0.0, sig0=None,
+                 S05=None):
+        self.S05 = S05
         self.dir = save_dir
         self.NL = NL
         self.saveevery = int(saveevery)
@@ -140,15 +159,21 @@ class SaveWriter:
         np.save(self._p("last_sig"), sig)
         np.save(self._p("last_time"), float(t))
         np.save(self._p("last_stepsize"), float(h))
+        if self.S05 is not None:
+            psi = np.einsum("kij,kaxcj->kaxci", self.S05, psi)
         np.save(self._p("last_psi"), psi)
         np.save(self._p("last_omg"), omg)
 
 
-def load_checkpoint(save_dir):
-    """(t, h, σ, Ψ, Ω) written by ``SaveWriter.checkpoint`` (or zand's full-state files)."""
+def load_checkpoint(save_dir, Sm05=None):
+    """(t, h, σ, Ψ, Ω) written by ``SaveWriter.checkpoint`` (or zand's full-state
+    files); with ``Sm05`` Ψ is carried to the non-orthogonal basis as nozand does."""
     p = lambda f: os.path.join(save_dir, f)
+    psi = np.load(p("last_psi.npy"))
+    if Sm05 is not None:
+        psi = np.einsum("kij,kaxcj->kaxci", Sm05, psi)
     return (float(np.load(p("last_time.npy"))), float(np.load(p("last_stepsize.npy"))),
-            np.load(p("last_sig.npy")), np.load(p("last_psi.npy")), np.load(p("last_omg.npy")))
+            np.load(p("last_sig.npy")), psi, np.load(p("last_omg.npy")))
 
 
 def propagate_to_save(prop, writer, t0, t1, h, eps, checkpoints=(), time_update="reference",
@@ -201,7 +226,10 @@ def main(argv=None):
     f = lambda k, d: float(kw.get(k, d))
     b = lambda k, d: str(kw.get(k, d)).lower() in ("1", "true", "yes")
     hbar = f("hbar", HBAR)
-    p = load_arrays(os.path.join(run_dir, name, "Arrays"), hbar=hbar, name=name)
+    nonortho = b("nonorthogonal", False)
+    p = load_arrays(os.path.join(run_dir, name, "Arrays"), hbar=hbar, name=name,
+                    nonorthogonal=nonortho)
+    Sm05 = p.Sm05 if nonortho else None
     p.pulse = Pulse(kw.get("pulse", "none"), f("amp", 0.0), f("p1", 0.0), f("p2", 0.0),
                     tuple(float(x) for x in kw.get("weights", "0.5,-0.5").split(",")))
     if f("hubbard_U", 0.0):
@@ -211,7 +239,7 @@ def main(argv=None):
     t0, t1, h = f("t0", 0.0), f("t1", 10.0), f("stepsize", 0.01)
     state = None
     if b("usesave", False):
-        t0, h, s, ps, om = load_checkpoint(save_dir)
+        t0, h, s, ps, om = load_checkpoint(save_dir, Sm05)
         state = (s, ps, om)
     prop = Propagator(p, kw.get("precision", "fp64"), state=state, autotune=True)
     if state is None and b("steady_state", False):
@@ -219,7 +247,7 @@ def main(argv=None):
         print(f"steady state: {its} iterations, relative residual {res:.2e}", flush=True)
     ck = [float(x) for x in kw.get("checkpoints", "").split(",") if x]
     w = SaveWriter(save_dir, p.dims[2], int(f("saveevery", 100)), int(f("n_dm_compress", 1)),
-                   t0, prop.sigma())
+                   t0, prop.sigma(), S05=None if Sm05 is None else np.linalg.inv(Sm05))
     t, h, acc, rej = propagate_to_save(prop, w, t0, t1, h, f("eps", 1e-8), ck,
                                        kw.get("time_update", "reference"))
     print(f"t={t:.6g} accepted={acc} rejected={rej} h={h:.6g}", flush=True)
