@@ -332,8 +332,8 @@ def to_nonorthogonal(p, Sm05, zero_tol=1e-10):
     H = 0.5 * (H + H.conj().transpose(0, 2, 1))
     sig = Sm05 @ np.asarray(p.sigma0, complex) @ Sm05
     NK, n, NL, NX, NOT = p.dims
-    xi = np.einsum("kij,kaxcj->kaxci", S05, p.xi)
-    Ixi = np.einsum("kaxcj,kji->kaxci", p.Ixi, S05)
+    xi = (np.asarray(p.xi, complex).reshape(NK, -1, n) @ S05.transpose(0, 2, 1)).reshape(p.xi.shape)
+    Ixi = (np.asarray(p.Ixi, complex).reshape(NK, -1, n) @ S05).reshape(p.Ixi.shape)
     for a in range(NL):
         live = (np.abs(xi[:, a]) > zero_tol).any(axis=(0, 1, 2)) | \
                (np.abs(Ixi[:, a]) > zero_tol).any(axis=(0, 1, 2))
