@@ -107,6 +107,18 @@ template <typename Epi>
 struct HasBlock4<Epi, std::void_t<decltype(Epi::kBlock4)>>
     : std::integral_constant<bool, Epi::kBlock4> {};
 
+// Row epilogues (static constexpr bool kRow4 = true) take, in FP32, four
+// consecutive columns of one output row per call,
+//   row4(z, row, col0, ncol, re[4], im[4], double& acc)   (ncol of them valid;
+// row = -1 outside the matrix),
+// so that a lane's loads and stores of them are 32 contiguous bytes.  The kernel
+// then issues the MFMAs with A and B swapped: the accumulator tile is the
+// transpose, whose four items per lane run along the output row.
+template <typename Epi, typename = void> struct HasRow4 : std::false_type {};
+template <typename Epi>
+struct HasRow4<Epi, std::void_t<decltype(Epi::kRow4)>>
+    : std::integral_constant<bool, Epi::kRow4> {};
+
 // Workgroup: 256 threads = 4 waves laid out WM x (4/WM) over a BM x BN output
 // tile; each wave owns a (BM/WM) x (BN*WM/4) block of 16x16 MFMA tiles.  The
 // global loads of K tile t+1 are issued before the MFMAs of tile t.  With DB the
@@ -118,6 +130,7 @@ template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
 __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi,
                                                               KSeg ks) {
   constexpr bool RA_ = RMODE == 1, RB_ = RMODE == 2;
+  constexpr bool SWP = std::is_same<T, float>::value && HasRow4<Epi>::value;
   constexpr int NBUF = DB ? 2 : 1;
   constexpr int WN = 4 / WM;
   constexpr int WTM = BM / WM, WTN = BN / WN;  // wave tile
@@ -270,21 +283,23 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
         br[j] = sBr[buf][kk + lk][n];
         if (!RB_) bi[j] = sBi[buf][kk + lk][n];
       }
+      // a x b on the matrix core, or b x a (the transposed tile) for row epilogues
+      auto mm = [&](T a, T b, acc_t c) { return SWP ? Mfma<T>::op(b, a, c) : Mfma<T>::op(a, b, c); };
       if (RA_) {  // Cre += Ar Br, Cim += Ar Bi
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
-            ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
+            cr[i][j] = mm(ar[i], br[j], cr[i][j]);
+            ci[i][j] = mm(ar[i], bi[j], ci[i][j]);
           }
       } else if (RB_) {  // Cre += Ar Br, Cim += Ai Br
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
-            ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
+            cr[i][j] = mm(ar[i], br[j], cr[i][j]);
+            ci[i][j] = mm(ai[i], br[j], ci[i][j]);
           }
       } else {
         T nai[TM];
@@ -294,10 +309,10 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            cr[i][j] = Mfma<T>::op(ar[i], br[j], cr[i][j]);
-            ci[i][j] = Mfma<T>::op(ar[i], bi[j], ci[i][j]);
-            cr[i][j] = Mfma<T>::op(nai[i], bi[j], cr[i][j]);
-            ci[i][j] = Mfma<T>::op(ai[i], br[j], ci[i][j]);
+            cr[i][j] = mm(ar[i], br[j], cr[i][j]);
+            ci[i][j] = mm(ar[i], bi[j], ci[i][j]);
+            cr[i][j] = mm(nai[i], bi[j], cr[i][j]);
+            ci[i][j] = mm(ai[i], br[j], ci[i][j]);
           }
       }
     }
@@ -306,7 +321,24 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
   }
 
   double red = 0.0;
-  if constexpr (HasBlock4<Epi>::value) {
+  if constexpr (SWP) {
+    // transposed tile: item r of lane l is row lr, column 4 lk + r of the 16 x 16 tile
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int row = m0 + wm * WTM + i * 16 + lr;
+        const int col0 = n0 + wn * WTN + j * 16 + 4 * lk;
+        double re[4], im[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          re[r] = (double)cr[i][j][r];
+          im[r] = (double)ci[i][j][r];
+        }
+        const int ncol = row < g.M ? max(0, min(4, g.N - col0)) : 0;
+        epi.row4(blockIdx.z, row < g.M ? row : -1, col0, ncol, re, im, red);
+      }
+  } else if constexpr (HasBlock4<Epi>::value) {
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll

@@ -583,6 +583,86 @@ struct EpiOmegaRK {
       }
     }
   }
+  // Four consecutive columns of one row (FP32 storage): 32-byte loads and stores
+  // when all four are inside the matrix; every slot load first, then the stores.
+  static constexpr bool kRow4 = true;
+  __device__ inline void row4(int b, int row, int col0, int ncol, const double* re,
+                              const double* im, double& acc) const {
+    if (ncol <= 0) return;
+    const long off = ((long)b * M + row) * M + col0;
+    const bool vec = ncol == 4 && sizeof(C) == 8 && (off & 3) == 0;
+    double sr[4], si[4], er[4], ei[4];
+    load4(y0, off, ncol, vec, sr, si);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { er[r] = 0.0; ei[r] = 0.0; }
+#pragma unroll
+    for (int j = 0; j < kMaxTerms; ++j)
+      if (j < nterms) {
+        double xr[4], xi[4];
+        load4(k[j], off, ncol, vec, xr, xi);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          sr[r] += a[j] * xr[r];
+          si[r] += a[j] * xi[r];
+          if constexpr (TE) {
+            er[r] += ct[j] * xr[r];
+            ei[r] += ct[j] * xi[r];
+          }
+        }
+      }
+    const double2 uu = u[(long)b * M + row];
+    double kr[4], ki[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double2 vv = r < ncol ? v[(long)b * M + col0 + r] : make_double2(0.0, 0.0);
+      const double dr = uu.x - vv.x, di = uu.y - vv.y;
+      // k_s = h acc + i (dr + i di) Ω_s, rounded to the storage type
+      const double2 t = cmul(make_double2(-di, dr), make_double2(sr[r], si[r]));
+      const C ks = from_d2<C>(make_double2(re[r] + t.x, im[r] + t.y));
+      kr[r] = (double)ks.x;
+      ki[r] = (double)ks.y;
+      if constexpr (TE) {
+        if (r < ncol) {
+          const double xr = er[r] + ct_self * kr[r], xi = ei[r] + ct_self * ki[r];
+          acc += xr * xr + xi * xi;
+        }
+      }
+    }
+    if (ynew) store4(ynew, off, ncol, vec, sr, si);
+    if (out) store4(out, off, ncol, vec, kr, ki);
+  }
+  __device__ static inline void load4(const C* p, long off, int ncol, bool vec, double* xr,
+                                      double* xi) {
+    if constexpr (sizeof(C) == 8) {
+      if (vec) {
+        const float4* q = reinterpret_cast<const float4*>(p + off);
+        const float4 lo = q[0], hi = q[1];
+        xr[0] = lo.x; xi[0] = lo.y; xr[1] = lo.z; xi[1] = lo.w;
+        xr[2] = hi.x; xi[2] = hi.y; xr[3] = hi.z; xi[3] = hi.w;
+        return;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double2 x = r < ncol ? to_d2(p[off + r]) : make_double2(0.0, 0.0);
+      xr[r] = x.x;
+      xi[r] = x.y;
+    }
+  }
+  __device__ static inline void store4(C* p, long off, int ncol, bool vec, const double* xr,
+                                       const double* xi) {
+    if constexpr (sizeof(C) == 8) {
+      if (vec) {
+        float4* q = reinterpret_cast<float4*>(p + off);
+        q[0] = make_float4((float)xr[0], (float)xi[0], (float)xr[1], (float)xi[1]);
+        q[1] = make_float4((float)xr[2], (float)xi[2], (float)xr[3], (float)xi[3]);
+        return;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (r < ncol) p[off + r] = from_d2<C>(make_double2(xr[r], xi[r]));
+  }
   __device__ inline void finish(int block, double sum) const { te_part[block] = sum; }
 };
 
