@@ -111,6 +111,21 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
 int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double width,
                       const double* sig_eq);
 
+/* Equilibrium density matrix by the contour sum of Zandpack/cmdtools/SCF:200-212
+ * (the SCF tool's funci): rho = sum_i (i/2pi)(W_i G_i - conj(W_i) G_i^H),
+ * G_i = (E_i - H - Sigma(E_i))^-1, one call per SCF iteration (its Jl.Parallel sum).
+ *   contour  (nE, 4) real: Re E_i, Im E_i, Re W_i, Im W_i (already shifted by mu)
+ *   evals    (NK, n) and evecs (NK, n, n) complex128: H = V diag(e) V^H per k
+ *            (column o of V is eigenvector o)
+ *   corb     (nc) orbitals that carry the lead self-energy, sigma_cc (NK, nE, nc, nc)
+ *            complex128 its values Sigma(E_i) on them (FittedSelfEnergy.evaluate,
+ *            summed over leads); nc = 0: no self-energy
+ *   rho      (NK, n, n) complex128 out.
+ * Stateless: allocates its device buffers on `device` and frees them. */
+int zh_contour_density(int NK, int n, int nE, int nc, const double* contour,
+                       const double* evals, const double* evecs, const int* corb,
+                       const double* sigma_cc, double* rho, int device);
+
 /* Non-orthogonal basis (Zandpack/mpi/nozand): Sinv (NK, n, n) complex128 is S^-1
  * (nozand's Sm1 = S^(-0.5) S^(-0.5), :259-265); H0, xi, Ixi, sigma and psi are then
  * in the non-orthogonal basis.  S^-1 enters as nozand applies it: S^-1 H sigma in

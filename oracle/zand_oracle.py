@@ -509,3 +509,57 @@ def steady_state_dense(consts, H, sig0, delta, hbar=HBAR):
     x = np.linalg.solve(A, np.concatenate([b.real, b.imag]))
     psi0 = (x[:N] + 1j * x[N:]).reshape(shape)
     return psi0, stationary_omega(consts, psi0, delta, hbar)
+
+
+# ---------------------------------------------------------------------------
+# Equilibrium density (cmdtools/SCF)
+# ---------------------------------------------------------------------------
+def lorentzian_basis(E, eps, gam):
+    """Γ-basis L(E) = γ²/((E-ε)²+γ²) and its Kramers-Kronig partner
+    KK(E) = γ(E-ε)/((E-ε)²+γ²), both continued to complex E (the Lorentzian basis of
+    the level-width fit that FittedSelfEnergy.py:35-59 evaluates through the
+    external Block_matrices.Croy routines; restated from the Lorentzian form)."""
+    d = E - eps
+    den = d * d + gam * gam
+    return gam * gam / den, gam * d / den
+
+
+def lead_self_energy(consts, E, bias=0.0):
+    """Σ_α(E) per lead, (NK, NL, n, n) at one complex E, as FittedSelfEnergy.evaluate
+    builds it (:35-53): Σ = 0.5·KK - 0.5i·Γ with Γ(E) = Σ_x C_x L_x(E - bias) and
+    C_x = Σ_c λ_xc ξ_xc ⊗ Ixi_xc (from_eigendecomp, :72-76).  The Lorentzian
+    modes' eigenvalues come from the propagation constants, λ = 2i(GG_P - GL_P)/γ
+    (TimedependentTransport.py:964-1000: GG_P - GL_P = -½iλγ); the Fermi-pole modes
+    have GG_P = GL_P and drop out."""
+    NK, NL, NX, NOT, n, M = _flat(consts)
+    xi = consts["xi"]
+    Ixi = consts["Ixi"]
+    dgp = consts["dGP"]
+    eps = consts["Xpp"].real
+    gam = consts["Xpp"].imag
+    b = np.broadcast_to(np.asarray(bias, float), (NL,))
+    out = np.zeros((NK, NL, n, n), complex)
+    for k in range(NK):
+        for a in range(NL):
+            for x in range(NX):
+                for c in range(NOT):
+                    if dgp[k, a, x, c] == 0:
+                        continue
+                    lam = 2j * dgp[k, a, x, c] / gam[k, a, x, c]
+                    L, KK = lorentzian_basis(E - b[a], eps[k, a, x, c], gam[k, a, x, c])
+                    out[k, a] += (0.5 * KK - 0.5j * L) * lam * np.outer(xi[k, a, x, c], Ixi[k, a, x, c])
+    return out
+
+
+def contour_density(H, contour, sigma_fn):
+    """ρ = Σ_i (i/2π)(W_i G_i - conj(W_i) G_i†), G_i = (E_i - H - Σ(E_i))⁻¹
+    (cmdtools/SCF:200-212, the non-memory-conserving funci).  ``contour`` (nE, 4):
+    Re E, Im E, Re W, Im W (already shifted by μ); ``sigma_fn(E)`` -> (NK, n, n)."""
+    NK, n, _ = H.shape
+    I = np.eye(n)
+    rho = np.zeros((NK, n, n), complex)
+    for Er, Ei, Wr, Wi in contour:
+        E, W = Er + 1j * Ei, Wr + 1j * Wi
+        G = np.linalg.inv(E * I - H - sigma_fn(E))
+        rho += (1j / (2 * np.pi)) * (W * G - np.conj(W) * G.conj().transpose(0, 2, 1))
+    return rho

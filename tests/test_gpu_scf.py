@@ -1,0 +1,83 @@
+"""GPU equilibrium density (cmdtools/SCF contour sum) against the oracle's direct
+inversion per contour point, the Fermi closed form without self-energy, and the
+fixed point of the self-consistent loop.  FP64 bar: 1e-10 relative to max|ρ|."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import zand_oracle as O
+from zandpack_amd import builder, scf
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _contour(mu=0.0):
+    C = np.load(os.path.join(GOLD, "scf_default_contour.npy"))
+    C = C.copy()
+    C[:, 0] += mu
+    return C
+
+
+CASES = {
+    "small": lambda: builder.config_small(seed=1),
+    "small_nk2": lambda: builder.config_small(NK=2, seed=3),
+    "chain20": lambda: builder.config_chain(),
+    "wide": lambda: builder.config_small(n_w=6, n_l=7, N_L=5, slots=6, NOT=3, seed=4),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_contour_density_matches_oracle(native, case):
+    p = CASES[case]()
+    C = _contour(0.05)
+    corb = scf.coupled_orbitals(p)
+    sig = scf.self_energy_cc(p, C[:, 0] + 1j * C[:, 1], corb)
+    say("contour density", case, p.dims, len(corb))
+    got = scf.contour_density(p.H0, C, corb, sig)
+    ref = O.contour_density(np.asarray(p.H0, complex), C,
+                            lambda E: O.lead_self_energy(p.consts(), E).sum(axis=1))
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+    assert np.abs(got - got.conj().transpose(0, 2, 1)).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_contour_density_without_self_energy_is_fermi(native):
+    rng = np.random.default_rng(0)
+    n = 40
+    A = rng.normal(size=(n, n)) + 0.3j * rng.normal(size=(n, n))
+    H = ((A + A.conj().T) / 2)[None]
+    C = _contour(0.2)
+    kT = (C[-1, 1] - C[-2, 1]) / (2 * np.pi)
+    say("contour density fermi")
+    got = scf.contour_density(H, C, np.zeros(0, np.int32), None)
+    e, V = np.linalg.eigh(H[0])
+    ref = (V / (1 + np.exp((e - 0.2) / kT))) @ V.conj().T
+    assert np.abs(got[0] - ref).max() < 5e-6
+
+
+def test_self_consistent_hubbard_fixed_point(native):
+    """Hubbard mean field: the loop's ρ is a fixed point of the oracle's map."""
+    p = builder.config_small(NK=2, hubbard_U=1.0, seed=2)
+    p.pulse = builder.Pulse()  # equilibrium: no bias at t0
+    C = _contour()
+    say("scf hubbard", p.dims)
+    rho, its, ok = scf.self_consistent_density(p, C, mu=0.0, tol=1e-8, max_iter=200)
+    assert ok, its
+    H = np.asarray(p.H0, complex) + p.dH(0.0, rho)
+    F = O.contour_density(H, C, lambda E: O.lead_self_energy(p.consts(), E).sum(axis=1))
+    assert np.abs(F - rho).max() < 1e-7
+    say("scf iterations", its)
+
+
+def test_nonequilibrium_is_refused(native):
+    p = builder.config_small(seed=1)
+    p.pulse = builder.Pulse("step", amp=0.2, p1=-1.0, weights=(0.5, -0.5))
+    with pytest.raises(NotImplementedError):
+        scf.self_consistent_density(p, _contour(), mu=0.0)

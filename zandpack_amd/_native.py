@@ -75,6 +75,8 @@ SIGNATURES = {
     "zh_set_dissipator": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                     _dp]),
     "zh_set_overlap": (C.c_int, [C.c_void_p, _dp]),
+    "zh_contour_density": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                                     C.POINTER(C.c_int), _dp, _dp, C.c_int]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_set_profile": (C.c_int, [C.c_void_p, C.c_int]),

@@ -31,6 +31,7 @@
 #include "../../include/zandhip.h"
 #include "zh_gemm.hpp"
 #include "zh_kernels.hpp"
+#include "zh_scf.hpp"
 
 namespace zh {
 
@@ -1766,6 +1767,112 @@ struct zh_problem {
   zh_dims dims;
 };
 
+// ---------------------------------------------------------- equilibrium density
+// Device buffers and stream of one contour_density call, released on every exit.
+struct ScfScratch {
+  hipStream_t st = nullptr;
+  std::vector<void*> bufs;
+  ~ScfScratch() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : bufs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  template <typename T> T* alloc(size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+      throw Error{ZH_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+    bufs.push_back(p);
+    return (T*)p;
+  }
+};
+
+static void contour_density(int NK, int n, int nE, int nc, const double* EW, const double* evals,
+                            const double* evecs, const int* corb, const double* sigma_cc,
+                            double* rho, int device) {
+  require(NK > 0 && n > 0 && nE > 0, "NK, n and the number of contour points must be positive");
+  require(nc >= 0 && nc <= n && nc <= kGJMax, "coupled orbitals: 0 <= nc <= min(n, 1024)");
+  require(nc == 0 || (corb && sigma_cc), "the self-energy needs its orbitals and values");
+  for (int c = 0; c < nc; ++c) require(corb[c] >= 0 && corb[c] < n, "orbital index out of range");
+  HIPCHK(hipSetDevice(device));
+  ScfScratch sc;
+  HIPCHK(hipStreamCreateWithFlags(&sc.st, hipStreamNonBlocking));
+  hipStream_t st = sc.st;
+  const long nn = (long)n * n, L = (long)nE * nc, cc = (long)nc * nc;
+  std::vector<double2> Eh(nE), Wh(nE);
+  for (int i = 0; i < nE; ++i) {
+    Eh[i] = make_double2(EW[4 * i], EW[4 * i + 1]);
+    Wh[i] = make_double2(EW[4 * i + 2], EW[4 * i + 3]);
+  }
+  double2* Ed = sc.alloc<double2>(nE);
+  double2* Wd = sc.alloc<double2>(nE);
+  double* ed = sc.alloc<double>(n);
+  int* cd = sc.alloc<int>(nc);
+  double2 *V = sc.alloc<double2>(nn), *Vh = sc.alloc<double2>(nn), *Vch = sc.alloc<double2>((size_t)n * nc);
+  double2 *w = sc.alloc<double2>(n), *Y = sc.alloc<double2>(nn), *Z = sc.alloc<double2>(nn);
+  double2 *X = sc.alloc<double2>(nn), *R = sc.alloc<double2>(nn);
+  double2 *Tp = sc.alloc<double2>((size_t)n * L), *T = sc.alloc<double2>((size_t)n * L);
+  double2 *U = sc.alloc<double2>((size_t)n * L);
+  double2 *Sg = sc.alloc<double2>((size_t)nE * cc), *G0 = sc.alloc<double2>((size_t)nE * cc);
+  double2 *Mm = sc.alloc<double2>((size_t)nE * cc), *Kk = sc.alloc<double2>((size_t)nE * cc);
+  HIPCHK(hipMemcpyAsync(Ed, Eh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(Wd, Wh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
+  if (nc > 0) HIPCHK(hipMemcpyAsync(cd, corb, sizeof(int) * nc, hipMemcpyHostToDevice, st));
+  typedef Tile<64, 64, 16, 2, false, 3> TBig;
+  typedef Tile<32, 32, 16, 2, false, 4> TSmall;
+  const dim3 g1(1024), b1(256);
+  for (int k = 0; k < NK; ++k) {
+    HIPCHK(hipMemcpyAsync(V, evecs + 2 * (size_t)k * nn, sizeof(double2) * nn,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ed, evals + (size_t)k * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scf_adjoint, g1, b1, 0, st, n, nc, (const double2*)V, (const int*)cd, Vh,
+                       Vch);
+    hipLaunchKernelGGL(k_scf_diag, dim3((n + 255) / 256), b1, 0, st, n, nE, (const double2*)Ed,
+                       (const double2*)Wd, (const double*)ed, w);
+    HIPCHK(hipGetLastError());
+    if (nc > 0) {
+      HIPCHK(hipMemcpyAsync(Sg, sigma_cc + 2 * (size_t)k * nE * cc, sizeof(double2) * nE * cc,
+                            hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_scf_factors, g1, b1, 0, st, n, nE, nc, (const double2*)Ed,
+                         (const double*)ed, (const double2*)V, (const int*)cd, Tp, T);
+      HIPCHK(hipGetLastError());
+      // G0_i,cc = T_i V_c†   (nc x n)(n x nc), one batch entry per contour point
+      GemmArgs<double, double> ga{nc, nc, n, T, n, (long)nc * n, Vch, nc, 0};
+      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall>(
+          ga, nE, EpiStore<double2>{G0, nc, cc, 1.0, 0.0}, st)));
+      // I - Σ_i G0_i,cc
+      GemmArgs<double, double> gb{nc, nc, nc, Sg, nc, cc, G0, nc, cc};
+      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall>(
+          gb, nE, EpiIdMinus{Mm, nc, cc}, st)));
+      // K_i = (I - Σ_i G0_i,cc)⁻¹ Σ_i
+      HIPCHK(hipMemcpyAsync(Kk, Sg, sizeof(double2) * nE * cc, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_gauss_jordan, dim3(nE), dim3(256), 0, st, nc, Mm, Kk);
+      HIPCHK(hipGetLastError());
+      // U_i = W_i T'_i K_i  (n x nc)(nc x nc), into the column block i of U (n, nE nc)
+      GemmArgs<double, double> gc{n, nc, nc, Tp, L, (long)nc, Kk, nc, cc};
+      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall>(
+          gc, nE, EpiScaleZ{U, L, (long)nc, Wd}, st)));
+    }
+    // Y = U T + diag(Σ_i W_i D_i)  (n x nE nc)(nE nc x n)
+    GemmArgs<double, double> gy{n, n, (int)L, U, L, 0, T, n, 0};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig>(gy, 1, EpiAddDiag{Y, n, w},
+                                                                     st)));
+    // X = V Y V†
+    GemmArgs<double, double> gz{n, n, n, V, n, 0, Y, n, 0};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig>(
+        gz, 1, EpiStore<double2>{Z, n, nn, 1.0, 0.0}, st)));
+    GemmArgs<double, double> gx{n, n, n, Z, n, 0, Vh, n, 0};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig>(
+        gx, 1, EpiStore<double2>{X, n, nn, 1.0, 0.0}, st)));
+    hipLaunchKernelGGL(k_scf_rho, g1, b1, 0, st, n, (const double2*)X, R);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rho + 2 * (size_t)k * nn, R, sizeof(double2) * nn, hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  for (size_t i = 0; i < 2 * (size_t)NK * nn; ++i)
+    require(std::isfinite(rho[i]), "non-finite density (singular Woodbury matrix?)", ZH_ENONFINITE);
+}
+
 template <typename F>
 static int zh_try(F&& f) {
   try {
@@ -1857,6 +1964,15 @@ int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double wi
     require(kind != ZH_DISS_STEP || width != 0.0, "zero dissipator step width");
     DissDesc d{kind, eta0, t0, width == 0.0 ? 1.0 : width};
     p->eng->set_dissipator(d, sig_eq);
+  });
+}
+
+int zh_contour_density(int NK, int n, int nE, int nc, const double* contour,
+                       const double* evals, const double* evecs, const int* corb,
+                       const double* sigma_cc, double* rho, int device) {
+  return zh_try([&] {
+    require(contour && evals && evecs && rho, "null argument");
+    contour_density(NK, n, nE, nc, contour, evals, evecs, corb, sigma_cc, rho, device);
   });
 }
 
