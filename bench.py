@@ -219,6 +219,65 @@ def cpu_baseline(p):
                       f"({el:.2f} s), x7 per DOP54 step"}
 
 
+def scf_contour():
+    """The SCF tool's default contour (102 points, kT = 0.02585 eV), kept as test data."""
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                "scf_default_contour.npy"))
+
+
+def scf_leg(p, reps=3):
+    """Equilibrium density (cmdtools/SCF equivalent) of the same workload: one SCF
+    iteration's density from H(σ₀) — host eigenpairs of H plus the device contour sum
+    over the 102-point default contour — and the once-per-run self-energy set-up."""
+    from zandpack_amd import scf
+    C = scf_contour()
+    corb = scf.coupled_orbitals(p)
+    t0 = time.perf_counter()
+    sig = scf.self_energy_cc(p, C[:, 0] + 1j * C[:, 1], corb)
+    t_setup = time.perf_counter() - t0
+    H = np.asarray(p.H0, complex) + p.dH(0.0, p.sigma0)
+    scf.contour_density(H, C, corb, sig)  # warm-up
+    t_eigh, t_all = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        np.linalg.eigh(H)
+        t_eigh.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        rho = scf.contour_density(H, C, corb, sig)
+        t_all.append(time.perf_counter() - t0)
+    tr = np.trace(rho, axis1=1, axis2=2).real
+    return {"contour_points": int(C.shape[0]), "n_c": int(corb.size),
+            "s_per_iteration": float(np.median(t_all)),
+            "host_eigh_s": float(np.median(t_eigh)),
+            "device_s": float(np.median(t_all) - np.median(t_eigh)),
+            "self_energy_setup_s": t_setup, "charge_per_k": [float(x) for x in tr]}
+
+
+def scf_cpu_sample(p, points=4):
+    """The reference form of one SCF iteration (one n x n inversion per contour point,
+    cmdtools/SCF:200-212) on the host, timed over ``points`` contour points and
+    scaled to the full contour."""
+    from oracle import zand_oracle as O
+    from zandpack_amd import scf
+    C = scf_contour()
+    H = np.asarray(p.H0, complex) + p.dH(0.0, p.sigma0)
+    sub = C[:: max(1, C.shape[0] // points)][:points]
+    # the tool evaluates the self-energies once before its loop (SCF:170-180)
+    corb = scf.coupled_orbitals(p)
+    Ec = sub[:, 0] + 1j * sub[:, 1]
+    scc = scf.self_energy_cc(p, Ec, corb)
+    full = {}
+    for i, E in enumerate(Ec):
+        S = np.zeros(H.shape, complex)
+        S[:, corb[:, None], corb[None, :]] = scc[:, i]
+        full[complex(E)] = S
+    t0 = time.perf_counter()
+    O.contour_density(H, sub, lambda E: full[complex(E)])
+    el = time.perf_counter() - t0
+    return {"s_per_iteration": el * C.shape[0] / len(sub),
+            "sample": f"{len(sub)} of {C.shape[0]} contour points ({el:.2f} s), scaled"}
+
+
 def replay_fp64_error(req):
     """FP64 run of one served request forced onto its accepted (t, h) sequence:
     max |J_mixed - J_fp64| / max |J_fp64|."""
@@ -383,6 +442,8 @@ def main():
         print("# batched (config 4)", file=sys.stderr, flush=True)
         print("# steady state", file=sys.stderr, flush=True)
         line["steady_state"] = steady_leg(p)
+        print("# equilibrium density", file=sys.stderr, flush=True)
+        line["scf"] = scf_leg(p)
         line["batched"] = batched_leg(args.batched_instances, args.batched_steps)
         print("# serving (config 5)", file=sys.stderr, flush=True)
         line["serving"] = serving_leg(args.serving_requests, args.serving_rate,
@@ -390,6 +451,8 @@ def main():
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(p)
+        if "scf" in line:
+            line["cpu_baseline"]["scf"] = scf_cpu_sample(p)
     print(json.dumps(line), flush=True)
 
 

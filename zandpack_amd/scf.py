@@ -53,7 +53,7 @@ def self_energy_cc(problem, E, corb, bias=None):
             X = problem.xi[k, a].reshape(-1, n)[m][:, corb]        # (Mm, nc)
             Y = problem.Ixi[k, a].reshape(-1, n)[m][:, corb]
             s = 1j * d[m][None, :] / (E[:, None] - b[a] - xm[None, :])   # (nE, Mm)
-            out[k] += np.einsum("em,mc,md->ecd", s, X, Y, optimize=True)
+            out[k] += np.matmul(np.ascontiguousarray((s[:, :, None] * X[None]).transpose(0, 2, 1)), Y)
     return out
 
 
