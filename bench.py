@@ -255,14 +255,14 @@ def scf_leg(p, reps=3):
 
 def scf_cpu_sample(p, points=4):
     """The reference form of one SCF iteration (one n x n inversion per contour point,
-    cmdtools/SCF:200-212) on the host, timed over ``points`` contour points and
+    cmdtools/SCF:209-215) on the host, timed over ``points`` contour points and
     scaled to the full contour."""
     from oracle import zand_oracle as O
     from zandpack_amd import scf
     C = scf_contour()
     H = np.asarray(p.H0, complex) + p.dH(0.0, p.sigma0)
     sub = C[:: max(1, C.shape[0] // points)][:points]
-    # the tool evaluates the self-energies once before its loop (SCF:170-180)
+    # the tool evaluates the self-energies once before its loop (SCF:181-189)
     corb = scf.coupled_orbitals(p)
     Ec = sub[:, 0] + 1j * sub[:, 1]
     scc = scf.self_energy_cc(p, Ec, corb)

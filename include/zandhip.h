@@ -111,7 +111,7 @@ int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, cons
 int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double width,
                       const double* sig_eq);
 
-/* Equilibrium density matrix by the contour sum of Zandpack/cmdtools/SCF:200-212
+/* Equilibrium density matrix by the contour sum of Zandpack/cmdtools/SCF:209-215
  * (the SCF tool's funci): rho = sum_i (i/2pi)(W_i G_i - conj(W_i) G_i^H),
  * G_i = (E_i - H - Sigma(E_i))^-1, one call per SCF iteration (its Jl.Parallel sum).
  *   contour  (nE, 4) real: Re E_i, Im E_i, Re W_i, Im W_i (already shifted by mu)

@@ -553,7 +553,7 @@ def lead_self_energy(consts, E, bias=0.0):
 
 def contour_density(H, contour, sigma_fn):
     """ρ = Σ_i (i/2π)(W_i G_i - conj(W_i) G_i†), G_i = (E_i - H - Σ(E_i))⁻¹
-    (cmdtools/SCF:200-212, the non-memory-conserving funci).  ``contour`` (nE, 4):
+    (cmdtools/SCF:209-215, the non-memory-conserving funci).  ``contour`` (nE, 4):
     Re E, Im E, Re W, Im W (already shifted by μ); ``sigma_fn(E)`` -> (NK, n, n)."""
     NK, n, _ = H.shape
     I = np.eye(n)

@@ -1,4 +1,4 @@
-// Equilibrium density matrix by the contour sum of Zandpack/cmdtools/SCF:200-212,
+// Equilibrium density matrix by the contour sum of Zandpack/cmdtools/SCF:209-215,
 //   ρ = Σ_i (i/2π) (W_i G_i - conj(W_i) G_i†),   G_i = (E_i - H - Σ(E_i))⁻¹,
 // with the self-energy Σ(E_i) supported on the n_c orbitals corb.  Instead of one
 // n x n inversion per contour point, H = V diag(e) V† (eigenpairs from the caller)
