@@ -26,7 +26,7 @@ enum { ZH_OK = 0, ZH_EINVAL = -1, ZH_EHIP = -2, ZH_ENOMEM = -3, ZH_ESTATE = -4, 
        ZH_ELIMIT = -6 };
 /* bias waveform kinds (Zandpack/Pulses.py:52-94 shapes) */
 enum { ZH_PULSE_NONE = 0, ZH_PULSE_STEP = 1, ZH_PULSE_GAUSS = 2, ZH_PULSE_RAMP = 3, ZH_PULSE_SIN = 4,
-       ZH_PULSE_BOX = 5 };
+       ZH_PULSE_BOX = 5, ZH_PULSE_GENERIC = 6, ZH_PULSE_STAIRS = 7, ZH_PULSE_SPLINE = 8 };
 /* dissipator rate kinds */
 enum { ZH_DISS_NONE = 0, ZH_DISS_CONST = 1, ZH_DISS_STEP = 2 };
 /* time bookkeeping after an accepted step */
@@ -99,6 +99,23 @@ int zh_set_hubbard(zh_problem* p, double U, const double* n0);
  *   STEP amp for t >= p1; GAUSS amp exp(-((t-p1)/p2)^2); RAMP amp (1 - exp(-t/p1));
  *   SIN amp sin(p1 t / hbar); BOX amp (tanh(t/p2) - tanh((t-p1)/p2))/2. */
 int zh_set_pulse(zh_problem* p, int kind, double amp, double p1, double p2, const double* weights);
+
+/* Any pulse kind with up to four parameters par[0..3] (zh_set_pulse is the two-
+ * parameter form):
+ *   GENERIC amp exp(-((t-d)/s)^2) sin(2 pi f (t-d) - phase), par = {s, f, phase, d}
+ *           (generic_pulse with the Gaussian envelope env1, Zandpack/Pulses.py:52-66)
+ *   STAIRS  amp sum_{i=1}^{n-1} (1 - 1/(1+exp((t - i w)/soft))), par = {w, n, soft}
+ *           (stairs, Pulses.py:83-88)
+ *   SPLINE  amp S(t) (tanh((t-a)/ts) - tanh((t-b)/ts))/2 inside the knots, 0 outside,
+ *           par = {a, b, ts}; S the piecewise cubic of zh_set_pulse_spline (the
+ *           data-driven air_photonics_pulse / toptica_pulse, Pulses.py:5-50) */
+int zh_set_pulse_ex(zh_problem* p, int kind, double amp, const double* par,
+                    const double* weights);
+
+/* Piecewise cubic of ZH_PULSE_SPLINE: knots x (nknots, increasing) and coefficients
+ * c (4, nknots-1) in scipy's CubicSpline.c layout, S(t) = sum_m c[m][i] (t-x_i)^(3-m)
+ * on the interval i with x_i <= t (the last interval at the last knot). */
+int zh_set_pulse_spline(zh_problem* p, int nknots, const double* x, const double* c);
 
 /* Device-evaluated dissipator of the sigma equation, dissipator(t, sigma) =
  * eta(t) (sigma - sigma_eq), subtracted as D -= hbar * dissipator(t, sigma)

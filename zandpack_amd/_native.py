@@ -15,7 +15,8 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzandhip
 ZH_FP64, ZH_MIXED = 0, 1
 ZH_TIME_REFERENCE, ZH_TIME_CORRECT = 0, 1
 ZH_ELIMIT = -6
-PULSE_KINDS = {"none": 0, "step": 1, "gauss": 2, "ramp": 3, "sin": 4, "box": 5}
+PULSE_KINDS = {"none": 0, "step": 1, "gauss": 2, "ramp": 3, "sin": 4, "box": 5, "generic": 6,
+               "stairs": 7, "spline": 8}
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE", -6: "ELIMIT"}
 
 
@@ -75,6 +76,8 @@ SIGNATURES = {
     "zh_set_dissipator": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                     _dp]),
     "zh_set_overlap": (C.c_int, [C.c_void_p, _dp]),
+    "zh_set_pulse_ex": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _dp, _dp]),
+    "zh_set_pulse_spline": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "zh_contour_density": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                      C.POINTER(C.c_int), _dp, _dp, C.c_int]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

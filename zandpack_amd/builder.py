@@ -184,15 +184,27 @@ class Pulse:
 
     kinds: 'none', 'step' (amp for t>=t0), 'gauss' amp*exp(-((t-tc)/w)^2),
     'ramp' amp*(1-exp(-t/tau)), 'sin' amp*sin(omega*t/ħ) (omega in eV),
-    'box' amp*(tanh(t/ts)-tanh((t-tp)/ts))/2 (Zandpack/Pulses.py:76-77).
+    'box' amp*(tanh(t/ts)-tanh((t-tp)/ts))/2 (Zandpack/Pulses.py:76-77),
+    'generic' amp*exp(-((t-d)/s)^2)*sin(2πf(t-d)-phase) with (p1, p2, p3, p4) =
+    (s, f, phase, d) (generic_pulse with env1, Pulses.py:52-66),
+    'stairs' amp*Σ_{i=1}^{n-1}(1-1/(1+exp((t-i w)/soft))) with (p1, p2, p3) =
+    (w, n, soft) (Pulses.py:79-88),
+    'spline' amp*S(t)*(tanh((t-a)/ts)-tanh((t-b)/ts))/2 inside the knots with
+    (p1, p2, p3) = (a, b, ts) and S the piecewise cubic (spline_x, spline_c) in
+    scipy's CubicSpline layout (the data-driven pulses, Pulses.py:5-50).
     """
     kind: str = "none"
     amp: float = 0.0
     p1: float = 0.0
     p2: float = 0.0
     weights: tuple = (0.5, -0.5)
+    p3: float = 0.0
+    p4: float = 0.0
+    spline_x: np.ndarray | None = None
+    spline_c: np.ndarray | None = None
 
-    KINDS = {"none": 0, "step": 1, "gauss": 2, "ramp": 3, "sin": 4, "box": 5}
+    KINDS = {"none": 0, "step": 1, "gauss": 2, "ramp": 3, "sin": 4, "box": 5, "generic": 6,
+             "stairs": 7, "spline": 8}
 
     def value(self, t):
         k = self.kind
@@ -208,10 +220,59 @@ class Pulse:
             return self.amp * np.sin(self.p1 * t / HBAR)
         if k == "box":
             return self.amp * (np.tanh(t / self.p2) - np.tanh((t - self.p1) / self.p2)) / 2
+        if k == "generic":
+            x = (t - self.p4) / self.p1
+            return self.amp * np.exp(-x * x) * np.sin(2 * np.pi * self.p2 * (t - self.p4) - self.p3)
+        if k == "stairs":
+            r = 0.0
+            with np.errstate(over="ignore"):  # exp -> inf gives the exact limit 0
+                for i in range(1, int(self.p2)):
+                    r += 1.0 - 1.0 / (1.0 + np.exp((t - i * self.p1) / self.p3))
+            return self.amp * r
+        if k == "spline":
+            x, c = self.spline_x, self.spline_c
+            if t < x[0] or t > x[-1]:
+                return 0.0
+            i = min(int(np.searchsorted(x, t, side="right")) - 1, len(x) - 2)
+            d = t - x[i]
+            S = ((c[0, i] * d + c[1, i]) * d + c[2, i]) * d + c[3, i]
+            return self.amp * S * (np.tanh((t - self.p1) / self.p3) - np.tanh((t - self.p2) / self.p3)) / 2
         raise ValueError(k)
 
     def bias(self, t, a):
         return float(self.weights[a] * self.value(t))
+
+
+def spline_pulse(t, y, window, amp=1.0, weights=(0.5, -0.5)):
+    """A 'spline' pulse through the samples (t, y): scipy's CubicSpline (its default
+    not-a-knot ends, as Pulses.py:19-22 builds it) inside the tanh window
+    (a, b, ts)."""
+    from scipy.interpolate import CubicSpline
+    cs = CubicSpline(np.asarray(t, float), np.asarray(y, float))
+    a, b, ts = window
+    return Pulse("spline", amp, a, b, tuple(weights), ts, 0.0,
+                 np.ascontiguousarray(cs.x, float), np.ascontiguousarray(cs.c, float))
+
+
+# The two measured waveforms of Zandpack/ExpPulses and how Pulses.py:5-50 prepares
+# them: rows [lo:hi] of the (t [ps], V) table, t in fs from 0, V / max|V|, window.
+EXPERIMENTAL_PULSES = {
+    "air_photonics": dict(rows=(10, 60), window=(100.0, 550.0, 15.0)),
+    "toptica": dict(rows=(90, 170), window=(300.0, 4000.0, 300.0)),
+}
+
+
+def experimental_pulse(name, table, amp=1.0, weights=(0.5, -0.5)):
+    """The data-driven pulse ``name`` ('air_photonics' or 'toptica') from its
+    ExpPulses table (the (N, 2) array of the reference's .npy file)."""
+    spec = EXPERIMENTAL_PULSES[name]
+    t, v = np.asarray(table, float).T
+    lo, hi = spec["rows"]
+    t = t[lo:hi] * 1000.0
+    v = v[lo:hi]
+    v = v / np.abs(v).max()
+    t = t - t.min()
+    return spline_pulse(t, v, spec["window"], amp, weights)
 
 
 @dataclass

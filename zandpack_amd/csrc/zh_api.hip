@@ -89,6 +89,7 @@ struct EngineBase {
                              const double* xpp, const double* xpm) = 0;
   virtual void set_hubbard(double U, const double* n0) = 0;
   virtual void set_pulse(const PulseDesc& p) = 0;
+  virtual void set_pulse_spline(int nknots, const double* x, const double* c) = 0;
   virtual void set_dissipator(const DissDesc& d, const double* sig_eq) = 0;
   virtual void set_overlap(const double* Sinv) = 0;
   virtual void state_put(const double* sig, const double* psi, const double* omg) = 0;
@@ -756,9 +757,33 @@ struct Engine : EngineBase {
   }
 
   void set_pulse(const PulseDesc& p) override {
+    require(p.kind != ZH_PULSE_SPLINE || spl_n >= 2, "set the pulse spline first", ZH_ESTATE);
     drop_graphs();
     pulse = p;
     pulse.hbar = hbar;
+    pulse.sx = spl_x;
+    pulse.sc = spl_c;
+    pulse.ns = spl_n;
+  }
+
+  // knots and coefficients of ZH_PULSE_SPLINE (device copies; replaced on each call)
+  double *spl_x = nullptr, *spl_c = nullptr;
+  int spl_n = 0, spl_cap = 0;
+  void set_pulse_spline(int nk, const double* x, const double* c) override {
+    HIPCHK(hipSetDevice(device));
+    drop_graphs();
+    if (nk > spl_cap) {
+      spl_x = dalloc<double>(nk);
+      spl_c = dalloc<double>(4 * (size_t)(nk - 1));
+      spl_cap = nk;
+    }
+    HIPCHK(hipMemcpyAsync(spl_x, x, sizeof(double) * nk, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(spl_c, c, sizeof(double) * 4 * (nk - 1), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    spl_n = nk;
+    pulse.sx = spl_x;
+    pulse.sc = spl_c;
+    pulse.ns = spl_n;
   }
 
   // ------------------------------------------------------------- transfers
@@ -1930,6 +1955,33 @@ int zh_set_hubbard(zh_problem* p, double U, const double* n0) {
   return zh_try([&] {
     require(p && (U == 0.0 || n0), "null argument");
     p->eng->set_hubbard(U, n0);
+  });
+}
+
+int zh_set_pulse_ex(zh_problem* p, int kind, double amp, const double* par, const double* w) {
+  return zh_try([&] {
+    require(p && w && par, "null argument");
+    require(kind >= ZH_PULSE_NONE && kind <= ZH_PULSE_SPLINE, "unknown pulse kind");
+    require(!((kind == ZH_PULSE_GAUSS || kind == ZH_PULSE_BOX) && par[1] == 0.0), "zero pulse width");
+    require(!(kind == ZH_PULSE_RAMP && par[0] == 0.0), "zero ramp time");
+    require(!(kind == ZH_PULSE_GENERIC && par[0] == 0.0), "zero envelope width");
+    require(!(kind == ZH_PULSE_STAIRS && (par[2] == 0.0 || par[1] < 0.0 || par[1] > 1e6)),
+            "stairs: softness must be non-zero and the count in [0, 1e6]");
+    require(!(kind == ZH_PULSE_SPLINE && par[2] == 0.0), "zero window softness");
+    PulseDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.kind = kind; d.amp = amp; d.p1 = par[0]; d.p2 = par[1]; d.p3 = par[2]; d.p4 = par[3];
+    for (int a = 0; a < p->dims.NL; ++a) d.w[a] = w[a];
+    p->eng->set_pulse(d);
+  });
+}
+
+int zh_set_pulse_spline(zh_problem* p, int nknots, const double* x, const double* c) {
+  return zh_try([&] {
+    require(p && x && c, "null argument");
+    require(nknots >= 2 && nknots <= (1 << 20), "spline: 2 to 2^20 knots");
+    for (int i = 1; i < nknots; ++i) require(x[i] > x[i - 1], "spline knots must increase");
+    p->eng->set_pulse_spline(nknots, x, c);
   });
 }
 

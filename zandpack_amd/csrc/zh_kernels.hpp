@@ -40,6 +40,10 @@ struct PulseDesc {
   int kind;
   double amp, p1, p2, hbar;
   double w[ZH_MAX_LEADS];
+  double p3, p4;
+  const double* sx;  // spline knots (device), ns of them
+  const double* sc;  // spline coefficients (4, ns-1) (device)
+  int ns;
 };
 
 // Dormand-Prince 5(4) (Zandpack/mpi_RK4pars_dev.py:121-138): nodes c, stage
@@ -69,6 +73,28 @@ __device__ inline double pulse_value(const PulseDesc& p, double t) {
     case ZH_PULSE_RAMP: return p.amp * (1.0 - exp(-t / p.p1));
     case ZH_PULSE_SIN: return p.amp * sin(p.p1 * t / p.hbar);
     case ZH_PULSE_BOX: return p.amp * (tanh(t / p.p2) - tanh((t - p.p1) / p.p2)) * 0.5;
+    case ZH_PULSE_GENERIC: {  // p1 = s, p2 = f, p3 = phase, p4 = delay
+      const double x = (t - p.p4) / p.p1;
+      return p.amp * exp(-x * x) * sin(2.0 * M_PI * p.p2 * (t - p.p4) - p.p3);
+    }
+    case ZH_PULSE_STAIRS: {  // p1 = width, p2 = n, p3 = softness
+      double r = 0.0;
+      const int ns = (int)p.p2;
+      for (int i = 1; i < ns; ++i) r += 1.0 - 1.0 / (1.0 + exp((t - i * p.p1) / p.p3));
+      return p.amp * r;
+    }
+    case ZH_PULSE_SPLINE: {  // p1 = a, p2 = b, p3 = ts
+      if (p.ns < 2 || t < p.sx[0] || t > p.sx[p.ns - 1]) return 0.0;
+      int lo = 0, hi = p.ns - 2;  // interval i: the last knot x_i <= t
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (p.sx[mid] <= t) lo = mid; else hi = mid - 1;
+      }
+      const double d = t - p.sx[lo];
+      const int m1 = p.ns - 1;
+      const double s = ((p.sc[lo] * d + p.sc[m1 + lo]) * d + p.sc[2 * m1 + lo]) * d + p.sc[3 * m1 + lo];
+      return p.amp * s * (tanh((t - p.p1) / p.p3) - tanh((t - p.p2) / p.p3)) * 0.5;
+    }
     default: return 0.0;
   }
 }

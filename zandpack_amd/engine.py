@@ -56,8 +56,14 @@ class Propagator:
         w = np.zeros(max(self.NL, 1))
         w[: len(pulse.weights)] = pulse.weights[: self.NL]
         w = np.ascontiguousarray(w[: self.NL])
-        N.check(self.lib.zh_set_pulse(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
-                                      float(pulse.p1), float(pulse.p2), N.ptr(w)))
+        if pulse.kind == "spline":
+            x = np.ascontiguousarray(pulse.spline_x, np.float64)
+            c = np.ascontiguousarray(pulse.spline_c, np.float64)
+            assert c.shape == (4, len(x) - 1)
+            N.check(self.lib.zh_set_pulse_spline(self.h, len(x), N.ptr(x), N.ptr(c)))
+        par = np.array([pulse.p1, pulse.p2, pulse.p3, pulse.p4], np.float64)
+        N.check(self.lib.zh_set_pulse_ex(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
+                                         N.ptr(par), N.ptr(w)))
 
     def set_overlap(self, Sinv):
         """Non-orthogonal (nozand) form with S⁻¹ (NK, n, n); None: orthogonal form."""

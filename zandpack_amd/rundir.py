@@ -8,7 +8,8 @@ reads the results:
 
     python -m zandpack_amd.rundir Dir=<dir> name=<name> t1=50 eps=1e-8 stepsize=0.01 \\
         [t0=0 saveevery=100 n_dm_compress=1 usesave=False checkpoints=10,20 \\
-         precision=fp64 hbar=0.6582119569 pulse=step amp=0.1 p1=0 p2=0 weights=0.5,-0.5 \\
+         precision=fp64 hbar=0.6582119569 pulse=step amp=0.1 p1=0 p2=0 p3=0 p4=0 \\
+         weights=0.5,-0.5 [pulse=air_photonics|toptica pulse_table=<ExpPulses .npy>] \\
          hubbard_U=0 hubbard_n0=<file.npy> steady_state=False nonorthogonal=False]
 
 Keywords follow ``Initial.py`` (Zandpack/mpi/zand:57-71).  ``nonorthogonal=True``
@@ -27,7 +28,8 @@ import sys
 
 import numpy as np
 
-from .builder import AMEProblem, HBAR, Pulse, to_nonorthogonal
+from .builder import (AMEProblem, EXPERIMENTAL_PULSES, HBAR, Pulse, experimental_pulse,
+                      to_nonorthogonal)
 
 ARRAY_KEYS = ("H_Ortho", "DM_Ortho", "psi_shape", "GG_P", "GL_P", "GG_M", "GL_M", "Xpp", "Xpm",
               "xi", "Ixi", "zero_tol")
@@ -230,8 +232,14 @@ def main(argv=None):
     p = load_arrays(os.path.join(run_dir, name, "Arrays"), hbar=hbar, name=name,
                     nonorthogonal=nonortho)
     Sm05 = p.Sm05 if nonortho else None
-    p.pulse = Pulse(kw.get("pulse", "none"), f("amp", 0.0), f("p1", 0.0), f("p2", 0.0),
-                    tuple(float(x) for x in kw.get("weights", "0.5,-0.5").split(",")))
+    weights = tuple(float(x) for x in kw.get("weights", "0.5,-0.5").split(","))
+    kind = kw.get("pulse", "none")
+    if kind in EXPERIMENTAL_PULSES:  # pulse=air_photonics pulse_table=<ExpPulses .npy>
+        p.pulse = experimental_pulse(kind, np.load(kw["pulse_table"], allow_pickle=False),
+                                     f("amp", 1.0), weights)
+    else:
+        p.pulse = Pulse(kind, f("amp", 0.0), f("p1", 0.0), f("p2", 0.0), weights,
+                        f("p3", 0.0), f("p4", 0.0))
     if f("hubbard_U", 0.0):
         p.hubbard_U = f("hubbard_U", 0.0)
         p.hubbard_n0 = np.load(kw["hubbard_n0"], allow_pickle=False)
