@@ -198,25 +198,48 @@ def freerun_h_diff(p, eps, h0, n_steps):
                 rejected_fp64=seqs["fp64"][1], rejected_mixed=seqs["mixed"][1])
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(p):
     """The CPU oracle (numpy/BLAS, zand-style orbital pruning) on one RHS
-    evaluation of the same workload, scaled to steps/s (7 evaluations per step)."""
+    evaluation of the same workload, scaled to steps/s (7 evaluations per step),
+    with all BLAS threads and with one."""
     from oracle import zand_oracle as O
     try:
-        from threadpoolctl import threadpool_info
+        from threadpoolctl import threadpool_info, threadpool_limits
         threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
     except Exception:
+        threadpool_limits = None
         threads = os.cpu_count() or 1
     rng = np.random.default_rng(0)
     sig, psi, omg = p.zero_state()
     psi = 1e-3 * (rng.normal(size=psi.shape) + 1j * rng.normal(size=psi.shape))
     Ht = p.H0 + p.dH(0.0, sig)
+    run = lambda: O.rhs(p.consts(), sig, psi, omg, Ht, [0.15, -0.15], h=0.01, prune=True)
     t0 = time.perf_counter()
-    O.rhs(p.consts(), sig, psi, omg, Ht, [0.15, -0.15], h=0.01, prune=True)
+    run()
     el = time.perf_counter() - t0
+    el1 = None
+    if threadpool_limits is not None:
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            run()
+            el1 = time.perf_counter() - t0
     return {"value": 1.0 / (7 * el), "unit": "steps/s", "cores": int(threads), "kind": "port",
             "sample": f"one RHS evaluation of the same n=1000, M=4000, NK=2 state "
-                      f"({el:.2f} s), x7 per DOP54 step"}
+                      f"({el:.2f} s), x7 per DOP54 step",
+            "s_per_rhs": el, "s_per_rhs_1_thread": el1,
+            "steps_per_s_1_thread": (1.0 / (7 * el1)) if el1 else None,
+            "cpu_model": _cpu_model(), "cpu_count": os.cpu_count()}
 
 
 def scf_contour():
