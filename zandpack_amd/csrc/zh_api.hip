@@ -1071,7 +1071,8 @@ struct Engine : EngineBase {
                               (long)Md * nc, kPiParts};
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
         KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, 0};
-        HIPCHK((launch_sel<double, R, double, true, false, 0>(tsel[TS_PI], g, NK, e, st, ks,
+        // FP64 MFMAs: three real products per complex tile product
+        HIPCHK((launch_sel<double, R, double, true, false, 3>(tsel[TS_PI], g, NK, e, st, ks,
                                                               CandPi())));
       });
     }
@@ -1139,8 +1140,11 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
         EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
         KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
-        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_OMGXI], g, NK, e, st, ks,
-                                                     typename Cands<R>::omgxi())));
+        // FP64: three real MFMAs per complex tile product (the FP32 mixed mode keeps
+        // four: the three-multiplication form's larger rounding reaches its error norm)
+        constexpr int rm = std::is_same<R, double>::value ? 3 : 0;
+        HIPCHK((launch_sel<R, R, R, false, false, rm>(tsel[TS_OMGXI], g, NK, e, st, ks,
+                                                      typename Cands<R>::omgxi())));
         WSumArgs<cs> w{};
         w.part = W;
         for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];

@@ -60,7 +60,11 @@ template <> struct Cplx<float> { typedef float2 type; };
 // A operand is the real matrix Areal (element (m,k) at Areal[m*ldar + k], or
 // [k*ldar + m] transposed); 2 the B operand is the real matrix Breal ((k,n) at
 // Breal[k*ldbr + n], or [n*ldbr + k]).  A real operand moves half the bytes and
-// its product needs two real MFMAs per tile pair instead of four.
+// its product needs two real MFMAs per tile pair instead of four.  Mode 3 is
+// complex x complex in the three-multiplication form
+//   t1 = (ar + ai) br,  t2 = ar (bi - br),  t3 = ai (br + bi);
+//   Re = t1 - t3,  Im = t1 + t2,
+// three real MFMAs per tile pair (the operand sums are formed from the LDS values).
 template <typename RA, typename RB>
 struct GemmArgs {
   int M, N, K;
@@ -129,7 +133,7 @@ template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
           typename KSeg>
 __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi,
                                                               KSeg ks) {
-  constexpr bool RA_ = RMODE == 1, RB_ = RMODE == 2;
+  constexpr bool RA_ = RMODE == 1, RB_ = RMODE == 2, G3 = RMODE == 3;
   constexpr bool SWP = std::is_same<T, float>::value && HasRow4<Epi>::value;
   constexpr int NBUF = DB ? 2 : 1;
   constexpr int WN = 4 / WM;
@@ -185,10 +189,17 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
   const RB* __restrict__ Br = g.Breal ? g.Breal + (long)batch * g.sBr : nullptr;
 
   acc_t cr[TM][TN], ci[TM][TN];
+  acc_t c3[G3 ? TM : 1][G3 ? TN : 1];  // t1 of the three-multiplication form
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) { cr[i][j] = Mfma<T>::zero(); ci[i][j] = Mfma<T>::zero(); }
+  if constexpr (G3) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) c3[i][j] = Mfma<T>::zero();
+  }
 
   ca_t ra[LA];
   cb_t rb[LB];
@@ -301,6 +312,20 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
             cr[i][j] = mm(ar[i], br[j], cr[i][j]);
             ci[i][j] = mm(ai[i], br[j], ci[i][j]);
           }
+      } else if constexpr (G3) {  // c3 += (ar+ai) br; ci += ar (bi-br); cr += -ai (br+bi)
+        T sa[TM], nai[TM], db[TN], sb[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) { sa[i] = ar[i] + ai[i]; nai[i] = -ai[i]; }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) { db[j] = bi[j] - br[j]; sb[j] = br[j] + bi[j]; }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            c3[i][j] = mm(sa[i], br[j], c3[i][j]);
+            ci[i][j] = mm(ar[i], db[j], ci[i][j]);
+            cr[i][j] = mm(nai[i], sb[j], cr[i][j]);
+          }
       } else {
         T nai[TM];
 #pragma unroll
@@ -320,6 +345,17 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
     __syncthreads();
   }
 
+  if constexpr (G3) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          cr[i][j][r] += c3[i][j][r];
+          ci[i][j][r] += c3[i][j][r];
+        }
+  }
   double red = 0.0;
   if constexpr (SWP) {
     // transposed tile: item r of lane l is row lr, column 4 lk + r of the 16 x 16 tile
