@@ -186,6 +186,10 @@ struct Engine : EngineBase {
   static constexpr int kPiParts = 8;  // split-K of Π_c = ψᵀ Ixi_c (K = modes)
   static constexpr int kWParts = 4;   // split-K of W = Ω ξ_c (K = modes)
   static constexpr int kEllMax = 64;  // most entries per row of the row-compressed H
+  // complex FP64 products on the matrix cores in the three-multiplication form (three
+  // real MFMAs per tile product); the FP32 mixed mode keeps four, since the form's
+  // larger rounding reaches its error norm
+  static constexpr int kRM3 = std::is_same<R, double>::value ? 3 : 0;
   static constexpr int kCGroup = 16;  // coupled-orbital column group (lead-aligned)
   int NK, n, NL, NX, NOT;
   int M;        // modes per k in the reference layout, NL*NX*NOT
@@ -1140,11 +1144,8 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
         EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
         KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
-        // FP64: three real MFMAs per complex tile product (the FP32 mixed mode keeps
-        // four: the three-multiplication form's larger rounding reaches its error norm)
-        constexpr int rm = std::is_same<R, double>::value ? 3 : 0;
-        HIPCHK((launch_sel<R, R, R, false, false, rm>(tsel[TS_OMGXI], g, NK, e, st, ks,
-                                                      typename Cands<R>::omgxi())));
+        HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_OMGXI], g, NK, e, st, ks,
+                                                        typename Cands<R>::omgxi())));
         WSumArgs<cs> w{};
         w.part = W;
         for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
@@ -1166,7 +1167,7 @@ struct Engine : EngineBase {
         HIPCHK(hipGetLastError());
         GemmArgs<R, R> g{Md, n, nc, Yov, nc, (long)Md * nc, Bs, n, (long)nc * n};
         EpiStore<cs> e{psi[kslot], n, (long)Md * n, 1.0, 0.0};
-        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, st, KFull(),
+        HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
       });
     }
@@ -1178,8 +1179,9 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
-          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, kseg_g0(),
-                                                       typename Cands<R>::psi())));
+          HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g0, NK, e0, st,
+                                                          kseg_g0(),
+                                                          typename Cands<R>::psi())));
         }
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
                      dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
@@ -1199,8 +1201,9 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
-          HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g0, NK, e0, st, kseg_g0(),
-                                                       typename Cands<R>::psi())));
+          HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g0, NK, e0, st,
+                                                          kseg_g0(),
+                                                          typename Cands<R>::psi())));
         }
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
@@ -1215,7 +1218,7 @@ struct Engine : EngineBase {
                          (long)(n + nc) * n};
         EpiPsi<cs> e{psi[kslot], ovl ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
                      dvec, lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md};
-        HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_PSI], g, NK, e, st, KFull(),
+        HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
       }
     });
