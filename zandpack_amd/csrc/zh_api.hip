@@ -602,7 +602,7 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpyAsync(Hcur, H, sizeof(double2) * NK * nn, hipMemcpyHostToDevice, stream));
     GemmArgs<double, double> g{n, n, n, Sdev, n, nn, Hcur, n, nn};
     EpiStore<double2> e{Qs, n, nn, 1.0, 0.0};
-    HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, stream,
+    HIPCHK((launch_sel<double, double, double, false, false, 3>(tsel[TS_HSIG], g, NK, e, stream,
                                                                 KFull(), CandHsig())));
     std::vector<double> out(2 * (size_t)NK * nn);
     HIPCHK(hipMemcpyAsync(out.data(), Qs, sizeof(double2) * NK * nn, hipMemcpyDeviceToHost,
@@ -1089,7 +1089,7 @@ struct Engine : EngineBase {
         HIPCHK(hipGetLastError());
         GemmArgs<double, double> g{n, n, nc, Pis, nc, (long)n * nc, Sc, n, (long)nc * n};
         EpiStore<double2> e{Qs, n, (long)n * n, 1.0, 0.0};
-        HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, st,
+        HIPCHK((launch_sel<double, double, double, false, false, 3>(tsel[TS_HSIG], g, NK, e, st,
                                                                     KFull(), CandHsig())));
       });
     }
@@ -1118,7 +1118,7 @@ struct Engine : EngineBase {
         HIPCHK((launch_sel<double, double, double, false, false, 1>(tsel[TS_HSIG], g, NK, e, st,
                                                                     KFull(), CandHsigR())));
       } else {
-        HIPCHK((launch_sel<double, double, double, false, false, 0>(tsel[TS_HSIG], g, NK, e, st,
+        HIPCHK((launch_sel<double, double, double, false, false, 3>(tsel[TS_HSIG], g, NK, e, st,
                                                                     KFull(), CandHsig())));
       }
     });
@@ -1849,6 +1849,7 @@ static void contour_density(int NK, int n, int nE, int nc, const double* EW, con
   HIPCHK(hipMemcpyAsync(Ed, Eh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(Wd, Wh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
   if (nc > 0) HIPCHK(hipMemcpyAsync(cd, corb, sizeof(int) * nc, hipMemcpyHostToDevice, st));
+  // FP64 products, three real MFMAs per complex tile product
   typedef Tile<64, 64, 16, 2, false, 3> TBig;
   typedef Tile<32, 32, 16, 2, false, 4> TSmall;
   const dim3 g1(1024), b1(256);
@@ -1869,11 +1870,11 @@ static void contour_density(int NK, int n, int nE, int nc, const double* EW, con
       HIPCHK(hipGetLastError());
       // G0_i,cc = T_i V_c†   (nc x n)(n x nc), one batch entry per contour point
       GemmArgs<double, double> ga{nc, nc, n, T, n, (long)nc * n, Vch, nc, 0};
-      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall>(
+      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
           ga, nE, EpiStore<double2>{G0, nc, cc, 1.0, 0.0}, st)));
       // I - Σ_i G0_i,cc
       GemmArgs<double, double> gb{nc, nc, nc, Sg, nc, cc, G0, nc, cc};
-      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall>(
+      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
           gb, nE, EpiIdMinus{Mm, nc, cc}, st)));
       // K_i = (I - Σ_i G0_i,cc)⁻¹ Σ_i
       HIPCHK(hipMemcpyAsync(Kk, Sg, sizeof(double2) * nE * cc, hipMemcpyDeviceToDevice, st));
@@ -1881,19 +1882,19 @@ static void contour_density(int NK, int n, int nE, int nc, const double* EW, con
       HIPCHK(hipGetLastError());
       // U_i = W_i T'_i K_i  (n x nc)(nc x nc), into the column block i of U (n, nE nc)
       GemmArgs<double, double> gc{n, nc, nc, Tp, L, (long)nc, Kk, nc, cc};
-      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall>(
+      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
           gc, nE, EpiScaleZ{U, L, (long)nc, Wd}, st)));
     }
     // Y = U T + diag(Σ_i W_i D_i)  (n x nE nc)(nE nc x n)
     GemmArgs<double, double> gy{n, n, (int)L, U, L, 0, T, n, 0};
-    HIPCHK((launch_zgemm<double, double, double, false, false, TBig>(gy, 1, EpiAddDiag{Y, n, w},
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(gy, 1, EpiAddDiag{Y, n, w},
                                                                      st)));
     // X = V Y V†
     GemmArgs<double, double> gz{n, n, n, V, n, 0, Y, n, 0};
-    HIPCHK((launch_zgemm<double, double, double, false, false, TBig>(
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
         gz, 1, EpiStore<double2>{Z, n, nn, 1.0, 0.0}, st)));
     GemmArgs<double, double> gx{n, n, n, Z, n, 0, Vh, n, 0};
-    HIPCHK((launch_zgemm<double, double, double, false, false, TBig>(
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
         gx, 1, EpiStore<double2>{X, n, nn, 1.0, 0.0}, st)));
     hipLaunchKernelGGL(k_scf_rho, g1, b1, 0, st, n, (const double2*)X, R);
     HIPCHK(hipGetLastError());
