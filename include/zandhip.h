@@ -143,6 +143,13 @@ int zh_contour_density(int NK, int n, int nE, int nc, const double* contour,
                        const double* evals, const double* evecs, const int* corb,
                        const double* sigma_cc, double* rho, int device);
 
+/* Per-lead Pi of the resident state, as the stage-0 Pi the next attempt starts from
+ * (the save_PI output of Zandpack/mpi/zand:376-378, 444-448): Q (NK, NL, n, n)
+ * complex128 = (Pi_a + Pi_a^H)/hbar with hermitian = 1 (zand's Q_np, :90-110) or
+ * Pi_a/hbar with hermitian = 0 (nozand's PI_np, mpi/nozand:120-140), where
+ * Pi_a[o, o'] = sum over the modes m of lead a of psi_m[o] Ixi_m[o']. */
+int zh_pi_get(zh_problem* p, int hermitian, double* Q);
+
 /* Non-orthogonal basis (Zandpack/mpi/nozand): Sinv (NK, n, n) complex128 is S^-1
  * (nozand's Sm1 = S^(-0.5) S^(-0.5), :259-265); H0, xi, Ixi, sigma and psi are then
  * in the non-orthogonal basis.  S^-1 enters as nozand applies it: S^-1 H sigma in

@@ -118,3 +118,37 @@ def test_command_line_nonorthogonal(native, tmp_path):
     assert k > 0
     np.testing.assert_allclose(T, ref["times"][:k], rtol=1e-10, atol=0)
     np.testing.assert_allclose(C[1], ref["J"][:k, 1], rtol=1e-9, atol=1e-13)
+
+
+@pytest.mark.parametrize("nonortho", [False, True])
+def test_pi_output(native, tmp_path, nonortho):
+    """Per-lead Π of the resident state (zand's Q_np / nozand's PI_np) against the
+    oracle, and the PIa files of a save_PI run."""
+    from oracle import zand_oracle as O
+    from zandpack_amd.engine import Propagator
+    p = _problem()
+    if nonortho:
+        NK, n = p.dims[:2]
+        p = builder.to_nonorthogonal(p, builder.lowdin_factors(builder.random_overlap(NK, n, seed=2)))
+    rng = np.random.default_rng(4)
+    s0, ps0, om0 = p.zero_state()
+    ps0 = 0.05 * (rng.normal(size=ps0.shape) + 1j * rng.normal(size=ps0.shape))
+    say("pi output", nonortho)
+    prop = Propagator(p, "fp64", state=(s0, ps0, om0))
+    NK, n, NL, NX, NOT = p.dims
+    P = ps0.reshape(NK, NL, NX * NOT, n)
+    X = p.Ixi.reshape(NK, NL, NX * NOT, n)
+    r = np.einsum("kamo,kamp->kaop", P, X) / p.hbar
+    want = r if nonortho else r + r.conj().transpose(0, 1, 3, 2)
+    got = prop.pi(hermitian=not nonortho)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    if not nonortho:
+        _, Q = O.rhs(p.consts(), s0, ps0, om0, p.H0, np.zeros(NL), return_pi=True)
+        assert np.abs(got - Q).max() < 1e-12 * np.abs(Q).max()
+    d = str(tmp_path / "x_save")
+    w = rundir.SaveWriter(d, NL, saveevery=3, t0=0.0, sig0=prop.sigma(), save_pi=True)
+    rundir.propagate_to_save(prop, w, 0.0, 0.05, 0.01, 1e-8)
+    pia = np.load(os.path.join(d, "PIa3.npy"))
+    assert pia.shape == (3, NK, NL, n, n) and pia.dtype == np.complex64
+    assert np.abs(pia[0] - want).max() < 1e-6 * np.abs(want).max()
+    prop.close()

@@ -76,6 +76,7 @@ SIGNATURES = {
     "zh_set_dissipator": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                     _dp]),
     "zh_set_overlap": (C.c_int, [C.c_void_p, _dp]),
+    "zh_pi_get": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "zh_set_pulse_ex": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _dp, _dp]),
     "zh_set_pulse_spline": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "zh_contour_density": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,

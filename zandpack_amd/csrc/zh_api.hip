@@ -92,6 +92,7 @@ struct EngineBase {
   virtual void set_pulse_spline(int nknots, const double* x, const double* c) = 0;
   virtual void set_dissipator(const DissDesc& d, const double* sig_eq) = 0;
   virtual void set_overlap(const double* Sinv) = 0;
+  virtual void pi_get(int hermitian, double* Q) = 0;
   virtual void state_put(const double* sig, const double* psi, const double* omg) = 0;
   virtual void state_get(double* sig, double* psi, double* omg) = 0;
   virtual void sigma_get(double* sig) = 0;
@@ -237,6 +238,7 @@ struct Engine : EngineBase {
   cs* Yov = nullptr;          // ħ GL_P ξ_c + Ω_s ξ_c / ħ (NK, Md, nc)
   int* cpos_none = nullptr;   // (n) all -1: the coupled-column epilogue terms off
   std::vector<int> cidx_h; // coupled orbitals in device column order
+  std::vector<int4> lrange_h;  // per lead: device modes in the active and passive blocks
 
   // device buffers
   int *cidx = nullptr, *cpos = nullptr, *lead = nullptr;
@@ -576,6 +578,7 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpyAsync(gmask, gm.data(), sizeof(int) * gm.size(), hipMemcpyHostToDevice,
                           stream));
     HIPCHK(hipMemcpyAsync(lrange, lr.data(), sizeof(int4) * NL, hipMemcpyHostToDevice, stream));
+    lrange_h = lr;
     HIPCHK(hipMemcpyAsync(lead, ld.data(), sizeof(int) * Md, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(glp, g1.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(chip, x1.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
@@ -694,6 +697,44 @@ struct Engine : EngineBase {
     if (U != 0.0) {
       HIPCHK(hipMemcpyAsync(n0, n0_, sizeof(double) * NK * n, hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
+    }
+  }
+
+  // Π_α of the resident state per lead (the stage-0 Π of the next attempt,
+  // Zandpack/mpi/zand:363-366 Q_np; nozand:484-486 PI_np): Q (NK, NL, n, n) complex128
+  // = (Π_α + Π_α†)/ħ (hermitian) or Π_α/ħ, Π_α[o, o'] = Σ_{m in α} ψ_m[o] Ixi_m[o'].
+  void pi_get(int hermitian, double* Q) override {
+    require(have_consts && have_state, "Pi needs constants and a state", ZH_ESTATE);
+    HIPCHK(hipSetDevice(device));
+    const long nn = (long)n * n;
+    std::memset(Q, 0, sizeof(double) * 2 * (size_t)NK * NL * nn);
+    if (nc == 0) return;
+    std::vector<double2> part((size_t)NK * n * nc);
+    for (int a = 0; a < NL; ++a) {
+      const int4 r = lrange_h[a];
+      GemmArgs<R, double> g{n, nc, Md, psi[iy0], n, (long)Md * n, Ixic, nc, (long)Md * nc};
+      EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
+      HIPCHK((launch_zgemm<double, R, double, true, false, Tile<64, 32, 16, 2, false, 3>, 3>(
+          g, NK, e, stream, KSegFixed{r.x, r.y, r.z, r.w})));
+      HIPCHK(hipMemcpyAsync(part.data(), Pic, sizeof(double2) * part.size(),
+                            hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      const double ih = 1.0 / hbar;
+      for (int b = 0; b < NK; ++b) {
+        double* q = Q + 2 * ((size_t)b * NL + a) * nn;
+        for (int o = 0; o < n; ++o)
+          for (int c = 0; c < nc; ++c) {
+            const int oc = cidx_h[c];
+            if (oc < 0) continue;
+            const double2 v = part[((size_t)b * n + o) * nc + c];
+            q[2 * ((size_t)o * n + oc)] += v.x * ih;
+            q[2 * ((size_t)o * n + oc) + 1] += v.y * ih;
+            if (hermitian) {
+              q[2 * ((size_t)oc * n + o)] += v.x * ih;
+              q[2 * ((size_t)oc * n + o) + 1] -= v.y * ih;
+            }
+          }
+      }
     }
   }
 
@@ -2033,6 +2074,13 @@ int zh_contour_density(int NK, int n, int nE, int nc, const double* contour,
   return zh_try([&] {
     require(contour && evals && evecs && rho, "null argument");
     contour_density(NK, n, nE, nc, contour, evals, evecs, corb, sigma_cc, rho, device);
+  });
+}
+
+int zh_pi_get(zh_problem* p, int hermitian, double* Q) {
+  return zh_try([&] {
+    require(p && Q, "null argument");
+    p->eng->pi_get(hermitian, Q);
   });
 }
 

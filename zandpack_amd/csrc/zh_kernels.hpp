@@ -793,6 +793,16 @@ struct KSegLead {
   __device__ inline bool skip(int, int, int, int) const { return false; }
 };
 
+// The same two K ranges for every tile (a lead's active and passive device modes).
+struct KSegFixed {
+  int lo0, hi0, lo1, hi1;
+  __device__ inline KRange get(int, int, int, int, int) const {
+    const bool a = hi0 > lo0, b = hi1 > lo1;
+    return KRange{a ? lo0 : lo1, a ? hi0 : hi1, lo1, hi1, (int)a + (int)b};
+  }
+  __device__ inline bool skip(int, int, int, int) const { return false; }
+};
+
 // [ΔP ξ_c][ħ σ_cᵀ] (K = coupled orbitals): a row tile's K range is its leads'
 // orbitals; tiles past ML (passive rows, ΔP = 0) are skipped when skip_passive
 // (their consumer then reads no partial product for those rows).

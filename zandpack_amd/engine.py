@@ -153,6 +153,13 @@ class Propagator:
         N.check(self.lib.zh_state_get(self.h, N.ptr(s), N.ptr(p), N.ptr(o)))
         return s, p, o
 
+    def pi(self, hermitian=True):
+        """Per-lead Π of the resident state (NK, NL, n, n): (Π_α + Π_α†)/ħ as zand's
+        Q_np, or Π_α/ħ as nozand's PI_np with ``hermitian=False``."""
+        Q = np.empty((self.NK, self.NL, self.n, self.n), complex)
+        N.check(self.lib.zh_pi_get(self.h, 1 if hermitian else 0, N.ptr(Q)))
+        return Q
+
     def sigma(self):
         s = np.empty((self.NK, self.n, self.n), complex)
         N.check(self.lib.zh_sigma_get(self.h, N.ptr(s)))

@@ -10,7 +10,7 @@ reads the results:
         [t0=0 saveevery=100 n_dm_compress=1 usesave=False checkpoints=10,20 \\
          precision=fp64 hbar=0.6582119569 pulse=step amp=0.1 p1=0 p2=0 p3=0 p4=0 \\
          weights=0.5,-0.5 [pulse=air_photonics|toptica pulse_table=<ExpPulses .npy>] \\
-         hubbard_U=0 hubbard_n0=<file.npy> steady_state=False nonorthogonal=False]
+         hubbard_U=0 hubbard_n0=<file.npy> steady_state=False nonorthogonal=False save_PI=False]
 
 Keywords follow ``Initial.py`` (Zandpack/mpi/zand:57-71).  ``nonorthogonal=True``
 runs the directory as ``mpi/nozand`` does: ``Arrays/S^(-0.5).npy`` carries the
@@ -117,8 +117,10 @@ class SaveWriter:
     carries back with S^-½ (Zandpack/mpi/nozand:720-723); σ stays in its own basis."""
 
     def __init__(self, save_dir, NL, saveevery=100, n_dm_compress=1, t0=0.0, sig0=None,
-                 S05=None):
+                 S05=None, save_pi=False):
         self.S05 = S05
+        self.save_pi = bool(save_pi)
+        self.PI = []
         self.dir = save_dir
         self.NL = NL
         self.saveevery = int(saveevery)
@@ -135,8 +137,11 @@ class SaveWriter:
     def _p(self, f):
         return os.path.join(self.dir, f)
 
-    def step(self, t, sig, J):
-        """One accepted step: the new time, the new σ and the stage-0 currents J (NL, NK)."""
+    def step(self, t, sig, J, pi=None):
+        """One accepted step: the new time, the new σ, the stage-0 currents J (NL, NK)
+        and, with save_pi, the stage-0 per-lead Π (NK, NL, n, n)."""
+        if self.save_pi:
+            self.PI.append(np.array(pi))
         for a in range(self.NL):
             self.currents[a].append(np.array(J[a], float))
         self.steps += 1
@@ -152,6 +157,9 @@ class SaveWriter:
         if len(dm_t) > L:
             dm_t = dm_t[1:]
         np.save(self._p(f"DMt{s}"), dm_t[::self.n])
+        if self.save_pi:  # PIa<step> (Zandpack/mpi/zand:444-448)
+            np.save(self._p(f"PIa{s}"), np.array(self.PI[::self.n]).astype(np.complex64))
+            self.PI = []
         np.save(self._p(f"times{s}"), self.times[-L:])
         for a, c in enumerate(self.currents):
             np.save(self._p(f"current_{a}_{s}"), np.array(c[-L:]))
@@ -192,6 +200,9 @@ def propagate_to_save(prop, writer, t0, t1, h, eps, checkpoints=(), time_update=
     while t < t1:
         due = (t > ck) & ~done
         snap = prop.get_state() if due.any() else None
+        # the stage-0 Π of the step (every attempt of the step starts from the same state):
+        # zand saves Q_np = (Π + Π†)/ħ, nozand PI_np = Π/ħ
+        pi = prop.pi(hermitian=prop.problem.Sinv is None) if writer.save_pi else None
         while True:
             attempts += 1
             if attempts > max_attempts:
@@ -208,7 +219,7 @@ def propagate_to_save(prop, writer, t0, t1, h, eps, checkpoints=(), time_update=
             writer.checkpoint(t, h_old, *snap)
             done[np.nonzero(due)[0][0]] = True
         t = t + (h if time_update == "reference" else h_old)
-        writer.step(t, prop.sigma(), r["J"])
+        writer.step(t, prop.sigma(), r["J"], pi)
     return t, h, acc, rej
 
 
@@ -255,7 +266,8 @@ def main(argv=None):
         print(f"steady state: {its} iterations, relative residual {res:.2e}", flush=True)
     ck = [float(x) for x in kw.get("checkpoints", "").split(",") if x]
     w = SaveWriter(save_dir, p.dims[2], int(f("saveevery", 100)), int(f("n_dm_compress", 1)),
-                   t0, prop.sigma(), S05=None if Sm05 is None else np.linalg.inv(Sm05))
+                   t0, prop.sigma(), S05=None if Sm05 is None else np.linalg.inv(Sm05),
+                   save_pi=b("save_PI", False))
     t, h, acc, rej = propagate_to_save(prop, w, t0, t1, h, f("eps", 1e-8), ck,
                                        kw.get("time_update", "reference"))
     print(f"t={t:.6g} accepted={acc} rejected={rej} h={h:.6g}", flush=True)
