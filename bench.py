@@ -346,6 +346,7 @@ def batched_leg(n_inst, steps, seed=0):
     st = Server(hbm_bytes=200 << 30, max_live=n_inst).run(reqs, realtime=False)
     return {"instances": n_inst, "n_values": sorted(int(x) for x in ns), "steps_each": steps,
             "steps_per_s": st["steps_per_s"], "wall_s": st["wall_s"], "failed": st["failed"],
+            "admit_s": st["admit_s"], "stepping_steps_per_s": st["stepping_steps_per_s"],
             "precision": "mixed"}
 
 
@@ -372,6 +373,7 @@ def serving_leg(n_req, rate, t_scale, seed=0):
     return {"requests": n_req, "rate_per_s": rate, "t_end_scale": t_scale,
             "p50_s": st["p50_s"], "p99_s": st["p99_s"], "steps_per_s": st["steps_per_s"],
             "wall_s": st["wall_s"], "failed": st["failed"], "steps": st["steps"],
+            "admit_s": st["admit_s"], "stepping_steps_per_s": st["stepping_steps_per_s"],
             "sampled_J_rel_err_vs_fp64": errs}
 
 

@@ -110,6 +110,7 @@ class Server:
         used = 0
         self.peak_bytes = 0
         ticks = 0
+        t_admit = t_step = 0.0
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
         while pending or live:
@@ -119,7 +120,9 @@ class Server:
                    <= self.budget):
                 r = pending.pop(0)
                 r.admitted = time.perf_counter() - t0
+                ta = time.perf_counter()
                 lv = _Live(r, self.graph, self.autotune)
+                t_admit += time.perf_counter() - ta
                 used += lv.bytes
                 self.peak_bytes = max(self.peak_bytes, used)
                 live.append(lv)
@@ -129,6 +132,7 @@ class Server:
                 continue
             finished = []
             launched = []
+            ts = time.perf_counter()
             for lv in live:
                 try:
                     lv.prop.launch(lv.t, lv.h, lv.req.eps)
@@ -159,6 +163,7 @@ class Server:
                 elif lv.attempts >= lv.req.max_attempts:
                     lv.req.error = "attempt limit reached"
                     finished.append(lv)
+            t_step += time.perf_counter() - ts
             ticks += 1
             for lv in finished:
                 live.remove(lv)
@@ -169,15 +174,20 @@ class Server:
                 used -= lv.bytes
                 done.append(lv.req)
         wall = time.perf_counter() - t0
-        return self.stats(done, wall, ticks)
+        return self.stats(done, wall, ticks, t_admit, t_step)
 
     @staticmethod
-    def stats(done, wall, ticks):
+    def stats(done, wall, ticks, t_admit=0.0, t_step=0.0):
+        """Aggregate figures; ``steps_per_s`` over the whole wall time (admission,
+        i.e. problem set-up and upload, included), ``stepping_steps_per_s`` over the
+        time spent launching and collecting attempts only."""
         ok = [r for r in done if r.error is None]
         lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
         steps = int(sum(r.steps for r in done))
         return dict(requests=len(done), failed=len(done) - len(ok), wall_s=wall, ticks=ticks,
                     steps=steps, steps_per_s=steps / wall if wall > 0 else 0.0,
+                    admit_s=t_admit, stepping_s=t_step,
+                    stepping_steps_per_s=steps / t_step if t_step > 0 else 0.0,
                     p50_s=float(np.percentile(lat, 50)) if lat.size else None,
                     p99_s=float(np.percentile(lat, 99)) if lat.size else None,
                     done=done)
