@@ -192,7 +192,9 @@ typedef struct zh_precond {
 int zh_steady_state(zh_problem* p, double t0, double tol, int maxiter, const zh_precond* pc,
                     int* iters, double* rel_resid);
 
-/* Resident state y0 = (sigma, Psi, Omega), reference layout, complex128. */
+/* Resident state y0 = (sigma, Psi, Omega), reference layout, complex128.  A NULL
+ * psi or omg puts zeros (cleared on the device, nothing uploaded): the usual start
+ * of a propagation from sigma0 (Zandpack/mpi/zand:535-538). */
 int zh_state_put(zh_problem* p, const double* sig, const double* psi, const double* omg);
 int zh_state_get(zh_problem* p, double* sig, double* psi, double* omg);
 int zh_sigma_get(zh_problem* p, double* sig);

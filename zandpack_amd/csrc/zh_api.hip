@@ -955,8 +955,19 @@ struct Engine : EngineBase {
     require(have_consts, "set the constants first", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
     upload(sig[iy0], n, s, (long)NK * n, n);
-    upload_psi(psi[iy0], n, p);
-    const bool ffz = upload_omega(omg[iy0], o);
+    // a null Ψ or Ω is zero: cleared on the device instead of uploaded
+    if (p) {
+      upload_psi(psi[iy0], n, p);
+    } else {
+      HIPCHK(hipMemsetAsync(psi[iy0], 0, sizeof(cs) * (size_t)NK * Md * n, stream));
+    }
+    bool ffz = true;
+    if (o) {
+      ffz = upload_omega(omg[iy0], o);
+    } else {
+      HIPCHK(hipMemsetAsync(omg[iy0], 0, sizeof(cs) * (size_t)NK * Md * Md, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
     if (!ffz) ff_dirty = true;
     set_ff(ffz);
     have_state = true;
@@ -2093,7 +2104,7 @@ int zh_set_overlap(zh_problem* p, const double* Sinv) {
 
 int zh_state_put(zh_problem* p, const double* sig, const double* psi, const double* omg) {
   return zh_try([&] {
-    require(p && sig && psi && omg, "null argument");
+    require(p && sig, "null argument");
     p->eng->state_put(sig, psi, omg);
   });
 }

@@ -48,7 +48,10 @@ class Propagator:
         if p.Sinv is not None:
             self.set_overlap(p.Sinv)
         N.check(self.lib.zh_set_graph(self.h, 1 if graph else 0))
-        self.put_state(*(state if state is not None else p.zero_state()))
+        if state is not None:
+            self.put_state(*state)
+        else:
+            self.put_state(p.sigma0)
         self.tiles = self.autotune() if autotune else None
 
     # --------------------------------------------------------------- set-up
@@ -139,12 +142,19 @@ class Propagator:
         N.check(self.lib.zh_set_profile(self.h, 1 if on else 0))
 
     # --------------------------------------------------------------- state
-    def put_state(self, sig, psi, omg):
-        s, p, o = N.c128(sig), N.c128(psi), N.c128(omg)
+    def put_state(self, sig, psi=None, omg=None):
+        """Resident state; a None Ψ or Ω is zero (cleared on the device)."""
+        s = N.c128(sig)
         assert s.shape == (self.NK, self.n, self.n)
-        assert p.shape == (self.NK, self.NL, self.NX, self.NOT, self.n)
-        assert o.shape == (self.NK, self.NL, self.NX, self.NOT, self.NL, self.NX, self.NOT)
-        N.check(self.lib.zh_state_put(self.h, N.ptr(s), N.ptr(p), N.ptr(o)))
+        p = o = None
+        if psi is not None:
+            p = N.c128(psi)
+            assert p.shape == (self.NK, self.NL, self.NX, self.NOT, self.n)
+        if omg is not None:
+            o = N.c128(omg)
+            assert o.shape == (self.NK, self.NL, self.NX, self.NOT, self.NL, self.NX, self.NOT)
+        N.check(self.lib.zh_state_put(self.h, N.ptr(s), None if p is None else N.ptr(p),
+                                      None if o is None else N.ptr(o)))
 
     def get_state(self):
         s = np.empty((self.NK, self.n, self.n), complex)
