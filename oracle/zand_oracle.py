@@ -30,6 +30,8 @@ Pinning (what makes this oracle trustworthy; see tests/test_oracle_*.py):
                          block of dΩ vanishes, von-Neumann closed form when ξ=0,
                          closed-form stationary Ω makes dΩ=0.
 * ``propagate``       -- von-Neumann closed form σ(t)=U σ0 U† to the DOP54 error level.
+* ``rkf45_propagate`` -- Fehlberg rationals (order conditions, row sums) and the same
+                         von-Neumann closed form.
 * ``rhs(Sinv=...)``   -- the nozand form equals the orthogonal form carried to the
                          non-orthogonal basis (dσ_n = S^-½ dσ_o S^-½, dΨ_n = S^-½ dΨ_o,
                          dΩ and J unchanged); S = I reduces it to zand.
@@ -63,6 +65,78 @@ def dop54_rationals():
     b4 = [Fr(5179, 57600), Fr(0), Fr(7571, 16695), Fr(393, 640), Fr(-92097, 339200),
           Fr(187, 2100), Fr(1, 40)]
     return c, a, b5, b4
+
+
+def rkf45_rationals():
+    """Exact Runge-Kutta-Fehlberg 4(5) coefficients of the in-process driver
+    AdaptiveRK4 (TimedependentTransport.py:3191-3203; Optimized_RK45.py:34-46):
+    nodes c, stage matrix a, the fifth-order weights CH the step is taken with and
+    the fourth-order weights C (error weights CT = CH - C)."""
+    c = [Fr(0), Fr(1, 4), Fr(3, 8), Fr(12, 13), Fr(1), Fr(1, 2)]
+    a = [
+        [],
+        [Fr(1, 4)],
+        [Fr(3, 32), Fr(9, 32)],
+        [Fr(1932, 2197), Fr(-7200, 2197), Fr(7296, 2197)],
+        [Fr(439, 216), Fr(-8), Fr(3680, 513), Fr(-845, 4104)],
+        [Fr(-8, 27), Fr(2), Fr(-3544, 2565), Fr(1859, 4104), Fr(-11, 40)],
+    ]
+    ch = [Fr(16, 135), Fr(0), Fr(6656, 12825), Fr(28561, 56430), Fr(-9, 50), Fr(2, 55)]
+    c4 = [Fr(25, 216), Fr(0), Fr(1408, 2565), Fr(2197, 4104), Fr(-1, 5), Fr(0)]
+    return c, a, ch, c4
+
+
+def rkf45_propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
+                    fixed_h=False, max_attempts=100000):
+    """The in-process adaptive RKF45 loop (AdaptiveRK4, TimedependentTransport.py:
+    3179-3390; AdaptiveRK4_Opti, Optimized_RK45.py:19-228): six stages per attempt,
+    TE = sqrt(Σ |Σ_j CT_j k_j|²) over σ, Ψ, Ω (:3215-3221), h <- 0.9 h (eps/TE)^(1/5)
+    after every attempt, accept iff TE <= eps (the ``while TE > eps`` loop), step
+    y += Σ CH_j k_j with the k of the accepted attempt, currents and σ recorded from
+    the state before the step, ``while t0 <= t1`` and t += h with the new h.
+    Returns dict(times (start time of every step), J, sigma, y, h_accepted, n_reject)."""
+    c, a, ch, c4 = rkf45_rationals()
+    c = [float(x) for x in c]
+    a = [[float(x) for x in row] for row in a]
+    ch = np.array([float(x) for x in ch])
+    ct = ch - np.array([float(x) for x in c4])
+    y = tuple(np.array(v, complex) for v in y0)
+    times, Js, sigmas, hs = [], [], [], []
+    n_rej, attempts, t = 0, 0, t0
+    while t <= t1:
+        while True:
+            attempts += 1
+            if attempts > max_attempts:
+                raise RuntimeError("attempt limit reached")
+            ks = []
+            for i in range(6):
+                ts = t + c[i] * h
+                yi = [y[p] + sum(a[i][j] * ks[j][p] for j in range(i)) for p in range(3)]
+                Ht = H0 + dH(ts, yi[0])
+                delta = np.array([bias(ts, al) for al in range(NL)], float)
+                ks.append(rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar))
+            h_used = h
+            if fixed_h:
+                break
+            te = 0.0
+            for p in range(3):
+                e = sum(ct[j] * ks[j][p] for j in range(6))
+                te += float(np.sum(e.real ** 2 + e.imag ** 2))
+            TE = te ** 0.5
+            if not np.isfinite(TE):
+                raise FloatingPointError("non-finite error estimate")
+            h = 0.9 * h * (eps / TE) ** 0.2
+            if TE <= eps:
+                break
+            n_rej += 1
+        sigmas.append(y[0].copy())
+        Js.append(currents(consts, y[1], hbar=hbar))
+        y = tuple(y[p] + sum(ch[j] * ks[j][p] for j in range(6)) for p in range(3))
+        times.append(t)
+        hs.append(h_used)
+        t = t + h
+    return dict(times=np.array(times), J=np.array(Js), sigma=np.array(sigmas), y=y,
+                h_accepted=np.array(hs), n_reject=n_rej)
 
 
 def dop54_tableau():

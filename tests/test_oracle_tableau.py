@@ -5,6 +5,7 @@ from fractions import Fraction as Fr
 import numpy as np
 
 from oracle import zand_oracle as O
+from zandpack_amd import builder
 
 
 def test_order_conditions_exact():
@@ -35,3 +36,39 @@ def test_float_layout():
 def test_hnew():
     assert O.hnew(1.0, 1.0, 1.0) == 0.9
     assert abs(O.hnew(0.1, 32e-6, 1e-6) - 0.9 * 0.1 * 0.5) < 1e-15
+
+
+def test_rkf45_order_conditions():
+    """Fehlberg 4(5) of the in-process driver: row sums, fifth-order conditions of
+    the step weights CH, fourth-order ones of the embedded C."""
+    from fractions import Fraction as Fr
+    c, a, ch, c4 = O.rkf45_rationals()
+    for i in range(1, 6):
+        assert sum(a[i]) == c[i]
+    for q in range(5):
+        assert sum(w * ci ** q for w, ci in zip(ch, c)) == Fr(1, q + 1)
+    for q in range(4):
+        assert sum(w * ci ** q for w, ci in zip(c4, c)) == Fr(1, q + 1)
+    # the order-3 condition Σ b_i Σ_j a_ij c_j = 1/6
+    s = sum(ch[i] * sum(a[i][j] * c[j] for j in range(i)) for i in range(6))
+    assert s == Fr(1, 6)
+
+
+def test_rkf45_von_neumann():
+    """ξ = 0: the RKF45 loop follows σ(t) = U σ0 U† to its error level."""
+    p = builder.config_small(seed=2)
+    p.pulse = builder.Pulse()
+    c = p.consts()
+    c = {k: (np.zeros_like(v) if k in ("xi", "Ixi") else v) for k, v in c.items()}
+    rng = np.random.default_rng(0)
+    n = p.dims[1]
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    s0 = (A @ A.conj().T)[None] / n
+    y0 = (s0, np.zeros_like(p.xi), p.zero_state()[2])
+    out = O.rkf45_propagate(c, y0, 0.0, 0.5, 0.01, 1e-9, p.H0, O.zero_dH, p.bias, 2)
+    T = float(np.sum(out["h_accepted"]))
+    e, V = np.linalg.eigh(p.H0[0])
+    U = (V * np.exp(-1j * e * T / O.HBAR)) @ V.conj().T
+    ref = U @ s0[0] @ U.conj().T
+    assert np.abs(out["y"][0][0] - ref).max() < 1e-7 * np.abs(ref).max()
+    assert len(out["times"]) > 3 and out["times"][-1] <= 0.5

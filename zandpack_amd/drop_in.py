@@ -13,7 +13,13 @@
   currents from the stage-0 state, and ``t += h_next`` on acceptance unless
   ``time_update='correct'``.
 
-Both run every evaluation on the GPU (host arrays in and out per call); the
+* ``AdaptiveRK4(...)`` / ``AdaptiveRK4_Opti(...)``: the in-process Runge-Kutta-
+  Fehlberg 4(5) drivers (Zandpack/TimedependentTransport.py:3179-3390,
+  Zandpack/Optimized_RK45.py:19-228) with their own semantics: six stages, step with
+  the fifth-order weights, accept iff TE <= eps, h <- 0.9 h (eps/TE)^(1/5) after every
+  attempt, ``while t0 <= t1`` and t += the new h, h_guess default (t1 - t0)/1000.
+
+All run every evaluation on the GPU (host arrays in and out per call); the
 device-resident path with no per-stage transfers is ``Propagator.propagate``.
 """
 from __future__ import annotations
@@ -97,3 +103,66 @@ def AdaptiveDOP(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_
     data["density matrix"] = np.array(dms)
     data["state"] = tuple(y)
     return np.array(times), data
+
+
+_RKF_C = np.array([0.0, 1 / 4, 3 / 8, 12 / 13, 1.0, 1 / 2])
+_RKF_A = [
+    [],
+    [1 / 4],
+    [3 / 32, 9 / 32],
+    [1932 / 2197, -7200 / 2197, 7296 / 2197],
+    [439 / 216, -8.0, 3680 / 513, -845 / 4104],
+    [-8 / 27, 2.0, -3544 / 2565, 1859 / 4104, -11 / 40],
+]
+_RKF_CH = np.array([16 / 135, 0.0, 6656 / 12825, 28561 / 56430, -9 / 50, 2 / 55])
+_RKF_CT = np.array([1 / 360, 0.0, -128 / 4275, -2197 / 75240, 1 / 50, 2 / 55])
+
+
+def AdaptiveRK4(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_guess=None,
+                dH_given=True, fixed_mode=False, elec_names=("left", "right"),
+                max_attempts=10 ** 6, **_unused):
+    """Adaptive RKF45 over an ``f`` made by ``make_f`` (the in-process AdaptiveRK4's
+    loop).  Returns (times, data): the start time of every accepted step, and
+    ``data['current_<elec>']`` (list of (NK,) arrays, from the state before the
+    step), ``data['density matrix']`` (σ before each step), ``data['state']``."""
+    h = (t1 - t0) / 1000 if h_guess is None else float(h_guess)
+    y = [np.array(sig0, complex), np.array(psi0, complex), np.array(omega0, complex)]
+    data = {f"current_{e}": [] for e in elec_names}
+    dms, times = [], []
+    t = float(t0)
+    attempts = 0
+    while t <= t1:
+        while True:
+            attempts += 1
+            if attempts > max_attempts:
+                raise RuntimeError("attempt limit reached")
+            ks = []
+            J0 = None
+            for i in range(6):
+                yi = [y[p] + sum(_RKF_A[i][j] * ks[j][p] for j in range(i)) for p in range(3)]
+                d = f(t + _RKF_C[i] * h, yi[0], yi[1], yi[2], dH, delta_func, dH_given=dH_given)
+                if i == 0:
+                    J0 = f.last_J
+                ks.append(tuple(h * v for v in d))
+            if fixed_mode:
+                break
+            te = np.sqrt(sum(float(np.sum(np.abs(sum(_RKF_CT[j] * ks[j][p] for j in range(6))) ** 2))
+                             for p in range(3)))
+            if not np.isfinite(te):
+                raise FloatingPointError("non-finite error estimate")
+            h = 0.9 * h * (eps / te) ** 0.2
+            if te <= eps:
+                break
+        dms.append(y[0].copy())
+        for a, e in enumerate(elec_names):
+            data[f"current_{e}"].append(J0[a].copy())
+        y = [y[p] + sum(_RKF_CH[j] * ks[j][p] for j in range(6)) for p in range(3)]
+        times.append(t)
+        t += h
+    data["density matrix"] = np.array(dms)
+    data["state"] = tuple(y)
+    return np.array(times), data
+
+
+# Optimized_RK45.AdaptiveRK4_Opti is the same method with vectorised stage sums.
+AdaptiveRK4_Opti = AdaptiveRK4
