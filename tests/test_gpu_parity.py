@@ -320,6 +320,40 @@ def test_drop_in_rkf45(native, driver):
     prop.close()
 
 
+def test_drop_in_scipy_ode(native):
+    """solve_ivp over the GPU f against solve_ivp over the oracle's RHS."""
+    from scipy.integrate import solve_ivp
+    from zandpack_amd import drop_in
+    from zandpack_amd.engine import Propagator
+    p = CASES["small"]()
+    prop = Propagator(p, "fp64")
+    f = drop_in.make_f(prop)
+    y0 = p.zero_state()
+    t_eval = np.array([0.1, 0.2, 0.3])
+    say("scipy_ode")
+    t, rho, cl = drop_in.scipy_ode(f, *y0, 0.0, t_eval, p.dH, p.bias, p.Ixi, atol=1e-8, rtol=1e-6)
+    shapes = [a.shape for a in y0]
+    sizes = [a.size for a in y0]
+
+    def F(tt, y):
+        parts = np.split(y, np.cumsum(sizes)[:-1])
+        s, ps, om = (v.reshape(sh) for v, sh in zip(parts, shapes))
+        delta = np.array([p.bias(tt, a) for a in range(2)])
+        d = O.rhs(p.consts(), s, ps, om, p.H0 + p.dH(tt, s), delta)
+        return np.concatenate([v.ravel() for v in d])
+
+    y = np.concatenate([v.ravel().astype(complex) for v in y0])
+    tp = 0.0
+    for i, te in enumerate(t_eval):
+        y = solve_ivp(F, (tp, te), y, method="RK45", t_eval=[te], rtol=1e-6, atol=1e-8).y[:, -1]
+        parts = np.split(y, np.cumsum(sizes)[:-1])
+        s, ps = parts[0].reshape(shapes[0]), parts[1].reshape(shapes[1])
+        assert rel(rho[i], s[0]) < 1e-9
+        assert rel(cl[i], O.currents(p.consts(), ps)[0]) < 1e-8
+        tp = te
+    prop.close()
+
+
 def test_drop_in_f_and_adaptive_dop(native):
     """The reference call surface: make_f + AdaptiveDOP vs the oracle driver."""
     from zandpack_amd.drop_in import AdaptiveDOP, make_f

@@ -19,6 +19,12 @@
   the fifth-order weights, accept iff TE <= eps, h <- 0.9 h (eps/TE)^(1/5) after every
   attempt, ``while t0 <= t1`` and t += the new h, h_guess default (t1 - t0)/1000.
 
+* ``scipy_ode(f, sig0, psi0, omg0, t0, t_eval, dH, delta_variant, Ixi, ...)``: the
+  reference's hand-off of ``f`` to ``scipy.integrate.solve_ivp``
+  (Zandpack/TimedependentTransport.py:3591-3673): σ, Ψ, Ω flattened into one complex
+  vector, one solve_ivp call per output interval; returns (t_eval, σ[k=0] at each
+  output time, the first lead's current (NK,) at each output time).
+
 All run every evaluation on the GPU (host arrays in and out per call); the
 device-resident path with no per-stage transfers is ``Propagator.propagate``.
 """
@@ -166,3 +172,46 @@ def AdaptiveRK4(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_
 
 # Optimized_RK45.AdaptiveRK4_Opti is the same method with vectorised stage sums.
 AdaptiveRK4_Opti = AdaptiveRK4
+
+
+def scipy_ode(f, sig0, psi0, omg0, t0, t_eval, dH, delta_variant, Ixi=None, dH_given=True,
+              method="RK45", dt_guess=None, atol=1e-6, rtol=1e-4):
+    """solve_ivp over an ``f`` made by ``make_f`` (TimedependentTransport.py:3625-3673).
+
+    One ``solve_ivp`` call per interval (t_prev, t_eval[i]) from the state reached at
+    the previous output time; after each, σ[k=0] and the first lead's k-resolved
+    current are recorded.  The current is the one the GPU forms with the problem's
+    Ixi (the reference recomputes it on the host from the ``Ixi`` argument, which must
+    be the same array; it is accepted here for the call signature).  ``method`` is
+    passed to solve_ivp (the reference accepts it but always runs solve_ivp's default
+    RK45); ``dt_guess`` is accepted and unused, as in the reference."""
+    from scipy.integrate import solve_ivp
+    shapes = [np.shape(sig0), np.shape(psi0), np.shape(omg0)]
+    sizes = [int(np.prod(s)) for s in shapes]
+
+    def split(y):
+        out, o = [], 0
+        for sh, sz in zip(shapes, sizes):
+            out.append(y[o:o + sz].reshape(sh))
+            o += sz
+        return out
+
+    def F(t, y):
+        s, p, o = split(y)
+        d = f(t, s, p, o, dH, delta_variant, dH_given=dH_given)
+        return np.concatenate([np.ravel(v) for v in d])
+
+    t_eval = np.asarray(t_eval, float)
+    assert (t0 < t_eval).all()
+    y = np.concatenate([np.ravel(np.asarray(v, complex)) for v in (sig0, psi0, omg0)])
+    rho, cl = [], []
+    t_prev = float(t0)
+    for te in t_eval:
+        sol = solve_ivp(F, (t_prev, te), y, method=method, t_eval=[te], rtol=rtol, atol=atol)
+        y = sol.y[:, -1]
+        s, p, o = split(y)
+        rho.append(s[0].copy())
+        f(te, s, p, o, dH, delta_variant, dH_given=dH_given)  # currents of the new state
+        cl.append(np.array(f.last_J[0]))
+        t_prev = float(te)
+    return t_eval, np.array(rho), np.array(cl)
