@@ -66,6 +66,18 @@ template <> __host__ __device__ inline float2 from_d2<float2>(double2 v) {
   return make_float2((float)v.x, (float)v.y);
 }
 
+// Load of a slot element that is streamed once per pass (non-temporal: it does not
+// displace the reused operands of the concurrent GEMMs from L2).
+template <typename C> __device__ inline C ld_stream(const C* p) {
+  typedef decltype(p->x) E;
+  typedef E v2 __attribute__((ext_vector_type(2)));
+  const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  C v;
+  v.x = x.x;
+  v.y = x.y;
+  return v;
+}
+
 __device__ inline double pulse_value(const PulseDesc& p, double t) {
   switch (p.kind) {
     case ZH_PULSE_STEP: return t >= p.p1 ? p.amp : 0.0;
@@ -146,9 +158,9 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<C> g, Map mp) {
     const int cols = mp.row(r, src, o1, o2);
     for (int c = threadIdx.x; c < cols; c += blockDim.x) {
       C kv[NT + 1];
-      kv[0] = g.y0[src + c];
+      kv[0] = ld_stream(g.y0 + src + c);
 #pragma unroll
-      for (int j = 0; j < NT; ++j) kv[j + 1] = g.k[j][src + c];
+      for (int j = 0; j < NT; ++j) kv[j + 1] = ld_stream(g.k[j] + src + c);
       double vr = (double)kv[0].x, vi = (double)kv[0].y;
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
@@ -376,7 +388,7 @@ __global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial
     for (int c = threadIdx.x; c < cols; c += blockDim.x) {
       C kv[NW];
 #pragma unroll
-      for (int j = 0; j < NW; ++j) kv[j] = g.k[j][src + c];
+      for (int j = 0; j < NW; ++j) kv[j] = ld_stream(g.k[j] + src + c);
       double vr = 0.0, vi = 0.0;
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
@@ -575,7 +587,7 @@ struct EpiOmegaRK {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       off[r] = ((long)b * M + (rows[r] < 0 ? 0 : rows[r])) * M + col;
-      const double2 y = rows[r] < 0 ? make_double2(0.0, 0.0) : to_d2(y0[off[r]]);
+      const double2 y = rows[r] < 0 ? make_double2(0.0, 0.0) : ld_nt(y0 + off[r]);
       sr[r] = y.x; si[r] = y.y; er[r] = 0.0; ei[r] = 0.0;
     }
 #pragma unroll
@@ -583,7 +595,7 @@ struct EpiOmegaRK {
       if (j < nterms) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const double2 x = rows[r] < 0 ? make_double2(0.0, 0.0) : to_d2(k[j][off[r]]);
+          const double2 x = rows[r] < 0 ? make_double2(0.0, 0.0) : ld_nt(k[j] + off[r]);
           sr[r] += a[j] * x.x;
           si[r] += a[j] * x.y;
           if constexpr (TE) {
@@ -592,22 +604,38 @@ struct EpiOmegaRK {
           }
         }
       }
-    const double2 vv = v[(long)b * M + col];
+    // col < M whenever any row of the block is inside
+    const bool any = rows[0] >= 0 || rows[1] >= 0 || rows[2] >= 0 || rows[3] >= 0;
+    const double2 vv = v[any ? (long)b * M + col : 0];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       if (rows[r] < 0) continue;
-      if (ynew) ynew[off[r]] = from_d2<C>(make_double2(sr[r], si[r]));
+      if (ynew) st_nt(ynew + off[r], make_double2(sr[r], si[r]));
       const double2 uu = u[(long)b * M + rows[r]];
       const double dr = uu.x - vv.x, di = uu.y - vv.y;
       // k_s = h acc + i (dr + i di) Ω_s
       const double2 t = cmul(make_double2(-di, dr), make_double2(sr[r], si[r]));
       const C ks = from_d2<C>(make_double2(re[r] + t.x, im[r] + t.y));
-      if (out) out[off[r]] = ks;
+      if (out) st_nt(out + off[r], to_d2(ks));
       if constexpr (TE) {
         const double xr = er[r] + ct_self * (double)ks.x, xi = ei[r] + ct_self * (double)ks.y;
         acc += xr * xr + xi * xi;
       }
     }
+  }
+  // Slot values are streamed once: non-temporal loads and stores keep them from
+  // evicting the GEMM operands from L2.
+  __device__ static inline double2 ld_nt(const C* p) {
+    typedef typename std::conditional<sizeof(C) == 16, double, float>::type E;
+    typedef E v2 __attribute__((ext_vector_type(2)));
+    const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return make_double2((double)x.x, (double)x.y);
+  }
+  __device__ static inline void st_nt(C* p, double2 v) {
+    typedef typename std::conditional<sizeof(C) == 16, double, float>::type E;
+    typedef E v2 __attribute__((ext_vector_type(2)));
+    const v2 x = {(E)v.x, (E)v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<v2*>(p));
   }
   // Four consecutive columns of one row (FP32 storage): 32-byte loads and stores
   // when all four are inside the matrix; every slot load first, then the stores.
@@ -661,8 +689,8 @@ struct EpiOmegaRK {
                                       double* xi) {
     if constexpr (sizeof(C) == 8) {
       if (vec) {
-        const float4* q = reinterpret_cast<const float4*>(p + off);
-        const float4 lo = q[0], hi = q[1];
+        const v4f* q = reinterpret_cast<const v4f*>(p + off);
+        const v4f lo = __builtin_nontemporal_load(q), hi = __builtin_nontemporal_load(q + 1);
         xr[0] = lo.x; xi[0] = lo.y; xr[1] = lo.z; xi[1] = lo.w;
         xr[2] = hi.x; xi[2] = hi.y; xr[3] = hi.z; xi[3] = hi.w;
         return;
@@ -679,9 +707,11 @@ struct EpiOmegaRK {
                                        const double* xi) {
     if constexpr (sizeof(C) == 8) {
       if (vec) {
-        float4* q = reinterpret_cast<float4*>(p + off);
-        q[0] = make_float4((float)xr[0], (float)xi[0], (float)xr[1], (float)xi[1]);
-        q[1] = make_float4((float)xr[2], (float)xi[2], (float)xr[3], (float)xi[3]);
+        v4f* q = reinterpret_cast<v4f*>(p + off);
+        const v4f lo = {(float)xr[0], (float)xi[0], (float)xr[1], (float)xi[1]};
+        const v4f hi = {(float)xr[2], (float)xi[2], (float)xr[3], (float)xi[3]};
+        __builtin_nontemporal_store(lo, q);
+        __builtin_nontemporal_store(hi, q + 1);
         return;
       }
     }
