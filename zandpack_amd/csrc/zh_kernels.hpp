@@ -78,6 +78,13 @@ template <typename C> __device__ inline C ld_stream(const C* p) {
   return v;
 }
 
+template <typename C> __device__ inline void st_stream(C* p, C v) {
+  typedef decltype(p->x) E;
+  typedef E v2 __attribute__((ext_vector_type(2)));
+  const v2 x = {v.x, v.y};
+  __builtin_nontemporal_store(x, reinterpret_cast<v2*>(p));
+}
+
 __device__ inline double pulse_value(const PulseDesc& p, double t) {
   switch (p.kind) {
     case ZH_PULSE_STEP: return t >= p.p1 ? p.amp : 0.0;
@@ -878,7 +885,7 @@ __global__ __launch_bounds__(256) void k_psi_ell(EpiPsi<C> e, long sAp, Ell h) {
     const int i = i0 + q;
     for (int o = threadIdx.x; o < n; o += blockDim.x)
       rows[q * n + o] =
-          (i < M) ? e.Ap[b * sAp + (long)i * e.ldAp + o] : from_d2<C>(make_double2(0, 0));
+          (i < M) ? ld_stream(e.Ap + b * sAp + (long)i * e.ldAp + o) : from_d2<C>(make_double2(0, 0));
   }
   if (threadIdx.x < RB) {
     const int i = i0 + threadIdx.x;
@@ -913,7 +920,7 @@ __global__ __launch_bounds__(256) void k_psi_ell(EpiPsi<C> e, long sAp, Ell h) {
         const int i = i0 + q;
         const long bm = (long)b * M + i;
         const bool rok = ok && i < M;
-        a0[u][q] = (rok && e.acc0 && i < e.acc0_rows) ? e.acc0[bm * n + o]
+        a0[u][q] = (rok && e.acc0 && i < e.acc0_rows) ? ld_stream(e.acc0 + bm * n + o)
                                                       : from_d2<C>(make_double2(0, 0));
         const bool cc = rok && cp[u] >= 0;
         xv[u][q] = cc ? e.xic[bm * e.nc + cp[u]] : from_d2<C>(make_double2(0, 0));
@@ -960,7 +967,7 @@ __global__ __launch_bounds__(256) void k_psi_ell(EpiPsi<C> e, long sAp, Ell h) {
           vr += gx.x + w.x * e.inv_hbar2;
           vi += gx.y + w.y * e.inv_hbar2;
         }
-        e.out[((long)b * M + i) * n + o] = from_d2<C>(make_double2(hs * vi, -hs * vr));
+        st_stream(e.out + ((long)b * M + i) * n + o, from_d2<C>(make_double2(hs * vi, -hs * vr)));
       }
     }
   }
