@@ -564,13 +564,17 @@ struct Engine : EngineBase {
       HIPCHK(hipMemcpyAsync(cidx, cidx_h.data(), sizeof(int) * nc, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(Ixic, ixic.data(), sizeof(double2) * mc, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(xic, xc.data(), sizeof(cs) * mc, hipMemcpyHostToDevice, stream));
-      // constant column blocks of Ap (right), Aomg (right) and Bomg (left)
-      HIPCHK(hipMemcpy2DAsync(Ap + n, sizeof(cs) * (n + nc), apr.data(), sizeof(cs) * nc,
-                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpy2DAsync(Aomg + nc, sizeof(cs) * 2 * nc, apr.data(), sizeof(cs) * nc,
-                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpy2DAsync(Bomg, sizeof(cs) * 2 * nc, bol.data(), sizeof(cs) * nc,
-                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyHostToDevice, stream));
+      // constant column blocks of Ap (right), Aomg (right) and Bomg (left): one
+      // contiguous upload into the Ωξ partials buffer (free outside an attempt), then
+      // strided device copies (a strided copy from pageable host memory goes row by row)
+      HIPCHK(hipMemcpyAsync(W, apr.data(), sizeof(cs) * mc, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(W + mc, bol.data(), sizeof(cs) * mc, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpy2DAsync(Ap + n, sizeof(cs) * (n + nc), W, sizeof(cs) * nc, sizeof(cs) * nc,
+                              (size_t)NK * Md, hipMemcpyDeviceToDevice, stream));
+      HIPCHK(hipMemcpy2DAsync(Aomg + nc, sizeof(cs) * 2 * nc, W, sizeof(cs) * nc,
+                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyDeviceToDevice, stream));
+      HIPCHK(hipMemcpy2DAsync(Bomg, sizeof(cs) * 2 * nc, W + mc, sizeof(cs) * nc,
+                              sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyDeviceToDevice, stream));
     }
     HIPCHK(hipMemcpyAsync(cpos, cp.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(t1, tr.data(), sizeof(int2) * ntile, hipMemcpyHostToDevice, stream));
@@ -833,18 +837,21 @@ struct Engine : EngineBase {
 
   // ------------------------------------------------------------- transfers
   // host complex128 sigma-like (rows x cols dense) -> device storage C (leading dim ld)
+  // (dense rows only: every caller passes ld == cols, and a strided copy from
+  // pageable host memory would go row by row)
   template <typename C>
   void upload(C* dst, long ld, const double* src, long rows, long cols) {
+    require(ld == cols, "upload: rows must be dense");
     if (sizeof(C) == sizeof(double2)) {
-      HIPCHK(hipMemcpy2DAsync(dst, sizeof(C) * ld, src, sizeof(double2) * cols,
-                              sizeof(double2) * cols, rows, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(dst, src, sizeof(double2) * rows * cols, hipMemcpyHostToDevice,
+                            stream));
       HIPCHK(hipStreamSynchronize(stream));
     } else {
       std::vector<C> tmp((size_t)rows * cols);
       for (size_t i = 0; i < tmp.size(); ++i)
         tmp[i] = from_d2<C>(make_double2(src[2 * i], src[2 * i + 1]));
-      HIPCHK(hipMemcpy2DAsync(dst, sizeof(C) * ld, tmp.data(), sizeof(C) * cols, sizeof(C) * cols,
-                              rows, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(dst, tmp.data(), sizeof(C) * tmp.size(), hipMemcpyHostToDevice,
+                            stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
   }
@@ -863,8 +870,9 @@ struct Engine : EngineBase {
       }
     }
   }
-  // Ψ (NK, M, n) reference rows -> device rows (leading dim ld), permuted
-  void upload_psi(cs* dst, long ld, const double* src) {
+  // Ψ (NK, M, n) reference rows -> device rows (leading dim ld), permuted; with
+  // ld > n the rows go through `stage` (NK Md n elements) and a strided device copy
+  void upload_psi(cs* dst, long ld, const double* src, cs* stage = nullptr) {
     std::vector<cs> tmp((size_t)NK * Md * n);
     for (int b = 0; b < NK; ++b)
       for (int d = 0; d < Md; ++d) {
@@ -876,8 +884,16 @@ struct Engine : EngineBase {
           row[o] = from_d2<cs>(make_double2(s[0], s[1]));
         }
       }
-    HIPCHK(hipMemcpy2DAsync(dst, sizeof(cs) * ld, tmp.data(), sizeof(cs) * n, sizeof(cs) * n,
-                            (size_t)NK * Md, hipMemcpyHostToDevice, stream));
+    if (ld == n) {
+      HIPCHK(hipMemcpyAsync(dst, tmp.data(), sizeof(cs) * tmp.size(), hipMemcpyHostToDevice,
+                            stream));
+    } else {
+      require(stage != nullptr, "upload_psi: strided rows need a staging buffer");
+      HIPCHK(hipMemcpyAsync(stage, tmp.data(), sizeof(cs) * tmp.size(), hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipMemcpy2DAsync(dst, sizeof(cs) * ld, stage, sizeof(cs) * n, sizeof(cs) * n,
+                              (size_t)NK * Md, hipMemcpyDeviceToDevice, stream));
+    }
     HIPCHK(hipStreamSynchronize(stream));
   }
   void download_psi(double* dst, const cs* src) {
@@ -1797,7 +1813,7 @@ struct Engine : EngineBase {
     for (int a = 0; a < NL; ++a) st.delta[a] = delta[a];
     HIPCHK(hipMemcpyAsync(sp, &st, sizeof(st), hipMemcpyHostToDevice, stream));
     upload(sig[9], n, s, (long)NK * n, n);
-    upload_psi(Ap, n + nc, p);
+    upload_psi(Ap, n + nc, p, psi[1]);  // k slot 1 is free outside an attempt
     upload_omega(omg[9], o);
     // the input's passive block may be non-zero: evaluate it in full
     const bool ff_keep = ff;
