@@ -309,7 +309,7 @@ def test_drop_in_rkf45(native, driver):
     # (with the driver's default first step, (t1-t0)/1000, the first error norms are at
     # the rounding level and the step sequence is not reproducible to 1e-10)
     times, data = getattr(drop_in, driver)(f, *y0, 1e-7, 0.0, 0.3, p.dH, p.bias, p.Ixi,
-                                           h_guess=0.01)
+                                           h_guess=0.01, print_to_file=False)
     ref = O.rkf45_propagate(p.consts(), y0, 0.0, 0.3, 0.01, 1e-7, p.H0, p.dH, p.bias, 2)
     assert len(times) == len(ref["times"])
     np.testing.assert_allclose(times, ref["times"], rtol=1e-10)
@@ -362,7 +362,8 @@ def test_drop_in_f_and_adaptive_dop(native):
     prop = Propagator(p, "fp64")
     f = make_f(prop)
     y0 = p.zero_state()
-    times, data = AdaptiveDOP(f, *y0, 1e-7, 0.0, 0.3, p.dH, p.bias, p.Ixi, h_guess=0.01)
+    times, data = AdaptiveDOP(f, *y0, 1e-7, 0.0, 0.3, p.dH, p.bias, p.Ixi, h_guess=0.01,
+                              print_to_file=False)
     ref = O.propagate(p.consts(), y0, 0.0, 0.3, 0.01, 1e-7, p.H0, p.dH, p.bias, 2)
     np.testing.assert_allclose(times, ref["times"][:-1], rtol=1e-10)
     J = np.array([data["current_left"], data["current_right"]]).transpose(1, 0, 2)

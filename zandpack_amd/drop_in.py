@@ -25,10 +25,18 @@
   vector, one solve_ivp call per output interval; returns (t_eval, σ[k=0] at each
   output time, the first lead's current (NK,) at each output time).
 
+The three adaptive drivers write the reference's run files into the working
+directory (``<name>.txt``, ``<name>_last_{sig,psi,omega,time}.npy``, ``_times.npy``,
+the current files and ``_#electrons_device.npy``) every ``print_step`` accepted steps,
+resume from ``<name>_last_*`` when the state is None, and take the reference's
+``atol``/``rtol`` error form; ``print_to_file=False`` writes nothing.
+
 All run every evaluation on the GPU (host arrays in and out per call); the
 device-resident path with no per-stage transfers is ``Propagator.propagate``.
 """
 from __future__ import annotations
+
+import time
 
 import numpy as np
 
@@ -49,6 +57,79 @@ _B4 = np.array([5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187
 _CT = _B5 - _B4
 
 
+class _RunFiles:
+    """The run files the in-process drivers write into the working directory
+    (Zandpack/TimedependentTransport.py:3279-3281, 3343-3375, 3384-3388; the same in
+    Zandpack/Optimized_RK45.py): ``<name>.txt`` (start line, then every ``print_step``
+    accepted steps the progress in %, the new step and the elapsed time, and
+    ``write_func(file, t, σ, Ψ, Ω)`` for the step just taken; at the end ``100%`` and
+    the runtime), and with each progress entry the snapshots ``<name>_last_sig``,
+    ``_last_psi``, ``_last_omega``, ``_last_time`` (the start time of the last step),
+    ``_times``, the currents, and ``_#electrons_device`` (Tr σ per step and k).
+    ``enabled=False`` writes nothing (the reference writes regardless of its
+    ``print_to_file`` argument)."""
+
+    def __init__(self, enabled, name, print_step, write_func, t_start, t_end, current_files):
+        self.on = bool(enabled)
+        self.name = str(name)
+        self.every = max(1, int(print_step))
+        self.write_func = write_func
+        self.T0, self.t1 = float(t_start), float(t_end)
+        self.current_files = current_files  # [(file stem, data key)]
+        self.step = 0
+        self.t_start = time.time()
+        if self.on:
+            with open(self.name + ".txt", "w") as fh:
+                fh.write("\n\n\n\n\nStart (Wait for compilation)\n")
+
+    def accepted(self, t_new, h, state, times, data, dms):
+        if self.on and self.step % self.every == 0:
+            with open(self.name + ".txt", "a") as fh:
+                fh.write(str(((t_new - self.T0) / (self.t1 - self.T0)) * 100) + " %\n")
+                fh.write("current timestep: " + str(h) + "fs\n")
+                fh.write("delta t: " + str(time.time() - self.t_start) + "seconds\n")
+                if self.write_func is not None:
+                    self.write_func(fh, t_new - h, state[0].copy(), state[1].copy(),
+                                    state[2].copy())
+            np.save(self.name + "_last_sig", state[0])
+            np.save(self.name + "_last_psi", state[1])
+            np.save(self.name + "_last_omega", state[2])
+            np.save(self.name + "_last_time", np.array(times[-1]))
+            np.save("_times", np.array(times))
+            for stem, key in self.current_files:
+                np.save(stem, np.asarray(data.get(key, [])))
+            np.save("_#electrons_device", np.trace(np.asarray(dms), axis1=2, axis2=3))
+        self.step += 1
+
+    def finish(self):
+        if self.on:
+            with open(self.name + ".txt", "a") as fh:
+                fh.write("100%\n")
+                fh.write("Runtime: " + str(time.time() - self.t_start) + "seconds")
+
+
+def _resume(name, sig0, psi0, omega0, t0):
+    """A missing state is read from ``<name>_last_*.npy`` and a missing t0 from
+    ``<name>_last_time.npy`` (Zandpack/TimedependentTransport.py:3258-3264)."""
+    if sig0 is None or psi0 is None or omega0 is None:
+        sig0 = np.load(name + "_last_sig.npy")
+        psi0 = np.load(name + "_last_psi.npy")
+        omega0 = np.load(name + "_last_omega.npy")
+        if t0 is None:
+            t0 = float(np.load(name + "_last_time.npy"))
+    return sig0, psi0, omega0, t0
+
+
+def _tolerance_error(err, state, atol, rtol):
+    """The drivers' error with ``atol``/``rtol`` given
+    (Zandpack/TimedependentTransport.py:3226-3235): f_x = max((|e_x| - atol_x)/rtol_x -
+    |x|) over σ, Ψ, Ω and error = 1 ** max(f); this is 1.0 for every finite or NaN
+    f, so a step is accepted iff eps >= 1 and h <- 0.9 h eps^(1/5)."""
+    f = [float(np.max((np.abs(e) - atol[i]) / rtol[i] - np.abs(x)))
+         for i, (e, x) in enumerate(zip(err, state))]
+    return 1.0 ** max(f)
+
+
 def make_f(prop: Propagator):
     """Reference-style RHS closure over the constants held by ``prop``."""
     H0 = prop.problem.H0
@@ -67,15 +148,23 @@ def make_f(prop: Propagator):
 
 
 def AdaptiveDOP(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_guess=None,
-                dH_given=True, fixed_mode=False, elec_names=("left", "right"),
-                time_update="reference", max_attempts=10 ** 6, **_unused):
+                dH_given=True, print_to_file=True, fixed_mode=False, name="Runge-Kutta",
+                write_func=None, print_step=50, atol=None, rtol=None,
+                elec_names=("left", "right"), time_update="reference", max_attempts=10 ** 6,
+                **_unused):
     """Adaptive DOP54 over an ``f`` made by ``make_f``.  Returns (times, data) with
-    ``data['current_<elec>']`` (list of (NK,) arrays) and ``data['density matrix']``."""
+    ``data['current_<elec>']`` (list of (NK,) arrays) and ``data['density matrix']``.
+    Run files as the reference driver writes them (see ``_RunFiles``); its ``_JL`` /
+    ``_JR`` hold the first two leads' currents here (the reference saves two lists it
+    never fills).  ``plot`` and ``use_GPU`` are accepted and ignored."""
+    sig0, psi0, omega0, t0 = _resume(name, sig0, psi0, omega0, t0)
     h = 0.01 if h_guess is None else float(h_guess)
     y = [np.array(sig0, complex), np.array(psi0, complex), np.array(omega0, complex)]
     data = {f"current_{e}": [] for e in elec_names}
     dms, times = [], []
     t = float(t0)
+    cur = [(stem, f"current_{e}") for stem, e in zip(("_JL", "_JR"), elec_names)]
+    files = _RunFiles(print_to_file, name, print_step, write_func, t0, t1, cur)
     attempts = 0
     while t < t1:
         while True:
@@ -90,8 +179,11 @@ def AdaptiveDOP(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_
                 if i == 0:
                     J0 = f.last_J
                 ks.append(tuple(h * v for v in d))
-            te = np.sqrt(sum(float(np.sum(np.abs(sum(_CT[j] * ks[j][p] for j in range(7))) ** 2))
-                             for p in range(3)))
+            err = [sum(_CT[j] * ks[j][p] for j in range(7)) for p in range(3)]
+            if atol is None:
+                te = np.sqrt(sum(float(np.sum(np.abs(e) ** 2)) for e in err))
+            else:
+                te = _tolerance_error(err, y, atol, rtol)
             if not np.isfinite(te):
                 raise FloatingPointError("non-finite error estimate")
             h_old = h
@@ -106,6 +198,8 @@ def AdaptiveDOP(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_
         y = [y[p] + sum(_B5[j] * ks[j][p] for j in range(7)) for p in range(3)]
         times.append(t)
         t += h if (time_update == "reference" and not fixed_mode) else h_old
+        files.accepted(t, h, y, times, data, dms)
+    files.finish()
     data["density matrix"] = np.array(dms)
     data["state"] = tuple(y)
     return np.array(times), data
@@ -125,17 +219,24 @@ _RKF_CT = np.array([1 / 360, 0.0, -128 / 4275, -2197 / 75240, 1 / 50, 2 / 55])
 
 
 def AdaptiveRK4(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_guess=None,
-                dH_given=True, fixed_mode=False, elec_names=("left", "right"),
-                max_attempts=10 ** 6, **_unused):
+                dH_given=True, print_to_file=True, fixed_mode=False, name="Runge-Kutta",
+                write_func=None, print_step=50, atol=None, rtol=None,
+                elec_names=("left", "right"), max_attempts=10 ** 6, **_unused):
     """Adaptive RKF45 over an ``f`` made by ``make_f`` (the in-process AdaptiveRK4's
     loop).  Returns (times, data): the start time of every accepted step, and
     ``data['current_<elec>']`` (list of (NK,) arrays, from the state before the
-    step), ``data['density matrix']`` (σ before each step), ``data['state']``."""
+    step), ``data['density matrix']`` (σ before each step), ``data['state']``.
+    Run files as the reference driver writes them (see ``_RunFiles``), with the
+    currents in ``_current_<elec>``; ``plot`` and ``use_GPU`` are accepted and
+    ignored."""
+    sig0, psi0, omega0, t0 = _resume(name, sig0, psi0, omega0, t0)
     h = (t1 - t0) / 1000 if h_guess is None else float(h_guess)
     y = [np.array(sig0, complex), np.array(psi0, complex), np.array(omega0, complex)]
     data = {f"current_{e}": [] for e in elec_names}
     dms, times = [], []
     t = float(t0)
+    cur = [(f"_current_{e}", f"current_{e}") for e in elec_names]
+    files = _RunFiles(print_to_file, name, print_step, write_func, t0, t1, cur)
     attempts = 0
     while t <= t1:
         while True:
@@ -152,8 +253,11 @@ def AdaptiveRK4(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_
                 ks.append(tuple(h * v for v in d))
             if fixed_mode:
                 break
-            te = np.sqrt(sum(float(np.sum(np.abs(sum(_RKF_CT[j] * ks[j][p] for j in range(6))) ** 2))
-                             for p in range(3)))
+            err = [sum(_RKF_CT[j] * ks[j][p] for j in range(6)) for p in range(3)]
+            if atol is None:
+                te = np.sqrt(sum(float(np.sum(np.abs(e) ** 2)) for e in err))
+            else:
+                te = _tolerance_error(err, y, atol, rtol)
             if not np.isfinite(te):
                 raise FloatingPointError("non-finite error estimate")
             h = 0.9 * h * (eps / te) ** 0.2
@@ -165,6 +269,8 @@ def AdaptiveRK4(f, sig0, psi0, omega0, eps, t0, t1, dH, delta_func, Ixi=None, h_
         y = [y[p] + sum(_RKF_CH[j] * ks[j][p] for j in range(6)) for p in range(3)]
         times.append(t)
         t += h
+        files.accepted(t, h, y, times, data, dms)
+    files.finish()
     data["density matrix"] = np.array(dms)
     data["state"] = tuple(y)
     return np.array(times), data
