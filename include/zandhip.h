@@ -223,6 +223,16 @@ int zh_attempt(zh_problem* p, zh_attempt_info* info, double* J);
 int zh_attempt_launch(zh_problem* p, const zh_attempt_info* info);
 int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J);
 
+/* One attempt of each of count problems: every launch, then every collect, in one
+ * call (the multi-instance step of the batched and serving configurations; the
+ * reference runs one problem per process, Zandpack/mpi/zand:297-431 per problem).
+ * infos[i] and J[i] (NL*NK doubles) are those of zh_attempt_collect for ps[i];
+ * status[i] receives problem i's return code.  A failure of one problem leaves the
+ * others running; the return value is the first failure's code (its message in
+ * zh_last_error) or ZH_OK. */
+int zh_batch_attempt(zh_problem* const* ps, int count, zh_attempt_info* infos, double* const* J,
+                     int* status);
+
 /* Tile autotuner: times a few compiled tile shapes of each GEMM of the right-hand
  * side on rejected attempts from the resident state (nothing is committed) and
  * keeps the fastest; choices are cached per (precision, size bucket) for the

@@ -75,3 +75,34 @@ def test_failing_request_is_retired(native):
     errs = {r.rid: r.error for r in st["done"]}
     assert errs[1] == "attempt limit reached"
     assert errs[0] is None and errs[2] is None
+
+
+def test_batch_attempt_matches_single(native):
+    """zh_batch_attempt: the same results as one zh_attempt per problem, bitwise, and
+    a problem whose attempt is refused (a non-finite step) leaves the others going."""
+    from zandpack_amd.engine import Propagator, batch_attempt
+    say("batch attempt vs single")
+    reqs = _requests("mixed")
+    a = [Propagator(r.problem, "mixed") for r in reqs]
+    b = [Propagator(r.problem, "mixed") for r in reqs]
+    t = [0.0] * 3
+    h = [0.004, 0.005, 0.006]
+    for _ in range(3):
+        res = batch_attempt(a, t, h, 1e-6)
+        for i in range(3):
+            one = b[i].attempt(t[i], h[i], 1e-6)
+            assert res[i]["accepted"] == one["accepted"]
+            assert res[i]["te"] == one["te"] and res[i]["h_next"] == one["h_next"]
+            np.testing.assert_array_equal(res[i]["J"], one["J"])
+            if one["accepted"]:
+                t[i] += one["h_next"]
+            h[i] = one["h_next"]
+    for i in range(3):
+        np.testing.assert_array_equal(a[i].sigma(), b[i].sigma())
+    res = batch_attempt(a, t, [h[0], float("inf"), h[2]], 1e-6)
+    assert isinstance(res[1], Exception) and "step size" in str(res[1])
+    for i in (0, 2):
+        one = b[i].attempt(t[i], h[i], 1e-6)
+        assert res[i]["te"] == one["te"]
+    for p in a + b:
+        p.close()

@@ -2216,6 +2216,29 @@ int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J) {
   });
 }
 
+int zh_batch_attempt(zh_problem* const* ps, int count, zh_attempt_info* infos, double* const* J,
+                     int* status) {
+  if (count < 0 || (count > 0 && (!ps || !infos || !J || !status))) {
+    g_err = "null argument";
+    return ZH_EINVAL;
+  }
+  // every launch first (each problem has its own streams), then every collect; one
+  // problem's failure is reported in its status and does not stop the others
+  int first = ZH_OK;
+  std::string msg;
+  for (int i = 0; i < count; ++i) {
+    status[i] = zh_attempt_launch(ps[i], &infos[i]);
+    if (status[i] != ZH_OK && first == ZH_OK) { first = status[i]; msg = g_err; }
+  }
+  for (int i = 0; i < count; ++i) {
+    if (status[i] != ZH_OK) continue;
+    status[i] = zh_attempt_collect(ps[i], &infos[i], J[i]);
+    if (status[i] != ZH_OK && first == ZH_OK) { first = status[i]; msg = g_err; }
+  }
+  if (first != ZH_OK) g_err = msg;
+  return first;
+}
+
 int zh_autotune(zh_problem* p, int force, int* choice) {
   return zh_try([&] {
     require(p, "null argument");

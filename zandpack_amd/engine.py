@@ -322,3 +322,32 @@ def selftest_gemm(A, B, transA=False, transB=False, precision="fp64"):
     N.check(N.lib().zh_selftest_gemm(PRECISIONS[precision], M, Nn, K, int(transA), int(transB),
                                      N.ptr(A), N.ptr(B), N.ptr(Cm)))
     return Cm
+
+
+def batch_attempt(props, t0s, hs, eps, force_accept=False):
+    """One attempt of each Propagator in ``props`` from times ``t0s`` with steps
+    ``hs`` (C: ``zh_batch_attempt``: every launch, then every collect).  Returns one
+    entry per problem: the ``collect`` dict, or a ``ZHError`` for a problem whose
+    attempt failed (the others go on)."""
+    n = len(props)
+    if n == 0:
+        return []
+    lib = N.lib()
+    eps = np.broadcast_to(np.asarray(eps, dtype=np.float64), (n,))
+    infos = (N.zh_attempt_info * n)(*[
+        N.zh_attempt_info(float(t), float(h), float(e), 1 if force_accept else 0, 0.0, 0.0, 0)
+        for t, h, e in zip(t0s, hs, eps)])
+    Js = [np.empty((p.NL, p.NK)) for p in props]
+    jp = (N._dp * n)(*[N.ptr(J) for J in Js])
+    hs_ = (C.c_void_p * n)(*[p.h.value for p in props])
+    status = (C.c_int * n)()
+    rc = lib.zh_batch_attempt(hs_, n, infos, jp, status)
+    msg = lib.zh_last_error().decode() if rc != 0 else ""
+    out = []
+    for i in range(n):
+        if status[i] != 0:
+            out.append(N.ZHError(status[i], msg if status[i] == rc else "attempt failed"))
+            continue
+        info = infos[i]
+        out.append(dict(te=info.te, h_next=info.h_next, accepted=bool(info.accepted), J=Js[i]))
+    return out

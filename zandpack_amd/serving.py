@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .builder import AMEProblem
-from .engine import Propagator
+from .engine import Propagator, batch_attempt
 
 
 def footprint_bytes(problem: AMEProblem, precision: str = "mixed") -> int:
@@ -131,20 +131,15 @@ class Server:
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
                 continue
             finished = []
-            launched = []
             ts = time.perf_counter()
-            for lv in live:
-                try:
-                    lv.prop.launch(lv.t, lv.h, lv.req.eps)
-                    launched.append(lv)
-                except Exception as e:  # e.g. a step size driven to zero or infinity
-                    lv.req.error = str(e)
-                    finished.append(lv)
-            for lv in launched:
-                try:
-                    res = lv.prop.collect()
-                except Exception as e:  # a failed request is retired, the rest go on
-                    lv.req.error = str(e)
+            # one attempt of every live request in one native call (all launches, then
+            # all collects); a failed request (e.g. a step size driven to zero or
+            # infinity) is retired and the rest go on
+            results = batch_attempt([lv.prop for lv in live], [lv.t for lv in live],
+                                    [lv.h for lv in live], [lv.req.eps for lv in live])
+            for lv, res in zip(live, results):
+                if isinstance(res, Exception):
+                    lv.req.error = str(res)
                     finished.append(lv)
                     continue
                 lv.attempts += 1
