@@ -143,6 +143,19 @@ int zh_contour_density(int NK, int n, int nE, int nc, const double* contour,
                        const double* evals, const double* evecs, const int* corb,
                        const double* sigma_cc, double* rho, int device);
 
+/* Bias-window (nonequilibrium) density of Zandpack/cmdtools/SCF:236-253, 299-306
+ * (G_l integrated over [mu_min - fact_kT kT_max, mu_max + fact_kT kT_max]):
+ * rho = sum_i w_i (1/2pi) G(E_i)[:, c] M_i G(E_i)[:, c]^H on real nodes E_i, with
+ * G = (E - H - Sigma(E))^-1 and M_i = sum_a (f_a(E_i) - f_min(E_i)) Gamma_a(E_i).
+ *   nodes    (nE, 2) real: E_i, w_i (the caller's quadrature)
+ *   evals, evecs, corb, sigma_cc as for zh_contour_density (sigma_cc at the nodes)
+ *   gamma_w  (NK, nE, nc, nc) complex128: M_i on the coupled orbitals
+ *   rho      (NK, n, n) complex128 out (Hermitian part).
+ * Stateless like zh_contour_density. */
+int zh_window_density(int NK, int n, int nE, int nc, const double* nodes, const double* evals,
+                      const double* evecs, const int* corb, const double* sigma_cc,
+                      const double* gamma_w, double* rho, int device);
+
 /* Per-lead Pi of the resident state, as the stage-0 Pi the next attempt starts from
  * (the save_PI output of Zandpack/mpi/zand:376-378, 444-448): Q (NK, NL, n, n)
  * complex128 = (Pi_a + Pi_a^H)/hbar with hermitian = 1 (zand's Q_np, :90-110) or

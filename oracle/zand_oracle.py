@@ -15,6 +15,7 @@ its orthogonal MPI propagator, written as one process:
 * loop form of the RHS     -- Zandpack/TimedependentTransport.py:2798-2873
 * Hu poles / Fermi expansion -- Zandpack/PadeDecomp.py:36-42, 84-152
 * non-orthogonal form      -- Zandpack/mpi/nozand:456-497, 1003, 1075-1085, 1208-1210
+* SCF contour and window   -- Zandpack/cmdtools/SCF:209-215, 236-253, 273-306
 
 Array layout is the reference's (per k):
 ``sigma (NK,n,n)``, ``psi (NK,NL,NX,NOT,n)``, ``omega (NK,NL,NX,NOT,NL,NX,NOT)``,
@@ -32,6 +33,11 @@ Pinning (what makes this oracle trustworthy; see tests/test_oracle_*.py):
 * ``propagate``       -- von-Neumann closed form σ(t)=U σ0 U† to the DOP54 error level.
 * ``rkf45_propagate`` -- Fehlberg rationals (order conditions, row sums) and the same
                          von-Neumann closed form.
+* ``window_density``  -- a single level between wide-band leads against the scalar
+                         Lorentzian integral; the fixed Gauss-Legendre panels against
+                         scipy's adaptive quad_vec (the reference's integrator) on a
+                         random device; zero for equal chemical potentials; with one
+                         lead decoupled, contour(mu_min) + window = contour(mu_max).
 * ``rhs(Sinv=...)``   -- the nozand form equals the orthogonal form carried to the
                          non-orthogonal basis (dσ_n = S^-½ dσ_o S^-½, dΨ_n = S^-½ dΨ_o,
                          dΩ and J unchanged); S = I reduces it to zand.
@@ -636,4 +642,53 @@ def contour_density(H, contour, sigma_fn):
         E, W = Er + 1j * Ei, Wr + 1j * Wi
         G = np.linalg.inv(E * I - H - sigma_fn(E))
         rho += (1j / (2 * np.pi)) * (W * G - np.conj(W) * G.conj().transpose(0, 2, 1))
+    return rho
+
+
+def fermi(E, mu, kT):
+    """1 / (1 + exp((E - mu)/kT)) (the exact Fermi function of cmdtools/SCF:228-229)."""
+    return 1.0 / (1.0 + np.exp((np.asarray(E, float) - mu) / kT))
+
+
+def window_limits(mus, kTs, fact_kT=10.0):
+    """[mu_min - fact_kT kT_max, mu_max + fact_kT kT_max] (cmdtools/SCF:273-276)."""
+    mus, kTs = np.asarray(mus, float), np.asarray(kTs, float)
+    return mus.min() - fact_kT * kTs.max(), mus.max() + fact_kT * kTs.max()
+
+
+def window_integrand(H, E, sigma_leads_fn, mus, kTs):
+    """G_l(E) of cmdtools/SCF:236-253 with the nonequilibrium fdf of :301-302:
+    (-i/2π) G SE_l G†, SE_l = Σ_α -(f_α(E) - f_min(E)) (Σ_α - Σ_α†), G = (E - H - ΣΣ_α)⁻¹;
+    ``sigma_leads_fn(E)`` -> (NK, NL, n, n).  Full n x n here (the reference keeps
+    the coupled columns only, which carry every non-zero of SE_l)."""
+    NK, n, _ = H.shape
+    mus, kTs = np.asarray(mus, float), np.asarray(kTs, float)
+    imin = int(np.argmin(mus))
+    S = sigma_leads_fn(E)
+    G = np.linalg.inv(E * np.eye(n) - H - S.sum(axis=1))
+    f_min = fermi(E, mus[imin], kTs[imin])
+    SE_l = np.zeros((NK, n, n), complex)
+    for a in range(S.shape[1]):
+        fd = fermi(E, mus[a], kTs[a]) - f_min
+        SE_l += -fd * (S[:, a] - S[:, a].conj().transpose(0, 2, 1))
+    return (-1j / (2 * np.pi)) * (G @ SE_l @ G.conj().transpose(0, 2, 1))
+
+
+def gauss_legendre_panels(e1, e2, panels, order=8):
+    """Composite Gauss-Legendre nodes and weights on [e1, e2] (the fixed quadrature
+    used in place of scipy's adaptive quad_vec of cmdtools/SCF:303-305)."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    edges = np.linspace(e1, e2, panels + 1)
+    nodes, weights = [], []
+    for a, b in zip(edges[:-1], edges[1:]):
+        nodes.append(0.5 * (b - a) * x + 0.5 * (a + b))
+        weights.append(0.5 * (b - a) * w)
+    return np.concatenate(nodes), np.concatenate(weights)
+
+
+def window_density(H, nodes, weights, sigma_leads_fn, mus, kTs):
+    """Σ_i w_i G_l(E_i): the bias-window density on given real nodes."""
+    rho = np.zeros(H.shape, complex)
+    for E, w in zip(nodes, weights):
+        rho += w * window_integrand(H, E, sigma_leads_fn, mus, kTs)
     return rho

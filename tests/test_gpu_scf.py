@@ -76,8 +76,67 @@ def test_self_consistent_hubbard_fixed_point(native):
     say("scf iterations", its)
 
 
-def test_nonequilibrium_is_refused(native):
-    p = builder.config_small(seed=1)
+def _biased(seed=1, U=0.0):
+    p = builder.config_small(seed=seed, hubbard_U=U, NK=2 if U else 1)
     p.pulse = builder.Pulse("step", amp=0.2, p1=-1.0, weights=(0.5, -0.5))
-    with pytest.raises(NotImplementedError):
-        scf.self_consistent_density(p, _contour(), mu=0.0)
+    return p
+
+
+def test_window_density_matches_oracle(native):
+    """Bias-window density on the device against the oracle's G_l sum on the same
+    real nodes (cmdtools/SCF:236-253, 299-306)."""
+    p = _biased(seed=3)
+    bias = np.array([p.bias(0.0, a) for a in range(2)])
+    mus, kTs = bias + 0.05, np.array([0.025, 0.03])
+    nodes, w = scf.bias_window(mus, kTs, panels=6, order=4)
+    corb = scf.coupled_orbitals(p)
+    sig_l = scf.self_energy_cc_leads(p, nodes.astype(complex), corb, bias)
+    gam = scf.window_weights(sig_l, nodes, mus, kTs)
+    H = np.asarray(p.H0, complex)
+    say("window density", p.dims, len(nodes))
+    got = scf.window_density(H, nodes, w, corb, sig_l.sum(axis=0), gam)
+    ref = O.window_density(H, nodes, w, lambda E: O.lead_self_energy(p.consts(), E, bias),
+                           mus, kTs)
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+    assert np.abs(got - got.conj().transpose(0, 2, 1)).max() == 0.0
+
+
+def test_unequal_potentials_without_window_warn(native):
+    """As the SCF tool: a warning, then the equilibrium density at the lowest
+    chemical potential."""
+    p = _biased()
+    with pytest.warns(UserWarning):
+        rho, its, ok = scf.self_consistent_density(p, _contour(), mu=0.0, max_iter=3)
+    C = _contour().copy()
+    C[:, 0] += min(p.bias(0.0, 0), p.bias(0.0, 1))
+    ref = O.contour_density(np.asarray(p.H0, complex), C,
+                            lambda E: O.lead_self_energy(p.consts(), E, np.array(
+                                [p.bias(0.0, a) for a in range(2)])).sum(axis=1))
+    assert np.abs(rho - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+def test_nonequilibrium_fixed_point(native):
+    """Hubbard mean field under bias: the loop's ρ is a fixed point of the oracle's
+    map ρ -> contour(μ_min) + window at H(ρ), on the same window nodes."""
+    p = _biased(seed=2, U=1.0)
+    bias = np.array([p.bias(0.0, a) for a in range(2)])
+    kT = 0.025
+    say("nonequilibrium scf", p.dims)
+    rho, its, ok = scf.self_consistent_density(p, _contour(), mu=0.0, tol=1e-8, max_iter=200,
+                                               kT=kT, nonequilibrium=True, window_panels=10)
+    assert ok, its
+    mus = bias
+    nodes, w = scf.bias_window(mus, [kT, kT], panels=10)
+    C = _contour().copy()
+    C[:, 0] += mus.min()
+    H = np.asarray(p.H0, complex) + p.dH(0.0, rho)
+    sig_l = lambda E: O.lead_self_energy(p.consts(), E, bias)
+    F = O.contour_density(H, C, lambda E: sig_l(E).sum(axis=1))
+    F = F + O.window_density(H, nodes, w, sig_l, mus, [kT, kT])
+    assert np.abs(F - rho).max() < 1e-7
+    # the biased density differs from the equilibrium one
+    with pytest.warns(UserWarning):
+        rho_eq, _, _ = scf.self_consistent_density(p, _contour(), mu=0.0, tol=1e-8,
+                                                   max_iter=200)
+    assert np.abs(rho - rho_eq).max() > 1e-4
+    say("scf iterations", its)

@@ -122,3 +122,84 @@ def test_pulay_mixer_finds_fixed_point():
             break
         x = x + 0.3 * r
     assert lin > 5 * it
+
+
+# ---------------------------------------------------------------- bias window
+def _wide_band(gl, gr):
+    return lambda E: np.array([[[[-0.5j * gl]], [[-0.5j * gr]]]])
+
+
+def test_window_single_level_closed_form():
+    """One level ε between wide-band leads (Σ_α = -iΓ_α/2): the window density is
+    (Γ_L/2π) ∫ (f_L - f_R) / ((E-ε)² + Γ²/4) dE, integrated here as a scalar."""
+    from scipy.integrate import quad
+    eps, gl, gr, kT = 0.1, 0.05, 0.08, 0.025
+    mus = np.array([0.25, 0.0])
+    H = np.array([[[eps]]], complex)
+    e1, e2 = O.window_limits(mus, [kT, kT])
+    x, w = O.gauss_legendre_panels(e1, e2, 200, 8)
+    rho = O.window_density(H, x, w, _wide_band(gl, gr), mus, [kT, kT])
+    g = gl + gr
+    ref = quad(lambda E: gl / (2 * np.pi) * (O.fermi(E, mus[0], kT) - O.fermi(E, mus[1], kT))
+               / ((E - eps) ** 2 + g * g / 4), e1, e2, epsabs=1e-14, epsrel=1e-12, limit=400)[0]
+    assert abs(rho[0, 0, 0] - ref) < 1e-10 * abs(ref)
+    # the sign of the bias sets which lead fills the window
+    rho2 = O.window_density(H, x, w, _wide_band(gr, gl), mus[::-1], [kT, kT])
+    assert rho2[0, 0, 0].real > 0 and abs(rho2[0, 0, 0].imag) < 1e-15
+
+
+def _small_device(seed=2):
+    p = builder.config_small(seed=seed)
+    c = p.consts()
+    H = np.asarray(p.H0, complex)
+    return p, c, H
+
+
+def test_window_panels_match_adaptive_quadrature():
+    """The fixed Gauss-Legendre panels against scipy's adaptive quad_vec (the
+    reference's integrator) on a small device with Lorentzian-fitted leads."""
+    from scipy.integrate import quad_vec
+    p, c, H = _small_device()
+    mus, kTs = np.array([0.3, -0.1]), np.array([0.025, 0.025])
+    sig = lambda E: O.lead_self_energy(c, E)
+    e1, e2 = O.window_limits(mus, kTs)
+    x, w = O.gauss_legendre_panels(e1, e2, 48, 8)
+    rho = O.window_density(H, x, w, sig, mus, kTs)
+    ref, err = quad_vec(lambda E: O.window_integrand(H, E, sig, mus, kTs), e1, e2,
+                        epsabs=1e-11, epsrel=1e-10, quadrature="gk21")
+    assert np.abs(rho - ref).max() < 1e-8 * np.abs(ref).max()
+    assert np.abs(rho - rho.conj().transpose(0, 2, 1)).max() < 1e-12 * np.abs(rho).max()
+    # the window adds charge for a positive bias on the lead at the higher potential
+    assert np.trace(rho[0]).real > 0
+
+
+def test_window_vanishes_in_equilibrium():
+    p, c, H = _small_device()
+    mus, kTs = np.array([0.1, 0.1]), np.array([0.025, 0.025])
+    x, w = O.gauss_legendre_panels(-0.5, 0.5, 8, 8)
+    rho = O.window_density(H, x, w, lambda E: O.lead_self_energy(c, E), mus, kTs)
+    assert np.abs(rho).max() == 0.0
+
+
+def test_window_closes_the_contour_with_one_lead():
+    """With the low-potential lead decoupled (Σ_R = 0), the device is in equilibrium
+    with the other lead: contour(μ_R) + window(f_L - f_R) = contour(μ_L)."""
+    C = np.load(os.path.join(GOLD, "scf_default_contour.npy"))
+    kT = (C[-1, 1] - C[-2, 1]) / (2 * np.pi)
+    p, c, H = _small_device(seed=4)
+
+    def sig_leads(E):
+        s = O.lead_self_energy(c, E)
+        s[:, 1] = 0.0
+        return s
+
+    sig_tot = lambda E: sig_leads(E).sum(axis=1)
+    mus, kTs = np.array([0.2, 0.0]), np.array([kT, kT])
+    lo, hi = C.copy(), C.copy()
+    lo[:, 0] += mus[1]
+    hi[:, 0] += mus[0]
+    e1, e2 = O.window_limits(mus, kTs, fact_kT=20.0)
+    x, w = O.gauss_legendre_panels(e1, e2, 96, 8)
+    rho = O.contour_density(H, lo, sig_tot) + O.window_density(H, x, w, sig_leads, mus, kTs)
+    ref = O.contour_density(H, hi, sig_tot)
+    assert np.abs(rho - ref).max() < 2e-5 * np.abs(ref).max()

@@ -1974,6 +1974,107 @@ static void contour_density(int NK, int n, int nE, int nc, const double* EW, con
     require(std::isfinite(rho[i]), "non-finite density (singular Woodbury matrix?)", ZH_ENONFINITE);
 }
 
+// Bias-window (nonequilibrium) density of Zandpack/cmdtools/SCF:236-253, 299-306,
+//   ρ_win = ∫ dE (1/2π) G(E)[:, c] M(E) G(E)[:, c]†,  M = Σ_α (f_α - f_min) Γ_α,
+// on real nodes E_i with weights w_i.  With H = V diag(e) V† and the Woodbury identity
+// on the coupled orbitals c, G[:, c] = V P_i with P_i = T'_i (I - Σ_i G0_i,cc)⁻¹
+// (T'_i = D_i V_c†, as for the contour sum), so
+//   ρ_win = V (Σ_i (w_i/2π) P_i M_i P_i†) V†,
+// the inner sum one (n x nE nc)(nE nc x n) product.
+static void window_density(int NK, int n, int nE, int nc, const double* Ew, const double* evals,
+                           const double* evecs, const int* corb, const double* sigma_cc,
+                           const double* gamma_w, double* rho, int device) {
+  require(NK > 0 && n > 0 && nE > 0, "NK, n and the number of nodes must be positive");
+  require(nc > 0 && nc <= n && nc <= kGJMax, "coupled orbitals: 0 < nc <= min(n, 1024)");
+  require(corb && sigma_cc && gamma_w, "the window needs its orbitals, self-energy and weights");
+  for (int c = 0; c < nc; ++c) require(corb[c] >= 0 && corb[c] < n, "orbital index out of range");
+  for (int i = 0; i < nE; ++i)
+    require(std::isfinite(Ew[2 * i]) && std::isfinite(Ew[2 * i + 1]), "nodes must be finite");
+  HIPCHK(hipSetDevice(device));
+  ScfScratch sc;
+  HIPCHK(hipStreamCreateWithFlags(&sc.st, hipStreamNonBlocking));
+  hipStream_t st = sc.st;
+  const long nn = (long)n * n, L = (long)nE * nc, cc = (long)nc * nc;
+  std::vector<double2> Eh(nE), Wh(nE);
+  for (int i = 0; i < nE; ++i) {
+    Eh[i] = make_double2(Ew[2 * i], 0.0);
+    Wh[i] = make_double2(Ew[2 * i + 1] / (2.0 * M_PI), 0.0);
+  }
+  double2* Ed = sc.alloc<double2>(nE);
+  double2* Wd = sc.alloc<double2>(nE);
+  double* ed = sc.alloc<double>(n);
+  int* cd = sc.alloc<int>(nc);
+  double2 *V = sc.alloc<double2>(nn), *Vh = sc.alloc<double2>(nn), *Vch = sc.alloc<double2>((size_t)n * nc);
+  double2 *Y = sc.alloc<double2>(nn), *Z = sc.alloc<double2>(nn), *X = sc.alloc<double2>(nn);
+  double2 *R = sc.alloc<double2>(nn);
+  double2 *Tp = sc.alloc<double2>((size_t)n * L), *T = sc.alloc<double2>((size_t)n * L);
+  double2 *P = sc.alloc<double2>((size_t)n * L), *Q = sc.alloc<double2>((size_t)n * L);
+  double2 *Ph = sc.alloc<double2>((size_t)n * L);
+  double2 *Sg = sc.alloc<double2>((size_t)nE * cc), *Gm = sc.alloc<double2>((size_t)nE * cc);
+  double2 *G0 = sc.alloc<double2>((size_t)nE * cc), *Mm = sc.alloc<double2>((size_t)nE * cc);
+  double2 *Ki = sc.alloc<double2>((size_t)nE * cc), *S = sc.alloc<double2>((size_t)nE * cc);
+  HIPCHK(hipMemcpyAsync(Ed, Eh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(Wd, Wh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cd, corb, sizeof(int) * nc, hipMemcpyHostToDevice, st));
+  typedef Tile<64, 64, 16, 2, false, 3> TBig;
+  typedef Tile<32, 32, 16, 2, false, 4> TSmall;
+  const dim3 g1(1024), b1(256);
+  for (int k = 0; k < NK; ++k) {
+    HIPCHK(hipMemcpyAsync(V, evecs + 2 * (size_t)k * nn, sizeof(double2) * nn,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ed, evals + (size_t)k * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(Sg, sigma_cc + 2 * (size_t)k * nE * cc, sizeof(double2) * nE * cc,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(Gm, gamma_w + 2 * (size_t)k * nE * cc, sizeof(double2) * nE * cc,
+                          hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scf_adjoint, g1, b1, 0, st, n, nc, (const double2*)V, (const int*)cd, Vh,
+                       Vch);
+    hipLaunchKernelGGL(k_scf_factors, g1, b1, 0, st, n, nE, nc, (const double2*)Ed,
+                       (const double*)ed, (const double2*)V, (const int*)cd, Tp, T);
+    HIPCHK(hipGetLastError());
+    // G0_i,cc = T_i V_c†, then I - Σ_i G0_i,cc, then its inverse K_i (R = I, solved in place)
+    GemmArgs<double, double> ga{nc, nc, n, T, n, (long)nc * n, Vch, nc, 0};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+        ga, nE, EpiStore<double2>{G0, nc, cc, 1.0, 0.0}, st)));
+    GemmArgs<double, double> gb{nc, nc, nc, Sg, nc, cc, G0, nc, cc};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+        gb, nE, EpiIdMinus{Mm, nc, cc}, st)));
+    hipLaunchKernelGGL(k_identity_batch, g1, b1, 0, st, nE, nc, Ki);
+    hipLaunchKernelGGL(k_gauss_jordan, dim3(nE), dim3(256), 0, st, nc, Mm, Ki);
+    HIPCHK(hipGetLastError());
+    // S_i = (w_i/2π) K_i M_i  (nc x nc x nc)
+    GemmArgs<double, double> gs{nc, nc, nc, Ki, nc, cc, Gm, nc, cc};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+        gs, nE, EpiScaleZ{S, nc, cc, Wd}, st)));
+    // P_i = T'_i K_i and Q_i = T'_i S_i, column block i of (n, nE nc)
+    GemmArgs<double, double> gp{n, nc, nc, Tp, L, (long)nc, Ki, nc, cc};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+        gp, nE, EpiStore<double2>{P, L, (long)nc, 1.0, 0.0}, st)));
+    GemmArgs<double, double> gq{n, nc, nc, Tp, L, (long)nc, S, nc, cc};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+        gq, nE, EpiStore<double2>{Q, L, (long)nc, 1.0, 0.0}, st)));
+    hipLaunchKernelGGL(k_conj_transpose, g1, b1, 0, st, n, (int)L, (const double2*)P, Ph);
+    HIPCHK(hipGetLastError());
+    // Y = Q P†  (n x nE nc)(nE nc x n); X = V Y V†; ρ = Hermitian part of X
+    GemmArgs<double, double> gy{n, n, (int)L, Q, L, 0, Ph, n, 0};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
+        gy, 1, EpiStore<double2>{Y, n, nn, 1.0, 0.0}, st)));
+    GemmArgs<double, double> gz{n, n, n, V, n, 0, Y, n, 0};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
+        gz, 1, EpiStore<double2>{Z, n, nn, 1.0, 0.0}, st)));
+    GemmArgs<double, double> gx{n, n, n, Z, n, 0, Vh, n, 0};
+    HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
+        gx, 1, EpiStore<double2>{X, n, nn, 1.0, 0.0}, st)));
+    hipLaunchKernelGGL(k_herm_part, g1, b1, 0, st, n, (const double2*)X, R);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rho + 2 * (size_t)k * nn, R, sizeof(double2) * nn, hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  for (size_t i = 0; i < 2 * (size_t)NK * nn; ++i)
+    require(std::isfinite(rho[i]), "non-finite density (singular Woodbury matrix?)", ZH_ENONFINITE);
+}
+
 template <typename F>
 static int zh_try(F&& f) {
   try {
@@ -2092,6 +2193,15 @@ int zh_set_dissipator(zh_problem* p, int kind, double eta0, double t0, double wi
     require(kind != ZH_DISS_STEP || width != 0.0, "zero dissipator step width");
     DissDesc d{kind, eta0, t0, width == 0.0 ? 1.0 : width};
     p->eng->set_dissipator(d, sig_eq);
+  });
+}
+
+int zh_window_density(int NK, int n, int nE, int nc, const double* nodes, const double* evals,
+                      const double* evecs, const int* corb, const double* sigma_cc,
+                      const double* gamma_w, double* rho, int device) {
+  return zh_try([&] {
+    require(nodes && evals && evecs && rho, "null argument");
+    window_density(NK, n, nE, nc, nodes, evals, evecs, corb, sigma_cc, gamma_w, rho, device);
   });
 }
 

@@ -155,6 +155,39 @@ __global__ __launch_bounds__(256) void k_gauss_jordan(int nc, double2* Mb, doubl
   }
 }
 
+// R[z] = I (nc x nc) for every batch entry z
+__global__ void k_identity_batch(int nb, int nc, double2* R) {
+  const long cc = (long)nc * nc, total = (long)nb * cc;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long q = idx % cc;
+    const long r = q / nc, c = q - r * nc;
+    R[idx] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// Ph = P† for P (rows, cols): Ph[c, r] = conj(P[r, c])
+__global__ void k_conj_transpose(int rows, int cols, const double2* P, double2* Ph) {
+  const long total = (long)rows * cols;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long r = idx / cols, c = idx - r * cols;
+    const double2 v = P[idx];
+    Ph[c * rows + r] = make_double2(v.x, -v.y);
+  }
+}
+
+// ρ = (X + X†)/2 (the Hermitian part of a product that is Hermitian up to rounding)
+__global__ void k_herm_part(int n, const double2* X, double2* rho) {
+  const long total = (long)n * n;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long r = idx / n, c = idx - r * n;
+    const double2 x = X[idx], y = X[c * n + r];
+    rho[idx] = make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+  }
+}
+
 // ρ = (i/2π)(X - X†)
 __global__ void k_scf_rho(int n, const double2* X, double2* rho) {
   const long total = (long)n * n;

@@ -1,5 +1,5 @@
-"""Equilibrium density matrix σ₀ on the GPU: the contour sum and self-consistency
-loop of ``Zandpack/cmdtools/SCF``.
+"""Density matrix σ₀ on the GPU: the contour sum, the bias-window term and the
+self-consistency loop of ``Zandpack/cmdtools/SCF``.
 
 The lead self-energy is the one the propagation constants encode
 (``FittedSelfEnergy.evaluate``: Σ = ½·KK[Γ] - ½i·Γ over the Lorentzian fit): for a
@@ -22,6 +22,7 @@ the lowest chemical potential (SCF:119).
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 
 import numpy as np
 
@@ -39,10 +40,15 @@ def coupled_orbitals(problem):
 
 def self_energy_cc(problem, E, corb, bias=None):
     """Σ(E_i) summed over leads on the orbitals corb: (NK, nE, nc, nc) complex128."""
+    return self_energy_cc_leads(problem, E, corb, bias).sum(axis=0)
+
+
+def self_energy_cc_leads(problem, E, corb, bias=None):
+    """Σ_α(E_i) per lead on the orbitals corb: (NL, NK, nE, nc, nc) complex128."""
     NK, n, NL, NX, NOT = problem.dims
     E = np.asarray(E, complex).reshape(-1)
     b = np.zeros(NL) if bias is None else np.asarray(bias, float)
-    out = np.zeros((NK, len(E), len(corb), len(corb)), complex)
+    out = np.zeros((NL, NK, len(E), len(corb), len(corb)), complex)
     for k in range(NK):
         for a in range(NL):
             d = problem.dGP[k, a].reshape(-1)
@@ -53,7 +59,7 @@ def self_energy_cc(problem, E, corb, bias=None):
             X = problem.xi[k, a].reshape(-1, n)[m][:, corb]        # (Mm, nc)
             Y = problem.Ixi[k, a].reshape(-1, n)[m][:, corb]
             s = 1j * d[m][None, :] / (E[:, None] - b[a] - xm[None, :])   # (nE, Mm)
-            out[k] += np.matmul(np.ascontiguousarray((s[:, :, None] * X[None]).transpose(0, 2, 1)), Y)
+            out[a, k] = np.matmul(np.ascontiguousarray((s[:, :, None] * X[None]).transpose(0, 2, 1)), Y)
     return out
 
 
@@ -78,6 +84,70 @@ def contour_density(H, contour, corb, sigma_cc, device=0):
         NK, n, cont.shape[0], nc, N.ptr(cont), N.ptr(e), N.ptr(V),
         corb.ctypes.data_as(C.POINTER(C.c_int)) if nc else None,
         N.ptr(sig) if nc else None, N.ptr(rho), int(device)))
+    return rho
+
+
+def fermi(E, mu, kT):
+    """1 / (1 + exp((E - mu)/kT))."""
+    return 1.0 / (1.0 + np.exp((np.asarray(E, float) - mu) / kT))
+
+
+def bias_window(mus, kTs, fact_kT=10.0, panels=None, order=8):
+    """Real-axis nodes and weights over [mu_min - fact_kT kT_max, mu_max + fact_kT kT_max]
+    (the integration range of cmdtools/SCF:273-276): composite Gauss-Legendre of
+    ``order`` points on ``panels`` equal panels (default: panels no wider than the
+    smallest kT).  The reference integrates adaptively (scipy quad_vec, gk21,
+    epsabs 1e-5, epsrel 1e-4); a fixed rule keeps the nodes, and so the self-energy
+    evaluations, the same in every SCF iteration."""
+    mus, kTs = np.asarray(mus, float), np.asarray(kTs, float)
+    e1 = mus.min() - fact_kT * kTs.max()
+    e2 = mus.max() + fact_kT * kTs.max()
+    if panels is None:
+        panels = max(8, int(np.ceil((e2 - e1) / kTs.min())))
+    x, w = np.polynomial.legendre.leggauss(int(order))
+    edges = np.linspace(e1, e2, int(panels) + 1)
+    half = 0.5 * np.diff(edges)
+    mid = 0.5 * (edges[:-1] + edges[1:])
+    nodes = (mid[:, None] + half[:, None] * x[None]).reshape(-1)
+    weights = (half[:, None] * w[None]).reshape(-1)
+    return nodes, weights
+
+
+def window_weights(sigma_leads, nodes, mus, kTs):
+    """M_i = Σ_α (f_α(E_i) - f_min(E_i)) Γ_α(E_i) with Γ_α = i(Σ_α - Σ_α†)
+    (the SE_l of cmdtools/SCF:246-247 with the fdf of :301-302, times -i):
+    (NK, nE, nc, nc) from ``sigma_leads`` (NL, NK, nE, nc, nc)."""
+    mus, kTs = np.asarray(mus, float), np.asarray(kTs, float)
+    imin = int(np.argmin(mus))
+    f_min = fermi(nodes, mus[imin], kTs[imin])
+    out = np.zeros(sigma_leads.shape[1:], complex)
+    for a in range(sigma_leads.shape[0]):
+        fd = fermi(nodes, mus[a], kTs[a]) - f_min
+        S = sigma_leads[a]
+        out += fd[None, :, None, None] * (1j * (S - S.conj().swapaxes(-1, -2)))
+    return out
+
+
+def window_density(H, nodes, weights, corb, sigma_cc, gamma_w, device=0):
+    """One device evaluation of Σ_i w_i (1/2π) G(E_i)[:, c] M_i G(E_i)[:, c]† for H
+    (NK, n, n): the bias-window density of cmdtools/SCF:299-306."""
+    lib = N.lib()
+    H = np.asarray(H, complex)
+    NK, n, _ = H.shape
+    e, V = np.linalg.eigh(H)
+    e = np.ascontiguousarray(e, np.float64)
+    V = np.ascontiguousarray(V, np.complex128)
+    nw = np.ascontiguousarray(np.stack([nodes, weights], axis=1), np.float64)
+    corb = np.ascontiguousarray(corb, np.int32)
+    nc = int(corb.size)
+    sig = N.c128(sigma_cc)
+    gam = N.c128(gamma_w)
+    assert sig.shape == gam.shape == (NK, nw.shape[0], nc, nc)
+    rho = np.empty((NK, n, n), complex)
+    N.check(lib.zh_window_density(
+        NK, n, nw.shape[0], nc, N.ptr(nw), N.ptr(e), N.ptr(V),
+        corb.ctypes.data_as(C.POINTER(C.c_int)), N.ptr(sig), N.ptr(gam), N.ptr(rho),
+        int(device)))
     return rho
 
 
@@ -114,21 +184,36 @@ class PulayMixer:
 
 
 def self_consistent_density(problem, contour, mu, rho0=None, t0=0.0, tol=1e-5, weight=0.2,
-                            history=6, max_iter=500, device=0, mixer=None, callback=None):
+                            history=6, max_iter=500, device=0, mixer=None, callback=None,
+                            kT=0.025, nonequilibrium=False, fact_kT=10.0, window_panels=None):
     """The SCF tool's loop (cmdtools/SCF:282-309): H = H0 + dH(t0, ρ_in), ρ_out from
-    the contour sum, stop when max|ρ_out - ρ_in| <= tol, else ρ_in <- mixer(ρ_in, δ).
-    Equilibrium only (one chemical potential ``mu`` for all leads, shifted by the
-    bias at t0 as SCF:95-99 does; unequal ones need the tool's real-axis
-    nonequilibrium term, not built).  Returns (ρ_out, iterations, converged)."""
+    the contour sum at the lowest chemical potential (the leads' ``mu`` shifted by the
+    bias at t0, SCF:93-99, 119), plus with ``nonequilibrium`` the bias-window term on
+    the real axis (SCF:299-306; ``kT`` per lead or one value), stop when
+    max|ρ_out - ρ_in| <= tol, else ρ_in <- mixer(ρ_in, δ).  Unequal chemical
+    potentials without ``nonequilibrium`` give a warning and the equilibrium density
+    at the lowest one, as the tool does (SCF:100-101).  Returns (ρ_out, iterations,
+    converged)."""
     NK, n, NL = problem.dims[:3]
     bias = np.array([problem.bias(t0, a) for a in range(NL)])
-    mus = float(mu) + bias
-    if mus.max() - mus.min() > 1e-6:
-        raise NotImplementedError("nonequilibrium SCF (unequal chemical potentials)")
+    mus = np.broadcast_to(np.asarray(mu, float), (NL,)) + bias
+    kTs = np.broadcast_to(np.asarray(kT, float), (NL,))
+    window = mus.max() - mus.min() > 1e-6
+    if window and not nonequilibrium:
+        warnings.warn("unequal chemical potentials without nonequilibrium=True: "
+                      "equilibrium density at the lowest one", stacklevel=2)
+        window = False
     cont = np.array(contour, float)
     cont[:, 0] += mus.min()
     corb = coupled_orbitals(problem)
     sig = self_energy_cc(problem, cont[:, 0] + 1j * cont[:, 1], corb, bias)
+    if window:
+        if corb.size == 0:
+            raise ValueError("nonequilibrium density needs lead-coupled orbitals")
+        nodes, wts = bias_window(mus, kTs, fact_kT, window_panels)
+        sig_l = self_energy_cc_leads(problem, nodes.astype(complex), corb, bias)
+        sig_w = sig_l.sum(axis=0)
+        gam_w = window_weights(sig_l, nodes, mus, kTs)
     rho_in = np.array(problem.sigma0 if rho0 is None else rho0, complex)
     mixer = mixer or PulayMixer(weight, history)
     it, ok = 0, False
@@ -136,6 +221,8 @@ def self_consistent_density(problem, contour, mu, rho0=None, t0=0.0, tol=1e-5, w
     while it < max_iter:
         H = np.asarray(problem.H0, complex) + problem.dH(t0, rho_in)
         rho_out = contour_density(H, cont, corb, sig, device)
+        if window:
+            rho_out = rho_out + window_density(H, nodes, wts, corb, sig_w, gam_w, device)
         delta = rho_out - rho_in
         it += 1
         if callback:
