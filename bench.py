@@ -269,11 +269,34 @@ def scf_leg(p, reps=3):
         rho = scf.contour_density(H, C, corb, sig)
         t_all.append(time.perf_counter() - t0)
     tr = np.trace(rho, axis1=1, axis2=2).real
-    return {"contour_points": int(C.shape[0]), "n_c": int(corb.size),
-            "s_per_iteration": float(np.median(t_all)),
-            "host_eigh_s": float(np.median(t_eigh)),
-            "device_s": float(np.median(t_all) - np.median(t_eigh)),
-            "self_energy_setup_s": t_setup, "charge_per_k": [float(x) for x in tr]}
+    out = {"contour_points": int(C.shape[0]), "n_c": int(corb.size),
+           "s_per_iteration": float(np.median(t_all)),
+           "host_eigh_s": float(np.median(t_eigh)),
+           "device_s": float(np.median(t_all) - np.median(t_eigh)),
+           "self_energy_setup_s": t_setup, "charge_per_k": [float(x) for x in tr]}
+    # under the workload's bias (cmdtools/SCF Nonequilibrium=True): the bias-window
+    # term on fixed Gauss-Legendre nodes, kT = 0.025 eV for both leads
+    NL = p.dims[2]
+    bias = np.array([p.bias(1.0, a) for a in range(NL)])
+    if bias.max() - bias.min() > 1e-6:
+        kTs = np.full(NL, 0.025)
+        nodes, w = scf.bias_window(bias, kTs)
+        t0 = time.perf_counter()
+        sig_l = scf.self_energy_cc_leads(p, nodes.astype(complex), corb, bias)
+        gam = scf.window_weights(sig_l, nodes, bias, kTs)
+        sig_w = sig_l.sum(axis=0)
+        t_wsetup = time.perf_counter() - t0
+        scf.window_density(H, nodes, w, corb, sig_w, gam)  # warm-up
+        t_w = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            rw = scf.window_density(H, nodes, w, corb, sig_w, gam)
+            t_w.append(time.perf_counter() - t0)
+        out["window"] = {"nodes": int(nodes.size), "bias_eV": [float(b) for b in bias],
+                         "s_per_iteration": float(np.median(t_w)),
+                         "setup_s": t_wsetup,
+                         "charge_per_k": [float(x) for x in np.trace(rw, axis1=1, axis2=2).real]}
+    return out
 
 
 def scf_cpu_sample(p, points=4):
