@@ -63,14 +63,19 @@ def self_energy_cc_leads(problem, E, corb, bias=None):
     return out
 
 
-def contour_density(H, contour, corb, sigma_cc, device=0):
-    """One device evaluation of Σ_i (i/2π)(W_i G_i - h.c.) for H (NK, n, n)."""
+def eigenpairs(H):
+    """H = V diag(e) V† per k (host LAPACK), in the layout the density entries take."""
+    e, V = np.linalg.eigh(np.asarray(H, complex))
+    return np.ascontiguousarray(e, np.float64), np.ascontiguousarray(V, np.complex128)
+
+
+def contour_density(H, contour, corb, sigma_cc, device=0, eig=None):
+    """One device evaluation of Σ_i (i/2π)(W_i G_i - h.c.) for H (NK, n, n);
+    ``eig`` = eigenpairs(H) when the caller has them already."""
     lib = N.lib()
     H = np.asarray(H, complex)
     NK, n, _ = H.shape
-    e, V = np.linalg.eigh(H)
-    e = np.ascontiguousarray(e, np.float64)
-    V = np.ascontiguousarray(V, np.complex128)
+    e, V = eigenpairs(H) if eig is None else eig
     cont = np.ascontiguousarray(contour, np.float64)
     assert cont.ndim == 2 and cont.shape[1] == 4
     corb = np.ascontiguousarray(corb, np.int32)
@@ -128,15 +133,14 @@ def window_weights(sigma_leads, nodes, mus, kTs):
     return out
 
 
-def window_density(H, nodes, weights, corb, sigma_cc, gamma_w, device=0):
+def window_density(H, nodes, weights, corb, sigma_cc, gamma_w, device=0, eig=None):
     """One device evaluation of Σ_i w_i (1/2π) G(E_i)[:, c] M_i G(E_i)[:, c]† for H
-    (NK, n, n): the bias-window density of cmdtools/SCF:299-306."""
+    (NK, n, n): the bias-window density of cmdtools/SCF:299-306; ``eig`` as for
+    ``contour_density``."""
     lib = N.lib()
     H = np.asarray(H, complex)
     NK, n, _ = H.shape
-    e, V = np.linalg.eigh(H)
-    e = np.ascontiguousarray(e, np.float64)
-    V = np.ascontiguousarray(V, np.complex128)
+    e, V = eigenpairs(H) if eig is None else eig
     nw = np.ascontiguousarray(np.stack([nodes, weights], axis=1), np.float64)
     corb = np.ascontiguousarray(corb, np.int32)
     nc = int(corb.size)
@@ -220,9 +224,10 @@ def self_consistent_density(problem, contour, mu, rho0=None, t0=0.0, tol=1e-5, w
     rho_out = rho_in
     while it < max_iter:
         H = np.asarray(problem.H0, complex) + problem.dH(t0, rho_in)
-        rho_out = contour_density(H, cont, corb, sig, device)
+        eig = eigenpairs(H)
+        rho_out = contour_density(H, cont, corb, sig, device, eig)
         if window:
-            rho_out = rho_out + window_density(H, nodes, wts, corb, sig_w, gam_w, device)
+            rho_out = rho_out + window_density(H, nodes, wts, corb, sig_w, gam_w, device, eig)
         delta = rho_out - rho_in
         it += 1
         if callback:
