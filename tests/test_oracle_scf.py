@@ -203,3 +203,22 @@ def test_window_closes_the_contour_with_one_lead():
     rho = O.contour_density(H, lo, sig_tot) + O.window_density(H, x, w, sig_leads, mus, kTs)
     ref = O.contour_density(H, hi, sig_tot)
     assert np.abs(rho - ref).max() < 2e-5 * np.abs(ref).max()
+
+
+def test_bias_window_nodes():
+    """The product's window nodes: the reference's limits (mu_min - 10 kT_max,
+    mu_max + 10 kT_max), panels no wider than the smallest kT, and a rule exact for
+    polynomials of degree 2*order - 1 on each panel."""
+    from zandpack_amd import scf
+    mus, kTs = np.array([0.15, -0.15]), np.array([0.025, 0.02])
+    x, w = scf.bias_window(mus, kTs)
+    e1, e2 = O.window_limits(mus, kTs)
+    assert e1 < x.min() and x.max() < e2
+    assert abs(w.sum() - (e2 - e1)) < 1e-13
+    panels = x.size // 8
+    assert (e2 - e1) / panels <= kTs.min() + 1e-15
+    for deg in (1, 5, 15):
+        exact = (e2 ** (deg + 1) - e1 ** (deg + 1)) / (deg + 1)
+        assert abs(np.dot(w, x ** deg) - exact) < 1e-12 * max(1.0, abs(exact))
+    xo, wo = O.gauss_legendre_panels(e1, e2, panels, 8)
+    np.testing.assert_allclose(np.sort(x), np.sort(xo), rtol=0, atol=1e-14)
