@@ -174,6 +174,16 @@ int zh_pi_get(zh_problem* p, int hermitian, double* Q);
  * device Hubbard term or zh_steady_state (ZH_EINVAL / ZH_ESTATE). */
 int zh_set_overlap(zh_problem* p, const double* Sinv);
 
+/* Error norm of the attempts.  (0, 0), the default, is zand's absolute norm
+ * TE = sqrt(sum |err|^2) over sigma, Psi and Omega (Zandpack/mpi_funcs.py:247-252,
+ * mpi/zand:390-396).  atol > 0 selects the scaled RMS norm
+ * TE = sqrt(mean |err / (atol + rtol max(|y0|, |y1|))|^2) over the NK(n^2 + Mn + M^2)
+ * entries, y0 and y1 the step's start and fifth-order end state, for runs specified
+ * by rtol/atol (the reference's in-process drivers take atol/rtol, but their branch
+ * evaluates to 1, TimedependentTransport.py:3226-3235); accept iff TE < eps (eps = 1
+ * is the usual choice) and h_next = 0.9 h (eps/TE)^(1/5) as for the absolute norm. */
+int zh_set_tolerance(zh_problem* p, double rtol, double atol);
+
 /* Preconditioner of zh_steady_state: (chi+_i + Delta - H + B_i)^-1 per mode i, with
  * H = H(t0, sigma0) through its eigenpairs (evecs (NK,n,n) complex128, column k =
  * vector k; evals (NK,n) real) and, optionally, the lead self-energy term

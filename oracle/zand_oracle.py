@@ -441,12 +441,14 @@ def relaxation_dissipator(eta, sig_eq):
 
 
 def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, dissipator=None,
-                  Sinv=None):
+                  Sinv=None, rtol=0.0, atol=0.0):
     """One 7-stage attempt (mpi/zand:301-396).  Returns (ks, TE, J_stage0, y_new).
 
     ``y0`` = (σ, Ψ, Ω); ``ks[i]`` are the h-scaled stage derivatives.  J is taken
     from the stage-0 state (mpi/zand:368-375).  ``y_new`` = y0 + Σ b5_j k_j
-    (step_fourth_*, mpi_funcs.py:254-259).
+    (step_fourth_*, mpi_funcs.py:254-259).  TE is zand's absolute norm
+    (mpi_funcs.py:247-252), or with atol > 0 the scaled RMS norm
+    sqrt(mean |err/(atol + rtol max(|y0|, |y_new|))|²) over all entries.
     """
     tab = tab or dop54_tableau()
     A, B, CT = tab["A"], tab["B"], tab["CT"]
@@ -467,13 +469,18 @@ def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, diss
         if i == 0:
             J0 = currents(consts, yi[1], hbar=hbar)
         ks.append(k)
-    te2 = 0.0
-    for part in range(3):
-        e = sum(CT[j] * ks[j][part] for j in range(7))
-        te2 += float(np.sum(e.real ** 2 + e.imag ** 2))
-    TE = te2 ** 0.5
     b5 = tab["b5"]
     y_new = tuple(y0[p] + sum(b5[j] * ks[j][p] for j in range(7)) for p in range(3))
+    te2, count = 0.0, 0
+    for part in range(3):
+        e = sum(CT[j] * ks[j][part] for j in range(7))
+        w = 1.0
+        if atol > 0:
+            sc = atol + rtol * np.maximum(np.abs(y0[part]), np.abs(y_new[part]))
+            w = 1.0 / sc ** 2
+        te2 += float(np.sum((e.real ** 2 + e.imag ** 2) * w))
+        count += e.size
+    TE = (te2 / count) ** 0.5 if atol > 0 else te2 ** 0.5
     return ks, TE, J0, y_new
 
 

@@ -320,6 +320,38 @@ def test_drop_in_rkf45(native, driver):
     prop.close()
 
 
+@pytest.mark.parametrize("case,precision", [("small", "fp64"), ("small", "mixed"),
+                                            ("tile_aligned", "fp64")])
+def test_scaled_norm_attempt(native, case, precision):
+    """Attempts under the scaled RMS norm (zh_set_tolerance) against the oracle: TE,
+    accept and the committed state; (0, 0) restores the absolute norm."""
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    y0 = _rand_state(p, 17, scale=0.02)
+    rtol, atol = 1e-4, 1e-8
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, p.dH, p.bias, 2,
+                                         rtol=rtol, atol=atol)
+    # mixed: the weights reach 1/atol² on near-zero entries, where the complex64
+    # rounding of the stored stage slots is not small against atol
+    tol = 1e-11 if precision == "fp64" else 1e-2
+    prop = Propagator(p, precision, state=y0)
+    prop.set_tolerance(rtol, atol)
+    say("scaled norm", case, precision, TE)
+    r = prop.attempt(0.1, 0.01, 1.0)
+    assert abs(r["te"] - TE) <= tol * TE, (r["te"], TE)
+    assert r["accepted"] == (TE < 1.0)
+    if r["accepted"]:
+        s, ps, om = prop.get_state()
+        for got, w in zip((s, ps, om), y_new):
+            assert rel(got, w) < (1e-12 if precision == "fp64" else 1e-5)
+    prop.put_state(*y0)
+    prop.set_tolerance(0.0, 0.0)
+    _, TE_abs, _, _ = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, p.dH, p.bias, 2)
+    r = prop.attempt(0.1, 0.01, 1.0)
+    assert abs(r["te"] - TE_abs) <= (1e-11 if precision == "fp64" else 1e-4) * TE_abs
+    prop.close()
+
+
 def test_drop_in_scipy_ode(native):
     """solve_ivp over the GPU f against solve_ivp over the oracle's RHS."""
     from scipy.integrate import solve_ivp

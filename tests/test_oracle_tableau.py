@@ -72,3 +72,28 @@ def test_rkf45_von_neumann():
     ref = U @ s0[0] @ U.conj().T
     assert np.abs(out["y"][0][0] - ref).max() < 1e-7 * np.abs(ref).max()
     assert len(out["times"]) > 3 and out["times"][-1] <= 0.5
+
+
+def test_scaled_error_norm():
+    """The scaled RMS norm: with rtol = 0 it is the absolute norm over atol sqrt(N),
+    and an entry-by-entry loop with max(|y0|, |y_new|) gives the same value."""
+    p = builder.config_small(seed=8)
+    rng = np.random.default_rng(8)
+    y0 = tuple(0.05 * (rng.normal(size=a.shape) + 1j * rng.normal(size=a.shape))
+               for a in p.zero_state())
+    args = (p.consts(), y0, 0.1, 0.01, p.H0, p.dH, p.bias, 2)
+    ks, te_abs, _, y_new = O.dop54_attempt(*args)
+    N = sum(a.size for a in y0)
+    _, te_a, _, _ = O.dop54_attempt(*args, rtol=0.0, atol=1e-7)
+    assert abs(te_a - te_abs / (1e-7 * N ** 0.5)) < 1e-12 * te_a
+    rtol, atol = 1e-3, 1e-6
+    _, te_s, _, _ = O.dop54_attempt(*args, rtol=rtol, atol=atol)
+    ct = O.dop54_tableau()["CT"]
+    acc = 0.0
+    for part in range(3):
+        e = sum(ct[j] * ks[j][part] for j in range(7)).ravel()
+        a, b = y0[part].ravel(), y_new[part].ravel()
+        for i in range(e.size):
+            sc = atol + rtol * max(abs(a[i]), abs(b[i]))
+            acc += abs(e[i]) ** 2 / sc ** 2
+    assert abs(te_s - (acc / N) ** 0.5) < 1e-12 * te_s

@@ -77,6 +77,7 @@ SIGNATURES = {
                                   C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "zh_set_dissipator": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                     _dp]),
+    "zh_set_tolerance": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "zh_set_overlap": (C.c_int, [C.c_void_p, _dp]),
     "zh_pi_get": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "zh_set_pulse_ex": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _dp, _dp]),

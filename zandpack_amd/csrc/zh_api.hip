@@ -92,6 +92,7 @@ struct EngineBase {
   virtual void set_pulse_spline(int nknots, const double* x, const double* c) = 0;
   virtual void set_dissipator(const DissDesc& d, const double* sig_eq) = 0;
   virtual void set_overlap(const double* Sinv) = 0;
+  virtual void set_tolerance(double rtol, double atol) = 0;
   virtual void pi_get(int hermitian, double* Q) = 0;
   virtual void state_put(const double* sig, const double* psi, const double* omg) = 0;
   virtual void state_get(double* sig, double* psi, double* omg) = 0;
@@ -308,6 +309,8 @@ struct Engine : EngineBase {
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     for (auto& e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl), hipHostMallocDefault));
+    std::memset(ctrl_h, 0, sizeof(Ctrl));
+    ctrl_h->norm_scale = 1.0;
     HIPCHK(hipHostMalloc((void**)&J_h, sizeof(double) * NL * NK, hipHostMallocDefault));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     ctrl = dalloc<Ctrl>(1);
@@ -743,6 +746,25 @@ struct Engine : EngineBase {
   }
 
   // S⁻¹ of the non-orthogonal form (null: back to the orthogonal form).
+  // Scaled error norm: TE = sqrt(mean |err/(atol + rtol max(|y0|, |y1|))|²) over the
+  // reference-layout entries of σ, Ψ and Ω; (0, 0) restores zand's absolute norm.
+  double tol_rtol = 0.0, tol_atol = 0.0;
+  bool tol_scaled = false;
+  void set_tolerance(double rtol, double atol) override {
+    require(rtol >= 0.0 && atol >= 0.0 && std::isfinite(rtol) && std::isfinite(atol),
+            "tolerances must be finite and non-negative");
+    require(rtol == 0.0 || atol > 0.0, "the scaled norm needs atol > 0 (zero entries)");
+    drop_graphs();
+    tol_rtol = rtol;
+    tol_atol = atol;
+    tol_scaled = atol > 0.0;
+  }
+  double norm_scale() const {
+    if (!tol_scaled) return 1.0;
+    const double cnt = (double)NK * ((double)n * n + (double)M * n + (double)M * M);
+    return 1.0 / cnt;
+  }
+
   void set_overlap(const double* S) override {
     HIPCHK(hipSetDevice(device));
     require(have_consts, "set the constants first", ZH_ESTATE);
@@ -1311,6 +1333,7 @@ struct Engine : EngineBase {
         e.u = uvec; e.v = vvec; e.M = Md;
         e.ct_self = os.ct_self;
         e.te_part = te_part + 2 * kTEBlocks;
+        e.rtol = tol_rtol; e.atol = tol_atol; e.scaled = tol_scaled ? 1 : 0;
       };
       if (os.te) {
         EpiOmegaRK<cs, true> e{};
@@ -1340,6 +1363,11 @@ struct Engine : EngineBase {
         ++nw;
       }
       require(nw == 6, "error weights: expected six non-zero entries");
+      if (tol_scaled) {
+        ts.y0 = sig[iy0]; ts.y1 = sig[iyn]; tp.y0 = psi[iy0]; tp.y1 = psi[iyn];
+        ts.rtol = tp.rtol = tol_rtol; ts.atol = tp.atol = tol_atol;
+        ts.scaled = tp.scaled = 1;
+      }
       MapDense ms{NK * n, n, n, n, n};
       MapDense mp{NK * Md, n, n, n, n};
       hipLaunchKernelGGL((k_te<double2, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, ts,
@@ -1478,6 +1506,7 @@ struct Engine : EngineBase {
     ctrl_h->h = info->h;
     ctrl_h->eps = info->eps;
     ctrl_h->force = info->force_accept ? 1 : 0;
+    ctrl_h->norm_scale = norm_scale();
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
     HIPCHK(hipEventRecord(ev[2 * KC_N], stream));
     const int par = (iy0 == 7) ? 0 : 1;
@@ -1546,6 +1575,7 @@ struct Engine : EngineBase {
     profile = false;
     // stage parameters at t0 with h = 1 (sp[0]: Δ(t0), the dissipator rate unused)
     ctrl_h->t0 = t0; ctrl_h->h = 1.0; ctrl_h->eps = 1.0; ctrl_h->force = 1;
+    ctrl_h->norm_scale = 1.0;
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_stage_params, dim3(1), dim3(1), 0, stream, (const Ctrl*)ctrl, pulse,
                        diss, tab, NL, sp);
@@ -2218,6 +2248,13 @@ int zh_pi_get(zh_problem* p, int hermitian, double* Q) {
   return zh_try([&] {
     require(p && Q, "null argument");
     p->eng->pi_get(hermitian, Q);
+  });
+}
+
+int zh_set_tolerance(zh_problem* p, double rtol, double atol) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_tolerance(rtol, atol);
   });
 }
 

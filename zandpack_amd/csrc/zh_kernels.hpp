@@ -34,7 +34,15 @@ struct Ctrl {
   int force;
   int accept;
   double te2, te, h_next;
+  double norm_scale;  // TE = sqrt(norm_scale * sum): 1 (absolute norm) or 1/count (scaled)
 };
+
+// Error weight of one complex entry in the scaled norm: 1/(atol + rtol max(|y0|, |y1|))²
+__device__ inline double te_weight(double atol, double rtol, double y0r, double y0i,
+                                   double y1r, double y1i) {
+  const double sc = atol + rtol * fmax(sqrt(y0r * y0r + y0i * y0i), sqrt(y1r * y1r + y1i * y1i));
+  return 1.0 / (sc * sc);
+}
 
 struct PulseDesc {
   int kind;
@@ -380,6 +388,11 @@ template <typename C>
 struct TEArgs {
   const C* k[7];
   double ct[7];
+  // scaled norm: the step's start and end states and the tolerances
+  const C* y0 = nullptr;
+  const C* y1 = nullptr;
+  double rtol = 0.0, atol = 0.0;
+  int scaled = 0;
 };
 
 constexpr int kTEBlocks = 1024;
@@ -402,7 +415,13 @@ __global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial
         vr += g.ct[j] * (double)kv[j].x;
         vi += g.ct[j] * (double)kv[j].y;
       }
-      acc += vr * vr + vi * vi;
+      if (g.scaled) {
+        const C a = g.y0[src + c], b = g.y1[src + c];
+        acc += (vr * vr + vi * vi) *
+               te_weight(g.atol, g.rtol, (double)a.x, (double)a.y, (double)b.x, (double)b.y);
+      } else {
+        acc += vr * vr + vi * vi;
+      }
     }
   }
   __shared__ double red[256];
@@ -431,7 +450,7 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller(const double* parti
   }
   if (threadIdx.x == 0) {
     const double te2 = red[0];
-    const double te = sqrt(te2);
+    const double te = sqrt(te2 * ctrl->norm_scale);
     ctrl->te2 = te2;
     ctrl->te = te;
     ctrl->accept = (ctrl->force || te < ctrl->eps) ? 1 : 0;
@@ -584,18 +603,20 @@ struct EpiOmegaRK {
   int M;
   double ct[kMaxTerms]; double ct_self;   // error weights of k[] and of k_s (TE)
   double* te_part;                        // one partial per workgroup (TE)
+  double rtol, atol; int scaled;          // scaled norm (TE): weights from |y0|, |Ω_7|
 
   // Four outputs of one column (rows[r] < 0: outside): every slot load first, then
   // the stores.
   __device__ inline void block(int b, const int* rows, int col, const double* re,
                                const double* im, double& acc) const {
     long off[4];
-    double sr[4], si[4], er[4], ei[4];
+    double sr[4], si[4], er[4], ei[4], yr0[4], yi0[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       off[r] = ((long)b * M + (rows[r] < 0 ? 0 : rows[r])) * M + col;
       const double2 y = rows[r] < 0 ? make_double2(0.0, 0.0) : ld_nt(y0 + off[r]);
       sr[r] = y.x; si[r] = y.y; er[r] = 0.0; ei[r] = 0.0;
+      if constexpr (TE) { yr0[r] = y.x; yi0[r] = y.y; }
     }
 #pragma unroll
     for (int j = 0; j < kMaxTerms; ++j)
@@ -626,7 +647,8 @@ struct EpiOmegaRK {
       if (out) st_nt(out + off[r], to_d2(ks));
       if constexpr (TE) {
         const double xr = er[r] + ct_self * (double)ks.x, xi = ei[r] + ct_self * (double)ks.y;
-        acc += xr * xr + xi * xi;
+        acc += scaled ? (xr * xr + xi * xi) * te_weight(atol, rtol, yr0[r], yi0[r], sr[r], si[r])
+                      : xr * xr + xi * xi;
       }
     }
   }
@@ -652,10 +674,13 @@ struct EpiOmegaRK {
     if (ncol <= 0) return;
     const long off = ((long)b * M + row) * M + col0;
     const bool vec = ncol == 4 && sizeof(C) == 8 && (off & 3) == 0;
-    double sr[4], si[4], er[4], ei[4];
+    double sr[4], si[4], er[4], ei[4], yr0[4], yi0[4];
     load4(y0, off, ncol, vec, sr, si);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) { er[r] = 0.0; ei[r] = 0.0; }
+    for (int r = 0; r < 4; ++r) {
+      er[r] = 0.0; ei[r] = 0.0;
+      if constexpr (TE) { yr0[r] = sr[r]; yi0[r] = si[r]; }
+    }
 #pragma unroll
     for (int j = 0; j < kMaxTerms; ++j)
       if (j < nterms) {
@@ -685,7 +710,9 @@ struct EpiOmegaRK {
       if constexpr (TE) {
         if (r < ncol) {
           const double xr = er[r] + ct_self * kr[r], xi = ei[r] + ct_self * ki[r];
-          acc += xr * xr + xi * xi;
+          acc += scaled ? (xr * xr + xi * xi) *
+                              te_weight(atol, rtol, yr0[r], yi0[r], sr[r], si[r])
+                        : xr * xr + xi * xi;
         }
       }
     }

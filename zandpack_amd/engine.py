@@ -68,6 +68,11 @@ class Propagator:
         N.check(self.lib.zh_set_pulse_ex(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
                                          N.ptr(par), N.ptr(w)))
 
+    def set_tolerance(self, rtol=0.0, atol=0.0):
+        """Scaled RMS error norm with these tolerances (attempts then take eps = 1);
+        (0, 0) restores zand's absolute norm."""
+        N.check(self.lib.zh_set_tolerance(self.h, float(rtol), float(atol)))
+
     def set_overlap(self, Sinv):
         """Non-orthogonal (nozand) form with S⁻¹ (NK, n, n); None: orthogonal form."""
         if Sinv is None:
