@@ -486,7 +486,7 @@ def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, diss
 
 def propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
               time_update="reference", max_attempts=100000, fixed_h=False, dissipator=None,
-              Sinv=None):
+              Sinv=None, rtol=0.0, atol=0.0):
     """zand's outer loop (mpi/zand:297-431).
 
     Accept iff TE < eps; h <- 0.9 h (eps/TE)^0.2 after every attempt; on accept
@@ -509,7 +509,8 @@ def propagate(consts, y0, t0, t1, h, eps, H0, dH, bias, NL, hbar=HBAR,
             if attempts > max_attempts:
                 raise RuntimeError("attempt limit reached")
             ks, TE, J0, y_new = dop54_attempt(consts, y, t, h, H0, dH, bias, NL, hbar, tab,
-                                              dissipator=dissipator, Sinv=Sinv)
+                                              dissipator=dissipator, Sinv=Sinv, rtol=rtol,
+                                              atol=atol)
             if not np.isfinite(TE):
                 raise FloatingPointError("non-finite error estimate")
             accept = fixed_h or (TE < eps)

@@ -481,6 +481,27 @@ def test_every_tile_candidate_matches_oracle(native, precision, case):
     prop.close()
 
 
+def test_config1_scaled_tolerances(native):
+    """Config 1 as specified (DOP54, rtol 1e-8, atol 1e-10; here to 4 fs): the
+    device propagation under the scaled norm takes the oracle's accepted steps, with
+    its currents and final state (FP64 ≤ 1e-10)."""
+    from zandpack_amd.engine import Propagator
+    p = builder.config_chain()
+    y0 = p.zero_state()
+    rtol, atol = 1e-8, 1e-10
+    say("c1 scaled tolerances")
+    ref = O.propagate(p.consts(), y0, 0.0, 4.0, 0.01, 1.0, p.H0, p.dH, p.bias, 2,
+                      rtol=rtol, atol=atol)
+    prop = Propagator(p, "fp64")
+    prop.set_tolerance(rtol, atol)
+    out = prop.propagate(0.0, 4.0, 0.01, 1.0)
+    assert out["n_steps"] == len(ref["times"]) - 1
+    np.testing.assert_allclose(out["times"], ref["times"], rtol=1e-10, atol=1e-12)
+    assert rel(out["J"], ref["J"]) < 1e-10
+    assert rel(prop.sigma(), ref["y"][0]) < 1e-10
+    prop.close()
+
+
 def test_config1_golden_fixture(native):
     """Config 1 to 50 fs against tests/golden/c1_chain20.npz: replay of the
     fixture's (t, h) sequence reproduces J(t) and σ(t) (FP64 ≤ 1e-10, mixed J ≤ 1e-6),
