@@ -335,6 +335,38 @@ def replay_fp64_error(req):
     return float(np.abs(Jm - J).max() / max(np.abs(J).max(), 1e-300))
 
 
+def c2_leg(steps=30, warmup=3):
+    """Config 2, the FP64 drop-in (n = 400, M = 1600, Gaussian pulse) with adaptive
+    steps under rtol = 1e-10 (the scaled norm also needs an atol: 1e-12), from the
+    zero state: accepted steps/s over ``steps`` steps after ``warmup``."""
+    from zandpack_amd import builder
+    from zandpack_amd.engine import Propagator
+    rtol, atol = 1e-10, 1e-12
+    p = builder.config_c2()
+    prop = Propagator(p, "fp64", autotune=True)
+    prop.set_tolerance(rtol, atol)
+    t, h = 0.0, 0.005
+    acc = rej = 0
+    t_start = None
+    while acc < warmup + steps:
+        if acc == warmup and t_start is None:
+            t_start = time.perf_counter()
+        r = prop.attempt(t, h, 1.0)
+        if r["accepted"]:
+            t += r["h_next"]
+            acc += 1
+        else:
+            rej += 1
+        h = r["h_next"]
+    dt = time.perf_counter() - t_start
+    prop.close()
+    NK, n, NL, NX, NOT = p.dims
+    return {"workload": "honeycomb 20x20 n=400, 2 leads x 8 ch, N_L=40 + 60 Fermi slots, "
+                        "M=1600, Gaussian pulse 0.5 eV", "precision": "fp64", "rtol": rtol,
+            "atol": atol, "steps": steps, "steps_per_s": steps / dt, "t_reached_fs": t,
+            "rejected": rej, "M": NL * NX * NOT, "n": n}
+
+
 def steady_leg(p, tol=1e-10, maxiter=200):
     """Steady state (cmdtools/psinought equivalent) of the same workload in FP64:
     wall time including the host eigendecomposition of H(σ₀), iterations, relative
@@ -490,6 +522,8 @@ def main():
         print("# batched (config 4)", file=sys.stderr, flush=True)
         print("# steady state", file=sys.stderr, flush=True)
         line["steady_state"] = steady_leg(p)
+        print("# config 2 (FP64, rtol 1e-10)", file=sys.stderr, flush=True)
+        line["config2"] = c2_leg()
         print("# equilibrium density", file=sys.stderr, flush=True)
         line["scf"] = scf_leg(p)
         line["batched"] = batched_leg(args.batched_instances, args.batched_steps)
