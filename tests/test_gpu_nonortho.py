@@ -106,6 +106,31 @@ def test_attempt_parity_nonortho(native, case, layout):
     prop.close()
 
 
+@pytest.mark.parametrize("case", ["small_nk2", "chain20"])
+@pytest.mark.parametrize("layout", ["reference", "pruned"])
+def test_scaled_norm_nonortho(native, case, layout):
+    """The scaled RMS norm (zh_set_tolerance) on the non-orthogonal right-hand side:
+    TE, accept and the committed state against the oracle."""
+    from zandpack_amd.engine import Propagator
+    q = CASES[case]()
+    q.pulse = builder.Pulse("step", amp=0.2, p1=0.0, weights=(0.5, -0.5))
+    y0 = _rand_state(q, 23, scale=0.02)
+    rtol, atol = 1e-4, 1e-8
+    t0, h = 0.3, 0.02
+    say("nonortho scaled norm", case, layout)
+    ks, TE, J0, y_new = O.dop54_attempt(q.consts(), y0, t0, h, q.H0, q.dH, q.bias, q.dims[2],
+                                        Sinv=q.Sinv, rtol=rtol, atol=atol)
+    prop = Propagator(q, "fp64", state=y0, layout=layout)
+    prop.set_tolerance(rtol, atol)
+    r = prop.attempt(t0, h, 1.0)
+    assert abs(r["te"] - TE) <= 1e-10 * TE, (r["te"], TE)
+    assert r["accepted"] == (TE < 1.0)
+    want = y_new if r["accepted"] else y0
+    for got, w in zip(prop.get_state(), want):
+        assert rel(got, w) < 1e-11
+    prop.close()
+
+
 def test_propagate_nonortho_chain20(native):
     """nozand's adaptive loop (error norm in the non-orthogonal basis), 0 -> 2 fs;
     then mixed precision replaying the accepted (t, h) sequence: J within 1e-6."""
