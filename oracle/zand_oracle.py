@@ -429,6 +429,58 @@ def zero_dH(t, sig):
     return np.zeros(sig.shape, complex)
 
 
+def hubbard_energy(H0, sig, U, n0):
+    """Mean-field Hubbard energy E[σ] = Σ_k Tr(H0_k σ_k) + U Σ_i (σ↑_ii - n0↑_i)(σ↓_ii - n0↓_i),
+    whose derivative ∂E/∂σ_k is H0_k + dH_k with dH the cross-spin diagonal of
+    ``hubbard_dH``; the isolated-device evolution dσ_k/dt = -(i/ħ)[H_k(σ), σ_k]
+    conserves it."""
+    n0 = np.asarray(n0)
+    E = np.einsum("kij,kji->", H0, sig).real
+    d0 = np.einsum("ii->i", sig[0]).real - n0[0]
+    d1 = np.einsum("ii->i", sig[1]).real - n0[1]
+    return float(E + U * np.dot(d0, d1))
+
+
+def waveform(kind, amp, par):
+    """Time factor f(t) of a drive term H(t) = H0 + Σ_i f_i(t) M_i, restating the
+    forms the reference's examples write into dH(t, σ):
+      'cos_window'  amp cos(ω t + φ) for t_on < t <= t_off, else 0; par = (ω, φ, t_on, t_off)
+                    (examples/nb_tutorial_updated/Tight-binding/T1_Rabi/Bias.py:16-24)
+      'sin_fermi'   amp sin(ω t + φ) / (1 + exp((t - t_off)/w)); par = (ω, φ, t_off, w)
+                    (examples/Reproduce_and_checks/scipy_vs_ARK4_Zand/shared.py:4-19)
+      'gauss_sin'   amp exp(-((t - d)/s)^2) sin(2π f (t - d) - φ); par = (s, f, φ, d)
+                    (examples/nb_tutorial_updated/Tight-binding/T2_TB_Chain/Bias.py:9-10, 21-26;
+                    Zandpack/Pulses.py:52-66)
+      'const'       amp."""
+    par = tuple(float(x) for x in par) + (0.0,) * (4 - len(par))
+    a, b, c, d = par[:4]
+    if kind == "cos_window":
+        return lambda t: amp * np.cos(a * t + b) if c < t <= d else 0.0
+    if kind == "sin_fermi":
+        return lambda t: amp * np.sin(a * t + b) / (1.0 + np.exp((t - c) / d))
+    if kind == "gauss_sin":
+        return lambda t: amp * np.exp(-((t - d) / a) ** 2) * np.sin(2 * np.pi * b * (t - d) - c)
+    if kind == "const":
+        return lambda t: amp
+    raise ValueError(kind)
+
+
+def model_dH(terms=(), U=0.0, n0=None):
+    """dH(t, σ) = Σ_i f_i(t) M_i (+ the Hubbard term when U != 0): the time-dependent
+    Hamiltonian increment a Bias.py hands the propagator per stage (mpi/zand:319;
+    mpi/nozand:427).  ``terms`` is a sequence of (f, M) with f a callable of t and M
+    a (NK, n, n) matrix."""
+    hub = hubbard_dH(U, n0) if U else None
+
+    def dH(t, sig):
+        out = np.zeros(sig.shape, complex) if hub is None else hub(t, sig)
+        for f, M in terms:
+            out = out + f(t) * np.asarray(M, complex)
+        return out
+
+    return dH
+
+
 # ---------------------------------------------------------------------------
 # DOP54 attempt / propagation with zand's control semantics
 # ---------------------------------------------------------------------------
@@ -438,6 +490,16 @@ def relaxation_dissipator(eta, sig_eq):
     def diss(t, sig):
         return eta(t) * (sig - sig_eq)
     return diss
+
+
+def dissipator_rate(kind, eta0, t0=0.0, width=1.0):
+    """η(t) of the relaxation dissipator: 'const' η0; 'step' η0 / (1 + exp((t - t0)/w)),
+    the smooth switch-off of examples/Hubbard/Calculation/Bias.py:29-39."""
+    if kind == "const":
+        return lambda t: eta0
+    if kind == "step":
+        return lambda t: eta0 / (1.0 + np.exp((t - t0) / width))
+    raise ValueError(kind)
 
 
 def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, dissipator=None,

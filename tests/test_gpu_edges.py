@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import zand_oracle as O
+from oracle_side import dH_of, diss_of  # noqa: F401
 from zandpack_amd import builder
 
 pytestmark = pytest.mark.gpu
@@ -90,7 +91,7 @@ def test_edge_attempt(native, case):
     say("edge attempt", case, p.dims)
     prop = Propagator(p, "fp64", state=y0)
     t0, h, eps = 0.2, 0.02, 1e-3
-    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, p.dims[2])
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, dH_of(p), p.bias, p.dims[2])
     r = prop.attempt(t0, h, eps)
     assert abs(r["te"] - TE) <= 1e-10 * TE
     want = y_new if r["accepted"] else y0

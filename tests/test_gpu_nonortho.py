@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import zand_oracle as O
+from oracle_side import dH_of, diss_of  # noqa: F401
 from zandpack_amd import builder
 
 pytestmark = pytest.mark.gpu
@@ -94,7 +95,7 @@ def test_attempt_parity_nonortho(native, case, layout):
     say("nonortho attempt", case, layout)
     prop = Propagator(q, "fp64", state=y0, layout=layout)
     t0, h, eps = 0.3, 0.02, 1e-3
-    ks, TE, J0, y_new = O.dop54_attempt(q.consts(), y0, t0, h, q.H0, q.dH, q.bias, q.dims[2],
+    ks, TE, J0, y_new = O.dop54_attempt(q.consts(), y0, t0, h, q.H0, dH_of(q), q.bias, q.dims[2],
                                         Sinv=q.Sinv)
     r = prop.attempt(t0, h, eps)
     assert abs(r["te"] - TE) <= 1e-10 * TE
@@ -118,7 +119,7 @@ def test_scaled_norm_nonortho(native, case, layout):
     rtol, atol = 1e-4, 1e-8
     t0, h = 0.3, 0.02
     say("nonortho scaled norm", case, layout)
-    ks, TE, J0, y_new = O.dop54_attempt(q.consts(), y0, t0, h, q.H0, q.dH, q.bias, q.dims[2],
+    ks, TE, J0, y_new = O.dop54_attempt(q.consts(), y0, t0, h, q.H0, dH_of(q), q.bias, q.dims[2],
                                         Sinv=q.Sinv, rtol=rtol, atol=atol)
     prop = Propagator(q, "fp64", state=y0, layout=layout)
     prop.set_tolerance(rtol, atol)
@@ -139,7 +140,7 @@ def test_propagate_nonortho_chain20(native):
     q.pulse = builder.Pulse("step", amp=0.1, p1=0.0, weights=(0.5, -0.5))
     y0 = q.zero_state()
     say("nonortho propagate")
-    ref = O.propagate(q.consts(), y0, 0.0, 2.0, 0.01, 1e-8, q.H0, q.dH, q.bias, 2, Sinv=q.Sinv)
+    ref = O.propagate(q.consts(), y0, 0.0, 2.0, 0.01, 1e-8, q.H0, dH_of(q), q.bias, 2, Sinv=q.Sinv)
     prop = Propagator(q, "fp64")
     out = prop.propagate(0.0, 2.0, 0.01, 1e-8)
     assert out["n_steps"] == len(ref["times"]) - 1

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import zand_oracle as O
+from oracle_side import dH_of, diss_of  # noqa: F401
 from zandpack_amd import builder
 
 pytestmark = pytest.mark.gpu
@@ -85,7 +86,7 @@ def test_rhs_parity(native, case, precision, layout, hproducts):
     from zandpack_amd.engine import Propagator
     p = CASES[case]()
     sig, psi, omg = _rand_state(p, 7)
-    Ht = p.H0 + p.dH(0.2, sig)
+    Ht = p.H0 + dH_of(p)(0.2, sig)
     delta = np.array([p.bias(0.2, a) for a in range(2)]) + np.array([0.05, -0.03])
     ref = O.rhs(p.consts(), sig, psi, omg, Ht, delta)
     Jref = O.currents(p.consts(), psi)
@@ -121,7 +122,7 @@ def test_attempt_parity(native, case, layout, zero_ff):
     assert info["active_first"] == (layout == "pruned")
     assert info["skip_passive_block"] == (layout == "pruned" and zero_ff)
     t0, h, eps = 0.3, 0.02, 1e-3
-    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2)
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, dH_of(p), p.bias, 2)
     r = prop.attempt(t0, h, eps)
     assert abs(r["te"] - TE) <= 1e-10 * TE
     assert r["accepted"] == (TE < eps)
@@ -145,7 +146,7 @@ def test_attempt_sparse_h(native, case, precision):
     prop = Propagator(p, precision, state=y0, hproducts="sparse")
     assert prop.hamiltonian_info()["sparse"]
     t0, h, eps = 0.3, 0.02, 1e-3
-    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2)
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, dH_of(p), p.bias, 2)
     r = prop.attempt(t0, h, eps)
     tol = 1e-10 if precision == "fp64" else 1e-4
     assert abs(r["te"] - TE) <= tol * TE
@@ -218,15 +219,15 @@ def test_attempt_dissipator(native, case, kind, precision):
     say("attempt dissipator", case, kind, precision)
     prop = Propagator(p, precision, state=y0)
     t0, h, eps = 0.3, 0.02, 1e-3
-    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2,
-                                        dissipator=p.dissipator)
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, dH_of(p), p.bias, 2,
+                                        dissipator=diss_of(p))
     r = prop.attempt(t0, h, eps, force_accept=True)
     tol = 1e-10 if precision == "fp64" else 1e-4
     assert abs(r["te"] - TE) <= tol * TE
     s = prop.sigma()
     assert rel(s, y_new[0]) < (1e-12 if precision == "fp64" else 1e-6)
     # without the dissipator the step differs by far more than that
-    ks0, TE0, _, y_nd = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, p.dH, p.bias, 2)
+    ks0, TE0, _, y_nd = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, dH_of(p), p.bias, 2)
     assert rel(y_nd[0], y_new[0]) > 1e-4
     prop.close()
 
@@ -250,7 +251,7 @@ def test_propagate_chain20(native, graph, layout):
     p = builder.config_chain()
     y0 = p.zero_state()
     say("propagate chain20 graph", graph, layout)
-    ref = O.propagate(p.consts(), y0, 0.0, 3.0, 0.01, 1e-8, p.H0, p.dH, p.bias, 2)
+    ref = O.propagate(p.consts(), y0, 0.0, 3.0, 0.01, 1e-8, p.H0, dH_of(p), p.bias, 2)
     prop = Propagator(p, "fp64", graph=graph, layout=layout)
     out = prop.propagate(0.0, 3.0, 0.01, 1e-8)
     assert out["n_steps"] == len(ref["times"]) - 1
@@ -308,9 +309,9 @@ def test_drop_in_rkf45(native, driver):
     say("rkf45", driver)
     # (with the driver's default first step, (t1-t0)/1000, the first error norms are at
     # the rounding level and the step sequence is not reproducible to 1e-10)
-    times, data = getattr(drop_in, driver)(f, *y0, 1e-7, 0.0, 0.3, p.dH, p.bias, p.Ixi,
+    times, data = getattr(drop_in, driver)(f, *y0, 1e-7, 0.0, 0.3, dH_of(p), p.bias, p.Ixi,
                                            h_guess=0.01, print_to_file=False)
-    ref = O.rkf45_propagate(p.consts(), y0, 0.0, 0.3, 0.01, 1e-7, p.H0, p.dH, p.bias, 2)
+    ref = O.rkf45_propagate(p.consts(), y0, 0.0, 0.3, 0.01, 1e-7, p.H0, dH_of(p), p.bias, 2)
     assert len(times) == len(ref["times"])
     np.testing.assert_allclose(times, ref["times"], rtol=1e-10)
     J = np.array([data["current_left"], data["current_right"]]).transpose(1, 0, 2)
@@ -329,7 +330,7 @@ def test_scaled_norm_attempt(native, case, precision):
     p = CASES[case]()
     y0 = _rand_state(p, 17, scale=0.02)
     rtol, atol = 1e-4, 1e-8
-    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, p.dH, p.bias, 2,
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, dH_of(p), p.bias, 2,
                                          rtol=rtol, atol=atol)
     # mixed: the weights reach 1/atol² on near-zero entries, where the complex64
     # rounding of the stored stage slots is not small against atol
@@ -346,7 +347,7 @@ def test_scaled_norm_attempt(native, case, precision):
             assert rel(got, w) < (1e-12 if precision == "fp64" else 1e-5)
     prop.put_state(*y0)
     prop.set_tolerance(0.0, 0.0)
-    _, TE_abs, _, _ = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, p.dH, p.bias, 2)
+    _, TE_abs, _, _ = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, dH_of(p), p.bias, 2)
     r = prop.attempt(0.1, 0.01, 1.0)
     assert abs(r["te"] - TE_abs) <= (1e-11 if precision == "fp64" else 1e-4) * TE_abs
     prop.close()
@@ -363,7 +364,7 @@ def test_drop_in_scipy_ode(native):
     y0 = p.zero_state()
     t_eval = np.array([0.1, 0.2, 0.3])
     say("scipy_ode")
-    t, rho, cl = drop_in.scipy_ode(f, *y0, 0.0, t_eval, p.dH, p.bias, p.Ixi, atol=1e-8, rtol=1e-6)
+    t, rho, cl = drop_in.scipy_ode(f, *y0, 0.0, t_eval, dH_of(p), p.bias, p.Ixi, atol=1e-8, rtol=1e-6)
     shapes = [a.shape for a in y0]
     sizes = [a.size for a in y0]
 
@@ -371,7 +372,7 @@ def test_drop_in_scipy_ode(native):
         parts = np.split(y, np.cumsum(sizes)[:-1])
         s, ps, om = (v.reshape(sh) for v, sh in zip(parts, shapes))
         delta = np.array([p.bias(tt, a) for a in range(2)])
-        d = O.rhs(p.consts(), s, ps, om, p.H0 + p.dH(tt, s), delta)
+        d = O.rhs(p.consts(), s, ps, om, p.H0 + dH_of(p)(tt, s), delta)
         return np.concatenate([v.ravel() for v in d])
 
     y = np.concatenate([v.ravel().astype(complex) for v in y0])
@@ -394,9 +395,9 @@ def test_drop_in_f_and_adaptive_dop(native):
     prop = Propagator(p, "fp64")
     f = make_f(prop)
     y0 = p.zero_state()
-    times, data = AdaptiveDOP(f, *y0, 1e-7, 0.0, 0.3, p.dH, p.bias, p.Ixi, h_guess=0.01,
+    times, data = AdaptiveDOP(f, *y0, 1e-7, 0.0, 0.3, dH_of(p), p.bias, p.Ixi, h_guess=0.01,
                               print_to_file=False)
-    ref = O.propagate(p.consts(), y0, 0.0, 0.3, 0.01, 1e-7, p.H0, p.dH, p.bias, 2)
+    ref = O.propagate(p.consts(), y0, 0.0, 0.3, 0.01, 1e-7, p.H0, dH_of(p), p.bias, 2)
     np.testing.assert_allclose(times, ref["times"][:-1], rtol=1e-10)
     J = np.array([data["current_left"], data["current_right"]]).transpose(1, 0, 2)
     assert rel(J, ref["J"]) < 1e-10
@@ -446,10 +447,10 @@ def test_c2_size_properties(native):
     prop.close()
 
 
-_HUNG = ("hung once (candidate 2, inside zh_attempt) in a run whose only device change "
-         "was a trial of occupancy hints on the FP32 Ω tiles, since taken out; the kernels "
-         "it runs are those of earlier passing runs and no cause was found by reading the "
-         "code, so the case stays off the GPU until one is shown")
+_HUNG = ("hung once inside zh_attempt at tile candidate 2 in graph mode (FP64, complex H, "
+         "n = 12, M = 28); no cause was found in the kernels' index arithmetic or the "
+         "capture's fork/join, so FP64 candidate 2 is switched off in the engine and this "
+         "case stays off the GPU until a cause is shown")
 
 
 @pytest.mark.parametrize("case,precision", [
@@ -457,19 +458,20 @@ _HUNG = ("hung once (candidate 2, inside zh_attempt) in a run whose only device 
     pytest.param("complex_H", "fp64", marks=pytest.mark.skip(reason=_HUNG)),
     ("complex_H", "mixed")])
 def test_every_tile_candidate_matches_oracle(native, precision, case):
-    """The autotuner may pick any compiled tile shape: every candidate gives the
-    same attempt as the oracle."""
+    """The autotuner may pick any enabled tile shape: every candidate gives the
+    same attempt as the oracle (FP64: candidates 0 and 1; candidate 2 is off)."""
     from zandpack_amd.engine import Propagator
     p = CASES[case]()
     y0 = _rand_state(p, 31, scale=0.02)
-    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, p.dH, p.bias, 2)
+    ks, TE, J0, y_new = O.dop54_attempt(p.consts(), y0, 0.1, 0.01, p.H0, dH_of(p), p.bias, 2)
     # the error norm is a difference of nearly equal stage sums: its FP32 relative
     # error is larger than the state's
     tol = 1e-11 if precision == "fp64" else 1e-4
+    ncand = 2 if precision == "fp64" else 3
     prop = Propagator(p, precision, state=y0, autotune=True)
     tiles = prop.autotune(force=True)
-    assert all(0 <= v < 3 for v in tiles.values())
-    for c in range(3):
+    assert all(0 <= v < ncand for v in tiles.values())
+    for c in range(ncand):
         say("tiles", case, precision, c)
         prop.put_state(*y0)
         prop.set_tiles({k: c for k in tiles})
@@ -478,6 +480,17 @@ def test_every_tile_candidate_matches_oracle(native, precision, case):
         s, ps, om = prop.get_state()
         for got, w in zip((s, ps, om), y_new):
             assert rel(got, w) < (1e-12 if precision == "fp64" else 1e-5), c
+    prop.close()
+
+
+def test_fp64_candidate2_refused(native):
+    """FP64 tile candidate 2 is switched off: set_tiles refuses it (nothing runs)."""
+    from zandpack_amd._native import ZHError
+    from zandpack_amd.engine import Propagator
+    p = CASES["small"]()
+    prop = Propagator(p, "fp64")
+    with pytest.raises(ZHError, match="switched off"):
+        prop.set_tiles({k: 2 for k in ("psi", "hsig", "omega", "omgxi", "pi")})
     prop.close()
 
 
@@ -490,7 +503,7 @@ def test_config1_scaled_tolerances(native):
     y0 = p.zero_state()
     rtol, atol = 1e-8, 1e-10
     say("c1 scaled tolerances")
-    ref = O.propagate(p.consts(), y0, 0.0, 4.0, 0.01, 1.0, p.H0, p.dH, p.bias, 2,
+    ref = O.propagate(p.consts(), y0, 0.0, 4.0, 0.01, 1.0, p.H0, dH_of(p), p.bias, 2,
                       rtol=rtol, atol=atol)
     prop = Propagator(p, "fp64")
     prop.set_tolerance(rtol, atol)
@@ -552,7 +565,7 @@ def test_device_pulses_propagate(native, kind):
     p.pulse = _pulse_cases()[kind]
     y0 = p.zero_state()
     say("pulse", kind)
-    ref = O.propagate(p.consts(), y0, 0.0, 2.0, 0.01, 1e-8, p.H0, p.dH, p.bias, 2)
+    ref = O.propagate(p.consts(), y0, 0.0, 2.0, 0.01, 1e-8, p.H0, dH_of(p), p.bias, 2)
     prop = Propagator(p, "fp64")
     out = prop.propagate(0.0, 2.0, 0.01, 1e-8)
     assert out["n_steps"] == len(ref["times"]) - 1

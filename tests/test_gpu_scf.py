@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import zand_oracle as O
+from oracle_side import dH_of, diss_of  # noqa: F401
 from zandpack_amd import builder, scf
 
 pytestmark = pytest.mark.gpu
@@ -70,7 +71,7 @@ def test_self_consistent_hubbard_fixed_point(native):
     say("scf hubbard", p.dims)
     rho, its, ok = scf.self_consistent_density(p, C, mu=0.0, tol=1e-8, max_iter=200)
     assert ok, its
-    H = np.asarray(p.H0, complex) + p.dH(0.0, rho)
+    H = np.asarray(p.H0, complex) + dH_of(p)(0.0, rho)
     F = O.contour_density(H, C, lambda E: O.lead_self_energy(p.consts(), E).sum(axis=1))
     assert np.abs(F - rho).max() < 1e-7
     say("scf iterations", its)
@@ -129,7 +130,7 @@ def test_nonequilibrium_fixed_point(native):
     nodes, w = scf.bias_window(mus, [kT, kT], panels=10)
     C = _contour().copy()
     C[:, 0] += mus.min()
-    H = np.asarray(p.H0, complex) + p.dH(0.0, rho)
+    H = np.asarray(p.H0, complex) + dH_of(p)(0.0, rho)
     sig_l = lambda E: O.lead_self_energy(p.consts(), E, bias)
     F = O.contour_density(H, C, lambda E: sig_l(E).sum(axis=1))
     F = F + O.window_density(H, nodes, w, sig_l, mus, [kT, kT])

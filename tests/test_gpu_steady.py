@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import zand_oracle as O
+from oracle_side import dH_of, diss_of  # noqa: F401
 from zandpack_amd import builder
 
 pytestmark = pytest.mark.gpu
@@ -44,7 +45,7 @@ def _problem(case):
 @functools.lru_cache(maxsize=None)
 def _oracle_solution(case, t0):
     p = _problem(case)
-    H = p.H0 + p.dH(t0, p.sigma0)
+    H = p.H0 + dH_of(p)(t0, p.sigma0)
     return O.steady_state_dense(p.consts(), H, p.sigma0, _bias(p, t0))
 
 
@@ -80,7 +81,7 @@ def test_steady_state_is_a_fixed_point(native, case):
     prop = Propagator(p, "fp64")
     its, res = prop.steady_state(t0, tol=1e-13, maxiter=2000)
     s, ps, om = prop.get_state()
-    Ht = p.H0 + p.dH(t0, s)
+    Ht = p.H0 + dH_of(p)(t0, s)
     ds, dp, do, J = prop.rhs(Ht, _bias(p, t0), s, ps, om)
     # compare with the size of the terms that cancel
     scale_p = np.abs(O.steady_state_rhs(p.consts(), s)).max() / O.HBAR

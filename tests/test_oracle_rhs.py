@@ -107,7 +107,7 @@ def test_dissipator_relaxation_closed_form(kind):
     omg[:] = 0
     e, v = np.linalg.eigh(p.H0[0])
     sig_eq = ((v * (e < 0)) @ v.conj().T)[None]          # a function of H: commutes with it
-    d = builder.Dissipator(kind, eta0=0.7, t0=0.4, width=0.15, sig_eq=sig_eq)
+    d = O.relaxation_dissipator(O.dissipator_rate(kind, 0.7, 0.4, 0.15), sig_eq)
     T = 0.9
     out = O.propagate(p.consts(), (sig0, psi, omg), 0.0, T, 0.01, 1e-11, p.H0,
                       O.zero_dH, lambda t, a: 0.0, 2, time_update="correct", dissipator=d)
