@@ -143,6 +143,22 @@ int zh_contour_density(int NK, int n, int nE, int nc, const double* contour,
                        const double* evals, const double* evecs, const int* corb,
                        const double* sigma_cc, double* rho, int device);
 
+/* zh_contour_density without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving or SCF loops that must not call hipMalloc per iteration).  Same
+ * arguments, results and errors as zh_contour_density (to rounding: the contour
+ * points are summed in chunks that fit the scratch); its device buffers are slices of
+ * a file-scope device array of ZH_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends, and it runs on the device's null stream.  It needs
+ * 16 (4 n^2 + n nc + 3 P n nc + 4 P nc^2) bytes plus a few kB for P >= 1 contour
+ * points per chunk (n = 1000, nc = 80: P = 16); larger n returns ZH_EINVAL before
+ * any device work.  Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (from several threads) race and must be serialised by the caller. */
+#define ZH_NOALLOC_SCRATCH_BYTES (128u << 20)
+int zh_contour_density_noalloc(int NK, int n, int nE, int nc, const double* contour,
+                               const double* evals, const double* evecs, const int* corb,
+                               const double* sigma_cc, double* rho, int device);
+
 /* Bias-window (nonequilibrium) density of Zandpack/cmdtools/SCF:236-253, 299-306
  * (G_l integrated over [mu_min - fact_kT kT_max, mu_max + fact_kT kT_max]):
  * rho = sum_i w_i (1/2pi) G(E_i)[:, c] M_i G(E_i)[:, c]^H on real nodes E_i, with

@@ -84,6 +84,8 @@ SIGNATURES = {
     "zh_set_pulse_spline": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "zh_contour_density": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                      C.POINTER(C.c_int), _dp, _dp, C.c_int]),
+    "zh_contour_density_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                                             C.POINTER(C.c_int), _dp, _dp, C.c_int]),
     "zh_window_density": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                     C.POINTER(C.c_int), _dp, _dp, _dp, C.c_int]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

@@ -1931,7 +1931,7 @@ struct zh_problem {
 };
 
 // ---------------------------------------------------------- equilibrium density
-// Device buffers and stream of one contour_density call, released on every exit.
+// Device buffers and stream of one window_density call, released on every exit.
 struct ScfScratch {
   hipStream_t st = nullptr;
   std::vector<void*> bufs;
@@ -1949,92 +1949,226 @@ struct ScfScratch {
   }
 };
 
-static void contour_density(int NK, int n, int nE, int nc, const double* EW, const double* evals,
-                            const double* evecs, const int* corb, const double* sigma_cc,
-                            double* rho, int device) {
+// Bytes a buffer of `count` elements takes in the scratch array (256-byte aligned).
+static inline size_t arena_bytes(size_t count, size_t elem) {
+  return ((std::max<size_t>(count, 1) * elem + 255) / 256) * 256;
+}
+// Counts what a buffer set needs, allocates nothing.
+struct CountArena {
+  size_t used = 0;
+  template <typename T> T* alloc(size_t count) {
+    used += arena_bytes(count, sizeof(T));
+    return nullptr;
+  }
+};
+// Slices of a caller-provided device region (the scratch array of the allocation-free
+// entry point); the region's size is checked before any buffer set is laid out.
+struct SliceArena {
+  char* base;
+  size_t cap, used = 0;
+  template <typename T> T* alloc(size_t count) {
+    const size_t b = arena_bytes(count, sizeof(T));
+    require(used + b <= cap, "scratch layout exceeds the scratch array", ZH_EINVAL);
+    T* p = reinterpret_cast<T*>(base + used);
+    used += b;
+    return p;
+  }
+};
+
+// Device buffers of one contour_density evaluation with P contour points per chunk.
+// X of V Y V† shares Y's buffer and ρ shares Z's (each is dead when the other is
+// written).
+struct DensityBufs {
+  double2 *Ed, *Wd;
+  double* ed;
+  int* cd;
+  double2 *V, *Vh, *Vch, *w, *Y, *Z;
+  double2 *Tp, *T, *U, *Sg, *G0, *Mm, *Kk;
+};
+template <typename Arena>
+static DensityBufs density_bufs(Arena& a, int n, int nE, int nc, int P) {
+  const size_t nn = (size_t)n * n, cc = (size_t)nc * nc, nL = (size_t)n * P * nc;
+  DensityBufs b;
+  b.Ed = a.template alloc<double2>(nE);
+  b.Wd = a.template alloc<double2>(nE);
+  b.ed = a.template alloc<double>(n);
+  b.cd = a.template alloc<int>(nc);
+  b.V = a.template alloc<double2>(nn);
+  b.Vh = a.template alloc<double2>(nn);
+  b.Vch = a.template alloc<double2>((size_t)n * nc);
+  b.w = a.template alloc<double2>(n);
+  b.Y = a.template alloc<double2>(nn);
+  b.Z = a.template alloc<double2>(nn);
+  b.Tp = a.template alloc<double2>(nL);
+  b.T = a.template alloc<double2>(nL);
+  b.U = a.template alloc<double2>(nL);
+  b.Sg = a.template alloc<double2>((size_t)P * cc);
+  b.G0 = a.template alloc<double2>((size_t)P * cc);
+  b.Mm = a.template alloc<double2>((size_t)P * cc);
+  b.Kk = a.template alloc<double2>((size_t)P * cc);
+  return b;
+}
+
+static void density_args(int NK, int n, int nE, int nc, const int* corb, const double* sigma_cc) {
   require(NK > 0 && n > 0 && nE > 0, "NK, n and the number of contour points must be positive");
   require(nc >= 0 && nc <= n && nc <= kGJMax, "coupled orbitals: 0 <= nc <= min(n, 1024)");
   require(nc == 0 || (corb && sigma_cc), "the self-energy needs its orbitals and values");
   for (int c = 0; c < nc; ++c) require(corb[c] >= 0 && corb[c] < n, "orbital index out of range");
-  HIPCHK(hipSetDevice(device));
-  ScfScratch sc;
-  HIPCHK(hipStreamCreateWithFlags(&sc.st, hipStreamNonBlocking));
-  hipStream_t st = sc.st;
-  const long nn = (long)n * n, L = (long)nE * nc, cc = (long)nc * nc;
+}
+
+// The contour sum on stream st over the buffers b, P contour points per chunk: the
+// chunks' (n x P n_c x n) products accumulate into Y = Σ_i W_i T'_i K_i T_i +
+// diag(Σ_i W_i D_i).  With P = nE it is one product.
+static void contour_density_run(const DensityBufs& b, hipStream_t st, int P, int NK, int n,
+                                int nE, int nc, const double* EW, const double* evals,
+                                const double* evecs, const int* corb, const double* sigma_cc,
+                                double* rho) {
+  const long nn = (long)n * n, cc = (long)nc * nc;
   std::vector<double2> Eh(nE), Wh(nE);
   for (int i = 0; i < nE; ++i) {
     Eh[i] = make_double2(EW[4 * i], EW[4 * i + 1]);
     Wh[i] = make_double2(EW[4 * i + 2], EW[4 * i + 3]);
   }
-  double2* Ed = sc.alloc<double2>(nE);
-  double2* Wd = sc.alloc<double2>(nE);
-  double* ed = sc.alloc<double>(n);
-  int* cd = sc.alloc<int>(nc);
-  double2 *V = sc.alloc<double2>(nn), *Vh = sc.alloc<double2>(nn), *Vch = sc.alloc<double2>((size_t)n * nc);
-  double2 *w = sc.alloc<double2>(n), *Y = sc.alloc<double2>(nn), *Z = sc.alloc<double2>(nn);
-  double2 *X = sc.alloc<double2>(nn), *R = sc.alloc<double2>(nn);
-  double2 *Tp = sc.alloc<double2>((size_t)n * L), *T = sc.alloc<double2>((size_t)n * L);
-  double2 *U = sc.alloc<double2>((size_t)n * L);
-  double2 *Sg = sc.alloc<double2>((size_t)nE * cc), *G0 = sc.alloc<double2>((size_t)nE * cc);
-  double2 *Mm = sc.alloc<double2>((size_t)nE * cc), *Kk = sc.alloc<double2>((size_t)nE * cc);
-  HIPCHK(hipMemcpyAsync(Ed, Eh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(Wd, Wh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
-  if (nc > 0) HIPCHK(hipMemcpyAsync(cd, corb, sizeof(int) * nc, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(b.Ed, Eh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(b.Wd, Wh.data(), sizeof(double2) * nE, hipMemcpyHostToDevice, st));
+  if (nc > 0) HIPCHK(hipMemcpyAsync(b.cd, corb, sizeof(int) * nc, hipMemcpyHostToDevice, st));
   // FP64 products, three real MFMAs per complex tile product
   typedef Tile<64, 64, 16, 2, false, 3> TBig;
   typedef Tile<32, 32, 16, 2, false, 4> TSmall;
   const dim3 g1(1024), b1(256);
   for (int k = 0; k < NK; ++k) {
-    HIPCHK(hipMemcpyAsync(V, evecs + 2 * (size_t)k * nn, sizeof(double2) * nn,
+    HIPCHK(hipMemcpyAsync(b.V, evecs + 2 * (size_t)k * nn, sizeof(double2) * nn,
                           hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ed, evals + (size_t)k * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_scf_adjoint, g1, b1, 0, st, n, nc, (const double2*)V, (const int*)cd, Vh,
-                       Vch);
-    hipLaunchKernelGGL(k_scf_diag, dim3((n + 255) / 256), b1, 0, st, n, nE, (const double2*)Ed,
-                       (const double2*)Wd, (const double*)ed, w);
+    HIPCHK(hipMemcpyAsync(b.ed, evals + (size_t)k * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scf_adjoint, g1, b1, 0, st, n, nc, (const double2*)b.V,
+                       (const int*)b.cd, b.Vh, b.Vch);
+    hipLaunchKernelGGL(k_scf_diag, dim3((n + 255) / 256), b1, 0, st, n, nE, (const double2*)b.Ed,
+                       (const double2*)b.Wd, (const double*)b.ed, b.w);
     HIPCHK(hipGetLastError());
-    if (nc > 0) {
-      HIPCHK(hipMemcpyAsync(Sg, sigma_cc + 2 * (size_t)k * nE * cc, sizeof(double2) * nE * cc,
-                            hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_scf_factors, g1, b1, 0, st, n, nE, nc, (const double2*)Ed,
-                         (const double*)ed, (const double2*)V, (const int*)cd, Tp, T);
-      HIPCHK(hipGetLastError());
-      // G0_i,cc = T_i V_c†   (nc x n)(n x nc), one batch entry per contour point
-      GemmArgs<double, double> ga{nc, nc, n, T, n, (long)nc * n, Vch, nc, 0};
-      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
-          ga, nE, EpiStore<double2>{G0, nc, cc, 1.0, 0.0}, st)));
-      // I - Σ_i G0_i,cc
-      GemmArgs<double, double> gb{nc, nc, nc, Sg, nc, cc, G0, nc, cc};
-      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
-          gb, nE, EpiIdMinus{Mm, nc, cc}, st)));
-      // K_i = (I - Σ_i G0_i,cc)⁻¹ Σ_i
-      HIPCHK(hipMemcpyAsync(Kk, Sg, sizeof(double2) * nE * cc, hipMemcpyDeviceToDevice, st));
-      hipLaunchKernelGGL(k_gauss_jordan, dim3(nE), dim3(256), 0, st, nc, Mm, Kk);
-      HIPCHK(hipGetLastError());
-      // U_i = W_i T'_i K_i  (n x nc)(nc x nc), into the column block i of U (n, nE nc)
-      GemmArgs<double, double> gc{n, nc, nc, Tp, L, (long)nc, Kk, nc, cc};
-      HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
-          gc, nE, EpiScaleZ{U, L, (long)nc, Wd}, st)));
+    for (int i0 = 0; i0 < (nc > 0 ? nE : 1); i0 += P) {
+      const int Pc = nc > 0 ? std::min(P, nE - i0) : 0;
+      const long L = (long)Pc * nc;
+      if (nc > 0) {
+        HIPCHK(hipMemcpyAsync(b.Sg, sigma_cc + 2 * ((size_t)k * nE + i0) * cc,
+                              sizeof(double2) * Pc * cc, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scf_factors, g1, b1, 0, st, n, Pc, nc, (const double2*)(b.Ed + i0),
+                           (const double*)b.ed, (const double2*)b.V, (const int*)b.cd, b.Tp, b.T);
+        HIPCHK(hipGetLastError());
+        // G0_i,cc = T_i V_c†   (nc x n)(n x nc), one batch entry per contour point
+        GemmArgs<double, double> ga{nc, nc, n, b.T, n, (long)nc * n, b.Vch, nc, 0};
+        HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+            ga, Pc, EpiStore<double2>{b.G0, nc, cc, 1.0, 0.0}, st)));
+        // I - Σ_i G0_i,cc
+        GemmArgs<double, double> gb{nc, nc, nc, b.Sg, nc, cc, b.G0, nc, cc};
+        HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+            gb, Pc, EpiIdMinus{b.Mm, nc, cc}, st)));
+        // K_i = (I - Σ_i G0_i,cc)⁻¹ Σ_i
+        HIPCHK(hipMemcpyAsync(b.Kk, b.Sg, sizeof(double2) * Pc * cc, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_gauss_jordan, dim3(Pc), dim3(256), 0, st, nc, b.Mm, b.Kk);
+        HIPCHK(hipGetLastError());
+        // U_i = W_i T'_i K_i  (n x nc)(nc x nc), into the column block i of U (n, Pc nc)
+        GemmArgs<double, double> gc{n, nc, nc, b.Tp, L, (long)nc, b.Kk, nc, cc};
+        HIPCHK((launch_zgemm<double, double, double, false, false, TSmall, 3>(
+            gc, Pc, EpiScaleZ{b.U, L, (long)nc, b.Wd + i0}, st)));
+      }
+      // Y = U T + diag(Σ_i W_i D_i) for the first chunk, Y += U T for the others
+      // (n x Pc nc)(Pc nc x n)
+      GemmArgs<double, double> gy{n, n, (int)L, b.U, L, 0, b.T, n, 0};
+      if (i0 == 0)
+        HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
+            gy, 1, EpiAddDiag{b.Y, n, b.w}, st)));
+      else
+        HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
+            gy, 1, EpiAddTo{b.Y, n}, st)));
     }
-    // Y = U T + diag(Σ_i W_i D_i)  (n x nE nc)(nE nc x n)
-    GemmArgs<double, double> gy{n, n, (int)L, U, L, 0, T, n, 0};
-    HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(gy, 1, EpiAddDiag{Y, n, w},
-                                                                     st)));
-    // X = V Y V†
-    GemmArgs<double, double> gz{n, n, n, V, n, 0, Y, n, 0};
+    // X = V Y V† (X into Y's buffer), ρ = (i/2π)(X - X†) (into Z's)
+    GemmArgs<double, double> gz{n, n, n, b.V, n, 0, b.Y, n, 0};
     HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
-        gz, 1, EpiStore<double2>{Z, n, nn, 1.0, 0.0}, st)));
-    GemmArgs<double, double> gx{n, n, n, Z, n, 0, Vh, n, 0};
+        gz, 1, EpiStore<double2>{b.Z, n, nn, 1.0, 0.0}, st)));
+    GemmArgs<double, double> gx{n, n, n, b.Z, n, 0, b.Vh, n, 0};
     HIPCHK((launch_zgemm<double, double, double, false, false, TBig, 3>(
-        gx, 1, EpiStore<double2>{X, n, nn, 1.0, 0.0}, st)));
-    hipLaunchKernelGGL(k_scf_rho, g1, b1, 0, st, n, (const double2*)X, R);
+        gx, 1, EpiStore<double2>{b.Y, n, nn, 1.0, 0.0}, st)));
+    hipLaunchKernelGGL(k_scf_rho, g1, b1, 0, st, n, (const double2*)b.Y, b.Z);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(rho + 2 * (size_t)k * nn, R, sizeof(double2) * nn, hipMemcpyDeviceToHost,
-                          st));
+    HIPCHK(hipMemcpyAsync(rho + 2 * (size_t)k * nn, b.Z, sizeof(double2) * nn,
+                          hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   for (size_t i = 0; i < 2 * (size_t)NK * nn; ++i)
     require(std::isfinite(rho[i]), "non-finite density (singular Woodbury matrix?)", ZH_ENONFINITE);
+}
+
+static void contour_density(int NK, int n, int nE, int nc, const double* EW, const double* evals,
+                            const double* evecs, const int* corb, const double* sigma_cc,
+                            double* rho, int device) {
+  density_args(NK, n, nE, nc, corb, sigma_cc);
+  HIPCHK(hipSetDevice(device));
+  ScfScratch sc;
+  HIPCHK(hipStreamCreateWithFlags(&sc.st, hipStreamNonBlocking));
+  const DensityBufs b = density_bufs(sc, n, nE, nc, nE);  // one chunk: every point at once
+  contour_density_run(b, sc.st, nE, NK, n, nE, nc, EW, evals, evecs, corb, sigma_cc, rho);
+}
+
+// ------------------------------------------- allocation-free contour density
+// Scratch of zh_contour_density_noalloc: one copy per GPU, held until the process
+// ends.  In host code `scratch` names a host-side stand-in; the device address of
+// this GPU's copy comes from hipGetSymbolAddress (scratch_ptr), once per device.
+static __device__ __align__(256) unsigned char scratch[ZH_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+static char* scratch_ptr(int device) {
+  require(device >= 0 && device < kMaxDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    require(ptr != nullptr, "no device address for the scratch array", ZH_EHIP);
+    g_scratch_ptr[device] = ptr;
+  }
+  return static_cast<char*>(g_scratch_ptr[device]);
+}
+
+// Contour points per chunk that fit in `cap` bytes (0: not even one).
+static int density_chunk(int n, int nE, int nc, size_t cap) {
+  auto need = [&](int P) {
+    CountArena a;
+    (void)density_bufs(a, n, nE, nc, P);
+    return a.used;
+  };
+  if (need(1) > cap) return 0;
+  if (nc == 0 || need(nE) <= cap) return nE;
+  int lo = 1, hi = nE;  // need(lo) fits, need(hi) does not
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    (need(mid) <= cap ? lo : hi) = mid;
+  }
+  return lo;
+}
+
+static void contour_density_noalloc(int NK, int n, int nE, int nc, const double* EW,
+                                    const double* evals, const double* evecs, const int* corb,
+                                    const double* sigma_cc, double* rho, int device) {
+  density_args(NK, n, nE, nc, corb, sigma_cc);
+  const size_t cap = ZH_NOALLOC_SCRATCH_BYTES;
+  const int P = density_chunk(n, nE, nc, cap);
+  if (P == 0) {
+    CountArena a;
+    (void)density_bufs(a, n, nE, nc, 1);
+    throw Error{ZH_EINVAL, "zh_contour_density_noalloc: n = " + std::to_string(n) + ", nc = " +
+                               std::to_string(nc) + " needs " + std::to_string(a.used) +
+                               " bytes of scratch, more than the " + std::to_string(cap) +
+                               " it has; use zh_contour_density"};
+  }
+  char* base = scratch_ptr(device);  // also makes `device` current
+  SliceArena arena{base, cap};
+  const DensityBufs b = density_bufs(arena, n, nE, nc, P);
+  // the null stream: no stream is created (every buffer is written before it is read,
+  // so nothing needs clearing between calls)
+  contour_density_run(b, 0, P, NK, n, nE, nc, EW, evals, evecs, corb, sigma_cc, rho);
 }
 
 // Bias-window (nonequilibrium) density of Zandpack/cmdtools/SCF:236-253, 299-306,
@@ -2274,6 +2408,15 @@ int zh_contour_density(int NK, int n, int nE, int nc, const double* contour,
   return zh_try([&] {
     require(contour && evals && evecs && rho, "null argument");
     contour_density(NK, n, nE, nc, contour, evals, evecs, corb, sigma_cc, rho, device);
+  });
+}
+
+int zh_contour_density_noalloc(int NK, int n, int nE, int nc, const double* contour,
+                               const double* evals, const double* evecs, const int* corb,
+                               const double* sigma_cc, double* rho, int device) {
+  return zh_try([&] {
+    require(contour && evals && evecs && rho, "null argument");
+    contour_density_noalloc(NK, n, nE, nc, contour, evals, evecs, corb, sigma_cc, rho, device);
   });
 }
 

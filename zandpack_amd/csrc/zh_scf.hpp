@@ -91,6 +91,15 @@ struct EpiAddDiag {
   }
 };
 
+// out += acc (the later contour chunks of Y)
+struct EpiAddTo {
+  double2* out; int n;
+  __device__ inline void operator()(int, int row, int col, double re, double im) const {
+    double2& o = out[(long)row * n + col];
+    o = make_double2(o.x + re, o.y + im);
+  }
+};
+
 // Solves M X = R for a batch of nc x nc systems (one workgroup each) by Gauss-Jordan
 // elimination with partial pivoting in global memory: R <- X, M destroyed.
 constexpr int kGJMax = 1024;

@@ -69,9 +69,11 @@ def eigenpairs(H):
     return np.ascontiguousarray(e, np.float64), np.ascontiguousarray(V, np.complex128)
 
 
-def contour_density(H, contour, corb, sigma_cc, device=0, eig=None):
+def contour_density(H, contour, corb, sigma_cc, device=0, eig=None, noalloc=False):
     """One device evaluation of Σ_i (i/2π)(W_i G_i - h.c.) for H (NK, n, n);
-    ``eig`` = eigenpairs(H) when the caller has them already."""
+    ``eig`` = eigenpairs(H) when the caller has them already.  ``noalloc`` runs it
+    without device allocation (``zh_contour_density_noalloc``: the library's fixed
+    scratch array, sizes up to about n = 1000)."""
     lib = N.lib()
     H = np.asarray(H, complex)
     NK, n, _ = H.shape
@@ -85,7 +87,8 @@ def contour_density(H, contour, corb, sigma_cc, device=0, eig=None):
         sig = N.c128(sigma_cc)
         assert sig.shape == (NK, cont.shape[0], nc, nc)
     rho = np.empty((NK, n, n), complex)
-    N.check(lib.zh_contour_density(
+    fn = lib.zh_contour_density_noalloc if noalloc else lib.zh_contour_density
+    N.check(fn(
         NK, n, cont.shape[0], nc, N.ptr(cont), N.ptr(e), N.ptr(V),
         corb.ctypes.data_as(C.POINTER(C.c_int)) if nc else None,
         N.ptr(sig) if nc else None, N.ptr(rho), int(device)))
