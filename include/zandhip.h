@@ -295,6 +295,14 @@ int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int
                  long max_attempts, long cap, double* times, double* J, double* h_used,
                  long* n_steps, long* n_rejected, double* h_final);
 
+/* Fused Omega stage kernel of the mixed-precision attempts: 0 auto (default: used when
+ * zh_autotune timed it faster than the unfused pair), 1 off, 2 on.  Fused, each stage's
+ * Omega kernel also forms the Omega xi_c product of the Psi equation
+ * (Zandpack/mpi/zand:771-778) from the k_s it has just stored (linearity over the
+ * stages) instead of a separate product streaming k_s back; results agree with the
+ * unfused path to rounding.  FP64 problems always run unfused. */
+int zh_set_fusion(zh_problem* p, int mode);
+
 /* Use hipGraphs for zh_attempt / zh_propagate (1, default) or direct launches (0). */
 int zh_set_graph(zh_problem* p, int enable);
 

@@ -573,3 +573,51 @@ def test_device_pulses_propagate(native, kind):
     assert rel(out["J"], ref["J"]) < 1e-10
     assert np.abs(ref["J"]).max() > 1e-6
     prop.close()
+
+
+@pytest.mark.parametrize("case", ["small", "tile_aligned", "small_spin_hubbard", "wide_edges",
+                                  "shared_orbitals", "chain20"])
+@pytest.mark.parametrize("layout", ["reference", "pruned"])
+@pytest.mark.parametrize("zero_ff", [False, True])
+def test_fused_omega_matches_unfused(native, case, layout, zero_ff):
+    """Mixed precision: the attempt with the Ωξ product fused into the Ω stage kernel
+    against the oracle and against the unfused attempt, over three attempts (the y0 ξ_c
+    carried from one accepted attempt to the next), for every chunk width."""
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    y0 = _rand_state(p, 41, scale=0.02)
+    if zero_ff:
+        passive = (p.dGP == 0) & (p.dGM == 0)
+        pm = passive.all(axis=0).reshape(-1)
+        om = y0[2].reshape(p.dims[0], p.M, p.M)
+        om[:, pm[:, None] & pm[None, :]] = 0
+    say("fused", case, layout, zero_ff)
+    ref_state = y0
+    seq = [(0.1, 0.01), (0.11, 0.012), (0.122, 0.008)]
+    refs = []
+    for t, h in seq:
+        ks, TE, J0, y_new = O.dop54_attempt(p.consts(), ref_state, t, h, p.H0, dH_of(p), p.bias, 2)
+        refs.append((TE, J0, y_new))
+        ref_state = y_new
+    for c in range(3):
+        a = Propagator(p, "mixed", state=y0, layout=layout)
+        b = Propagator(p, "mixed", state=y0, layout=layout)
+        a.set_fusion("on")
+        b.set_fusion("off")
+        tiles = {k: 0 for k in ("psi", "hsig", "omega", "omgxi", "pi")}
+        tiles["omega"] = c
+        a.set_tiles(tiles)
+        for (t, h), (TE, J0, y_new) in zip(seq, refs):
+            ra = a.attempt(t, h, 1.0, force_accept=True)
+            rb = b.attempt(t, h, 1.0, force_accept=True)
+            say("  te fused/unfused/oracle", c, ra["te"], rb["te"], TE)
+            # the error norm is a difference of nearly equal stage sums: FP32 rounding of
+            # the stages reaches it at up to ~1e-3 on small steps, on either path alike
+            assert abs(ra["te"] - TE) <= 2e-3 * TE, (c, ra["te"], TE)
+            assert abs(ra["te"] - rb["te"]) <= 2e-3 * TE
+            assert rel(ra["J"], J0) < 1e-5
+            sa, pa, oa = a.get_state()
+            for got, w in zip((sa, pa, oa), y_new):
+                assert rel(got, w) < 1e-5, c
+        a.close()
+        b.close()

@@ -89,6 +89,7 @@ SIGNATURES = {
     "zh_window_density": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                     C.POINTER(C.c_int), _dp, _dp, _dp, C.c_int]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "zh_set_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_set_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, _dp, _lp]),

@@ -32,6 +32,7 @@
 #include "zh_gemm.hpp"
 #include "zh_kernels.hpp"
 #include "zh_scf.hpp"
+#include "zh_omega.hpp"
 
 namespace zh {
 
@@ -113,6 +114,7 @@ struct EngineBase {
   virtual int layout_info(int* out) = 0;
   virtual void set_sparse(int mode) = 0;
   virtual void hamiltonian_info(int* out) = 0;
+  virtual void set_fusion(int mode) = 0;
   bool use_graph = true;
   bool profile = false;
   double kc_ms[KC_N] = {0};
@@ -161,7 +163,7 @@ struct TuneKey {
   }
 };
 static std::mutex g_tune_mu;
-static std::map<TuneKey, std::array<int, TS_N>> g_tune_cache;
+static std::map<TuneKey, std::array<int, TS_N + 1>> g_tune_cache;  // + fused Ω kernel
 static int pow2_bucket(int v) {
   int b = 1;
   while (b < v) b <<= 1;
@@ -247,6 +249,7 @@ struct Engine : EngineBase {
   int *cidx = nullptr, *cpos = nullptr, *lead = nullptr;
   int2* t1 = nullptr;      // per Ω column tile: term1 K range
   int2* t2 = nullptr;      // per Ω row tile: term2 K range (relative to nc)
+  int2* txi = nullptr;     // per Ω column tile: ξ_c column range of its modes' leads
   int* gmask = nullptr;    // per 16-column group of the coupled order: lead mask
   int4* lrange = nullptr;  // per lead: device rows in the active and passive blocks
   double2 *Hcur = nullptr, *Ixic = nullptr, *Pic = nullptr, *G = nullptr, *dvec = nullptr;
@@ -262,8 +265,37 @@ struct Engine : EngineBase {
   cs *Bp = nullptr, *Ap = nullptr, *Aomg = nullptr, *Bomg = nullptr, *xic = nullptr, *W = nullptr;
   cs* Pred[7] = {nullptr};  // y0 ξ_c and k_j ξ_c of the current attempt (Ω_s ξ_c by linearity)
   cs* Wst = nullptr;        // Ω_s ξ_c of the current stage
+  // Fused Ω stage kernel (mixed precision): each stage's Ω kernel also writes k_s ξ_c
+  // (the last stage y_new ξ_c) as per-chunk partials, double-buffered by stage parity;
+  // PredY[par] holds y0 ξ_c (par: which of the two y0 / y_new slots is y0), PredY[1-par]
+  // receives y_new ξ_c, so an accepted attempt leaves the next one's y0 ξ_c in place.
+  static constexpr int kFusedNG = 8;            // ξ_c column groups of 16 the kernel carries
+  static constexpr int kFusedChunk[3] = {512, 256, 768};
+  int fusion_req = 0;       // 0 auto (the autotuner's choice; unfused until tuned), 1 off, 2 on
+  bool fused_tuned = false; // the autotuner found the fused kernel faster
+  cs* Wpart[2] = {nullptr, nullptr};
+  cs* PredY[2] = {nullptr, nullptr};
+  int wpart_max = 0;        // chunks the partial buffers hold
+  bool y0xi_valid = false;  // PredY[par] is y0 ξ_c of the resident state
+  bool fused_used = false;  // the attempt in flight ran the fused kernel
+  bool fusable() const {
+    return std::is_same<R, float>::value && nc > 0 && nc <= 16 * kFusedNG && !overlap &&
+           (fusion_req == 2 || (fusion_req == 0 && fused_tuned));
+  }
+  // Chunk layout of the fused kernel for chunk width C.
+  OmegaFusedArgs fused_args(int C) const {
+    OmegaFusedArgs a{};
+    a.Md = Md; a.K = 2 * nc; a.nc = nc; a.C = C;
+    a.ML = ML;
+    a.SA = (ML + C - 1) / C;
+    a.SP = (Md - ML + C - 1) / C;
+    a.nbands = (Md + kFusedBM - 1) / kFusedBM;
+    a.nfull = ff ? ML / kFusedBM : a.nbands;
+    return a;
+  }
   // error-norm partials written by the last stage's Ω kernel: one per workgroup
   int te_omega_blocks() const {
+    if (fused_used) return fused_items(fused_args(kFusedChunk[tsel[TS_OMEGA]])) * NK;
     int bm = 0, bn = 0;
     tile_dims(tsel[TS_OMEGA], typename Cands<R>::omega(), bm, bn);
     return ((Md + bm - 1) / bm) * ((Md + bn - 1) / bn) * NK;
@@ -310,6 +342,7 @@ struct Engine : EngineBase {
     for (auto& a : aux) HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     for (auto& e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+
     HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl), hipHostMallocDefault));
     std::memset(ctrl_h, 0, sizeof(Ctrl));
     ctrl_h->norm_scale = 1.0;
@@ -332,6 +365,7 @@ struct Engine : EngineBase {
       if (e) (void)hipEventDestroy(e);
     for (auto& a : aux)
       if (a) (void)hipStreamDestroy(a);
+
     for (void* p : allocs) (void)hipFree(p);
     if (ctrl_h) (void)hipHostFree(ctrl_h);
     if (J_h) (void)hipHostFree(J_h);
@@ -425,6 +459,7 @@ struct Engine : EngineBase {
     cpos = dalloc<int>(n);
     t1 = dalloc<int2>((Md + kTile - 1) / kTile);
     t2 = dalloc<int2>((Md + kTile - 1) / kTile);
+    txi = dalloc<int2>((Md + kTile - 1) / kTile);
     gmask = dalloc<int>((nc + kCGroup - 1) / kCGroup + 1);
     lrange = dalloc<int4>(NL);
     Ixic = dalloc<double2>(mc);
@@ -433,6 +468,13 @@ struct Engine : EngineBase {
     W = dalloc<cs>(mc * kWParts);
     for (auto& p : Pred) p = dalloc<cs>(mc);
     Wst = dalloc<cs>(mc);
+    if (std::is_same<R, float>::value && nc > 0 && nc <= 16 * kFusedNG) {
+      // the narrowest chunk gives the most partials
+      const int C = kFusedChunk[1];
+      wpart_max = (Md + C - 1) / C + 1;
+      for (auto& w : Wpart) w = dalloc<cs>(mc * wpart_max);
+      for (auto& q : PredY) q = dalloc<cs>(mc);
+    }
     // room for the error-norm partials of the finest Ω tile candidate (32 x 32)
     const long t32 = (Md + 31) / 32;
     te_part = dalloc<double>((size_t)2 * kTEBlocks + t32 * t32 * NK);
@@ -502,6 +544,18 @@ struct Engine : EngineBase {
       tr[t] = make_int2(lo < hi ? lo : 0, lo < hi ? hi : 0);
     }
     // per row tile: term2 K range (ΔP ξ_c vanishes outside the row leads' orbitals)
+    // per column tile: ξ_c column range of the leads of its modes (the Ωξ product's
+    // contraction over the tile's columns touches only those ξ_c columns)
+    std::vector<int2> trx(ntile);
+    for (int t = 0; t < ntile; ++t) {
+      int lo = nc, hi = 0;
+      for (int d = t * kTile; d < std::min(Md, (t + 1) * kTile); ++d) {
+        if (perm[d] < 0) continue;
+        lo = std::min(lo, llo[ld[d]]);
+        hi = std::max(hi, lhi[ld[d]]);
+      }
+      trx[t] = make_int2(lo < hi ? lo : 0, lo < hi ? hi : 0);
+    }
     std::vector<int2> tr2(ntile);
     for (int t = 0; t < ntile; ++t) {
       int lo = nc, hi = 0;
@@ -584,6 +638,8 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpyAsync(cpos, cp.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(t1, tr.data(), sizeof(int2) * ntile, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(t2, tr2.data(), sizeof(int2) * ntile, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(txi, trx.data(), sizeof(int2) * ntile, hipMemcpyHostToDevice, stream));
+    y0xi_valid = false;
     HIPCHK(hipMemcpyAsync(gmask, gm.data(), sizeof(int) * gm.size(), hipMemcpyHostToDevice,
                           stream));
     HIPCHK(hipMemcpyAsync(lrange, lr.data(), sizeof(int4) * NL, hipMemcpyHostToDevice, stream));
@@ -978,6 +1034,7 @@ struct Engine : EngineBase {
   // zero the passive x passive block of every Ω slot
   void clear_ff() {
     if (ML >= Md) return;
+    y0xi_valid = false;
     for (int s = 0; s < 10; ++s)
       for (int b = 0; b < NK && omg[s]; ++b)
         HIPCHK(hipMemset2DAsync(omg[s] + ((size_t)b * Md + ML) * Md + ML, sizeof(cs) * Md, 0,
@@ -1011,6 +1068,7 @@ struct Engine : EngineBase {
     if (!ffz) ff_dirty = true;
     set_ff(ffz);
     have_state = true;
+    y0xi_valid = false;
   }
   void state_get(double* s, double* p, double* o) override {
     require(have_state, "no resident state", ZH_ESTATE);
@@ -1065,6 +1123,11 @@ struct Engine : EngineBase {
         throw;
       }
     }
+  }
+  void set_fusion(int mode) override {
+    require(mode >= 0 && mode <= 2, "fusion: 0 auto, 1 off, 2 on");
+    fusion_req = mode;
+    drop_graphs();
   }
   void hamiltonian_info(int* out) override {
     out[0] = h_real ? 1 : 0; out[1] = h_sparse ? 1 : 0; out[2] = ell_R;
@@ -1127,6 +1190,10 @@ struct Engine : EngineBase {
     cs* ynew = nullptr;
     bool store_k = true;
     bool te = false;
+    // fused Ωξ (attempts in mixed precision): the partials buffer this stage's Ω kernel
+    // writes, and the slot parity of y0 (PredY[par] = y0 ξ_c)
+    cs* wpart = nullptr;
+    int par = 0;
   };
 
   // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
@@ -1227,17 +1294,34 @@ struct Engine : EngineBase {
   void stage_psi(int s, const OmegaStage& os, int kslot, hipStream_t st) {
     const StageParams* sps = sp + s;
     const double ih = 1.0 / hbar;
-    if (nc > 0) {
+    cs* Wc = Wst;  // Ω_s ξ_c of this stage
+    if (nc > 0 && os.wpart) {
+      // fused: y0 ξ_c is resident and the earlier stages' Ω kernels left k_j ξ_c as
+      // per-chunk partials; stage 0 needs y0 ξ_c alone
+      if (s == 0) {
+        Wc = PredY[os.par];
+      } else {
+        timed(KC_GEMM_OMGXI, [&] {
+          const OmegaFusedArgs fa = fused_args(kFusedChunk[tsel[TS_OMEGA]]);
+          WSumArgs<cs> w{};
+          w.part = Wpart[(s - 1) % 2];
+          w.pred[0] = PredY[os.par];
+          for (int j = 1; j < 7; ++j) w.pred[j] = Pred[j];
+          w.W = Wst;
+          for (int j = 0; j < 6; ++j) w.a[j] = (j < s) ? tab.a[s][j] : 0.0;
+          w.s = s; w.parts = fa.SA + fa.SP; w.NK = NK; w.M = Md; w.nc = nc;
+          w.parts_pass = ff ? fa.SA : fa.SA + fa.SP;
+          w.ML = ML;
+          hipLaunchKernelGGL(k_wsum<cs>, dim3(1024), dim3(256), 0, st, w);
+          HIPCHK(hipGetLastError());
+        });
+      }
+    } else if (nc > 0) {
       timed(KC_GEMM_OMGXI, [&] {
         // X ξ_c  (Md x Md)(Md x nc) for X = y0 (first stage) or the previous stage's k,
         // then Ω_s ξ_c = y0 ξ_c + Σ_j a_sj k_j ξ_c: Ω is streamed once per stage and no
         // stage-input copy of it is needed
-        const cs* X = (s == 0) ? os.y0 : omg[s - 1];
-        GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
-        EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
-        KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
-        HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_OMGXI], g, NK, e, st, ks,
-                                                        typename Cands<R>::omgxi())));
+        omgxi_gemm((s == 0) ? os.y0 : omg[s - 1], W, st);
         WSumArgs<cs> w{};
         w.part = W;
         for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
@@ -1255,7 +1339,7 @@ struct Engine : EngineBase {
         // Ψ epilogue adds it to ψ(S⁻¹H)ᵀ + ħ ΔPξ_c σ_cᵀ  (Zandpack/mpi/nozand:1003, 1208-1210)
         const long total = (long)NK * Md * nc;
         hipLaunchKernelGGL(k_overlap_y<cs>, dim3(1024), dim3(256), 0, st, total, nc,
-                           (const double2*)glp, (const cs*)xic, (const cs*)Wst, Yov, hbar);
+                           (const double2*)glp, (const cs*)xic, (const cs*)Wc, Yov, hbar);
         HIPCHK(hipGetLastError());
         GemmArgs<R, R> g{Md, n, nc, Yov, nc, (long)Md * nc, Bs, n, (long)nc * n};
         EpiStore<cs> e{psi[kslot], n, (long)Md * n, 1.0, 0.0};
@@ -1275,7 +1359,7 @@ struct Engine : EngineBase {
                                                           kseg_g0(),
                                                           typename Cands<R>::psi())));
         }
-        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
+        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
                      dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
         const size_t lds = (size_t)hpsi_rb * n * sizeof(cs);
         const long sAp = (long)Md * (n + nc);
@@ -1299,7 +1383,7 @@ struct Engine : EngineBase {
         }
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
-        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
+        EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
                      dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
         HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psiR())));
@@ -1308,7 +1392,7 @@ struct Engine : EngineBase {
         // H0 is S⁻¹H and the coupled-column terms come in through acc0
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
-        EpiPsi<cs> e{psi[kslot], ovl ? psi[kslot] : nullptr, Ap, n + nc, xic, Wst, glp, chip,
+        EpiPsi<cs> e{psi[kslot], ovl ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
                      dvec, lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md};
         HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
@@ -1317,7 +1401,24 @@ struct Engine : EngineBase {
   }
 
   // Ω branch: sources, stage input, outer term and (last stage) the error norm.
+  // X ξ_c (Md x Md)(Md x nc) into the split-K partials `part` (X = y0 or a k slot).
+  void omgxi_gemm(const cs* X, cs* part, hipStream_t st) {
+    GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
+    EpiStore<cs> e{part, nc, (long)Md * nc, 1.0, 0.0};
+    KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
+    HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_OMGXI], g, NK, e, st, ks,
+                                                    typename Cands<R>::omgxi())));
+  }
+
   void stage_omega(const OmegaStage& os, int kslot, hipStream_t st) {
+    if (os.wpart) {
+      stage_omega_fused(os, kslot, st);
+      return;
+    }
+    stage_omega_gemm(os, kslot, st);
+  }
+
+  void stage_omega_gemm(const OmegaStage& os, int kslot, hipStream_t st) {
     timed(KC_GEMM_OMEGA, [&] {
       // [ψ_c | ΔPξ_c] [ΔM Ixi_c | ψ_c*]ᵀ  (Md x 2nc)(2nc x Md), with the stage input
       // formed, the outer-subtraction term applied and (last stage) the error norm
@@ -1349,6 +1450,67 @@ struct Engine : EngineBase {
                                                     typename Cands<R>::omega())));
       }
     });
+  }
+
+  // Ω branch with the Ωξ product fused (mixed precision, attempts): the kernel of
+  // zh_omega.hpp on chunks of kFusedChunk[tsel[TS_OMEGA]] columns.
+  template <typename E> void launch_fused(const OmegaFusedArgs& fa, const E& e, hipStream_t st) {
+    KSegOmega ks{t1, t2, nc, ML, ff ? 1 : 0, (Md + kTile - 1) / kTile};
+    const dim3 grid(fused_items(fa), NK);
+    const int ng = (nc + 15) / 16;
+    if (ng <= 2) hipLaunchKernelGGL((k_omega_fused<2, E>), grid, dim3(kThreads), 0, st, fa, e, ks);
+    else if (ng <= 4) hipLaunchKernelGGL((k_omega_fused<4, E>), grid, dim3(kThreads), 0, st, fa, e, ks);
+    else if (ng <= 6) hipLaunchKernelGGL((k_omega_fused<6, E>), grid, dim3(kThreads), 0, st, fa, e, ks);
+    else hipLaunchKernelGGL((k_omega_fused<8, E>), grid, dim3(kThreads), 0, st, fa, e, ks);
+    HIPCHK(hipGetLastError());
+  }
+  void stage_omega_fused(const OmegaStage& os, int kslot, hipStream_t st) {
+    if constexpr (std::is_same<R, float>::value) {
+      timed(KC_GEMM_OMEGA, [&] {
+        OmegaFusedArgs fa = fused_args(kFusedChunk[tsel[TS_OMEGA]]);
+        require(fa.SA + fa.SP <= wpart_max, "fused Omega kernel: partial buffer too small");
+        const long t32 = (Md + 31) / 32;
+        require((long)fused_items(fa) * NK <= t32 * t32 * NK,
+                "fused Omega kernel: more error-norm partials than te_part holds");
+        fa.A = Aomg; fa.B = Bomg; fa.xic = xic; fa.txi = txi; fa.wpart = os.wpart;
+        auto fill = [&](auto& e) {
+          e.out = os.store_k ? omg[kslot] : nullptr;
+          e.y0 = os.y0;
+          e.nterms = os.nterms;
+          for (int j = 0; j < 6; ++j) { e.k[j] = os.k[j]; e.a[j] = os.a[j]; e.ct[j] = os.ct[j]; }
+          e.ynew = os.ynew;
+          e.u = uvec; e.v = vvec; e.M = Md;
+          e.ct_self = os.ct_self;
+          e.te_part = te_part + 2 * kTEBlocks;
+          e.rtol = tol_rtol; e.atol = tol_atol; e.scaled = tol_scaled ? 1 : 0;
+        };
+        if (os.te) {
+          EpiOmegaRK<cs, true> e{};
+          fill(e);
+          launch_fused(fa, e, st);
+        } else {
+          EpiOmegaRK<cs, false> e{};
+          fill(e);
+          launch_fused(fa, e, st);
+        }
+      });
+    } else {
+      require(false, "the fused Omega kernel is FP32 only");
+    }
+  }
+
+  // y0 ξ_c of the resident state into PredY[par] (the first fused attempt from a new
+  // state): the Ωξ product of the unfused path on y0.
+  void compute_y0xi(int par, hipStream_t st) {
+    omgxi_gemm(omg[iy0], W, st);
+    WSumArgs<cs> w{};
+    w.part = W;
+    w.pred[0] = PredY[par];
+    for (int j = 1; j < 7; ++j) w.pred[j] = Pred[j];
+    w.W = Wst;
+    w.s = 0; w.parts = kWParts; w.NK = NK; w.M = Md; w.nc = nc;
+    hipLaunchKernelGGL(k_wsum<cs>, dim3(1024), dim3(256), 0, st, w);
+    HIPCHK(hipGetLastError());
   }
 
   void te_and_control() {
@@ -1416,7 +1578,22 @@ struct Engine : EngineBase {
         for (int j = 0; j < 6; ++j) covered += (tab.ct[j] == 0.0 || tab.a[6][j] != 0.0);
         require(covered == 6, "error weights need a stage-7 term the tableau does not carry");
       }
+      if (fused_used) {
+        os.wpart = Wpart[s % 2];
+        os.par = (iy0 == 7) ? 0 : 1;
+      }
       rhs_stage(s, sin_, os, s, s == 0, true);
+    }
+    if (fused_used) {
+      // y_new ξ_c from the last stage's partials: the next attempt's y0 ξ_c if accepted
+      const OmegaFusedArgs fa = fused_args(kFusedChunk[tsel[TS_OMEGA]]);
+      const int par = (iy0 == 7) ? 0 : 1;
+      timed(KC_GEMM_OMGXI, [&] {
+        hipLaunchKernelGGL(k_sum_wpart<cs>, dim3(1024), dim3(256), 0, stream,
+                           (const cs*)Wpart[0], fa.SA + fa.SP, ff ? fa.SA : fa.SA + fa.SP, ML, NK,
+                           Md, nc, PredY[1 - par]);
+      });
+      HIPCHK(hipGetLastError());
     }
     te_and_control();
   }
@@ -1466,10 +1643,12 @@ struct Engine : EngineBase {
       auto it = g_tune_cache.find(key);
       if (it != g_tune_cache.end()) {
         for (int i = 0; i < TS_N; ++i) tsel[i] = it->second[i];
+        fused_tuned = it->second[TS_N] != 0;
         cached = true;
       }
     }
     if (!cached) {
+      fused_tuned = false;  // the tile candidates are timed on the unfused path
       static const int kc_of[TS_N] = {KC_GEMM_PSI, KC_GEMM_HSIG, KC_GEMM_OMEGA, KC_GEMM_OMGXI,
                                       KC_GEMM_PI};
       double best[TS_N];
@@ -1505,14 +1684,36 @@ struct Engine : EngineBase {
           if (t < best[i]) { best[i] = t; arg[i] = c; }
         }
       }
+      for (int i = 0; i < TS_N; ++i) tsel[i] = arg[i];
+      // the fused Ω kernel (mixed precision) against the Ω and Ωξ products it replaces,
+      // for each chunk width (the Ω candidate index selects it)
+      fused_tuned = false;
+      if (std::is_same<R, float>::value && fusion_req == 0 && nc > 0 && nc <= 16 * kFusedNG &&
+          !overlap) {
+        const double unfused = best[TS_OMEGA] + best[TS_OMGXI];
+        double fbest = 1e300;
+        int fc = 0;
+        fused_tuned = true;
+        for (int c = 0; c < kCands; ++c) {
+          tsel[TS_OMEGA] = c;
+          attempt(&info, nullptr);
+          for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
+          for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
+          const double t = (kc_count[KC_GEMM_OMEGA] ? kc_ms[KC_GEMM_OMEGA] / kc_count[KC_GEMM_OMEGA] : 0.0) +
+                           (kc_count[KC_GEMM_OMGXI] ? kc_ms[KC_GEMM_OMGXI] / kc_count[KC_GEMM_OMGXI] : 0.0);
+          if (t < fbest) { fbest = t; fc = c; }
+        }
+        fused_tuned = fbest < unfused;
+        tsel[TS_OMEGA] = fused_tuned ? fc : arg[TS_OMEGA];
+      }
       for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
       attempt_ms = 0; attempt_count = 0;
       profile = prof0;
-      for (int i = 0; i < TS_N; ++i) tsel[i] = arg[i];
       restore.done = true;
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      std::array<int, TS_N> a;
+      std::array<int, TS_N + 1> a;
       for (int i = 0; i < TS_N; ++i) a[i] = tsel[i];
+      a[TS_N] = fused_tuned ? 1 : 0;
       g_tune_cache[key] = a;
     }
     drop_graphs();
@@ -1534,8 +1735,15 @@ struct Engine : EngineBase {
     ctrl_h->force = info->force_accept ? 1 : 0;
     ctrl_h->norm_scale = norm_scale();
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipEventRecord(ev[2 * KC_N], stream));
     const int par = (iy0 == 7) ? 0 : 1;
+    const bool fz = fusable();
+    if (fz != fused_used) drop_graphs();
+    fused_used = fz;
+    if (fused_used && !y0xi_valid) {
+      compute_y0xi(par, stream);
+      y0xi_valid = true;
+    }
+    HIPCHK(hipEventRecord(ev[2 * KC_N], stream));
     if (use_graph && !profile) {
       if (!graph[par]) {
         hipGraph_t g;
@@ -1577,7 +1785,10 @@ struct Engine : EngineBase {
                                      std::to_string(info->te) + ") at t=" +
                                      std::to_string(info->t0) + "; the step is not committed"};
     }
-    if (info->accepted) std::swap(iy0, iyn);
+    if (info->accepted) {
+      std::swap(iy0, iyn);
+      y0xi_valid = fused_used;  // the last fused stage left y_new ξ_c in the new y0's slot
+    }
   }
 
   void attempt(zh_attempt_info* info, double* J, bool finite_next = false) override {
@@ -1857,6 +2068,7 @@ struct Engine : EngineBase {
     omega0(omg[iy0]);
     HIPCHK(hipStreamSynchronize(stream));
     set_ff(true);
+    y0xi_valid = false;
     if (iters) *iters = it;
     if (resid) *resid = res;
     require(ok, "steady state: not converged to the tolerance within the iteration limit "
@@ -2614,6 +2826,13 @@ int zh_hamiltonian_info(zh_problem* p, int* out) {
   return zh_try([&] {
     require(p && out, "null argument");
     p->eng->hamiltonian_info(out);
+  });
+}
+
+int zh_set_fusion(zh_problem* p, int mode) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_fusion(mode);
   });
 }
 

@@ -668,10 +668,19 @@ struct EpiOmegaRK {
   }
   // Four consecutive columns of one row (FP32 storage): 32-byte loads and stores
   // when all four are inside the matrix; every slot load first, then the stores.
+  // qr, qi (optional) receive the four values as stored (k_s, or Ω_s when ynew is set;
+  // 0 outside the matrix) for a fused product with them.
   static constexpr bool kRow4 = true;
   __device__ inline void row4(int b, int row, int col0, int ncol, const double* re,
-                              const double* im, double& acc) const {
-    if (ncol <= 0) return;
+                              const double* im, double& acc, double* qr = nullptr,
+                              double* qi = nullptr) const {
+    if (ncol <= 0) {
+      if (qr) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { qr[r] = 0.0; qi[r] = 0.0; }
+      }
+      return;
+    }
     const long off = ((long)b * M + row) * M + col0;
     const bool vec = ncol == 4 && sizeof(C) == 8 && (off & 3) == 0;
     double sr[4], si[4], er[4], ei[4], yr0[4], yi0[4];
@@ -718,6 +727,20 @@ struct EpiOmegaRK {
     }
     if (ynew) store4(ynew, off, ncol, vec, sr, si);
     if (out) store4(out, off, ncol, vec, kr, ki);
+    if (qr) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool in = r < ncol;
+        if (ynew) {
+          const C v = from_d2<C>(make_double2(sr[r], si[r]));
+          qr[r] = in ? (double)v.x : 0.0;
+          qi[r] = in ? (double)v.y : 0.0;
+        } else {
+          qr[r] = in ? kr[r] : 0.0;
+          qi[r] = in ? ki[r] : 0.0;
+        }
+      }
+    }
   }
   __device__ static inline void load4(const C* p, long off, int ncol, bool vec, double* xr,
                                       double* xi) {
@@ -760,6 +783,8 @@ struct EpiOmegaRK {
 // at the first stage, else the previous stage's k) are summed into pred[s], then
 //   W = pred[0] + Σ_{j<s} a_j pred[j+1]
 // (pred[0] = y0 ξ_c, pred[j] = k_j ξ_c).
+// With the fused Ω stage kernel the partials are per column chunk, and rows from ML on
+// (passive, with the passive x passive block skipped) have only parts_pass of them.
 template <typename C>
 struct WSumArgs {
   const C* part;            // (NK, parts, M, nc)
@@ -767,6 +792,7 @@ struct WSumArgs {
   C* W;                     // (NK, M, nc)
   double a[6];
   int s, parts, NK, M, nc;
+  int parts_pass = -1, ML = 0;
 };
 
 template <typename C>
@@ -775,8 +801,9 @@ __global__ __launch_bounds__(256) void k_wsum(WSumArgs<C> g) {
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
        idx += (long)gridDim.x * blockDim.x) {
     const long b = idx / per, rem = idx - b * per;
+    const int np = (g.parts_pass >= 0 && rem / g.nc >= g.ML) ? g.parts_pass : g.parts;
     double pr = 0.0, pi = 0.0;
-    for (int p = 0; p < g.parts; ++p) {
+    for (int p = 0; p < np; ++p) {
       const double2 x = to_d2(g.part[(b * g.parts + p) * per + rem]);
       pr += x.x;
       pi += x.y;
@@ -796,6 +823,26 @@ __global__ __launch_bounds__(256) void k_wsum(WSumArgs<C> g) {
       }
     }
     g.W[idx] = from_d2<C>(make_double2(wr, wi));
+  }
+}
+
+// out = Σ_p part[p] (fixed order; rows from ML on sum parts_pass partials): the y_new ξ_c
+// of the last fused Ω stage, the next attempt's y0 ξ_c.
+template <typename C>
+__global__ __launch_bounds__(256) void k_sum_wpart(const C* part, int parts, int parts_pass,
+                                                   int ML, int NK, int M, int nc, C* out) {
+  const long per = (long)M * nc, total = (long)NK * per;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long b = idx / per, rem = idx - b * per;
+    const int np = rem / nc >= ML ? parts_pass : parts;
+    double pr = 0.0, pi = 0.0;
+    for (int p = 0; p < np; ++p) {
+      const double2 x = to_d2(part[(b * parts + p) * per + rem]);
+      pr += x.x;
+      pi += x.y;
+    }
+    out[idx] = from_d2<C>(make_double2(pr, pi));
   }
 }
 

@@ -140,6 +140,14 @@ class Propagator:
         N.check(self.lib.zh_hamiltonian_info(self.h, out))
         return dict(real=bool(out[0]), sparse=bool(out[1]), row_entries=out[2])
 
+    def set_fusion(self, mode="auto"):
+        """Fused Ω stage kernel of mixed precision (the Ωξ product inside the Ω kernel):
+        "auto" (the autotuner's timed choice), "on" or "off"; same results to rounding.
+        True / False mean "on" / "off"."""
+        if mode is True or mode is False:
+            mode = "on" if mode else "off"
+        N.check(self.lib.zh_set_fusion(self.h, {"auto": 0, "off": 1, "on": 2}[mode]))
+
     def set_graph(self, on: bool):
         N.check(self.lib.zh_set_graph(self.h, 1 if on else 0))
 

@@ -42,6 +42,8 @@ def footprint_bytes(problem: AMEProblem, precision: str = "mixed") -> int:
                 + 5 * n * n * 16                                 # H, its real copies, G, σ_eq
                 + 64 * n * 20                                    # row-compressed H
                 + 8 * Md * 16)                                   # per-mode constants
+    if precision == "mixed" and 0 < ncp <= 128:  # fused Ω kernel: chunk partials, y0/y_new ξ_c
+        ops += NK * (2 * ((Md + 255) // 256 + 1) + 2) * Md * ncp * cs
     te = 8 * (2 * 1024 + ((Md + 15) // 16) ** 2 * NK)
     return int(slots + ops + te + (1 << 20))
 
