@@ -20,6 +20,7 @@ extern "C" {
 
 #define ZH_ABI_VERSION 1
 #define ZH_MAX_LEADS 8
+#define ZH_MAX_DRIVE 8
 
 enum { ZH_FP64 = 0, ZH_MIXED = 1 };
 enum { ZH_OK = 0, ZH_EINVAL = -1, ZH_EHIP = -2, ZH_ENOMEM = -3, ZH_ESTATE = -4, ZH_ENONFINITE = -5,
@@ -27,6 +28,8 @@ enum { ZH_OK = 0, ZH_EINVAL = -1, ZH_EHIP = -2, ZH_ENOMEM = -3, ZH_ESTATE = -4, 
 /* bias waveform kinds (Zandpack/Pulses.py:52-94 shapes) */
 enum { ZH_PULSE_NONE = 0, ZH_PULSE_STEP = 1, ZH_PULSE_GAUSS = 2, ZH_PULSE_RAMP = 3, ZH_PULSE_SIN = 4,
        ZH_PULSE_BOX = 5, ZH_PULSE_GENERIC = 6, ZH_PULSE_STAIRS = 7, ZH_PULSE_SPLINE = 8 };
+/* time factors of the drive terms of zh_set_drive */
+enum { ZH_WAVE_CONST = 0, ZH_WAVE_COS_WINDOW = 1, ZH_WAVE_SIN_FERMI = 2, ZH_WAVE_GAUSS_SIN = 3 };
 /* dissipator rate kinds */
 enum { ZH_DISS_NONE = 0, ZH_DISS_CONST = 1, ZH_DISS_STEP = 2 };
 /* time bookkeeping after an accepted step */
@@ -91,8 +94,28 @@ int zh_set_constants(zh_problem* p, const double* H0, const double* xi, const do
 
 /* Device-evaluated H(t, sigma) = H0 + U diag(sigma[partner]_ii - n0[partner]_i) for NK = 2
  * spin blocks (examples/Hubbard/Calculation/Bias.py:13-20).  U = 0 disables it.
- * n0 is (NK, n) real. */
+ * n0 is (NK, n) real.  With an overlap (zh_set_overlap) the term enters as nozand applies
+ * dH (S^-1 (H0 + dH), Zandpack/mpi/nozand:427, 456-461, 1075-1085). */
 int zh_set_hubbard(zh_problem* p, double U, const double* n0);
+
+/* Drive terms of the device-evaluated Hamiltonian: H(t, sigma) = H0 + (Hubbard term) +
+ * sum_i f_i(t) M_i, the time-dependent dH(t, sigma) a run's Bias.py hands the propagator
+ * every stage (Zandpack/mpi/zand:318-319; mpi/nozand:427), e.g. a Rabi drive
+ * (examples/nb_tutorial_updated/Tight-binding/T1_Rabi/Bias.py:16-24), switched site
+ * potentials (examples/Reproduce_and_checks/scipy_vs_ARK4_Zand/shared.py:4-19) or a
+ * dipole ramp (examples/nb_tutorial_updated/Tight-binding/T2_TB_Chain/Bias.py:21-25).
+ * nterms <= ZH_MAX_DRIVE terms; term i has the time factor kinds[i] (ZH_WAVE_*) with
+ * amps[i] and par[4 i .. 4 i + 3]:
+ *   CONST       amp
+ *   COS_WINDOW  amp cos(w t + phi) for t_on < t <= t_off, else 0;   par = {w, phi, t_on, t_off}
+ *   SIN_FERMI   amp sin(w t + phi) / (1 + exp((t - t_off)/width));  par = {w, phi, t_off, width}
+ *   GAUSS_SIN   amp exp(-((t-d)/s)^2) sin(2 pi f (t-d) - phi);       par = {s, f, phi, d}
+ * and the nnz[i] entries (ks, rows, cols, vals) of M_i (vals complex128 interleaved),
+ * listed term after term.  nterms = 0 removes the drive.  A row-compressed H then holds
+ * the union of H0's and the drive's entries. */
+int zh_set_drive(zh_problem* p, int nterms, const int* kinds, const double* amps,
+                 const double* par, const int* nnz, const int* ks, const int* rows,
+                 const int* cols, const double* vals);
 
 /* Device-evaluated bias Delta_a(t) = weights[a] * V(t) (Bias.bias(t, a) hook,
  * Zandpack/mpi/zand:320-321).  kind: ZH_PULSE_*; amp, p1, p2 per kind:
@@ -186,8 +209,9 @@ int zh_pi_get(zh_problem* p, int hermitian, double* Q);
  * psi (:1075-1085), S^-1 on the GL_P xi constant (:1003) and on Omega xi
  * (:1208-1210); Omega, the currents and the error norm are unchanged in form.  An
  * explicit Ht (zh_rhs) is multiplied by S^-1 on the device.  Call after
- * zh_set_constants; NULL returns to the orthogonal form.  Not combined with the
- * device Hubbard term or zh_steady_state (ZH_EINVAL / ZH_ESTATE). */
+ * zh_set_constants; NULL returns to the orthogonal form.  With the Hubbard term or
+ * drive terms, S^-1 H(t, sigma) is formed on the device every stage.  Not combined
+ * with zh_steady_state (ZH_EINVAL). */
 int zh_set_overlap(zh_problem* p, const double* Sinv);
 
 /* Error norm of the attempts.  (0, 0), the default, is zand's absolute norm

@@ -190,7 +190,11 @@ def test_overlap_rejections(native):
     with pytest.raises(Exception, match="overlap"):
         prop.set_hproducts("sparse")
     prop.close()
+    # the Hubbard term with an overlap is accepted (S⁻¹H(t, σ) per stage; parity in
+    # tests/test_gpu_models.py), the steady state still refused
     h = builder.config_small(NK=2, hubbard_U=2.0, seed=2)
     h.Sinv = np.broadcast_to(np.eye(h.dims[1]), (2, h.dims[1], h.dims[1])).astype(complex)
+    prop = Propagator(h, "fp64")
     with pytest.raises(Exception, match="orthogonal"):
-        Propagator(h, "fp64")
+        prop.steady_state(0.0)
+    prop.close()

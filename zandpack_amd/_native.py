@@ -15,6 +15,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzandhip
 ZH_FP64, ZH_MIXED = 0, 1
 ZH_TIME_REFERENCE, ZH_TIME_CORRECT = 0, 1
 ZH_ELIMIT = -6
+WAVE_KINDS = {"const": 0, "cos_window": 1, "sin_fermi": 2, "gauss_sin": 3}
 PULSE_KINDS = {"none": 0, "step": 1, "gauss": 2, "ramp": 3, "sin": 4, "box": 5, "generic": 6,
                "stairs": 7, "spline": 8}
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE", -6: "ELIMIT"}
@@ -55,6 +56,9 @@ SIGNATURES = {
     "zh_problem_destroy": (None, [C.c_void_p]),
     "zh_set_constants": (C.c_int, [C.c_void_p] + [_dp] * 8),
     "zh_set_hubbard": (C.c_int, [C.c_void_p, C.c_double, _dp]),
+    "zh_set_drive": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp, _dp,
+                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int), _dp]),
     "zh_set_pulse": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _dp]),
     "zh_state_put": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "zh_state_get": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),

@@ -305,6 +305,36 @@ class Dissipator:
 
 
 @dataclass
+class Drive:
+    """One drive term f(t) M of a time-dependent Hamiltonian H(t) = H0 + Σ f_i(t) M_i,
+    the dH(t, σ) of a Bias.py (Zandpack/mpi/zand:318-319).  kinds of f:
+      'const'       amp
+      'cos_window'  amp cos(ω t + φ) for t_on < t <= t_off, else 0; par = (ω, φ, t_on, t_off)
+                    (the Rabi drive of examples/nb_tutorial_updated/Tight-binding/T1_Rabi/Bias.py:16-24)
+      'sin_fermi'   amp sin(ω t + φ) / (1 + exp((t - t_off)/w)); par = (ω, φ, t_off, w)
+                    (examples/Reproduce_and_checks/scipy_vs_ARK4_Zand/shared.py:4-19)
+      'gauss_sin'   amp exp(-((t - d)/s)^2) sin(2π f (t - d) - φ); par = (s, f, φ, d)
+                    (examples/nb_tutorial_updated/Tight-binding/T2_TB_Chain/Bias.py:9-10, 21-25)
+    M is (NK, n, n); only its non-zero entries are stored on the device."""
+    kind: str
+    amp: float
+    par: tuple
+    M: np.ndarray
+
+    def value(self, t):
+        a, b, c, d = (tuple(float(x) for x in self.par) + (0.0,) * 4)[:4]
+        if self.kind == "const":
+            return self.amp
+        if self.kind == "cos_window":
+            return self.amp * np.cos(a * t + b) if c < t <= d else 0.0
+        if self.kind == "sin_fermi":
+            return self.amp * np.sin(a * t + b) / (1.0 + np.exp((t - c) / d))
+        if self.kind == "gauss_sin":
+            return self.amp * np.exp(-((t - d) / a) ** 2) * np.sin(2 * np.pi * b * (t - d) - c)
+        raise ValueError(self.kind)
+
+
+@dataclass
 class AMEProblem:
     """Everything one propagation needs, in the reference's array layout."""
     H0: np.ndarray          # (NK, n, n)
@@ -322,6 +352,7 @@ class AMEProblem:
     hubbard_n0: np.ndarray | None = None  # (NK, n) reference densities
     dissipator: Dissipator = field(default_factory=Dissipator)
     Sinv: np.ndarray | None = None  # (NK, n, n) S⁻¹: the non-orthogonal (nozand) form
+    drive: list = field(default_factory=list)  # Drive terms of H(t)
     hbar: float = HBAR
     name: str = "problem"
     t_end: float = 10.0
@@ -366,6 +397,8 @@ class AMEProblem:
             i = np.arange(n)
             out[0, i, i] = self.hubbard_U * (sig[1, i, i] - self.hubbard_n0[1])
             out[1, i, i] = self.hubbard_U * (sig[0, i, i] - self.hubbard_n0[0])
+        for d in self.drive:
+            out = out + d.value(t) * np.asarray(d.M, complex)
         return out
 
     def bias(self, t, a):
@@ -382,7 +415,8 @@ def lowdin_factors(S):
 def to_nonorthogonal(p, Sm05, zero_tol=1e-10):
     """The problem in the non-orthogonal basis that ``mpi/nozand`` propagates, from
     the orthogonalised arrays of ``p`` and S^(-1/2) (Zandpack/mpi/nozand:259-300,
-    376, 808-812): H = S^½ H_o S^½ (entries below 1e-13 dropped, Hermitian part),
+    376, 808-812): H = S^½ H_o S^½ (entries below 1e-13 dropped, Hermitian part; drive
+    matrices the same way, the Hubbard parameters as they are),
     σ₀ = S^-½ σ_o S^-½, ξ = S^½ ξ_o, Ixi = Ixi_o S^½, S⁻¹ = (S^-½)².  Per lead, the
     orbitals where every ξ and Ixi entry is at most ``zero_tol`` are dropped, as
     nozand's orbital index lists do (``dot_prod_zero_tol``, :242-245, 378-383, 917-928)."""
@@ -400,10 +434,13 @@ def to_nonorthogonal(p, Sm05, zero_tol=1e-10):
                (np.abs(Ixi[:, a]) > zero_tol).any(axis=(0, 1, 2))
         xi[:, a, ..., ~live] = 0.0
         Ixi[:, a, ..., ~live] = 0.0
+    # drive matrices are Hamiltonian terms: carried like H (S^½ M S^½)
+    drive = [Drive(d.kind, d.amp, d.par, S05 @ np.asarray(d.M, complex) @ S05) for d in p.drive]
     q = AMEProblem(H0=H, sigma0=sig, xi=xi, Ixi=Ixi, GG_P=p.GG_P, GL_P=p.GL_P, GG_M=p.GG_M,
                    GL_M=p.GL_M, Xpp=p.Xpp, Xpm=p.Xpm, pulse=p.pulse, dissipator=p.dissipator,
                    Sinv=Sm05 @ Sm05, hbar=p.hbar, name=p.name + "_nonortho", t_end=p.t_end,
-                   eps=p.eps, h0=p.h0)
+                   eps=p.eps, h0=p.h0, hubbard_U=p.hubbard_U, hubbard_n0=p.hubbard_n0,
+                   drive=drive)
     return q
 
 

@@ -89,6 +89,9 @@ struct EngineBase {
                              const double* glp, const double* dgp, const double* dgm,
                              const double* xpp, const double* xpm) = 0;
   virtual void set_hubbard(double U, const double* n0) = 0;
+  virtual void set_drive(int nt, const int* kinds, const double* amps, const double* par,
+                         const int* nnz, const int* ks, const int* rows, const int* cols,
+                         const double* vals) = 0;
   virtual void set_pulse(const PulseDesc& p) = 0;
   virtual void set_pulse_spline(int nknots, const double* x, const double* c) = 0;
   virtual void set_dissipator(const DissDesc& d, const double* sig_eq) = 0;
@@ -217,6 +220,24 @@ struct Engine : EngineBase {
   PulseDesc pulse;
   DissDesc diss{ZH_DISS_NONE, 0.0, 0.0, 1.0};
   double2* sig_eq = nullptr;  // σ_eq of the dissipator (allocated on first use)
+  // Drive terms H(t) = H0 + Σ_i f_i(t) M_i: the distinct entries (k, row, col) of the M_i,
+  // each with its H0 value and its terms' coefficients (CSR), written into the H
+  // operands every stage (k_drive_update)
+  DriveDesc drive{};
+  std::vector<int3> de_h;                 // (k, row, col)
+  std::vector<int> de_beg_h, de_term_h;
+  std::vector<double2> de_coef_h;
+  std::vector<int> de_slot_h;
+  int *de_k = nullptr, *de_r = nullptr, *de_c = nullptr, *de_slot = nullptr, *de_beg = nullptr,
+      *de_term = nullptr;
+  double2 *de_h0 = nullptr, *de_coef = nullptr;
+  size_t de_cap = 0, dt_cap = 0;
+  // Overlap with a σ- or t-dependent H (Hubbard or drive): S⁻¹H0 and S⁻¹M_i (dense),
+  // S⁻¹H(t, σ) formed per stage; the epilogues then see no separate Hubbard diagonal
+  double2 *SH0 = nullptr, *SM = nullptr, *dzero = nullptr;
+  int sm_cap = -1;
+  bool model_dirty = true;
+  bool ovl_model() const { return overlap && (U != 0.0 || drive.nt > 0); }
   double U = 0.0;
   bool have_consts = false, have_state = false, h_dirty = false;
   bool h_real = false;     // the Hamiltonian in Hcur / Bp has no imaginary part
@@ -448,6 +469,7 @@ struct Engine : EngineBase {
     HreT = dalloc<R>(nn);
     G = dalloc<double2>(nn);
     dvec = dalloc<double2>((size_t)NK * n);
+    dzero = dalloc<double2>((size_t)NK * n);
     glp = dalloc<double2>((size_t)NK * Md);
     chip = dalloc<double2>((size_t)NK * Md);
     chim = dalloc<double2>((size_t)NK * Md);
@@ -649,6 +671,7 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpyAsync(chip, x1.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(chim, x2.data(), sizeof(double2) * NK * Md, hipMemcpyHostToDevice, stream));
     H0h.assign(H0, H0 + 2 * (size_t)NK * n * n);
+    model_dirty = true;
     install_h(H0);
     HIPCHK(hipStreamSynchronize(stream));
     h_dirty = false;
@@ -679,23 +702,42 @@ struct Engine : EngineBase {
     return out;
   }
 
-  void install_h(const double* H) {
+  // Installs H (host, (NK, n, n)) into the Hamiltonian operands.  With `model` (H0 of
+  // the attempts) the drive's entries are part of the structure: a row-compressed H holds
+  // them even where H0 is zero, complex drive values make H complex, and the entries'
+  // H0 values and compressed-row slots go to the device for k_drive_update.
+  void install_h(const double* H, bool model = true) {
     std::vector<double> SH;
     if (overlap) {
       SH = apply_sinv(H);
       H = SH.data();
     }
+    const bool with_drive = model && drive.nt > 0 && !overlap;
+    // drive entries per (k, row), for the compressed rows
+    std::map<long, std::vector<int>> dcols;
+    if (with_drive)
+      for (const int3& e : de_h) dcols[(long)e.x * n + e.y].push_back(e.z);
     // a real Hamiltonian halves the MFMA work of the Hσ and ψHᵀ products
     bool real = !overlap;
     for (size_t i = 0; i < (size_t)NK * n * n && real; ++i) real = H[2 * i + 1] == 0.0;
+    if (with_drive)
+      for (const double2& c : de_coef_h) real = real && c.y == 0.0;
     // entries per row: with few, both H products run as gathers (row-compressed H)
-    int rmax = 0;
-    for (long r = 0; r < (long)NK * n; ++r) {
-      int c = 0;
+    auto row_cols = [&](long r) {
+      std::vector<int> cols;
       const double* row = H + 2 * r * n;
-      for (int o = 0; o < n; ++o) c += (row[2 * o] != 0.0 || row[2 * o + 1] != 0.0);
-      rmax = std::max(rmax, c);
-    }
+      for (int o = 0; o < n; ++o)
+        if (row[2 * o] != 0.0 || row[2 * o + 1] != 0.0) cols.push_back(o);
+      auto it = dcols.find(r);
+      if (it != dcols.end()) {
+        cols.insert(cols.end(), it->second.begin(), it->second.end());
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+      }
+      return cols;
+    };
+    int rmax = 0;
+    for (long r = 0; r < (long)NK * n; ++r) rmax = std::max(rmax, (int)row_cols(r).size());
     const int rb = hpsi_rows();
     const bool fits = rmax <= kEllMax && rb > 0 && !overlap;
     bool sparse = false;
@@ -721,20 +763,38 @@ struct Engine : EngineBase {
       for (int b = 0; b < NK; ++b)
         for (int o = 0; o < n; ++o) {
           const double* row = H + 2 * ((long)b * n + o) * n;
+          const std::vector<int> cols = row_cols((long)b * n + o);
           int r = 0;
-          for (int c = 0; c < n; ++c)
-            if (row[2 * c] != 0.0 || row[2 * c + 1] != 0.0) {
-              const size_t at = ((size_t)b * R_ + r) * n + o;
-              ell_col_h[at] = c;
-              ell_val_h[at] = make_double2(row[2 * c], row[2 * c + 1]);
-              ++r;
-            }
+          for (int c : cols) {
+            const size_t at = ((size_t)b * R_ + r) * n + o;
+            ell_col_h[at] = c;
+            ell_val_h[at] = make_double2(row[2 * c], row[2 * c + 1]);
+            ++r;
+          }
           for (; r < R_; ++r) ell_col_h[((size_t)b * R_ + r) * n + o] = o;  // zero padding
         }
       HIPCHK(hipMemcpyAsync(ell_col, ell_col_h.data(), sizeof(int) * ell_col_h.size(),
                             hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(ell_val, ell_val_h.data(), sizeof(double2) * ell_val_h.size(),
                             hipMemcpyHostToDevice, stream));
+    }
+    if (with_drive) {
+      // the entries' H0 values and compressed-row slots
+      const size_t E = de_h.size();
+      std::vector<double2> h0(E);
+      de_slot_h.assign(E, -1);
+      for (size_t e = 0; e < E; ++e) {
+        const int3 q = de_h[e];
+        const double* v = H + 2 * (((long)q.x * n + q.y) * n + q.z);
+        h0[e] = make_double2(v[0], v[1]);
+        if (sparse)
+          for (int r = 0; r < R_; ++r)
+            if (ell_col_h[((size_t)q.x * R_ + r) * n + q.y] == q.z) { de_slot_h[e] = r; break; }
+      }
+      HIPCHK(hipMemcpyAsync(de_h0, h0.data(), sizeof(double2) * E, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(de_slot, de_slot_h.data(), sizeof(int) * E, hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipStreamSynchronize(stream));  // h0 and de_slot_h are read asynchronously
     }
     dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
     hipLaunchKernelGGL((k_transpose_h<cs, R>), grid, dim3(32, 8), 0, stream, n,
@@ -743,6 +803,126 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));  // the host copies above are read asynchronously
   }
+
+  // Drive terms: the distinct entries of the M_i with their terms' coefficients.
+  void set_drive(int nt, const int* kinds, const double* amps, const double* par, const int* nnz,
+                 const int* ks, const int* rows, const int* cols, const double* vals) override {
+    require(have_consts, "set the constants first", ZH_ESTATE);
+    require(nt >= 0 && nt <= ZH_MAX_DRIVE, "at most ZH_MAX_DRIVE drive terms");
+    HIPCHK(hipSetDevice(device));
+    DriveDesc d{};
+    d.nt = nt;
+    std::map<std::tuple<int, int, int>, std::vector<std::pair<int, double2>>> ent;
+    long off = 0;
+    for (int i = 0; i < nt; ++i) {
+      require(kinds[i] >= ZH_WAVE_CONST && kinds[i] <= ZH_WAVE_GAUSS_SIN, "unknown drive waveform");
+      require(!(kinds[i] == ZH_WAVE_SIN_FERMI && par[4 * i + 3] == 0.0), "zero switch-off width");
+      require(!(kinds[i] == ZH_WAVE_GAUSS_SIN && par[4 * i] == 0.0), "zero envelope width");
+      require(nnz[i] >= 0, "negative entry count");
+      d.kind[i] = kinds[i];
+      d.amp[i] = amps[i];
+      for (int j = 0; j < 4; ++j) d.par[i][j] = par[4 * i + j];
+      for (int q = 0; q < nnz[i]; ++q, ++off) {
+        require(ks[off] >= 0 && ks[off] < NK && rows[off] >= 0 && rows[off] < n && cols[off] >= 0 &&
+                    cols[off] < n, "drive entry out of range");
+        ent[std::make_tuple(ks[off], rows[off], cols[off])].push_back(
+            {i, make_double2(vals[2 * off], vals[2 * off + 1])});
+      }
+    }
+    drop_graphs();
+    de_h.clear(); de_beg_h.assign(1, 0); de_term_h.clear(); de_coef_h.clear();
+    for (auto& kv : ent) {
+      de_h.push_back(make_int3(std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)));
+      for (auto& tc : kv.second) { de_term_h.push_back(tc.first); de_coef_h.push_back(tc.second); }
+      de_beg_h.push_back((int)de_term_h.size());
+    }
+    const size_t E = de_h.size(), T = de_term_h.size();
+    if (E > de_cap) {
+      de_k = dalloc<int>(E); de_r = dalloc<int>(E); de_c = dalloc<int>(E); de_slot = dalloc<int>(E);
+      de_h0 = dalloc<double2>(E); de_beg = dalloc<int>(E + 1);
+      de_cap = E;
+    }
+    if (T > dt_cap) {
+      de_term = dalloc<int>(T); de_coef = dalloc<double2>(T);
+      dt_cap = T;
+    }
+    if (E > 0) {
+      std::vector<int> kk(E), rr(E), cc(E);
+      for (size_t e = 0; e < E; ++e) { kk[e] = de_h[e].x; rr[e] = de_h[e].y; cc[e] = de_h[e].z; }
+      HIPCHK(hipMemcpyAsync(de_k, kk.data(), sizeof(int) * E, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(de_r, rr.data(), sizeof(int) * E, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(de_c, cc.data(), sizeof(int) * E, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(de_beg, de_beg_h.data(), sizeof(int) * (E + 1), hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipMemcpyAsync(de_term, de_term_h.data(), sizeof(int) * T, hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipMemcpyAsync(de_coef, de_coef_h.data(), sizeof(double2) * T, hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    drive = d;
+    if (E == 0) drive.nt = 0;
+    model_dirty = true;
+    install_h(H0h.data());
+    h_dirty = false;
+  }
+
+  // S⁻¹H0 and S⁻¹M_i of the overlap form with a σ- or t-dependent H (before an attempt).
+  void prepare_model() {
+    if (!model_dirty) return;
+    model_dirty = false;
+    if (!ovl_model()) return;
+    const long nn = (long)n * n;
+    if (!SH0) SH0 = dalloc<double2>((size_t)NK * nn);
+    if (drive.nt > sm_cap) {
+      SM = dalloc<double2>((size_t)std::max(drive.nt, 1) * NK * nn);
+      sm_cap = drive.nt;
+    }
+    // S⁻¹ X for X = H0 and each dense M_i (FP64 MFMA)
+    auto sinv_times = [&](const double* X, double2* out) {
+      HIPCHK(hipMemcpyAsync(G, X, sizeof(double2) * NK * nn, hipMemcpyHostToDevice, stream));
+      GemmArgs<double, double> g{n, n, n, Sdev, n, nn, G, n, nn};
+      EpiStore<double2> e{out, n, nn, 1.0, 0.0};
+      HIPCHK((launch_zgemm<double, double, double, false, false, Tile<64, 64, 16, 2, false, 3>, 3>(
+          g, NK, e, stream)));
+      HIPCHK(hipStreamSynchronize(stream));
+    };
+    sinv_times(H0h.data(), SH0);
+    std::vector<double> M((size_t)2 * NK * nn);
+    for (int i = 0; i < drive.nt; ++i) {
+      std::fill(M.begin(), M.end(), 0.0);
+      for (size_t e = 0; e < de_h.size(); ++e)
+        for (int q = de_beg_h[e]; q < de_beg_h[e + 1]; ++q)
+          if (de_term_h[q] == i) {
+            const size_t at = ((size_t)de_h[e].x * n + de_h[e].y) * n + de_h[e].z;
+            M[2 * at] = de_coef_h[q].x;
+            M[2 * at + 1] = de_coef_h[q].y;
+          }
+      sinv_times(M.data(), SM + (size_t)i * NK * nn);
+    }
+  }
+
+  // H(t_s, σ_s) of stage sps into the Hamiltonian operands (attempts and the steady
+  // state; the explicit zh_rhs takes the caller's H): drive entries, or with an overlap
+  // the dense S⁻¹H(t, σ) with the stage's Hubbard diagonal.
+  void apply_model(const StageParams* sps, hipStream_t st) {
+    if (ovl_model()) {
+      hipLaunchKernelGGL((k_overlap_model<cs>), dim3(2048), dim3(256), 0, st, NK, n, nc, drive.nt,
+                         (const double2*)SH0, (const double2*)SM, (const double2*)Sdev,
+                         (const double2*)(U != 0.0 ? dvec : nullptr), sps, Hcur, Bp);
+      HIPCHK(hipGetLastError());
+    } else if (drive.nt > 0) {
+      DriveArgs<cs, R> a{(int)de_h.size(), n, nc, ell_R, de_k, de_r, de_c, de_slot, de_h0,
+                         de_beg, de_term, de_coef, Hcur, Bp, h_real ? Hre : nullptr,
+                         h_real ? HreT : nullptr, h_sparse ? ell_val : nullptr, sps};
+      hipLaunchKernelGGL((k_drive_update<cs, R>), dim3(std::min<int>(1024, (a.E + 255) / 256)),
+                         dim3(256), 0, st, a);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  // The Hubbard diagonal the σ and Ψ epilogues apply (with an overlap it is part of
+  // S⁻¹H(t, σ) instead).
+  const double2* dvec_epi() const { return ovl_model() ? dzero : dvec; }
 
   Ell ell() const { return Ell{ell_col, ell_val, ell_R, n}; }
 
@@ -755,10 +935,10 @@ struct Engine : EngineBase {
 
   void set_hubbard(double U_, const double* n0_) override {
     require(U_ == 0.0 || NK == 2, "the Hubbard model couples two spin blocks: NK must be 2");
-    require(U_ == 0.0 || !overlap, "the device Hubbard term is defined for an orthogonal basis");
     require(have_consts, "set the constants first", ZH_ESTATE);
     drop_graphs();
     U = U_;
+    model_dirty = true;
     if (U != 0.0) {
       HIPCHK(hipMemcpyAsync(n0, n0_, sizeof(double) * NK * n, hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -826,8 +1006,8 @@ struct Engine : EngineBase {
   void set_overlap(const double* S) override {
     HIPCHK(hipSetDevice(device));
     require(have_consts, "set the constants first", ZH_ESTATE);
-    require(!S || U == 0.0, "the device Hubbard term is defined for an orthogonal basis");
     drop_graphs();
+    model_dirty = true;
     if (!S) {
       overlap = false;
       install_h(H0h.data());
@@ -1200,7 +1380,10 @@ struct Engine : EngineBase {
   void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, bool currents,
                  bool hub) {
     const StageParams* sps = sp + s;
-    timed(KC_GATHER, [&] { gather(sps, sig_in, hub); });
+    timed(KC_GATHER, [&] {
+      gather(sps, sig_in, hub);
+      if (hub) apply_model(sps, stream);  // H(t_s, σ_s) of the attempts
+    });
     // the three branches (serial on the main stream when profiling, so that the
     // per-class event timings stay meaningful)
     const bool conc = !profile;
@@ -1265,7 +1448,7 @@ struct Engine : EngineBase {
     }
     timed(KC_GEMM_HSIG, [&] {
       GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n};
-      EpiSigma e{G, sig_in, dvec, Pic, overlap ? cpos_none : cpos, n, nc, ih, kPiParts,
+      EpiSigma e{G, sig_in, dvec_epi(), Pic, overlap ? cpos_none : cpos, n, nc, ih, kPiParts,
                  ovl ? Qs : nullptr};
       if (h_sparse) {
         constexpr int rb = 8;
@@ -1360,7 +1543,7 @@ struct Engine : EngineBase {
                                                           typename Cands<R>::psi())));
         }
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
-                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
+                     dvec_epi(), lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
         const size_t lds = (size_t)hpsi_rb * n * sizeof(cs);
         const long sAp = (long)Md * (n + nc);
         const dim3 grid((Md + hpsi_rb - 1) / hpsi_rb, NK);
@@ -1384,7 +1567,7 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
-                     dvec, lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
+                     dvec_epi(), lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
         HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psiR())));
       } else {
@@ -1393,7 +1576,7 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
         EpiPsi<cs> e{psi[kslot], ovl ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
-                     dvec, lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md};
+                     dvec_epi(), lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md};
         HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
       }
@@ -1551,7 +1734,7 @@ struct Engine : EngineBase {
   void attempt_body() {
     timed(KC_PARAMS, [&] {
       hipLaunchKernelGGL(k_stage_params, dim3(1), dim3(1), 0, stream, (const Ctrl*)ctrl, pulse,
-                         diss, tab, NL, sp);
+                         diss, drive, tab, NL, sp);
     });
     HIPCHK(hipGetLastError());
     for (int s = 0; s < 7; ++s) {
@@ -1728,6 +1911,7 @@ struct Engine : EngineBase {
     require(!pending, "an attempt is already in flight on this problem", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
     ensure_h0();
+    prepare_model();
     if (ff && ff_dirty) { clear_ff(); ff_dirty = false; }
     ctrl_h->t0 = info->t0;
     ctrl_h->h = info->h;
@@ -1822,9 +2006,10 @@ struct Engine : EngineBase {
     ctrl_h->norm_scale = 1.0;
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_stage_params, dim3(1), dim3(1), 0, stream, (const Ctrl*)ctrl, pulse,
-                       diss, tab, NL, sp);
+                       diss, drive, tab, NL, sp);
     HIPCHK(hipGetLastError());
     const StageParams* sp0 = sp;
+    apply_model(sp0, stream);  // drive terms at t0 (an overlap is refused above)
     const long N = (long)NK * Md * n;
     const long mc = (long)NK * Md * nc;
     const double2* sig0 = sig[iy0];
@@ -2079,7 +2264,7 @@ struct Engine : EngineBase {
            const double* o, double* ds, double* dp, double* dom, double* J) override {
     require(have_consts, "constants not set", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
-    install_h(Ht);
+    install_h(Ht, false);
     h_dirty = true;
     StageParams st;
     std::memset(&st, 0, sizeof(st));
@@ -2541,6 +2726,19 @@ int zh_set_hubbard(zh_problem* p, double U, const double* n0) {
   return zh_try([&] {
     require(p && (U == 0.0 || n0), "null argument");
     p->eng->set_hubbard(U, n0);
+  });
+}
+
+int zh_set_drive(zh_problem* p, int nterms, const int* kinds, const double* amps,
+                 const double* par, const int* nnz, const int* ks, const int* rows,
+                 const int* cols, const double* vals) {
+  return zh_try([&] {
+    require(p, "null argument");
+    require(nterms == 0 || (kinds && amps && par && nnz), "null argument");
+    long tot = 0;
+    for (int i = 0; i < nterms; ++i) tot += nnz[i] > 0 ? nnz[i] : 0;
+    require(tot == 0 || (ks && rows && cols && vals), "null argument");
+    p->eng->set_drive(nterms, kinds, amps, par, nnz, ks, rows, cols, vals);
   });
 }
 

@@ -13,7 +13,30 @@ struct StageParams {
   double ts, h;
   double delta[ZH_MAX_LEADS];
   double eta;  // dissipator rate at ts
+  double fdrive[ZH_MAX_DRIVE];  // time factors of the drive terms at ts
 };
+
+// Time factors f_i(t) of the drive terms H(t) = H0 + Σ_i f_i(t) M_i (the waveforms the
+// reference's examples write into dH(t, σ); ZH_WAVE_* in zandhip.h).
+struct DriveDesc {
+  int nt;
+  int kind[ZH_MAX_DRIVE];
+  double amp[ZH_MAX_DRIVE];
+  double par[ZH_MAX_DRIVE][4];
+};
+__device__ inline double drive_value(const DriveDesc& d, int i, double t) {
+  const double a = d.par[i][0], b = d.par[i][1], c = d.par[i][2], e = d.par[i][3];
+  switch (d.kind[i]) {
+    case ZH_WAVE_CONST: return d.amp[i];
+    case ZH_WAVE_COS_WINDOW: return (t > c && t <= e) ? d.amp[i] * cos(a * t + b) : 0.0;
+    case ZH_WAVE_SIN_FERMI: return d.amp[i] * sin(a * t + b) / (1.0 + exp((t - c) / e));
+    case ZH_WAVE_GAUSS_SIN: {
+      const double x = (t - e) / a;
+      return d.amp[i] * exp(-x * x) * sin(2.0 * M_PI * b * (t - e) - c);
+    }
+    default: return 0.0;
+  }
+}
 
 // dissipator(t, σ) = η(t)(σ - σ_eq) (Zandpack/mpi/zand:343 hook; the form of
 // examples/Hubbard/Calculation/Bias.py:29-39).
@@ -127,8 +150,8 @@ __device__ inline double pulse_value(const PulseDesc& p, double t) {
 }
 
 // Stage times and biases of one attempt from the device control block.
-__global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, DissDesc diss, Tableau tab,
-                               int NL, StageParams* sp) {
+__global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, DissDesc diss, DriveDesc drive,
+                               Tableau tab, int NL, StageParams* sp) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const double t0 = ctrl->t0, h = ctrl->h;
   for (int s = 0; s < 7; ++s) {
@@ -138,6 +161,71 @@ __global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, DissDesc diss,
     const double v = pulse_value(pulse, ts);
     for (int a = 0; a < NL; ++a) sp[s].delta[a] = pulse.w[a] * v;
     sp[s].eta = diss_rate(diss, ts);
+    for (int i = 0; i < ZH_MAX_DRIVE; ++i) sp[s].fdrive[i] = i < drive.nt ? drive_value(drive, i, ts) : 0.0;
+  }
+}
+
+// H(t_s) = H0 + Σ_i f_i(t_s) M_i written at the drive's entries into the Hamiltonian
+// operands of the right-hand side: the complex copies (A of Hσ, Hᵀ block of the Ψ
+// product's B), the real copies when H is real, the row-compressed values when the H
+// products are gathers.  Entry e: (k, row, col), H0 value, its terms' coefficients
+// term[beg[e] .. beg[e+1]).  The entries are distinct, so writes never collide.
+template <typename C, typename RT>
+struct DriveArgs {
+  int E, n, nc, R;
+  const int *k, *r, *c, *slot;  // slot: position in the row-compressed row (sparse)
+  const double2* h0;
+  const int* beg;
+  const int* term;
+  const double2* coef;
+  double2* Hcur; C* Bp; double* Hre; RT* HreT; double2* ell_val;
+  const StageParams* sp;
+};
+template <typename C, typename RT>
+__global__ void k_drive_update(DriveArgs<C, RT> a) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < a.E; e += gridDim.x * blockDim.x) {
+    double2 v = a.h0[e];
+    for (int q = a.beg[e]; q < a.beg[e + 1]; ++q) {
+      const double f = a.sp->fdrive[a.term[q]];
+      v.x += f * a.coef[q].x;
+      v.y += f * a.coef[q].y;
+    }
+    const long b = a.k[e], r = a.r[e], c = a.c[e], n = a.n;
+    a.Hcur[(b * n + r) * n + c] = v;
+    a.Bp[(b * (n + a.nc) + c) * n + r] = from_d2<C>(v);
+    if (a.Hre) a.Hre[(b * n + r) * n + c] = v.x;
+    if (a.HreT) a.HreT[(b * n + c) * n + r] = (RT)v.x;
+    if (a.ell_val && a.slot[e] >= 0) a.ell_val[(b * a.R + a.slot[e]) * n + r] = v;
+  }
+}
+
+// Non-orthogonal form with a σ- or t-dependent Hamiltonian (Zandpack/mpi/nozand:427,
+// 456-461, 1075-1085): S⁻¹H(t, σ) = S⁻¹H0 + Σ_i f_i(t) S⁻¹M_i + S⁻¹ diag(d), d the Hubbard
+// diagonal of the stage, into the complex A operand of Hσ and (transposed) the Hᵀ block
+// of the Ψ product's B.
+template <typename C>
+__global__ void k_overlap_model(int NK, int n, int nc, int nt, const double2* SH0,
+                                const double2* SM, const double2* Sinv, const double2* dvec,
+                                const StageParams* sp, double2* Hcur, C* Bp) {
+  const long nn = (long)n * n, total = (long)NK * nn;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long b = idx / nn, rc = idx - b * nn;
+    const long r = rc / n, c = rc - r * n;
+    double2 v = SH0[idx];
+    for (int i = 0; i < nt; ++i) {
+      const double f = sp->fdrive[i];
+      const double2 m = SM[(long)i * total + idx];
+      v.x += f * m.x;
+      v.y += f * m.y;
+    }
+    if (dvec) {
+      const double2 t = cmul(Sinv[idx], dvec[b * n + c]);
+      v.x += t.x;
+      v.y += t.y;
+    }
+    Hcur[idx] = v;
+    Bp[(b * (n + nc) + c) * n + r] = from_d2<C>(v);
   }
 }
 

@@ -43,6 +43,8 @@ class Propagator:
             n0 = np.ascontiguousarray(p.hubbard_n0, dtype=np.float64)
             N.check(self.lib.zh_set_hubbard(self.h, float(p.hubbard_U), N.ptr(n0)))
         self.set_pulse(p.pulse)
+        if p.drive:
+            self.set_drive(p.drive)
         if p.dissipator.kind != "none":
             self.set_dissipator(p.dissipator)
         if p.Sinv is not None:
@@ -67,6 +69,30 @@ class Propagator:
         par = np.array([pulse.p1, pulse.p2, pulse.p3, pulse.p4], np.float64)
         N.check(self.lib.zh_set_pulse_ex(self.h, N.PULSE_KINDS[pulse.kind], float(pulse.amp),
                                          N.ptr(par), N.ptr(w)))
+
+    def set_drive(self, terms):
+        """Drive terms H(t) = H0 + Σ f_i(t) M_i (builder.Drive list) evaluated on the
+        device every stage; an empty list removes them."""
+        kinds = (C.c_int * max(1, len(terms)))(*[N.WAVE_KINDS[d.kind] for d in terms])
+        amps = np.array([float(d.amp) for d in terms] or [0.0])
+        par = np.zeros((max(1, len(terms)), 4))
+        ks, rows, cols, vals, nnz = [], [], [], [], []
+        for i, d in enumerate(terms):
+            pv = tuple(float(x) for x in d.par)[:4]
+            par[i, :len(pv)] = pv
+            M = np.asarray(d.M, complex)
+            assert M.shape == (self.NK, self.n, self.n)
+            k, r, c = np.nonzero(M)
+            ks.append(k); rows.append(r); cols.append(c); vals.append(M[k, r, c])
+            nnz.append(len(k))
+        cat = lambda xs, dt: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros(0), dt)
+        ks, rows, cols = (cat(x, np.int32) for x in (ks, rows, cols))
+        vals = cat(vals, np.complex128)
+        nnz_a = (C.c_int * max(1, len(terms)))(*nnz)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        self._drive_keep = (amps, par, ks, rows, cols, vals)
+        N.check(self.lib.zh_set_drive(self.h, len(terms), kinds, N.ptr(amps), N.ptr(par), nnz_a,
+                                      ip(ks), ip(rows), ip(cols), N.ptr(vals)))
 
     def set_tolerance(self, rtol=0.0, atol=0.0):
         """Scaled RMS error norm with these tolerances (attempts then take eps = 1);
