@@ -151,10 +151,13 @@ def profile_leg(p, precision, attempts, h0):
 
 
 def replay_error(p, eps, h0, n_steps):
-    """Mixed precision forced onto the FP64 run's accepted (t, h) sequence."""
+    """Mixed precision forced onto the FP64 run's accepted (t, h) sequence.  J error at
+    every accepted step, max over leads and spins of |J_mixed - J_fp64| / max_t |J_fp64|,
+    and σ's relative Frobenius error every 50 steps and at the end."""
     from zandpack_amd.engine import Propagator
+    every = 50
     a = Propagator(p, "fp64", autotune=True)
-    seq, J64 = [], []
+    seq, J64, s64 = [], [], []
     t, h = 0.0, h0
     while len(seq) < n_steps:
         r = a.attempt(t, h, eps)
@@ -163,16 +166,24 @@ def replay_error(p, eps, h0, n_steps):
             seq.append((t, h_old))
             J64.append(r["J"])
             t += h
-    s64 = a.sigma()
+            if len(seq) % every == 0 or len(seq) == n_steps:
+                s64.append(a.sigma())
     a.close()
     b = Propagator(p, "mixed", autotune=True)
-    Jm = [b.attempt(ts, hs, 1.0, force_accept=True)["J"] for ts, hs in seq]
-    sm = b.sigma()
+    Jm, sm = [], []
+    for i, (ts, hs) in enumerate(seq):
+        Jm.append(b.attempt(ts, hs, 1.0, force_accept=True)["J"])
+        if (i + 1) % every == 0 or i + 1 == n_steps:
+            sm.append(b.sigma())
     b.close()
     J64, Jm = np.array(J64), np.array(Jm)
-    jerr = float(np.abs(Jm - J64).max() / max(np.abs(J64).max(), 1e-300))
-    serr = float(np.linalg.norm(sm - s64) / np.linalg.norm(s64))
-    return jerr, serr, float(np.abs(J64).max())
+    scale = max(np.abs(J64).max(), 1e-300)
+    per_step = np.abs(Jm - J64).max(axis=(1, 2)) / scale
+    serr = [float(np.linalg.norm(x - y) / np.linalg.norm(y)) for x, y in zip(sm, s64)]
+    return dict(steps=n_steps, t_reached_fs=float(t), J_rel_err_max=float(per_step.max()),
+                J_rel_err_every_50=[float(x) for x in per_step[every - 1::every]],
+                sigma_rel_err_every_50=serr[:-1] if n_steps % every == 0 else serr,
+                sigma_rel_err_final=serr[-1], J_max=float(scale))
 
 
 def freerun_h_diff(p, eps, h0, n_steps):
@@ -209,10 +220,10 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(p):
-    """The CPU oracle (numpy/BLAS, zand-style orbital pruning) on one RHS
-    evaluation of the same workload, scaled to steps/s (7 evaluations per step),
-    with all BLAS threads and with one."""
+def cpu_baseline(p, attempts=3):
+    """The CPU oracle (numpy/BLAS, zand-style orbital pruning in every right-hand side) on
+    ``attempts`` fixed-h DOP54 attempts of the same workload with all BLAS threads, and
+    one attempt with one thread: seconds per attempt (= per accepted step) and per RHS."""
     from oracle import zand_oracle as O
     try:
         from threadpoolctl import threadpool_info, threadpool_limits
@@ -223,22 +234,29 @@ def cpu_baseline(p):
     rng = np.random.default_rng(0)
     sig, psi, omg = p.zero_state()
     psi = 1e-3 * (rng.normal(size=psi.shape) + 1j * rng.normal(size=psi.shape))
-    Ht = p.H0 + p.dH(0.0, sig)
-    run = lambda: O.rhs(p.consts(), sig, psi, omg, Ht, [0.15, -0.15], h=0.01, prune=True)
+    y = (sig, psi, omg)
+    NL = p.dims[2]
+
+    dH = O.hubbard_dH(p.hubbard_U, p.hubbard_n0)
+
+    def attempt(y, t):
+        return O.dop54_attempt(p.consts(), y, t, 0.002, p.H0, dH, p.bias, NL, prune=True)[3]
+
     t0 = time.perf_counter()
-    run()
-    el = time.perf_counter() - t0
+    for i in range(attempts):
+        y = attempt(y, 0.002 * i)
+    el = (time.perf_counter() - t0) / attempts
     el1 = None
     if threadpool_limits is not None:
         with threadpool_limits(limits=1):
             t0 = time.perf_counter()
-            run()
+            attempt(y, 0.002 * attempts)
             el1 = time.perf_counter() - t0
-    return {"value": 1.0 / (7 * el), "unit": "steps/s", "cores": int(threads), "kind": "port",
-            "sample": f"one RHS evaluation of the same n=1000, M=4000, NK=2 state "
-                      f"({el:.2f} s), x7 per DOP54 step",
-            "s_per_rhs": el, "s_per_rhs_1_thread": el1,
-            "steps_per_s_1_thread": (1.0 / (7 * el1)) if el1 else None,
+    return {"value": 1.0 / el, "unit": "steps/s", "cores": int(threads), "kind": "port",
+            "sample": f"{attempts} fixed-h DOP54 attempts (7 RHS each) of the same n=1000, "
+                      f"M=4000, NK=2 workload, {el:.2f} s per attempt",
+            "s_per_attempt": el, "s_per_rhs": el / 7, "s_per_attempt_1_thread": el1,
+            "steps_per_s_1_thread": (1.0 / el1) if el1 else None,
             "cpu_model": _cpu_model(), "cpu_count": os.cpu_count()}
 
 
@@ -335,10 +353,11 @@ def replay_fp64_error(req):
     return float(np.abs(Jm - J).max() / max(np.abs(J).max(), 1e-300))
 
 
-def c2_leg(steps=30, warmup=3):
+def c2_leg(t_end=100.0, time_cap=40.0, warmup=3):
     """Config 2, the FP64 drop-in (n = 400, M = 1600, Gaussian pulse) with adaptive
     steps under rtol = 1e-10 (the scaled norm also needs an atol: 1e-12), from the
-    zero state: accepted steps/s over ``steps`` steps after ``warmup``."""
+    zero state, to t_end = 100 fs or ``time_cap`` seconds of stepping: accepted
+    steps/s after ``warmup`` steps and the time reached."""
     from zandpack_amd import builder
     from zandpack_amd.engine import Propagator
     rtol, atol = 1e-10, 1e-12
@@ -348,9 +367,11 @@ def c2_leg(steps=30, warmup=3):
     t, h = 0.0, 0.005
     acc = rej = 0
     t_start = None
-    while acc < warmup + steps:
+    while t < t_end:
         if acc == warmup and t_start is None:
             t_start = time.perf_counter()
+        if t_start is not None and time.perf_counter() - t_start > time_cap:
+            break
         r = prop.attempt(t, h, 1.0)
         if r["accepted"]:
             t += r["h_next"]
@@ -359,11 +380,13 @@ def c2_leg(steps=30, warmup=3):
             rej += 1
         h = r["h_next"]
     dt = time.perf_counter() - t_start
+    steps = acc - warmup
     prop.close()
     NK, n, NL, NX, NOT = p.dims
     return {"workload": "honeycomb 20x20 n=400, 2 leads x 8 ch, N_L=40 + 60 Fermi slots, "
                         "M=1600, Gaussian pulse 0.5 eV", "precision": "fp64", "rtol": rtol,
             "atol": atol, "steps": steps, "steps_per_s": steps / dt, "t_reached_fs": t,
+            "t_end_fs": t_end,
             "rejected": rej, "M": NL * NX * NOT, "n": n}
 
 
@@ -401,8 +424,7 @@ def batched_leg(n_inst, steps, seed=0):
     st = Server(hbm_bytes=200 << 30, max_live=n_inst).run(reqs, realtime=False)
     return {"instances": n_inst, "n_values": sorted(int(x) for x in ns), "steps_each": steps,
             "steps_per_s": st["steps_per_s"], "wall_s": st["wall_s"], "failed": st["failed"],
-            "admit_s": st["admit_s"], "stepping_steps_per_s": st["stepping_steps_per_s"],
-            "precision": "mixed"}
+            "admit_s_on_worker": st["admit_s"], "precision": "mixed"}
 
 
 def serving_leg(n_req, rate, t_scale, seed=0):
@@ -428,8 +450,8 @@ def serving_leg(n_req, rate, t_scale, seed=0):
     return {"requests": n_req, "rate_per_s": rate, "t_end_scale": t_scale,
             "p50_s": st["p50_s"], "p99_s": st["p99_s"], "steps_per_s": st["steps_per_s"],
             "wall_s": st["wall_s"], "failed": st["failed"], "steps": st["steps"],
-            "admit_s": st["admit_s"], "stepping_steps_per_s": st["stepping_steps_per_s"],
-            "sampled_J_rel_err_vs_fp64": errs}
+            "admit_s_on_worker": st["admit_s"], "sampled_J_rel_err_vs_fp64": errs,
+            "t_end_fs": "U[20, 200] x t_end_scale"}
 
 
 def main():
@@ -439,14 +461,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--h0", type=float, default=0.002)
-    ap.add_argument("--replay-steps", type=int, default=20)
+    ap.add_argument("--replay-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--batched-instances", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=10)
-    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--serving-rate", type=float, default=20.0)
-    ap.add_argument("--serving-t-scale", type=float, default=0.005)
+    ap.add_argument("--serving-t-scale", type=float, default=0.01)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -463,8 +485,8 @@ def main():
         print(f"# timing {prec}", file=sys.stderr, flush=True)
         legs[prec] = run_leg(p, prec, args.steps, args.warmup, args.eps, args.h0)
     print("# replay error", file=sys.stderr, flush=True)
-    jerr, serr, jmax = replay_error(p, args.eps, args.h0, args.replay_steps)
-    free = freerun_h_diff(p, args.eps, args.h0, args.replay_steps)
+    replay = replay_error(p, args.eps, args.h0, args.replay_steps)
+    free = freerun_h_diff(p, args.eps, args.h0, 20)
 
     flops, na, nc = algorithmic_flops(p)
     roof = {}
@@ -512,8 +534,7 @@ def main():
         "fp64": legs["fp64"],
         "mixed": legs["mixed"],
         "speedup_mixed_vs_fp64": value / legs["fp64"]["steps_per_s"],
-        "replay": {"steps": args.replay_steps, "J_rel_err_mixed": jerr,
-                   "sigma_rel_err_mixed": serr, "J_max": jmax, "free_running": free},
+        "replay": dict(replay, free_running=free),
         "roofline": {k: v for k, v in roof["mixed"].items() if k != "kernel_ms"},
         "roofline_fp64": {k: v for k, v in roof["fp64"].items() if k != "kernel_ms"},
         "kernel_ms": {"fp64": roof["fp64"]["kernel_ms"], "mixed": roof["mixed"]["kernel_ms"]},

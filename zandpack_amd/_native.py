@@ -67,6 +67,7 @@ SIGNATURES = {
     "zh_attempt": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info), _dp]),
     "zh_attempt_launch": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info)]),
     "zh_attempt_collect": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info), _dp]),
+    "zh_attempt_ready": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_batch_attempt": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(zh_attempt_info),
                                    C.POINTER(_dp), C.POINTER(C.c_int)]),
     "zh_problem_bytes": (C.c_int, [C.c_void_p, _lp]),

@@ -517,12 +517,13 @@ def config_chain(n=20, N_L=8, N_F=20, kT=0.025, seed=0, hop=-1.0):
 
 def config_honeycomb(width, length, N_L, n_fermi_slots, NOT, NK=1, seed=0, kT=0.025,
                      hop=-2.7, disorder=0.1, pulse=None, hubbard_U=0.0, name="honeycomb",
-                     t_end=10.0, eps=1e-6, h0=0.01, gamma_scale=1.0):
+                     t_end=10.0, eps=1e-6, h0=0.01, gamma_scale=1.0, k_shared=False):
     """Honeycomb patch with two leads on its zigzag edges, rank-NOT PSD Lorentzian weights.
 
     Each lead has one orthonormal channel basis V (n_α x NOT) and non-negative
     weights w_{x,c}: G_x = V diag(w_x) V†, so every Γ(z) has rank NOT and the NOT
-    cut is exact.  ``n_fermi_slots`` = 2 N_F (B and C halves).
+    cut is exact.  ``n_fermi_slots`` = 2 N_F (B and C halves).  With ``k_shared`` every k
+    block (e.g. the two spins of a Hubbard problem) has the same lead couplings.
     """
     assert n_fermi_slots % 2 == 0
     N_F = n_fermi_slots // 2
@@ -535,11 +536,13 @@ def config_honeycomb(width, length, N_L, n_fermi_slots, NOT, NK=1, seed=0, kT=0.
         centres = np.sort(rng.uniform(-9.0, 9.0, N_L))
         widths = rng.uniform(0.6, 1.2, N_L)
         Gk = []
-        for k in range(NK):
+        for k in range(1 if k_shared else NK):
             Z = rng.normal(size=(na, NOT)) + 1j * rng.normal(size=(na, NOT))
             V, _ = np.linalg.qr(Z)
             w = gamma_scale * rng.uniform(0.05, 1.0, (N_L, NOT))
             Gk.append(np.array([(V * w[x]) @ V.conj().T for x in range(N_L)]))
+        if k_shared:
+            Gk = Gk * NK
         leads.append(dict(orbitals=orb, centres=centres, widths=widths, G=Gk))
     p = assemble(H0k, leads, N_F, NOT, mu=[0.0, 0.0], kT=kT,
                  pulse=pulse or Pulse("none"), hubbard_U=hubbard_U, name=name,
@@ -557,11 +560,12 @@ def config_c2(seed=0):
 
 
 def config_c3(seed=0):
-    """n=1000 honeycomb, NK=2 spin, Hubbard U=2, 2 leads x rank 20, N_L=50 + 50 slots -> M=4000."""
+    """n=1000 honeycomb, NK=2 spin, Hubbard U=2, 2 leads x rank 20, N_L=50 + 50 slots -> M=4000;
+    both spins see the same (spin-independent) lead couplings."""
     return config_honeycomb(40, 25, 50, 50, 20, NK=2, seed=seed,
                             pulse=Pulse("step", 0.3, 0.0, 0.0, (0.5, -0.5)),
                             hubbard_U=2.0, name="c3_n1000_M4000", t_end=10.0, eps=1e-8,
-                            h0=0.005)
+                            h0=0.005, k_shared=True)
 
 
 def config_small(n_w=4, n_l=3, N_L=3, slots=4, NOT=2, NK=1, seed=0, hubbard_U=0.0,

@@ -110,6 +110,7 @@ struct EngineBase {
   virtual void attempt(zh_attempt_info* info, double* J, bool finite_next = false) = 0;
   virtual void launch(const zh_attempt_info* info) = 0;
   virtual void collect(zh_attempt_info* info, double* J, bool finite_next = false) = 0;
+  virtual int ready() = 0;
   virtual long bytes() const = 0;
   virtual void autotune(int force, int* out) = 0;
   virtual void set_tiles(const int* sel) = 0;
@@ -1948,6 +1949,17 @@ struct Engine : EngineBase {
     pending = true;
   }
 
+  // 1 when the attempt in flight has finished (collect will not wait), else 0.
+  int ready() override {
+    require(pending, "no attempt in flight on this problem", ZH_ESTATE);
+    HIPCHK(hipSetDevice(device));
+    const hipError_t e = hipStreamQuery(stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    HIPCHK(e);
+    return 0;
+  }
+
   void collect(zh_attempt_info* info, double* J, bool finite_next = false) override {
     require(pending, "no attempt in flight on this problem", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
@@ -2952,6 +2964,13 @@ int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J) {
     require(p && info, "null argument");
     p->eng->collect(info, J);
     if (!std::isfinite(info->te)) throw Error{ZH_ENONFINITE, "non-finite error estimate"};
+  });
+}
+
+int zh_attempt_ready(zh_problem* p, int* ready) {
+  return zh_try([&] {
+    require(p && ready, "null argument");
+    *ready = p->eng->ready();
   });
 }
 

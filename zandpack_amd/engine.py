@@ -242,6 +242,12 @@ class Propagator:
                                           1 if force_accept else 0, 0.0, 0.0, 0)
         N.check(self.lib.zh_attempt_launch(self.h, C.byref(self._pending)))
 
+    def ready(self):
+        """True when the launched attempt has finished (``collect`` will not wait)."""
+        r = C.c_int(0)
+        N.check(self.lib.zh_attempt_ready(self.h, C.byref(r)))
+        return bool(r.value)
+
     def collect(self):
         """Wait for the launched attempt; commits the state when accepted."""
         info = self._pending

@@ -4,10 +4,13 @@ Requests (one AME propagation each: a device, its lead constants, a bias pulse
 and an end time) arrive over time.  The server admits them in arrival order as
 HBM frees up (by the footprint of σ, Ψ, Ω, the seven RK stages and the GEMM
 operands at the request's precision), keeps every admitted state resident on
-the device, and advances all live requests together one DOP54 attempt per
-tick: each attempt is launched on its request's own HIP stream before any is
-collected, so the attempts of different requests run concurrently on the GPU.
-Finished requests are retired at once and their memory admits the next ones.
+the device and keeps one DOP54 attempt of every live request in flight on its
+own HIP streams, so the attempts of different requests run concurrently on the
+GPU.  Each request is collected as soon as its attempt has finished and its next
+attempt launched at once: a small request does not wait for the largest one.
+Admission (problem set-up, upload and tile tuning) runs on a host worker thread
+while the live requests keep stepping.  Finished requests are retired at once and
+their memory admits the next ones.
 
 Every request's arithmetic is the same kernels on its own buffers in its own
 order, so its result is bitwise the same alone and in any mix.
@@ -18,12 +21,13 @@ the step semantics per request are zand's (Zandpack/mpi/zand:297-431).
 from __future__ import annotations
 
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from .builder import AMEProblem
-from .engine import Propagator, batch_attempt
+from .engine import Propagator
 
 
 def footprint_bytes(problem: AMEProblem, precision: str = "mixed") -> int:
@@ -81,24 +85,53 @@ class _Live:
         self.req = req
         # tile choices are cached per (precision, size bucket): only the first
         # request of a bucket pays for the timing attempts
+        t0 = time.perf_counter()
         self.prop = Propagator(req.problem, req.precision, graph=graph, autotune=autotune)
+        self.setup_s = time.perf_counter() - t0
         self.bytes = footprint_bytes(req.problem, req.precision)
         self.t = 0.0
         self.h = req.h0
         self.attempts = 0
+        self.inflight = False
         req.times.append(0.0)
+
+    def launch(self):
+        self.prop.launch(self.t, self.h, self.req.eps)
+        self.inflight = True
+
+    def step_done(self, res):
+        """Bookkeeping of a collected attempt; True when the request is finished."""
+        req = self.req
+        self.attempts += 1
+        h_old, self.h = self.h, res["h_next"]
+        if res["accepted"]:
+            req.J.append(res["J"])
+            req.h.append(h_old)
+            self.t += self.h if req.time_update == "reference" else h_old
+            req.times.append(self.t)
+            req.steps += 1
+        else:
+            req.rejected += 1
+        if self.t >= req.t_end or (req.max_steps is not None and req.steps >= req.max_steps):
+            return True
+        if self.attempts >= req.max_attempts:
+            req.error = "attempt limit reached"
+            return True
+        return False
 
 
 class Server:
-    """Admission by HBM footprint, step-granular continuous batching."""
+    """Admission by HBM footprint (on a host worker thread), per-request continuous
+    stepping with one attempt of every live request in flight."""
 
     def __init__(self, hbm_bytes: int = 64 << 30, max_live: int = 64, graph: bool = True,
-                 keep_sigma: bool = True, autotune: bool = True):
+                 keep_sigma: bool = True, autotune: bool = True, admit_workers: int = 1):
         self.budget = int(hbm_bytes)
         self.max_live = int(max_live)
         self.graph = graph
         self.keep_sigma = keep_sigma
         self.autotune = autotune
+        self.admit_workers = int(admit_workers)
 
     def run(self, requests, realtime: bool = True):
         """Serve ``requests`` (arrival times in seconds).  With ``realtime`` the
@@ -108,83 +141,99 @@ class Server:
             if footprint_bytes(r.problem, r.precision) > self.budget:
                 raise ValueError(f"request {r.rid} alone exceeds the HBM budget")
         live: list[_Live] = []
+        admitting = []            # (request, reserved bytes, future)
         done: list[Request] = []
         used = 0
         self.peak_bytes = 0
-        ticks = 0
-        t_admit = t_step = 0.0
+        attempts = 0
+        t_admit = 0.0
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
-        while pending or live:
-            now = clock()
-            while (pending and pending[0].arrival <= now and len(live) < self.max_live
-                   and used + footprint_bytes(pending[0].problem, pending[0].precision)
-                   <= self.budget):
-                r = pending.pop(0)
-                r.admitted = time.perf_counter() - t0
-                ta = time.perf_counter()
-                lv = _Live(r, self.graph, self.autotune)
-                t_admit += time.perf_counter() - ta
-                used += lv.bytes
-                self.peak_bytes = max(self.peak_bytes, used)
-                live.append(lv)
-            if not live:
-                if pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
-                continue
-            finished = []
-            ts = time.perf_counter()
-            # one attempt of every live request in one native call (all launches, then
-            # all collects); a failed request (e.g. a step size driven to zero or
-            # infinity) is retired and the rest go on
-            results = batch_attempt([lv.prop for lv in live], [lv.t for lv in live],
-                                    [lv.h for lv in live], [lv.req.eps for lv in live])
-            for lv, res in zip(live, results):
-                if isinstance(res, Exception):
-                    lv.req.error = str(res)
-                    finished.append(lv)
-                    continue
-                lv.attempts += 1
-                h_old, lv.h = lv.h, res["h_next"]
-                if res["accepted"]:
-                    lv.req.J.append(res["J"])
-                    lv.req.h.append(h_old)
-                    lv.t += lv.h if lv.req.time_update == "reference" else h_old
-                    lv.req.times.append(lv.t)
-                    lv.req.steps += 1
-                else:
-                    lv.req.rejected += 1
-                if lv.t >= lv.req.t_end or (lv.req.max_steps is not None
-                                              and lv.req.steps >= lv.req.max_steps):
-                    finished.append(lv)
-                elif lv.attempts >= lv.req.max_attempts:
-                    lv.req.error = "attempt limit reached"
-                    finished.append(lv)
-            t_step += time.perf_counter() - ts
-            ticks += 1
-            for lv in finished:
-                live.remove(lv)
-                lv.req.finished = time.perf_counter() - t0
-                if self.keep_sigma and lv.req.error is None:
-                    lv.req.sigma = lv.prop.sigma()
-                lv.prop.close()
-                used -= lv.bytes
-                done.append(lv.req)
+
+        def retire(lv):
+            nonlocal used
+            live.remove(lv)
+            lv.req.finished = time.perf_counter() - t0
+            if self.keep_sigma and lv.req.error is None:
+                lv.req.sigma = lv.prop.sigma()
+            lv.prop.close()
+            used -= lv.bytes
+            done.append(lv.req)
+
+        with ThreadPoolExecutor(max_workers=self.admit_workers) as pool:
+            while pending or live or admitting:
+                now = clock()
+                # admission: reserve the footprint, set the problem up on the worker
+                while (pending and pending[0].arrival <= now
+                       and len(live) + len(admitting) < self.max_live
+                       and used + footprint_bytes(pending[0].problem, pending[0].precision)
+                       <= self.budget):
+                    r = pending.pop(0)
+                    r.admitted = time.perf_counter() - t0
+                    fp = footprint_bytes(r.problem, r.precision)
+                    used += fp
+                    self.peak_bytes = max(self.peak_bytes, used)
+                    admitting.append((r, fp, pool.submit(_Live, r, self.graph, self.autotune)))
+                for item in [a for a in admitting if a[2].done()]:
+                    admitting.remove(item)
+                    r, fp, fut = item
+                    try:
+                        lv = fut.result()
+                    except Exception as e:  # set-up failed: retired with its error
+                        r.error = f"admission failed: {e}"
+                        r.finished = time.perf_counter() - t0
+                        used -= fp
+                        done.append(r)
+                        continue
+                    t_admit += lv.setup_s
+                    lv.bytes = fp
+                    live.append(lv)
+                    try:
+                        lv.launch()
+                    except Exception as e:
+                        lv.req.error = str(e)
+                        retire(lv)
+                # collect every finished attempt and launch the next one at once
+                progressed = False
+                for lv in list(live):
+                    if not lv.inflight or not lv.prop.ready():
+                        continue
+                    progressed = True
+                    lv.inflight = False
+                    try:
+                        res = lv.prop.collect()
+                    except Exception as e:  # e.g. a step size driven to zero or infinity
+                        lv.req.error = str(e)
+                        retire(lv)
+                        continue
+                    attempts += 1
+                    if lv.step_done(res):
+                        retire(lv)
+                        continue
+                    try:
+                        lv.launch()
+                    except Exception as e:
+                        lv.req.error = str(e)
+                        retire(lv)
+                if not progressed:
+                    if not live and not admitting and pending:
+                        time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                    else:
+                        time.sleep(2e-5)
         wall = time.perf_counter() - t0
-        return self.stats(done, wall, ticks, t_admit, t_step)
+        return self.stats(done, wall, attempts, t_admit)
 
     @staticmethod
-    def stats(done, wall, ticks, t_admit=0.0, t_step=0.0):
-        """Aggregate figures; ``steps_per_s`` over the whole wall time (admission,
-        i.e. problem set-up and upload, included), ``stepping_steps_per_s`` over the
-        time spent launching and collecting attempts only."""
+    def stats(done, wall, attempts, t_admit=0.0):
+        """Aggregate figures over the whole wall time: steps/s, p50/p99 time to
+        completion (arrival to result), and the host time spent setting requests up
+        (on the admission worker, beside the stepping)."""
         ok = [r for r in done if r.error is None]
         lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
         steps = int(sum(r.steps for r in done))
-        return dict(requests=len(done), failed=len(done) - len(ok), wall_s=wall, ticks=ticks,
-                    steps=steps, steps_per_s=steps / wall if wall > 0 else 0.0,
-                    admit_s=t_admit, stepping_s=t_step,
-                    stepping_steps_per_s=steps / t_step if t_step > 0 else 0.0,
+        return dict(requests=len(done), failed=len(done) - len(ok), wall_s=wall,
+                    attempts=attempts, steps=steps,
+                    steps_per_s=steps / wall if wall > 0 else 0.0, admit_s=t_admit,
                     p50_s=float(np.percentile(lat, 50)) if lat.size else None,
                     p99_s=float(np.percentile(lat, 99)) if lat.size else None,
                     done=done)
