@@ -97,6 +97,27 @@ def test_command_line_on_arrays_dir(native, tmp_path):
     np.testing.assert_allclose(C[0], ref["J"][:k, 0], rtol=1e-9, atol=1e-13)
 
 
+def test_command_line_reads_initial_py(native, tmp_path):
+    """The run's settings come from its Initial.py (read, not imported); only the bias
+    model is given as keywords."""
+    p = _problem()
+    say("rundir cli Initial.py", p.dims)
+    ref = _reference_run(p)
+    rundir.write_arrays(p, str(tmp_path / "x" / "Arrays"))
+    (tmp_path / "Initial.py").write_text(
+        "import numpy as np\n"
+        f"name = 'x'\neps = {EPS!r}\nt0, t1 = 0.0, {T1!r}\nstepsize = {H0!r}\n"
+        "saveevery = 4\nusesave = False\ncheckpoints = [0.3]\nsave_checkpoints = False\n")
+    rc = rundir.main([f"Dir={tmp_path}", "pulse=step", "amp=0.2", "p1=0", "weights=0.5,-0.5"])
+    assert rc == 0
+    T, C = rundir.combine_currents([str(tmp_path / "x_save")], p.dims[2])
+    k = len(T)
+    assert k > 0
+    np.testing.assert_allclose(T, ref["times"][:k], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(C[0], ref["J"][:k, 0], rtol=1e-9, atol=1e-13)
+    assert not any(f.startswith("last_") for f in os.listdir(tmp_path / "x_save"))
+
+
 def test_command_line_nonorthogonal(native, tmp_path):
     """A directory with S^(-0.5) run as nozand runs it."""
     from zandpack_amd.engine import Propagator

@@ -82,3 +82,43 @@ def test_nonorthogonal_checkpoint_psi_basis(tmp_path):
     assert np.allclose(np.einsum("kij,kaxcj->kaxci", Sm05, on_disk), psi, atol=1e-12)
     t, h, s2, psi2, o2 = rundir.load_checkpoint(str(tmp_path), Sm05)
     assert (t, h) == (1.5, 0.01) and np.allclose(psi2, psi, atol=1e-12)
+
+
+def test_read_initial_without_importing(tmp_path):
+    """Initial.py in the forms of the reference's run directories (header imports, tuple
+    unpacking, np.linspace checkpoints, a value from an .npy file, the arrays loaded by a
+    function call): the settings are evaluated from the syntax tree, the rest skipped,
+    and nothing in the file runs."""
+    np.save(tmp_path / "epsilon.npy", np.array(2.5e-7))
+    (tmp_path / "Initial.py").write_text(
+        "import numpy as np\n"
+        "from Zandpack.Loader import load_dictionary\n"
+        "raise SystemExit('must not run')\n"
+        "name = 'Dev'\n"
+        "eps = float(np.load('epsilon.npy'))\n"
+        "t0, t1 = -20.0, 2 * 40.0\n"
+        "usesave = False\n"
+        "saveevery = 25\n"
+        "checkpoints = list(np.linspace(-10, 30, 5))\n"
+        "save_checkpoints = True\n"
+        "stepsize = 0.05\n"
+        "n_dm_compress = 3\n"
+        "save_PI = True\n"
+        "Adir = name + '/Arrays/'\n"
+        "Arrs = load_dictionary(Adir)\n")
+    v, skipped = rundir.read_initial(str(tmp_path))
+    assert v["name"] == "Dev" and v["eps"] == 2.5e-7
+    assert (v["t0"], v["t1"]) == (-20.0, 80.0)
+    assert v["usesave"] is False and v["save_checkpoints"] is True and v["save_PI"] is True
+    assert (v["saveevery"], v["n_dm_compress"], v["stepsize"]) == (25, 3, 0.05)
+    np.testing.assert_allclose(v["checkpoints"], [-10, 0, 10, 20, 30])
+    assert skipped == ["Arrs"]
+
+
+def test_read_initial_refuses_files_outside_the_run_dir(tmp_path):
+    d = tmp_path / "run"
+    d.mkdir()
+    np.save(tmp_path / "outside.npy", np.array(1.0))
+    (d / "Initial.py").write_text("eps = float(np.load('../outside.npy'))\nname = 'x'\n")
+    v, skipped = rundir.read_initial(str(d))
+    assert "eps" not in v and skipped == ["eps"] and v["name"] == "x"

@@ -1,10 +1,13 @@
 """Zand run directories on the GPU engine.
 
 Reads the ``<name>/Arrays/`` inputs of ``Zandpack/mpi/zand`` and writes its
-``<name>_save/`` outputs with the same file names, dtypes and bookkeeping, so a
-directory prepared for ``mpirun -np P zand Dir=<dir>`` runs here unchanged and
-the reference's post-processing (``mpi_tools.combine_currents``, ``td_info``)
-reads the results:
+``<name>_save/`` outputs with the same file names, dtypes and bookkeeping.  A
+directory prepared for ``mpirun -np P zand Dir=<dir>`` runs here with its
+``Arrays/`` and the settings of its ``Initial.py`` (read from the syntax tree, not
+imported: ``read_initial``); its ``Bias.py`` (arbitrary Python importing numba and
+Zandpack) is not run, and the device models it would describe are given as keywords
+instead.  The reference's post-processing (``mpi_tools.combine_currents``,
+``td_info``) reads the results:
 
     python -m zandpack_amd.rundir Dir=<dir> name=<name> t1=50 eps=1e-8 stepsize=0.01 \\
         [t0=0 saveevery=100 n_dm_compress=1 usesave=False checkpoints=10,20 \\
@@ -231,11 +234,107 @@ def _parse(argv):
     return kw
 
 
+# the settings zand imports from a run's Initial.py (Zandpack/mpi/zand:57-71)
+INITIAL_KEYS = ("name", "eps", "t0", "t1", "stepsize", "usesave", "saveevery", "checkpoints",
+                "save_checkpoints", "n_dm_compress", "save_PI", "print_timings", "LoadFromFull")
+
+
+def read_initial(run_dir):
+    """The settings of ``<run_dir>/Initial.py`` without importing or running it.
+
+    zand imports the file (Zandpack/mpi/zand:57-71); its settings are plain top-level
+    assignments, which are evaluated here from the syntax tree with a closed set of
+    forms: literals, names assigned earlier in the file, tuple unpacking, unary and
+    binary arithmetic and string concatenation, ``list``/``tuple``/``float``/``int``/
+    ``bool``, ``np.linspace``/``np.arange``/``np.array`` on such values and
+    ``np.load`` of an ``.npy`` file in the run directory (no pickles).  Returns
+    ``(values, skipped)``: the evaluated settings, and the names of assignments in
+    another form (the loaded arrays dictionary among them), which are left out.
+    """
+    import ast
+    path = os.path.join(run_dir, "Initial.py")
+    with open(path) as fh:
+        tree = ast.parse(fh.read(), filename=path)
+    env, skipped = {}, []
+
+    class Unsupported(Exception):
+        pass
+
+    def npy(fname):
+        if not isinstance(fname, str) or not fname.endswith(".npy"):
+            raise Unsupported
+        full = os.path.realpath(os.path.join(run_dir, fname))
+        if os.path.dirname(full) != os.path.realpath(run_dir):
+            raise Unsupported
+        return np.load(full, allow_pickle=False)
+
+    calls = {"list": list, "tuple": tuple, "float": float, "int": int, "bool": bool}
+    npcalls = {"linspace": np.linspace, "arange": np.arange, "array": np.array, "load": npy}
+    binops = {ast.Add: lambda a, b: a + b, ast.Sub: lambda a, b: a - b,
+              ast.Mult: lambda a, b: a * b, ast.Div: lambda a, b: a / b}
+
+    def ev(node):
+        if isinstance(node, ast.Constant):
+            return node.value
+        if isinstance(node, (ast.Tuple, ast.List)):
+            return [ev(e) for e in node.elts]
+        if isinstance(node, ast.Name):
+            if node.id in env:
+                return env[node.id]
+            raise Unsupported
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+            v = ev(node.operand)
+            return -v if isinstance(node.op, ast.USub) else +v
+        if isinstance(node, ast.BinOp) and type(node.op) in binops:
+            return binops[type(node.op)](ev(node.left), ev(node.right))
+        if isinstance(node, ast.Call) and not node.keywords:
+            f = node.func
+            args = [ev(a) for a in node.args]
+            if isinstance(f, ast.Name) and f.id in calls:
+                return calls[f.id](*args)
+            if (isinstance(f, ast.Attribute) and isinstance(f.value, ast.Name)
+                    and f.value.id in ("np", "numpy") and f.attr in npcalls):
+                return npcalls[f.attr](*args)
+        raise Unsupported
+
+    for st in tree.body:
+        if not isinstance(st, ast.Assign) or len(st.targets) != 1:
+            continue
+        tgt = st.targets[0]
+        names = ([tgt.id] if isinstance(tgt, ast.Name) else
+                 [e.id for e in tgt.elts if isinstance(e, ast.Name)]
+                 if isinstance(tgt, ast.Tuple) else [])
+        try:
+            v = ev(st.value)
+            if isinstance(tgt, ast.Tuple):
+                v = list(v)
+                if len(v) != len(tgt.elts) or len(names) != len(v):
+                    raise Unsupported
+                env.update(zip(names, v))
+            elif names:
+                env[names[0]] = v
+        except (Unsupported, TypeError, ValueError, ZeroDivisionError, OSError):
+            skipped.extend(names)
+    return {k: env[k] for k in INITIAL_KEYS if k in env}, skipped
+
+
 def main(argv=None):
     from .engine import Propagator
     kw = _parse(sys.argv[1:] if argv is None else argv)
     run_dir = kw.pop("Dir", ".")
-    name = kw.pop("name")
+    # Initial.py's settings, which keywords on the command line override
+    init = {}
+    if os.path.exists(os.path.join(run_dir, "Initial.py")):
+        init, _ = read_initial(run_dir)
+        if not init.get("save_checkpoints", True):
+            init["checkpoints"] = []
+        if "checkpoints" in init:
+            init["checkpoints"] = ",".join(repr(float(c)) for c in np.ravel(init["checkpoints"]))
+    for k, v in init.items():
+        kw.setdefault(k, v)
+    if "name" not in kw:
+        raise ValueError("no run name: give name=<name> or an Initial.py that sets it")
+    name = str(kw.pop("name"))
     f = lambda k, d: float(kw.get(k, d))
     b = lambda k, d: str(kw.get(k, d)).lower() in ("1", "true", "yes")
     hbar = f("hbar", HBAR)
