@@ -30,7 +30,16 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X dense FP64 / FP32 matri
 HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
 
 
-def algorithmic_flops(p):
+def tri_fraction(p):
+    """Share of the Ω entries outside the passive x passive block that are independent
+    when Ω is anti-Hermitian: the pairs (i, j), (j, i) once and the active diagonal."""
+    act = ((p.dGP != 0) | (p.dGM != 0)).any(axis=0)
+    M, ML = p.M, int(act.sum())
+    nonff = M * M - (M - ML) ** 2
+    return (nonff + ML) / (2.0 * nonff)
+
+
+def algorithmic_flops(p, tri=False):
     """Real flops per launch of each GEMM class for the work the path performs.
     Dense complex count per k: n³ (Hσ) + M n² + M n n_c (ψHᵀ, ξσᵀ) + Σ_α n n_α M_α
     (Π) + M² n_c (Ωξ) + Σ_α M M_α n_α + M² n_c (Ω sources), 8 flops per complex MAC
@@ -38,7 +47,9 @@ def algorithmic_flops(p):
     Hamiltonian halves ψHᵀ and Hσ and a sparse one reduces them to its non-zeros;
     the coupling vectors of a mode live on its own lead's orbitals (n_α of them), so
     ξσᵀ, Ωξ, Π and both Ω sources contract over those only; Fermi-block pruning drops
-    the Ω sources on passive rows/columns and the passive x passive block of Ωξ."""
+    the Ω sources on passive rows/columns and the passive x passive block of Ωξ.
+    With Ω held as its block upper triangle (tri) the Ω sources are formed for its
+    independent entries only; Ωξ stays the full product."""
     NK, n, NL, NX, NOT = p.dims
     M = p.M
     Ma = NX * NOT
@@ -66,14 +77,18 @@ def algorithmic_flops(p):
         "gemm_omega": 8.0 * sum(M * int(colM[a]) * na[a] + int(rowP[a]) * M * na[a]
                                 for a in range(NL)),
     }
+    if tri:
+        c["gemm_omega"] *= tri_fraction(p)
     return {k: NK * v for k, v in c.items()}, na, nc
 
 
-def algorithmic_bytes(p, precision):
+def algorithmic_bytes(p, precision, tri=False):
     """HBM bytes per launch of the Ω kernel (average over the seven stages): stage s
     reads y0 and the s-1 earlier stage derivatives with a non-zero coefficient and
     writes k_s (the last stage writes the new state instead), over the Ω elements
-    outside the passive x passive block; plus its two coupled-orbital operands."""
+    outside the passive x passive block (with Ω anti-Hermitian, tri, its independent
+    entries); plus its two coupled-orbital operands.  Ωξ: one Ω slot outside the passive
+    x passive block, ξ_c, and its four split-K partials of Ω ξ_c."""
     NK, n, NL, NX, NOT = p.dims
     M = p.M
     Ma = NX * NOT
@@ -81,11 +96,13 @@ def algorithmic_bytes(p, precision):
     ML = int(act.sum())
     nc = int(np.count_nonzero(np.any(p.xi.reshape(-1, n) != 0, axis=0)))
     elem = 16 if precision == "fp64" else 8
-    slot = NK * (M * M - (M - ML) ** 2) * elem
+    full = NK * (M * M - (M - ML) ** 2) * elem
+    slot = full * (tri_fraction(p) if tri else 1.0)
     reads = [1, 2, 3, 4, 5, 6, 6]   # y0 + k_j with a_sj != 0 (a_72 = 0)
     writes = [1] * 7
     per = slot * (sum(reads) + sum(writes)) / 7.0
-    return {"gemm_omega": per + 2 * NK * M * 2 * nc * elem}
+    return {"gemm_omega": per + 2 * NK * M * 2 * nc * elem,
+            "gemm_omgxi": full + NK * M * nc * elem * (1 + 4)}
 
 
 def measured_traffic(kernel, precision):
@@ -127,7 +144,9 @@ def run_leg(p, precision, K, W, eps, h0):
     el = time.perf_counter() - t_start
     dev = prop.timing()
     lay = prop.layout_info()
-    out = dict(layout=lay, tiles=prop.tiles, steps_per_s=acc / el, attempts_per_s=K / el,
+    out = dict(layout=lay, tiles=prop.tiles,
+               omega_antihermitian_triangle=prop.antihermitian_info()["active"],
+               steps_per_s=acc / el, attempts_per_s=K / el,
                rhs_evals_per_s=7 * K / el, reject_ratio=rej / K, ms_per_attempt=1e3 * el / K,
                device_ms_per_attempt=dev["attempt"][0], accepted=acc, rejected=rej,
                h_last=h, t_reached=t)
@@ -146,8 +165,9 @@ def profile_leg(p, precision, attempts, h0):
         prop.attempt(t, h0, 1.0, force_accept=True)
         t += h0
     tm = prop.timing()
+    tri = prop.antihermitian_info()["active"]
     prop.close()
-    return {k: v[0] for k, v in tm.items() if v[1] > 0}
+    return {k: v[0] for k, v in tm.items() if v[1] > 0}, tri
 
 
 def replay_error(p, eps, h0, n_steps):
@@ -488,18 +508,18 @@ def main():
     replay = replay_error(p, args.eps, args.h0, args.replay_steps)
     free = freerun_h_diff(p, args.eps, args.h0, 20)
 
-    flops, na, nc = algorithmic_flops(p)
     roof = {}
     for prec in ("fp64", "mixed"):
         print(f"# kernel profile {prec}", file=sys.stderr, flush=True)
-        tm = profile_leg(p, prec, 3, args.h0)
+        tm, tri = profile_leg(p, prec, 3, args.h0)
+        flops, na, nc = algorithmic_flops(p, tri)
         gemms = {k: v for k, v in tm.items() if k in flops}
         dom = max(gemms, key=lambda k: gemms[k])
         # kernel-class time is summed over NK-batched launches of one stage; the bound
         # is the roof (matrix-core flops or HBM bytes) the kernel's work takes longest on
         sec = gemms[dom] * 1e-3
         peak_f = PEAK[prec] if dom not in ("gemm_hsig", "gemm_pi") else PEAK["fp64"]
-        nbytes = algorithmic_bytes(p, prec).get(dom)
+        nbytes = algorithmic_bytes(p, prec, tri).get(dom)
         t_f = flops[dom] / (peak_f * 1e12)
         t_b = nbytes / (HBM_PEAK * 1e9) if nbytes else 0.0
         if t_b > t_f:
@@ -511,6 +531,7 @@ def main():
                       "traffic": measured_traffic(dom, prec),
                       "algorithmic_bytes": nbytes, "algorithmic_flops": flops[dom],
                       "mfma_tflops": flops[dom] / sec / 1e12,
+                      "omega_antihermitian_triangle": tri,
                       "avg_ms": gemms[dom], "kernel_ms": tm}
 
     value = legs["mixed"]["steps_per_s"]

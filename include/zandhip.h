@@ -331,6 +331,25 @@ int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int
  * unfused path to rounding.  FP64 problems always run unfused. */
 int zh_set_fusion(zh_problem* p, int mode);
 
+/* Anti-Hermitian Omega: 0 auto (default), 1 off, 2 on.  dOmega_ij/dt (Zandpack/mpi/zand:
+ * 784-809) maps Omega_ij -> -conj(Omega_ji) onto itself when every mode has
+ * dGM_j Ixi_j = -conj(dGP_j xi_j) and Xpm_j = conj(Xpp_j) (the reference's Lorentzian
+ * modes, Ixi = conj(xi) and dGP = dGM imaginary, TimedependentTransport.py:1032; modes
+ * with dGP = dGM = 0, the Fermi poles), whatever psi and H are.  From an anti-Hermitian
+ * Omega (zero, or psinought's Omega0) it stays so, and only its block upper triangle
+ * (64 x 64 blocks of the device mode order) is computed and stored: half of the Omega
+ * stage kernel's bytes and products.  Auto uses it when the constants have the symmetry
+ * (to 1e-13 relative) and the Omega last given (zh_state_put, or zero) is anti-Hermitian
+ * to 1e-13 of its largest entry; zh_rhs uses it per call under the same test of its
+ * input.  zh_state_get and zh_rhs return Omega in full.  On (2) refuses constants or a
+ * state without the symmetry (ZH_EINVAL).  No counterpart in the reference (which
+ * computes every entry). */
+int zh_set_antihermitian(zh_problem* p, int mode);
+
+/* bit 0: Omega is held as its block upper triangle now; bit 1: the constants have the
+ * symmetry; bit 2: the resident Omega is anti-Hermitian. */
+int zh_antihermitian_info(zh_problem* p, int* out);
+
 /* Use hipGraphs for zh_attempt / zh_propagate (1, default) or direct launches (0). */
 int zh_set_graph(zh_problem* p, int enable);
 

@@ -119,6 +119,8 @@ struct EngineBase {
   virtual void set_sparse(int mode) = 0;
   virtual void hamiltonian_info(int* out) = 0;
   virtual void set_fusion(int mode) = 0;
+  virtual void set_antihermitian(int mode) = 0;
+  virtual int antihermitian_info() const = 0;
   bool use_graph = true;
   bool profile = false;
   double kc_ms[KC_N] = {0};
@@ -145,12 +147,16 @@ template <> struct Cands<double> {
                    Tile<32, 32, 8, 2, false, 4>> omega;
   typedef TileList<Tile<64, 48, 16, 4, false, 3>, Tile<64, 32, 16, 2, false, 3>,
                    Tile<64, 32, 32, 2, false, 3>> omgxi;
+  // X ξ_c of an anti-Hermitian X held as its triangle (mirrored first K segment)
+  typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 48, 16, 4, false, 2>,
+                   Tile<64, 32, 8, 2, false, 3>> omgxiT;
 };
 template <> struct Cands<float> {
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 64, 16, 2>> psi;
   typedef TileList<Tile<64, 64, 32, 2>, Tile<64, 64, 16, 2>, Tile<128, 128, 16, 2>> psiR;
   typedef TileList<Tile<64, 64, 8, 2>, Tile<64, 32, 8, 2>, Tile<32, 32, 8, 2>> omega;
   typedef TileList<Tile<64, 48, 16, 4>, Tile<64, 48, 32, 4>, Tile<64, 32, 16, 2>> omgxi;
+  typedef TileList<Tile<64, 48, 16, 4>, Tile<64, 32, 16, 2>, Tile<64, 32, 8, 2>> omgxiT;
 };
 // σ-path products are FP64 in both precision modes
 typedef TileList<Tile<64, 32, 16, 2, false, 3>, Tile<64, 64, 8, 2, false, 3>,
@@ -161,9 +167,9 @@ typedef TileList<Tile<64, 48, 16, 4, false, 3>, Tile<64, 32, 16, 2, false, 3>,
                  Tile<32, 32, 16, 2, false, 4>> CandPi;
 
 struct TuneKey {
-  int prec, nb, mb, cb, hr;
+  int prec, nb, mb, cb, hr, tri;
   bool operator<(const TuneKey& o) const {
-    return std::tie(prec, nb, mb, cb, hr) < std::tie(o.prec, o.nb, o.mb, o.cb, o.hr);
+    return std::tie(prec, nb, mb, cb, hr, tri) < std::tie(o.prec, o.nb, o.mb, o.cb, o.hr, o.tri);
   }
 };
 static std::mutex g_tune_mu;
@@ -302,8 +308,19 @@ struct Engine : EngineBase {
   bool fused_used = false;  // the attempt in flight ran the fused kernel
   bool fusable() const {
     return std::is_same<R, float>::value && nc > 0 && nc <= 16 * kFusedNG && !overlap &&
-           (fusion_req == 2 || (fusion_req == 0 && fused_tuned));
+           !tri_on && (fusion_req == 2 || (fusion_req == 0 && fused_tuned));
   }
+  // Anti-Hermitian Ω.  dΩ_ij/dt = ΔM_j (ψ_i·Ixi_j) + ΔP_i (ξ_i·ψ_j*) + i(u_i - v_j) Ω_ij
+  // maps Ω_ij -> -conj(Ω_ji) onto itself when every mode has ΔM_j Ixi_j = -conj(ΔP_j ξ_j)
+  // and χ⁻_j = conj(χ⁺_j) (Lorentzian modes: Ixi = ξ*, ΔP = ΔM imaginary; Fermi-pole
+  // modes: ΔP = ΔM = 0), whatever ψ and H are.  From an anti-Hermitian start (zero, or
+  // psinought's Ω₀) Ω then stays anti-Hermitian, and only its block upper triangle (64 x
+  // 64 blocks) is computed and stored: half of the Ω stage kernel's bytes and products.
+  int anti_req = 0;         // 0 auto, 1 off, 2 on
+  bool anti_ok = false;     // the constants have the symmetry
+  bool state_anti = false;  // the resident Ω is anti-Hermitian
+  bool tri_on = false;      // the resident Ω is held as its block upper triangle
+  static constexpr double kAntiTol = 1e-13;  // relative, for both checks
   // Chunk layout of the fused kernel for chunk width C.
   OmegaFusedArgs fused_args(int C) const {
     OmegaFusedArgs a{};
@@ -676,7 +693,80 @@ struct Engine : EngineBase {
     install_h(H0);
     HIPCHK(hipStreamSynchronize(stream));
     h_dirty = false;
+    anti_ok = constants_antihermitian((long)NK * M, n, xi, Ixi, dgp, dgm, xpp, xpm);
     have_consts = true;
+    update_tri();
+  }
+
+  // Whether every mode has ΔM_j Ixi_j = -conj(ΔP_j ξ_j) and χ⁻_j = conj(χ⁺_j) (to
+  // kAntiTol of the largest term): the constants keep Ω anti-Hermitian.
+  static bool constants_antihermitian(long nm, int n, const std::complex<double>* xi,
+                                      const std::complex<double>* Ixi,
+                                      const std::complex<double>* dgp,
+                                      const std::complex<double>* dgm,
+                                      const std::complex<double>* xpp,
+                                      const std::complex<double>* xpm) {
+    double scale = 0.0, err = 0.0, xs = 0.0, xe = 0.0;
+    for (long m = 0; m < nm; ++m) {
+      xs = std::max(xs, std::abs(xpp[m]));
+      xe = std::max(xe, std::abs(xpm[m] - std::conj(xpp[m])));
+      if (dgp[m] == 0.0 && dgm[m] == 0.0) continue;
+      for (int o = 0; o < n; ++o) {
+        const std::complex<double> a = dgm[m] * Ixi[m * n + o], b = -std::conj(dgp[m] * xi[m * n + o]);
+        scale = std::max(scale, std::max(std::abs(a), std::abs(b)));
+        err = std::max(err, std::abs(a - b));
+      }
+    }
+    return err <= kAntiTol * scale && xe <= kAntiTol * xs;
+  }
+  // Whether Ω (NK, M, M, reference layout, complex128) is anti-Hermitian to kAntiTol of
+  // its largest entry (64 x 64 blocks, for the cache).
+  bool omega_antihermitian(const double* o) const {
+    typedef std::complex<double> cd;
+    const cd* w = (const cd*)o;
+    double scale = 0.0, err = 0.0;
+    for (int b = 0; b < NK; ++b)
+      for (int i0 = 0; i0 < M; i0 += 64)
+        for (int j0 = i0; j0 < M; j0 += 64)
+          for (int i = i0; i < std::min(M, i0 + 64); ++i)
+            for (int j = std::max(j0, i); j < std::min(M, j0 + 64); ++j) {
+              const cd a = w[((size_t)b * M + i) * M + j], c = w[((size_t)b * M + j) * M + i];
+              scale = std::max(scale, std::max(std::abs(a), std::abs(c)));
+              err = std::max(err, std::abs(a + std::conj(c)));
+            }
+    return err <= kAntiTol * scale;
+  }
+  // Hold Ω as its block upper triangle when the constants, the request and the resident
+  // state allow it; leaving the mode fills the resident lower triangle back in.
+  void update_tri() {
+    const bool want = anti_ok && anti_req != 1 && state_anti;
+    if (want == tri_on) return;
+    if (tri_on && have_state) mirror_lower(omg[iy0]);
+    tri_on = want;
+    y0xi_valid = false;
+    drop_graphs();
+  }
+  void mirror_lower(cs* X) {
+    HIPCHK(hipSetDevice(device));
+    hipLaunchKernelGGL(k_mirror_lower<cs>, dim3(2048), dim3(256), 0, stream, NK, Md, X);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void set_antihermitian(int mode) override {
+    require(mode >= 0 && mode <= 2, "anti-Hermitian Omega: 0 auto, 1 off, 2 on");
+    if (mode == 2) {
+      require(have_consts, "set the constants first", ZH_ESTATE);
+      require(anti_ok, "anti-Hermitian Omega: the constants do not keep Omega anti-Hermitian "
+                       "(every mode needs dGM_j Ixi_j = -conj(dGP_j xi_j) and "
+                       "Xpm_j = conj(Xpp_j))");
+      require(!have_state || state_anti,
+              "anti-Hermitian Omega: the resident Omega is not anti-Hermitian");
+    }
+    anti_req = mode;
+    update_tri();
+  }
+  int antihermitian_info() const override {
+    return (tri_on ? 1 : 0) | (anti_ok ? 2 : 0) | (state_anti ? 4 : 0);
   }
 
   // Mode rows per workgroup of k_psi_ell: those rows live in LDS (at most 32 KiB,
@@ -1194,6 +1284,7 @@ struct Engine : EngineBase {
     HIPCHK(hipStreamSynchronize(stream));
     return ffz;
   }
+  // With Ω held as its block upper triangle the rest is filled in, Ω_ij = -conj(Ω_ji).
   void download_omega(double* dst, const cs* src) {
     std::vector<cs> tmp((size_t)NK * Md * Md);
     HIPCHK(hipMemcpyAsync(tmp.data(), src, sizeof(cs) * tmp.size(), hipMemcpyDeviceToHost, stream));
@@ -1205,10 +1296,16 @@ struct Engine : EngineBase {
         for (int d2 = 0; d2 < Md; ++d2) {
           const int m2 = perm[d2];
           if (m2 < 0) continue;
-          const cs v = tmp[((size_t)b * Md + d1) * Md + d2];
           double* t = dst + 2 * (((size_t)b * M + m1) * M + m2);
-          t[0] = (double)v.x;
-          t[1] = (double)v.y;
+          if (tri_on && (d1 >> 6) > (d2 >> 6)) {
+            const cs v = tmp[((size_t)b * Md + d2) * Md + d1];
+            t[0] = -(double)v.x;
+            t[1] = (double)v.y;
+          } else {
+            const cs v = tmp[((size_t)b * Md + d1) * Md + d2];
+            t[0] = (double)v.x;
+            t[1] = (double)v.y;
+          }
         }
       }
   }
@@ -1232,6 +1329,14 @@ struct Engine : EngineBase {
   void state_put(const double* s, const double* p, const double* o) override {
     require(have_consts, "set the constants first", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
+    // a null Ω is zero, anti-Hermitian
+    const bool anti_in = o ? (anti_ok && omega_antihermitian(o)) : true;
+    require(anti_req != 2 || anti_in,
+            "anti-Hermitian Omega is on and the Omega given is not anti-Hermitian");
+    if (tri_on) {  // the resident state is replaced in full
+      tri_on = false;
+      drop_graphs();
+    }
     upload(sig[iy0], n, s, (long)NK * n, n);
     // a null Ψ or Ω is zero: cleared on the device instead of uploaded
     if (p) {
@@ -1250,6 +1355,8 @@ struct Engine : EngineBase {
     set_ff(ffz);
     have_state = true;
     y0xi_valid = false;
+    state_anti = anti_in;
+    update_tri();
   }
   void state_get(double* s, double* p, double* o) override {
     require(have_state, "no resident state", ZH_ESTATE);
@@ -1505,7 +1612,7 @@ struct Engine : EngineBase {
         // X ξ_c  (Md x Md)(Md x nc) for X = y0 (first stage) or the previous stage's k,
         // then Ω_s ξ_c = y0 ξ_c + Σ_j a_sj k_j ξ_c: Ω is streamed once per stage and no
         // stage-input copy of it is needed
-        omgxi_gemm((s == 0) ? os.y0 : omg[s - 1], W, st);
+        omgxi_run<kRM3>((s == 0) ? os.y0 : omg[s - 1], st);
         WSumArgs<cs> w{};
         w.part = W;
         for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
@@ -1585,13 +1692,24 @@ struct Engine : EngineBase {
   }
 
   // Ω branch: sources, stage input, outer term and (last stage) the error norm.
-  // X ξ_c (Md x Md)(Md x nc) into the split-K partials `part` (X = y0 or a k slot).
-  void omgxi_gemm(const cs* X, cs* part, hipStream_t st) {
+  // X ξ_c (Md x Md)(Md x nc) into the split-K partials W (X = y0 or a k slot).  With Ω
+  // held as its block upper triangle each row block also runs the mirrored part,
+  // -X[J < I, I]^H ξ_c, in the same K loop (KSegTri).
+  template <int RM> void omgxi_run(const cs* X, hipStream_t st) {
     GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
-    EpiStore<cs> e{part, nc, (long)Md * nc, 1.0, 0.0};
+    EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
     KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
-    HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_OMGXI], g, NK, e, st, ks,
+    if (!tri_on) {
+      HIPCHK((launch_sel<R, R, R, false, false, RM>(tsel[TS_OMGXI], g, NK, e, st, ks,
                                                     typename Cands<R>::omgxi())));
+      return;
+    }
+    int bm = 0, bn = 0;
+    tile_dims(tsel[TS_OMGXI], typename Cands<R>::omgxiT(), bm, bn);
+    require(bm == 64, "anti-Hermitian Omega: the X xi_c tiles must be 64 rows");
+    HIPCHK((launch_sel<R, R, R, false, false, RM>(tsel[TS_OMGXI], g, NK, e, st,
+                                                  KSegTri<KSegLead>{ks},
+                                                  typename Cands<R>::omgxiT())));
   }
 
   void stage_omega(const OmegaStage& os, int kslot, hipStream_t st) {
@@ -1608,10 +1726,12 @@ struct Engine : EngineBase {
       // formed, the outer-subtraction term applied and (last stage) the error norm
       // accumulated in the epilogue
       KSegOmega ks{t1, t2, nc, ML, ff ? 1 : 0, (Md + kTile - 1) / kTile};
+      ks.tri = tri_on ? 1 : 0;
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
       g.xcd_order = 0;  // the skipped passive block would unbalance contiguous XCD bands
       auto fill = [&](auto& e) {
+        e.tri = tri_on ? 1 : 0;
         e.out = os.store_k ? omg[kslot] : nullptr;
         e.y0 = os.y0;
         e.nterms = os.nterms;
@@ -1686,7 +1806,7 @@ struct Engine : EngineBase {
   // y0 ξ_c of the resident state into PredY[par] (the first fused attempt from a new
   // state): the Ωξ product of the unfused path on y0.
   void compute_y0xi(int par, hipStream_t st) {
-    omgxi_gemm(omg[iy0], W, st);
+    omgxi_run<kRM3>(omg[iy0], st);
     WSumArgs<cs> w{};
     w.part = W;
     w.pred[0] = PredY[par];
@@ -1812,7 +1932,7 @@ struct Engine : EngineBase {
 
   TuneKey tune_key() const {
     return TuneKey{(int)sizeof(R), pow2_bucket(n), pow2_bucket(Md), pow2_bucket(std::max(nc, 1)),
-                   h_sparse ? 2 : (h_real ? 1 : 0)};
+                   h_sparse ? 2 : (h_real ? 1 : 0), tri_on ? 1 : 0};
   }
 
   // Time each candidate tile of every product on rejected attempts from the
@@ -1873,7 +1993,7 @@ struct Engine : EngineBase {
       // for each chunk width (the Ω candidate index selects it)
       fused_tuned = false;
       if (std::is_same<R, float>::value && fusion_req == 0 && nc > 0 && nc <= 16 * kFusedNG &&
-          !overlap) {
+          !overlap && !tri_on) {
         const double unfused = best[TS_OMEGA] + best[TS_OMGXI];
         double fbest = 1e300;
         int fc = 0;
@@ -2121,6 +2241,7 @@ struct Engine : EngineBase {
     // Ω₀(ψ) of the ψ block of Ap into om (FF tiles skipped when the block is zero)
     auto omega0 = [&](cs* om) {
       KSegOmega ks{t1, t2, nc, ML, ff ? 1 : 0, (Md + kTile - 1) / kTile};
+      ks.tri = tri_on ? 1 : 0;
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
       g.xcd_order = 0;
@@ -2137,11 +2258,7 @@ struct Engine : EngineBase {
       to_ap(xin);
       gather(sp0, sig0, true);
       omega0(omg[0]);
-      GemmArgs<R, R> g{Md, nc, Md, omg[0], Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
-      EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
-      KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
-      HIPCHK((launch_sel<R, R, R, false, false, 0>(tsel[TS_OMGXI], g, NK, e, stream, ks,
-                                                   typename Cands<R>::omgxi())));
+      omgxi_run<0>(omg[0], stream);
       WSumArgs<cs> w{};
       w.part = W;
       for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
@@ -2287,14 +2404,19 @@ struct Engine : EngineBase {
     upload(sig[9], n, s, (long)NK * n, n);
     upload_psi(Ap, n + nc, p, psi[1]);  // k slot 1 is free outside an attempt
     upload_omega(omg[9], o);
-    // the input's passive block may be non-zero: evaluate it in full
-    const bool ff_keep = ff;
+    // the input's passive block may be non-zero: evaluate it in full; its block upper
+    // triangle stands for all of it only when the input is anti-Hermitian
+    const bool ff_keep = ff, tri_keep = tri_on;
+    struct Keep {
+      bool& f; bool& t; bool fv, tv;
+      ~Keep() { f = fv; t = tv; }
+    } keep{ff, tri_on, ff_keep, tri_keep};
     ff = false;
+    tri_on = anti_ok && anti_req != 1 && omega_antihermitian(o);
     OmegaStage os;
     os.y0 = omg[9];
     rhs_stage(0, sig[9], os, 0, J != nullptr, false);
     HIPCHK(hipStreamSynchronize(stream));
-    ff = ff_keep;
     ff_dirty = true;
     if (ds) download(ds, sig[0], (size_t)NK * n * n);
     if (dp) download_psi(dp, psi[0]);
@@ -3050,6 +3172,20 @@ int zh_set_fusion(zh_problem* p, int mode) {
   return zh_try([&] {
     require(p, "null argument");
     p->eng->set_fusion(mode);
+  });
+}
+
+int zh_set_antihermitian(zh_problem* p, int mode) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_antihermitian(mode);
+  });
+}
+
+int zh_antihermitian_info(zh_problem* p, int* out) {
+  return zh_try([&] {
+    require(p && out, "null argument");
+    *out = p->eng->antihermitian_info();
   });
 }
 

@@ -79,13 +79,24 @@ struct GemmArgs {
 constexpr int kThreads = 256;
 
 // K-range selection per output tile (first row m0, first column n0, bm x bn): up
-// to two half-open segments [lo0, hi0), [lo1, hi1) of the K axis (nseg of them);
-// tiles whose operands are known to vanish are skipped, and a tile with no segment
-// gets acc = 0 (its epilogue still runs).  Returned by value and built with
-// selects, so that the ranges stay in registers.
+// to three half-open segments [lo0, hi0), [lo1, hi1), [lo2, hi2) of the K axis (the
+// first nseg of them; an empty one is lo = hi); tiles whose operands are known to
+// vanish are skipped, and a tile with no segment gets acc = 0 (its epilogue still
+// runs).  Returned by value and built with selects, so that the ranges stay in
+// registers.
 struct KRange {
   int lo0, hi0, lo1, hi1, nseg;
+  int lo2, hi2;
 };
+
+// Selections with static constexpr bool kMirrorA = true read A in their first segment
+// as the mirror of a square row-major A: op(A)(m, k) = -conj(A[k*lda + m]) there, and
+// A[m*lda + k] in the others (the two halves of an anti-Hermitian matrix held as one
+// triangle).  Complex A, not transposed.
+template <typename KS, typename = void> struct MirrorA : std::false_type {};
+template <typename KS>
+struct MirrorA<KS, std::void_t<decltype(KS::kMirrorA)>>
+    : std::integral_constant<bool, KS::kMirrorA> {};
 struct KFull {
   __device__ inline KRange get(int, int, int, int, int K) const { return KRange{0, K, 0, 0, 1}; }
   __device__ inline bool skip(int, int, int, int) const { return false; }
@@ -134,6 +145,8 @@ template <typename T, typename RA, typename RB, bool TRANS_A, bool TRANS_B,
 __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g, Epi epi,
                                                               KSeg ks) {
   constexpr bool RA_ = RMODE == 1, RB_ = RMODE == 2, G3 = RMODE == 3;
+  constexpr bool MIR = MirrorA<KSeg>::value;
+  static_assert(!MIR || (!TRANS_A && !RA_), "mirrored A: complex, not transposed");
   constexpr bool SWP = std::is_same<T, float>::value && HasRow4<Epi>::value;
   constexpr int NBUF = DB ? 2 : 1;
   constexpr int WN = 4 / WM;
@@ -203,19 +216,35 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
 
   ca_t ra[LA];
   cb_t rb[LB];
+  bool mir = false;  // the K tile in the registers reads A mirrored (MIR, first segment)
   auto load_tiles = [&](int k0, int khi) {
 #pragma unroll
     for (int it = 0; it < LA; ++it) {
       const int e = tid + it * kThreads;
-      int m, k;
-      if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
-      const int gm = m0 + m, gk = k0 + k;
-      ca_t v; v.x = 0; v.y = 0;
-      if (gm < g.M && gk < khi) {
-        if (RA_) v.x = TRANS_A ? Ar[(long)gk * g.ldar + gm] : Ar[(long)gm * g.ldar + gk];
-        else v = TRANS_A ? A[(long)gk * g.lda + gm] : A[(long)gm * g.lda + gk];
+      if constexpr (MIR) {
+        // one address form for both segment kinds: element (R0 + r, C0 + c) of A, where
+        // the stored segments read (m0 + m, k0 + k) and the mirrored one (k0 + k, m0 + m)
+        const bool q = mir;
+        const int r = q ? e / BM : e / BK, c = q ? e % BM : e % BK;
+        const int R0 = q ? k0 : m0, C0 = q ? m0 : k0;
+        const int limR = q ? khi : g.M, limC = q ? g.M : khi;
+        ca_t v; v.x = 0; v.y = 0;
+        if (R0 + r < limR && C0 + c < limC) {
+          v = A[(long)(R0 + r) * g.lda + (C0 + c)];
+          if (q) v.x = -v.x;  // -conj
+        }
+        ra[it] = v;
+      } else {
+        int m, k;
+        if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
+        const int gm = m0 + m, gk = k0 + k;
+        ca_t v; v.x = 0; v.y = 0;
+        if (gm < g.M && gk < khi) {
+          if (RA_) v.x = TRANS_A ? Ar[(long)gk * g.ldar + gm] : Ar[(long)gm * g.ldar + gk];
+          else v = TRANS_A ? A[(long)gk * g.lda + gm] : A[(long)gm * g.lda + gk];
+        }
+        ra[it] = v;
       }
-      ra[it] = v;
     }
 #pragma unroll
     for (int it = 0; it < LB; ++it) {
@@ -236,7 +265,7 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
     for (int it = 0; it < LA; ++it) {
       const int e = tid + it * kThreads;
       int m, k;
-      if (!TRANS_A) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
+      if (!TRANS_A && !(MIR && mir)) { m = e / BK; k = e % BK; } else { k = e / BM; m = e % BM; }
       sAr[buf][k][m] = (T)ra[it].x;
       if (!RA_) sAi[buf][k][m] = (T)ra[it].y;
     }
@@ -253,15 +282,30 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
   // K tiles of this tile's segments, split evenly over the splitk parts
   const KRange kr = ks.get(m0, n0, BM, BN, g.K);
   const int lo0 = kr.lo0, hi0 = kr.hi0, lo1 = kr.lo1, hi1 = kr.hi1, nseg = kr.nseg;
+  const int lo2 = MIR ? kr.lo2 : 0, hi2 = MIR ? kr.hi2 : 0;  // third segment: MIR only
   const int nt0 = nseg > 0 ? (hi0 - lo0 + BK - 1) / BK : 0;
   const int nt1 = nseg > 1 ? (hi1 - lo1 + BK - 1) / BK : 0;
-  const int ktot = nt0 + nt1;
+  const int nt2 = (MIR && nseg > 2) ? (hi2 - lo2 + BK - 1) / BK : 0;
+  const int ktot = nt0 + nt1 + nt2;
   const int kper = (ktot + g.splitk - 1) / g.splitk;
   const int kt0 = part * kper;
   const int ktiles = max(0, min(ktot, kt0 + kper) - kt0);
   auto tile_load = [&](int t) {
-    const bool first = t < nt0;
-    load_tiles(first ? lo0 + t * BK : lo1 + (t - nt0) * BK, first ? hi0 : hi1);
+    if constexpr (MIR) {
+      // a branch on the uniform t and a two-way select (a three-way select here made the
+      // compiler index a private-memory table)
+      mir = t < nt0;
+      if (t < nt0) {
+        load_tiles(lo0 + t * BK, hi0);
+      } else {
+        const int u = t - nt0;
+        const bool a = u < nt1;
+        load_tiles(a ? lo1 + u * BK : lo2 + (u - nt1) * BK, a ? hi1 : hi2);
+      }
+    } else {
+      const bool first = t < nt0;
+      load_tiles(first ? lo0 + t * BK : lo1 + (t - nt0) * BK, first ? hi0 : hi1);
+    }
   };
   const int lr = lane & 15, lk = lane >> 4;
   if (ktiles > 0) {

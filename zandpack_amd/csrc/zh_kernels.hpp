@@ -692,6 +692,12 @@ struct EpiOmegaRK {
   double ct[kMaxTerms]; double ct_self;   // error weights of k[] and of k_s (TE)
   double* te_part;                        // one partial per workgroup (TE)
   double rtol, atol; int scaled;          // scaled norm (TE): weights from |y0|, |Ω_7|
+  int tri = 0;  // only the block upper triangle is computed: entries outside the diagonal
+                // 64 x 64 blocks stand for two in the error norm
+
+  __device__ inline double tri_weight(int row, int col) const {
+    return (tri && (row >> 6) != (col >> 6)) ? 2.0 : 1.0;
+  }
 
   // Four outputs of one column (rows[r] < 0: outside): every slot load first, then
   // the stores.
@@ -735,8 +741,9 @@ struct EpiOmegaRK {
       if (out) st_nt(out + off[r], to_d2(ks));
       if constexpr (TE) {
         const double xr = er[r] + ct_self * (double)ks.x, xi = ei[r] + ct_self * (double)ks.y;
-        acc += scaled ? (xr * xr + xi * xi) * te_weight(atol, rtol, yr0[r], yi0[r], sr[r], si[r])
-                      : xr * xr + xi * xi;
+        acc += tri_weight(rows[r], col) *
+               (scaled ? (xr * xr + xi * xi) * te_weight(atol, rtol, yr0[r], yi0[r], sr[r], si[r])
+                       : xr * xr + xi * xi);
       }
     }
   }
@@ -794,6 +801,7 @@ struct EpiOmegaRK {
         }
       }
     const double2 uu = u[(long)b * M + row];
+    const double tw = tri_weight(row, col0);  // col0 .. col0 + 3 lie in one 64-block
     double kr[4], ki[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -807,9 +815,9 @@ struct EpiOmegaRK {
       if constexpr (TE) {
         if (r < ncol) {
           const double xr = er[r] + ct_self * kr[r], xi = ei[r] + ct_self * ki[r];
-          acc += scaled ? (xr * xr + xi * xi) *
-                              te_weight(atol, rtol, yr0[r], yi0[r], sr[r], si[r])
-                        : xr * xr + xi * xi;
+          acc += tw * (scaled ? (xr * xr + xi * xi) *
+                                    te_weight(atol, rtol, yr0[r], yi0[r], sr[r], si[r])
+                              : xr * xr + xi * xi);
         }
       }
     }
@@ -934,6 +942,26 @@ __global__ __launch_bounds__(256) void k_sum_wpart(const C* part, int parts, int
   }
 }
 
+// Fill the strictly lower block triangle (64 x 64 blocks) of an anti-Hermitian X held
+// as its block upper triangle: X_ij = -conj(X_ji) for block(i) > block(j).  Reads only
+// the upper part, writes only the lower.
+template <typename C>
+__global__ __launch_bounds__(256) void k_mirror_lower(int NK, int M, C* X) {
+  const long mm = (long)M * M, total = (long)NK * mm;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long b = idx / mm, rc = idx - b * mm;
+    const int i = (int)(rc / M), j = (int)(rc - (long)i * M);
+    if ((i >> 6) > (j >> 6)) {
+      const C v = X[b * mm + (long)j * M + i];
+      C w;
+      w.x = -v.x;
+      w.y = v.y;
+      X[idx] = w;
+    }
+  }
+}
+
 // ------------------------------------------------------------ K segments
 // Device mode order: active modes (ΔP or ΔM non-zero) in [0, ML), ML a multiple
 // of the Ω tiles, passive (Fermi) modes after.  For the Ω sources
@@ -956,6 +984,10 @@ struct KSegOmega {
   const int2* t1;  // per 64 columns: term1 K range [x, y)
   const int2* t2;  // per 64 rows: term2 K range [nc + x, nc + y) (the row leads' orbitals)
   int nc, ML, ff, nt;  // nt: 64-wide tiles in the table
+  // tri: Ω is anti-Hermitian and only its block upper triangle (64 x 64 blocks, the
+  // diagonal blocks in full) is computed; tiles (at most 64 x 64, aligned) below it
+  // are skipped
+  int tri = 0;
   __device__ inline KRange get(int m0, int n0, int bm, int bn, int) const {
     const int2 r = tile_hull(t1, n0 / 64, min(nt - 1, (n0 + bn - 1) / 64));
     const int2 q = tile_hull(t2, m0 / 64, min(nt - 1, (m0 + bm - 1) / 64));
@@ -965,7 +997,39 @@ struct KSegOmega {
                   (int)s1 + (int)s2};
   }
   __device__ inline bool skip(int m0, int n0, int, int) const {
-    return ff && m0 >= ML && n0 >= ML;
+    return (ff && m0 >= ML && n0 >= ML) || (tri && (m0 >> 6) > (n0 >> 6));
+  }
+};
+
+// Products X ξ_c of an anti-Hermitian X held as its block upper triangle (64 x 64
+// blocks): row block I of X ξ_c is -X[J < I, I]^H ξ_c[J < I] + X[I, J >= I] ξ_c[J >= I].
+// One K loop per output tile runs both: the first segment (K < m0) reads A mirrored,
+// -conj(X[k][m]) from the column strip above the row block, then up to two segments (K
+// >= m0) read the row strip.  The K pieces of the inner selection are cut at m0; the
+// pieces below m0 (at most two, ordered) become their hull, where the gap's modes do not
+// couple the tile's columns and add zero.  Output tiles are 64 rows, aligned, so m0 is
+// the first column of the row's diagonal block.
+template <typename KS>
+struct KSegTri {
+  static constexpr bool kMirrorA = true;
+  KS in;
+  __device__ inline KRange get(int m0, int n0, int bm, int bn, int K) const {
+    const KRange r = in.get(m0, n0, bm, bn, K);
+    const bool p0 = r.nseg > 0 && r.hi0 > r.lo0, p1 = r.nseg > 1 && r.hi1 > r.lo1;
+    // below m0
+    const bool b0 = p0 && r.lo0 < m0, b1 = p1 && r.lo1 < m0;
+    const int mlo = b0 ? r.lo0 : (b1 ? r.lo1 : 0);
+    const int mhi = b1 ? min(r.hi1, m0) : (b0 ? min(r.hi0, m0) : 0);
+    // from m0 on
+    const int a0 = max(r.lo0, m0), a1 = max(r.lo1, m0);
+    const bool s0 = p0 && r.hi0 > a0, s1 = p1 && r.hi1 > a1;
+    KRange o{mlo, mhi, s0 ? a0 : 0, s0 ? r.hi0 : 0, 3};
+    o.lo2 = s1 ? a1 : 0;
+    o.hi2 = s1 ? r.hi1 : 0;
+    return o;
+  }
+  __device__ inline bool skip(int m0, int n0, int bm, int bn) const {
+    return in.skip(m0, n0, bm, bn);
   }
 };
 

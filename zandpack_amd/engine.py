@@ -23,7 +23,7 @@ SPARSE = {"auto": 0, "dense": 1, "sparse": 2}
 class Propagator:
     def __init__(self, problem: AMEProblem, precision: str = "fp64", device: int = 0,
                  graph: bool = True, state=None, layout: str = "auto", autotune: bool = False,
-                 hproducts: str = "auto"):
+                 hproducts: str = "auto", antihermitian: str = "auto"):
         self.lib = N.lib()
         self.problem = problem
         self.precision = precision
@@ -50,10 +50,14 @@ class Propagator:
         if p.Sinv is not None:
             self.set_overlap(p.Sinv)
         N.check(self.lib.zh_set_graph(self.h, 1 if graph else 0))
+        if antihermitian != "on":  # "on" is checked against the state, which comes next
+            self.set_antihermitian(antihermitian)
         if state is not None:
             self.put_state(*state)
         else:
             self.put_state(p.sigma0)
+        if antihermitian == "on":
+            self.set_antihermitian("on")
         self.tiles = self.autotune() if autotune else None
 
     # --------------------------------------------------------------- set-up
@@ -173,6 +177,24 @@ class Propagator:
         if mode is True or mode is False:
             mode = "on" if mode else "off"
         N.check(self.lib.zh_set_fusion(self.h, {"auto": 0, "off": 1, "on": 2}[mode]))
+
+    def set_antihermitian(self, mode="auto"):
+        """Ω held as its block upper triangle (half the Ω stage kernel's work) when the
+        constants keep Ω anti-Hermitian (ΔM_j Ixi_j = -conj(ΔP_j ξ_j), χ⁻ = conj χ⁺) and
+        the resident Ω is anti-Hermitian: "auto" (when both hold), "on" (refused
+        otherwise) or "off"; same results to rounding."""
+        if mode is True or mode is False:
+            mode = "on" if mode else "off"
+        N.check(self.lib.zh_set_antihermitian(self.h, {"auto": 0, "off": 1, "on": 2}[mode]))
+
+    def antihermitian_info(self):
+        """{"active", "constants", "state"}: whether Ω is held as its block upper triangle
+        now, whether the constants have the symmetry, whether the resident Ω is
+        anti-Hermitian."""
+        v = C.c_int()
+        N.check(self.lib.zh_antihermitian_info(self.h, C.byref(v)))
+        return {"active": bool(v.value & 1), "constants": bool(v.value & 2),
+                "state": bool(v.value & 4)}
 
     def set_graph(self, on: bool):
         N.check(self.lib.zh_set_graph(self.h, 1 if on else 0))
