@@ -508,12 +508,12 @@ struct Engine : EngineBase {
     W = dalloc<cs>(mc * kWParts);
     for (auto& p : Pred) p = dalloc<cs>(mc);
     Wst = dalloc<cs>(mc);
+    for (auto& q : PredY) q = dalloc<cs>(mc);
     if (std::is_same<R, float>::value && nc > 0 && nc <= 16 * kFusedNG) {
       // the narrowest chunk gives the most partials
       const int C = kFusedChunk[1];
       wpart_max = (Md + C - 1) / C + 1;
       for (auto& w : Wpart) w = dalloc<cs>(mc * wpart_max);
-      for (auto& q : PredY) q = dalloc<cs>(mc);
     }
     // room for the error-norm partials of the finest Ω tile candidate (32 x 32)
     const long t32 = (Md + 31) / 32;
@@ -1482,6 +1482,9 @@ struct Engine : EngineBase {
     // writes, and the slot parity of y0 (PredY[par] = y0 ξ_c)
     cs* wpart = nullptr;
     int par = 0;
+    // attempts: y0 ξ_c is resident in PredY[par] and the last stage leaves y_new ξ_c in
+    // PredY[1 - par]; otherwise (a single evaluation) the first stage forms y0 ξ_c
+    bool y0xi_resident = false;
   };
 
   // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
@@ -1607,20 +1610,29 @@ struct Engine : EngineBase {
           HIPCHK(hipGetLastError());
         });
       }
+    } else if (nc > 0 && s == 0 && os.y0xi_resident) {
+      // y0 ξ_c is resident (PredY[par]): the previous attempt's last stage left it (y_new
+      // ξ_c after an accepted step), or it was formed before the attempt
+      Wc = PredY[os.par];
     } else if (nc > 0) {
       timed(KC_GEMM_OMGXI, [&] {
-        // X ξ_c  (Md x Md)(Md x nc) for X = y0 (first stage) or the previous stage's k,
-        // then Ω_s ξ_c = y0 ξ_c + Σ_j a_sj k_j ξ_c: Ω is streamed once per stage and no
-        // stage-input copy of it is needed
+        // X ξ_c  (Md x Md)(Md x nc) for X = y0 (a single evaluation's first stage) or the
+        // previous stage's k, then
+        // Ω_s ξ_c = y0 ξ_c + Σ_j a_sj k_j ξ_c: Ω is streamed once per stage and no
+        // stage-input copy of it is needed.  The last stage's Ω_7 ξ_c is y_new ξ_c and
+        // goes to PredY[1 - par], the next attempt's y0 ξ_c if this one is accepted.
+        const bool res = os.y0xi_resident;
         omgxi_run<kRM3>((s == 0) ? os.y0 : omg[s - 1], st);
         WSumArgs<cs> w{};
         w.part = W;
         for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
-        w.W = Wst;
+        if (res) w.pred[0] = PredY[os.par];
+        w.W = (res && s == 6) ? PredY[1 - os.par] : Wst;
         for (int j = 0; j < 6; ++j) w.a[j] = (j < s) ? tab.a[s][j] : 0.0;
         w.s = s; w.parts = kWParts; w.NK = NK; w.M = Md; w.nc = nc;
         hipLaunchKernelGGL(k_wsum<cs>, dim3(1024), dim3(256), 0, st, w);
         HIPCHK(hipGetLastError());
+        Wc = w.W;
       });
     }
     const bool ovl = overlap && nc > 0;
@@ -1882,10 +1894,9 @@ struct Engine : EngineBase {
         for (int j = 0; j < 6; ++j) covered += (tab.ct[j] == 0.0 || tab.a[6][j] != 0.0);
         require(covered == 6, "error weights need a stage-7 term the tableau does not carry");
       }
-      if (fused_used) {
-        os.wpart = Wpart[s % 2];
-        os.par = (iy0 == 7) ? 0 : 1;
-      }
+      os.par = (iy0 == 7) ? 0 : 1;
+      os.y0xi_resident = true;
+      if (fused_used) os.wpart = Wpart[s % 2];
       rhs_stage(s, sin_, os, s, s == 0, true);
     }
     if (fused_used) {
@@ -2044,7 +2055,7 @@ struct Engine : EngineBase {
     const bool fz = fusable();
     if (fz != fused_used) drop_graphs();
     fused_used = fz;
-    if (fused_used && !y0xi_valid) {
+    if (nc > 0 && !y0xi_valid) {
       compute_y0xi(par, stream);
       y0xi_valid = true;
     }
@@ -2103,7 +2114,7 @@ struct Engine : EngineBase {
     }
     if (info->accepted) {
       std::swap(iy0, iyn);
-      y0xi_valid = fused_used;  // the last fused stage left y_new ξ_c in the new y0's slot
+      y0xi_valid = true;  // the last stage left y_new ξ_c in the new y0's PredY slot
     }
   }
 
