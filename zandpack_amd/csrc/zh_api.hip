@@ -1852,8 +1852,17 @@ struct Engine : EngineBase {
       MapDense mp{NK * Md, n, n, n, n};
       hipLaunchKernelGGL((k_te<double2, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, ts,
                          ms, te_part);
-      hipLaunchKernelGGL((k_te<cs, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, tp, mp,
-                         te_part + kTEBlocks);
+      bool done = false;
+      if constexpr (sizeof(cs) == 8) {
+        if (mp.pairs()) {
+          hipLaunchKernelGGL((k_te_pairs<cs, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream,
+                             tp, mp, te_part + kTEBlocks);
+          done = true;
+        }
+      }
+      if (!done)
+        hipLaunchKernelGGL((k_te<cs, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, tp,
+                           mp, te_part + kTEBlocks);
     });
     HIPCHK(hipGetLastError());
     timed(KC_CTRL, [&] {

@@ -234,6 +234,10 @@ __global__ void k_overlap_model(int NK, int n, int nc, int nt, const double2* SH
 struct MapDense {  // rows x cols block, source row stride lds, destination strides ld1 / ld2
   int rows, cols; long lds, ld1, ld2;
   __host__ __device__ inline int nrows() const { return rows; }
+  // every row's offsets and length even: complex64 elements go in 16-byte pairs
+  __host__ __device__ inline bool pairs() const {
+    return cols % 2 == 0 && lds % 2 == 0 && ld1 % 2 == 0 && ld2 % 2 == 0;
+  }
   __device__ inline int row(int r, long& src, long& d1, long& d2) const {
     src = r * lds; d1 = r * ld1; d2 = r * ld2;
     return cols;
@@ -277,8 +281,52 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<C> g, Map mp) {
   }
 }
 
+// Complex64 elements two per thread (16-byte loads and stores), for maps whose rows
+// have even offsets and lengths (slot bases are allocation-aligned).
+template <typename C, typename Map, int NT>
+__global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp) {
+  static_assert(sizeof(C) == 8, "complex64 pairs");
+  const int nr = mp.nrows();
+  for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+    long src, o1, o2;
+    const int cols = mp.row(r, src, o1, o2);
+    for (int c = 2 * threadIdx.x; c < cols; c += 2 * blockDim.x) {
+      v4f kv[NT + 1];
+      kv[0] = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(g.y0 + src + c));
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        kv[j + 1] = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(g.k[j] + src + c));
+      double v0r = kv[0].x, v0i = kv[0].y, v1r = kv[0].z, v1i = kv[0].w;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        v0r += g.a[j] * (double)kv[j + 1].x;
+        v0i += g.a[j] * (double)kv[j + 1].y;
+        v1r += g.a[j] * (double)kv[j + 1].z;
+        v1i += g.a[j] * (double)kv[j + 1].w;
+      }
+      const v4f out = {(float)v0r, (float)v0i, (float)v1r, (float)v1i};
+      *reinterpret_cast<v4f*>(g.d1 + o1 + c) = out;
+      if (g.d2) *reinterpret_cast<v4f*>(g.d2 + o2 + c) = out;
+    }
+  }
+}
+
 template <typename C, typename Map>
 inline void launch_combine(const CombineArgs<C>& g, const Map& mp, int blocks, hipStream_t s) {
+  if constexpr (sizeof(C) == 8) {
+    if (mp.pairs()) {
+      switch (g.nterms) {
+#define ZH_COMBINE_CASE(N)                                                                   \
+  case N:                                                                                    \
+    hipLaunchKernelGGL((k_combine_pairs<C, Map, N>), dim3(blocks), dim3(256), 0, s, g, mp); break;
+        ZH_COMBINE_CASE(0) ZH_COMBINE_CASE(1) ZH_COMBINE_CASE(2) ZH_COMBINE_CASE(3)
+        ZH_COMBINE_CASE(4) ZH_COMBINE_CASE(5) ZH_COMBINE_CASE(6)
+#undef ZH_COMBINE_CASE
+        default: break;
+      }
+      return;
+    }
+  }
   switch (g.nterms) {
 #define ZH_COMBINE_CASE(N)                                                                   \
   case N:                                                                                    \
@@ -509,6 +557,52 @@ __global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial
                te_weight(g.atol, g.rtol, (double)a.x, (double)a.y, (double)b.x, (double)b.y);
       } else {
         acc += vr * vr + vi * vi;
+      }
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// k_te for complex64 slots two elements per thread (16-byte loads), for maps whose rows
+// have even offsets and lengths.
+template <typename C, typename Map, int NW>
+__global__ __launch_bounds__(256) void k_te_pairs(TEArgs<C> g, Map mp, double* partial) {
+  static_assert(sizeof(C) == 8, "complex64 pairs");
+  double acc = 0.0;
+  const int nr = mp.nrows();
+  for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+    long src, o1, o2;
+    const int cols = mp.row(r, src, o1, o2);
+    for (int c = 2 * threadIdx.x; c < cols; c += 2 * blockDim.x) {
+      v4f kv[NW];
+#pragma unroll
+      for (int j = 0; j < NW; ++j)
+        kv[j] = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(g.k[j] + src + c));
+      double v0r = 0.0, v0i = 0.0, v1r = 0.0, v1i = 0.0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        v0r += g.ct[j] * (double)kv[j].x;
+        v0i += g.ct[j] * (double)kv[j].y;
+        v1r += g.ct[j] * (double)kv[j].z;
+        v1i += g.ct[j] * (double)kv[j].w;
+      }
+      if (g.scaled) {
+        const C a0 = g.y0[src + c], b0 = g.y1[src + c];
+        const C a1 = g.y0[src + c + 1], b1 = g.y1[src + c + 1];
+        acc += (v0r * v0r + v0i * v0i) *
+               te_weight(g.atol, g.rtol, (double)a0.x, (double)a0.y, (double)b0.x, (double)b0.y);
+        acc += (v1r * v1r + v1i * v1i) *
+               te_weight(g.atol, g.rtol, (double)a1.x, (double)a1.y, (double)b1.x, (double)b1.y);
+      } else {
+        acc += v0r * v0r + v0i * v0i;
+        acc += v1r * v1r + v1i * v1i;
       }
     }
   }
