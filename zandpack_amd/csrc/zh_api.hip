@@ -78,10 +78,10 @@ static Tableau dop54() {
 
 // Kernel classes timed with HIP events when profiling is on.
 enum KClass { KC_PARAMS, KC_COMBINE, KC_GATHER, KC_GEMM_PI, KC_CURRENTS, KC_GEMM_HSIG, KC_SIGSYM,
-              KC_GEMM_OMGXI, KC_GEMM_PSI, KC_GEMM_OMEGA, KC_TE, KC_CTRL, KC_N };
+              KC_GEMM_OMGXI, KC_GEMM_PSI, KC_GEMM_OMEGA, KC_TE, KC_CTRL, KC_WSUM, KC_N };
 static const char* kc_names[KC_N] = {"stage_params", "combine",   "gather",     "gemm_pi",
                                       "currents",     "gemm_hsig", "sigma_sym",  "gemm_omgxi",
-                                      "gemm_psi",     "gemm_omega", "te_reduce", "controller"};
+                                      "gemm_psi",     "gemm_omega", "te_reduce", "controller", "wsum"};
 
 struct EngineBase {
   virtual ~EngineBase() {}
@@ -1595,7 +1595,7 @@ struct Engine : EngineBase {
       if (s == 0) {
         Wc = PredY[os.par];
       } else {
-        timed(KC_GEMM_OMGXI, [&] {
+        timed(KC_WSUM, [&] {
           const OmegaFusedArgs fa = fused_args(kFusedChunk[tsel[TS_OMEGA]]);
           WSumArgs<cs> w{};
           w.part = Wpart[(s - 1) % 2];
@@ -1615,25 +1615,25 @@ struct Engine : EngineBase {
       // ξ_c after an accepted step), or it was formed before the attempt
       Wc = PredY[os.par];
     } else if (nc > 0) {
-      timed(KC_GEMM_OMGXI, [&] {
-        // X ξ_c  (Md x Md)(Md x nc) for X = y0 (a single evaluation's first stage) or the
-        // previous stage's k, then
-        // Ω_s ξ_c = y0 ξ_c + Σ_j a_sj k_j ξ_c: Ω is streamed once per stage and no
-        // stage-input copy of it is needed.  The last stage's Ω_7 ξ_c is y_new ξ_c and
-        // goes to PredY[1 - par], the next attempt's y0 ξ_c if this one is accepted.
-        const bool res = os.y0xi_resident;
-        omgxi_run<kRM3>((s == 0) ? os.y0 : omg[s - 1], st);
-        WSumArgs<cs> w{};
-        w.part = W;
-        for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
-        if (res) w.pred[0] = PredY[os.par];
-        w.W = (res && s == 6) ? PredY[1 - os.par] : Wst;
-        for (int j = 0; j < 6; ++j) w.a[j] = (j < s) ? tab.a[s][j] : 0.0;
-        w.s = s; w.parts = kWParts; w.NK = NK; w.M = Md; w.nc = nc;
+      // X ξ_c  (Md x Md)(Md x nc) for X = y0 (a single evaluation's first stage) or the
+      // previous stage's k, then Ω_s ξ_c = y0 ξ_c + Σ_j a_sj k_j ξ_c: Ω is streamed once
+      // per stage and no stage-input copy of it is needed.  The last stage's Ω_7 ξ_c is
+      // y_new ξ_c and goes to PredY[1 - par], the next attempt's y0 ξ_c if this one is
+      // accepted.
+      const bool res = os.y0xi_resident;
+      timed(KC_GEMM_OMGXI, [&] { omgxi_run<kRM3>((s == 0) ? os.y0 : omg[s - 1], st); });
+      WSumArgs<cs> w{};
+      w.part = W;
+      for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
+      if (res) w.pred[0] = PredY[os.par];
+      w.W = (res && s == 6) ? PredY[1 - os.par] : Wst;
+      for (int j = 0; j < 6; ++j) w.a[j] = (j < s) ? tab.a[s][j] : 0.0;
+      w.s = s; w.parts = kWParts; w.NK = NK; w.M = Md; w.nc = nc;
+      timed(KC_WSUM, [&] {
         hipLaunchKernelGGL(k_wsum<cs>, dim3(1024), dim3(256), 0, st, w);
         HIPCHK(hipGetLastError());
-        Wc = w.W;
       });
+      Wc = w.W;
     }
     const bool ovl = overlap && nc > 0;
     if (ovl) {
@@ -1912,7 +1912,7 @@ struct Engine : EngineBase {
       // y_new ξ_c from the last stage's partials: the next attempt's y0 ξ_c if accepted
       const OmegaFusedArgs fa = fused_args(kFusedChunk[tsel[TS_OMEGA]]);
       const int par = (iy0 == 7) ? 0 : 1;
-      timed(KC_GEMM_OMGXI, [&] {
+      timed(KC_WSUM, [&] {
         hipLaunchKernelGGL(k_sum_wpart<cs>, dim3(1024), dim3(256), 0, stream,
                            (const cs*)Wpart[0], fa.SA + fa.SP, ff ? fa.SA : fa.SA + fa.SP, ML, NK,
                            Md, nc, PredY[1 - par]);
@@ -2014,17 +2014,21 @@ struct Engine : EngineBase {
       fused_tuned = false;
       if (std::is_same<R, float>::value && fusion_req == 0 && nc > 0 && nc <= 16 * kFusedNG &&
           !overlap && !tri_on) {
-        const double unfused = best[TS_OMEGA] + best[TS_OMGXI];
+        // per-attempt time of the Ω kernels, the Ωξ products and their sums, on the
+        // chosen tiles unfused, then fused for each chunk width
+        auto omega_side = [&]() {
+          attempt(&info, nullptr);
+          for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
+          for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
+          return 0.5 * (kc_ms[KC_GEMM_OMEGA] + kc_ms[KC_GEMM_OMGXI] + kc_ms[KC_WSUM]);
+        };
+        const double unfused = omega_side();
         double fbest = 1e300;
         int fc = 0;
         fused_tuned = true;
         for (int c = 0; c < kCands; ++c) {
           tsel[TS_OMEGA] = c;
-          attempt(&info, nullptr);
-          for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
-          for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
-          const double t = (kc_count[KC_GEMM_OMEGA] ? kc_ms[KC_GEMM_OMEGA] / kc_count[KC_GEMM_OMEGA] : 0.0) +
-                           (kc_count[KC_GEMM_OMGXI] ? kc_ms[KC_GEMM_OMGXI] / kc_count[KC_GEMM_OMGXI] : 0.0);
+          const double t = omega_side();
           if (t < fbest) { fbest = t; fc = c; }
         }
         fused_tuned = fbest < unfused;

@@ -5,7 +5,8 @@
     rocprofv3 --pmc FETCH_SIZE -d out -o fetch -- python3 -m zandpack_amd.profile_c3 --copy-mib 256
 
 ``--copy-mib`` also runs one device copy of that many MiB (torch), whose counters
-calibrate FETCH_SIZE / WRITE_SIZE against a known byte count.  The tiles are
+calibrate FETCH_SIZE / WRITE_SIZE against a known byte count; ``--copy-only`` runs
+just that copy (one calibration for both precisions).  The tiles are
 autotuned first (those timing attempts show up in a trace as well); the last
 ``--attempts`` attempts run with the chosen tiles.  ``--serial`` runs them the way
 bench.py's per-class profile does (one stream, HIP events around each class).
@@ -19,6 +20,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--attempts", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--copy-only", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--serial", action="store_true",
                     help="per-class timing mode: one stream, no graph (as bench.py's kernel profile)")
@@ -31,6 +33,9 @@ def main():
         y = x.clone()
         torch.cuda.synchronize()
         del x, y
+        if a.copy_only:
+            print(f"profile_c3: copy of {a.copy_mib} MiB", flush=True)
+            return 0
     p = builder.config_c3(seed=0)
     prop = Propagator(p, a.precision, graph=not a.no_graph, autotune=True)
     if a.serial:
