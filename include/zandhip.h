@@ -331,6 +331,20 @@ int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int
  * unfused path to rounding.  FP64 problems always run unfused. */
 int zh_set_fusion(zh_problem* p, int mode);
 
+/* Kernel of the Omega xi_c product of the Psi equation (Zandpack/mpi/zand:771-778):
+ * 0 auto (default: the streaming kernel when zh_autotune timed it faster than the tiled
+ * GEMM), 1 the tiled GEMM, 2 the streaming kernel (Omega and xi_c staged through LDS by
+ * direct global-to-LDS loads, one lead's columns per tile).  The streaming kernel needs
+ * every 16-column group of the coupled orbitals to belong to one lead, an even mode and
+ * coupled-orbital count and at most 8 leads; where it does not apply, 2 falls back to the
+ * GEMM (zh_omgxi_info tells which ran).  Both agree to rounding.  No counterpart in the
+ * reference (a tensor contraction there). */
+int zh_set_omgxi(zh_problem* p, int mode);
+
+/* out[0] = 1 when the attempts run the streaming Omega xi_c kernel, out[1] = 1 when the
+ * problem's layout allows it. */
+int zh_omgxi_info(zh_problem* p, int* out);
+
 /* Anti-Hermitian Omega: 0 auto (default), 1 off, 2 on.  dOmega_ij/dt (Zandpack/mpi/zand:
  * 784-809) maps Omega_ij -> -conj(Omega_ji) onto itself when every mode has
  * dGM_j Ixi_j = -conj(dGP_j xi_j) and Xpm_j = conj(Xpp_j) (the reference's Lorentzian

@@ -95,6 +95,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_int), _dp, _dp, _dp, C.c_int]),
     "zh_hamiltonian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_fusion": (C.c_int, [C.c_void_p, C.c_int]),
+    "zh_set_omgxi": (C.c_int, [C.c_void_p, C.c_int]),
+    "zh_omgxi_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_antihermitian": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_antihermitian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_graph": (C.c_int, [C.c_void_p, C.c_int]),

@@ -33,6 +33,7 @@
 #include "zh_kernels.hpp"
 #include "zh_scf.hpp"
 #include "zh_omega.hpp"
+#include "zh_omgxi.hpp"
 
 namespace zh {
 
@@ -121,6 +122,8 @@ struct EngineBase {
   virtual void set_fusion(int mode) = 0;
   virtual void set_antihermitian(int mode) = 0;
   virtual int antihermitian_info() const = 0;
+  virtual void set_omgxi(int mode) = 0;
+  virtual void omgxi_info(int* out) const = 0;
   bool use_graph = true;
   bool profile = false;
   double kc_ms[KC_N] = {0};
@@ -173,7 +176,7 @@ struct TuneKey {
   }
 };
 static std::mutex g_tune_mu;
-static std::map<TuneKey, std::array<int, TS_N + 1>> g_tune_cache;  // + fused Ω kernel
+static std::map<TuneKey, std::array<int, TS_N + 2>> g_tune_cache;  // + fused Ω, streaming Ωξ
 static int pow2_bucket(int v) {
   int b = 1;
   while (b < v) b <<= 1;
@@ -306,6 +309,14 @@ struct Engine : EngineBase {
   int wpart_max = 0;        // chunks the partial buffers hold
   bool y0xi_valid = false;  // PredY[par] is y0 ξ_c of the resident state
   bool fused_used = false;  // the attempt in flight ran the fused kernel
+  // Streaming Ωξ kernel (zh_omgxi.hpp): column tiles of one lead each, built with the
+  // layout; used when the autotuner timed it faster than the tiled GEMM (or forced).
+  int xi_req = 0;          // 0 auto, 1 tiled GEMM, 2 streaming kernel
+  bool xi_tuned = false;   // the autotuner found the streaming kernel faster
+  bool xi_ok = false;      // the layout allows it
+  int xi_ng = 1;           // 16-column groups per column tile
+  std::vector<int4> xi_items;
+  bool xi_stream() const { return xi_ok && (xi_req == 2 || (xi_req == 0 && xi_tuned)); }
   bool fusable() const {
     return std::is_same<R, float>::value && nc > 0 && nc <= 16 * kFusedNG && !overlap &&
            !tri_on && (fusion_req == 2 || (fusion_req == 0 && fused_tuned));
@@ -626,6 +637,25 @@ struct Engine : EngineBase {
       lr[a] = make_int4(alo < ahi ? alo : 0, alo < ahi ? ahi : 0, plo < phi ? plo : 0,
                         plo < phi ? phi : 0);
     }
+    // column tiles of the streaming Ωξ kernel: runs of up to 4 groups of one lead (a
+    // group coupled by no lead gets a tile of zeros).  Its K chunks may reach past a
+    // lead's modes to the chunk boundary, which stays inside the active or the passive
+    // block when the active block is 64-aligned (or all of the modes).
+    xi_items.clear();
+    xi_ng = 1;
+    xi_ok = nc > 0 && NL <= kXiMaxLeads && Md % 2 == 0 && nc % 2 == 0 &&
+            (ML % kTile == 0 || ML == Md) && (long)Md * Md < (1L << 31) &&
+            (long)Md * nc < (1L << 31);
+    for (int g = 0; xi_ok && g < ngroup;) {
+      const int mk = gm[g];
+      if (mk & (mk - 1)) { xi_ok = false; break; }
+      int e = g + 1;
+      while (e < ngroup && gm[e] == mk && e - g < 4) ++e;
+      xi_items.push_back(make_int4(kCGroup * g, kCGroup * (e - g), mk ? __builtin_ffs(mk) - 1 : -1, 0));
+      xi_ng = std::max(xi_ng, e - g);
+      g = e;
+    }
+    if ((int)xi_items.size() > kXiMaxItems) xi_ok = false;
     const size_t mc = (size_t)NK * Md * nc;
     std::vector<double2> ixic(mc), g1((size_t)NK * Md), x1((size_t)NK * Md), x2((size_t)NK * Md);
     std::vector<cs> xc(mc), apr(mc), bol(mc);
@@ -1417,6 +1447,15 @@ struct Engine : EngineBase {
     fusion_req = mode;
     drop_graphs();
   }
+  void set_omgxi(int mode) override {
+    require(mode >= 0 && mode <= 2, "Omega xi_c kernel: 0 auto, 1 tiled GEMM, 2 streaming");
+    xi_req = mode;
+    drop_graphs();
+  }
+  void omgxi_info(int* out) const override {
+    out[0] = xi_stream() && !fusable() ? 1 : 0;
+    out[1] = xi_ok ? 1 : 0;
+  }
   void hamiltonian_info(int* out) override {
     out[0] = h_real ? 1 : 0; out[1] = h_sparse ? 1 : 0; out[2] = ell_R;
   }
@@ -1707,7 +1746,35 @@ struct Engine : EngineBase {
   // X ξ_c (Md x Md)(Md x nc) into the split-K partials W (X = y0 or a k slot).  With Ω
   // held as its block upper triangle each row block also runs the mirrored part,
   // -X[J < I, I]^H ξ_c, in the same K loop (KSegTri).
-  template <int RM> void omgxi_run(const cs* X, hipStream_t st) {
+  template <int NG> void xi_launch(const cs* X, hipStream_t st) {
+    XiArgs<cs> a{};
+    a.X = X; a.sX = (long)Md * Md;
+    a.xi = xic; a.sXi = (long)Md * nc;
+    a.part = W; a.sPart = (long)Md * nc;
+    a.Md = Md; a.nc = nc; a.parts = kWParts; a.tri = tri_on ? 1 : 0; a.ML = ML; a.ff = ff ? 1 : 0;
+    a.nitem = (int)xi_items.size();
+    for (int i = 0; i < a.nitem; ++i) a.item[i] = xi_items[i];
+    for (int l = 0; l < NL; ++l) a.lr[l] = lrange_h[l];
+    const dim3 grid((Md + 63) / 64, a.nitem, NK * kWParts);
+    // complex64: 3 chunk buffers (48 KB of LDS), 3 workgroups per CU; complex128 (212
+    // registers at NG = 3): 4 buffers, 2 workgroups per CU (measured at C3)
+    constexpr bool F32 = std::is_same<R, float>::value;
+    hipLaunchKernelGGL((k_xi_stream<cs, NG, F32 ? 3 : 4, F32 ? 3 : 2>), grid, dim3(256), 0, st,
+                       a);
+    HIPCHK(hipGetLastError());
+  }
+  // stream: the attempts' products may take the streaming kernel (the steady-state
+  // operator keeps the four-multiplication GEMM)
+  template <int RM> void omgxi_run(const cs* X, hipStream_t st, bool stream = true) {
+    if (stream && xi_stream()) {
+      require(xi_ng >= 1 && xi_ng <= 4 && (int)lrange_h.size() == NL && NL <= kXiMaxLeads,
+              "streaming Omega xi_c kernel: bad column tiles");
+      if (xi_ng == 1) xi_launch<1>(X, st);
+      else if (xi_ng == 2) xi_launch<2>(X, st);
+      else if (xi_ng == 3) xi_launch<3>(X, st);
+      else xi_launch<4>(X, st);
+      return;
+    }
     GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
     EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
     KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
@@ -1968,11 +2035,13 @@ struct Engine : EngineBase {
       if (it != g_tune_cache.end()) {
         for (int i = 0; i < TS_N; ++i) tsel[i] = it->second[i];
         fused_tuned = it->second[TS_N] != 0;
+        xi_tuned = it->second[TS_N + 1] != 0;
         cached = true;
       }
     }
     if (!cached) {
       fused_tuned = false;  // the tile candidates are timed on the unfused path
+      xi_tuned = false;     // and on the tiled Ωξ GEMM
       static const int kc_of[TS_N] = {KC_GEMM_PSI, KC_GEMM_HSIG, KC_GEMM_OMEGA, KC_GEMM_OMGXI,
                                       KC_GEMM_PI};
       double best[TS_N];
@@ -2009,6 +2078,16 @@ struct Engine : EngineBase {
         }
       }
       for (int i = 0; i < TS_N; ++i) tsel[i] = arg[i];
+      // the streaming Ωξ kernel against the chosen Ωξ tile
+      if (xi_ok && xi_req == 0) {
+        xi_tuned = true;
+        attempt(&info, nullptr);
+        for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
+        for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
+        const double t = kc_count[KC_GEMM_OMGXI] ? kc_ms[KC_GEMM_OMGXI] / kc_count[KC_GEMM_OMGXI]
+                                                 : 1e300;
+        xi_tuned = t < best[TS_OMGXI];
+      }
       // the fused Ω kernel (mixed precision) against the Ω and Ωξ products it replaces,
       // for each chunk width (the Ω candidate index selects it)
       fused_tuned = false;
@@ -2039,9 +2118,10 @@ struct Engine : EngineBase {
       profile = prof0;
       restore.done = true;
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      std::array<int, TS_N + 1> a;
+      std::array<int, TS_N + 2> a;
       for (int i = 0; i < TS_N; ++i) a[i] = tsel[i];
       a[TS_N] = fused_tuned ? 1 : 0;
+      a[TS_N + 1] = xi_tuned ? 1 : 0;
       g_tune_cache[key] = a;
     }
     drop_graphs();
@@ -2282,7 +2362,7 @@ struct Engine : EngineBase {
       to_ap(xin);
       gather(sp0, sig0, true);
       omega0(omg[0]);
-      omgxi_run<0>(omg[0], stream);
+      omgxi_run<0>(omg[0], stream, false);
       WSumArgs<cs> w{};
       w.part = W;
       for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
@@ -3196,6 +3276,20 @@ int zh_set_fusion(zh_problem* p, int mode) {
   return zh_try([&] {
     require(p, "null argument");
     p->eng->set_fusion(mode);
+  });
+}
+
+int zh_set_omgxi(zh_problem* p, int mode) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_omgxi(mode);
+  });
+}
+
+int zh_omgxi_info(zh_problem* p, int* out) {
+  return zh_try([&] {
+    require(p && out, "null argument");
+    p->eng->omgxi_info(out);
   });
 }
 

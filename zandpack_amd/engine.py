@@ -178,6 +178,20 @@ class Propagator:
             mode = "on" if mode else "off"
         N.check(self.lib.zh_set_fusion(self.h, {"auto": 0, "off": 1, "on": 2}[mode]))
 
+    def set_omgxi(self, mode="auto"):
+        """Kernel of the Ωξ product of the Ψ equation: "auto" (the autotuner's timed
+        choice), "gemm" (the tiled GEMM) or "stream" (the streaming kernel: Ω and ξ_c
+        staged through LDS by direct global-to-LDS loads; falls back to the GEMM where
+        the layout does not allow it); same results to rounding."""
+        N.check(self.lib.zh_set_omgxi(self.h, {"auto": 0, "gemm": 1, "stream": 2}[mode]))
+
+    def omgxi_info(self):
+        """{"stream", "possible"}: whether the attempts run the streaming Ωξ kernel,
+        whether the problem's layout allows it."""
+        out = (C.c_int * 2)()
+        N.check(self.lib.zh_omgxi_info(self.h, out))
+        return {"stream": bool(out[0]), "possible": bool(out[1])}
+
     def set_antihermitian(self, mode="auto"):
         """Ω held as its block upper triangle (half the Ω stage kernel's work) when the
         constants keep Ω anti-Hermitian (ΔM_j Ixi_j = -conj(ΔP_j ξ_j), χ⁻ = conj χ⁺) and
