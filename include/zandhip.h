@@ -62,6 +62,20 @@ const char* zh_last_error(void);
 int zh_problem_create(const zh_dims* dims, zh_problem** out);
 void zh_problem_destroy(zh_problem* p);
 
+/* A problem of `instances` independent propagations of the same shape (dims->NK k
+ * blocks each) stacked on the k axis, so that every kernel of an attempt runs once for
+ * all of them (one launch per kernel class across the instances).  Every other call
+ * then speaks NK = instances * dims->NK: constants, state and currents of instance g are
+ * its k blocks [g dims->NK, (g+1) dims->NK) (zh_instances reports the count).  Each
+ * instance has its own time, step, tolerance, pulse (zh_instance_pulse), error norm and
+ * acceptance: zh_attempt_instances.  Results of an instance are bitwise those of the same
+ * instance alone on the same tiles.  The instances share the device mode order (the
+ * union of their couplings), the Hubbard U, drive and dissipator.  No counterpart in the
+ * reference (one problem per mpirun, Zandpack/mpi/zand:297-431); for the batched
+ * workload of uneven instances, grouped by shape. */
+int zh_problem_create_instances(const zh_dims* dims, int instances, zh_problem** out);
+int zh_instances(zh_problem* p, int* instances);
+
 /* Device mode order, chosen before zh_set_constants: 0 auto (default), 1 the
  * reference order, 2 active modes (dGP or dGM non-zero) first, padded to a
  * 64-mode tile, then the passive Fermi modes.  Order 2 lets the Omega kernels skip
@@ -139,6 +153,11 @@ int zh_set_pulse_ex(zh_problem* p, int kind, double amp, const double* par,
  * c (4, nknots-1) in scipy's CubicSpline.c layout, S(t) = sum_m c[m][i] (t-x_i)^(3-m)
  * on the interval i with x_i <= t (the last interval at the last knot). */
 int zh_set_pulse_spline(zh_problem* p, int nknots, const double* x, const double* c);
+
+/* The bias pulse of one instance (zh_set_pulse_ex's arguments; zh_set_pulse_ex sets
+ * every instance's).  Replaces the per-run Bias.py of Zandpack/mpi/zand:113-124. */
+int zh_instance_pulse(zh_problem* p, int instance, int kind, double amp, const double* par,
+                      const double* weights);
 
 /* Device-evaluated dissipator of the sigma equation, dissipator(t, sigma) =
  * eta(t) (sigma - sigma_eq), subtracted as D -= hbar * dissipator(t, sigma)
@@ -289,6 +308,19 @@ int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J);
  * next zh_attempt_collect returns at once), else 0.  For callers that collect each
  * problem as soon as its attempt is done (the serving loop). */
 int zh_attempt_ready(zh_problem* p, int* ready);
+
+/* One DOP54 attempt (Zandpack/mpi/zand:301-408) of every instance of the problem
+ * together: info[g] (t0, h, eps, force_accept) in, (te, h_next, accepted) out for
+ * instance g; J (NL, NK) the stage-0 currents of all k blocks.  Accepted instances
+ * advance, the others keep their state; an instance whose next step size is not
+ * finite is not committed (accepted = 0; h_next tells).  h = 0 with force_accept holds
+ * an instance that has finished (its state stays bitwise; h_next is then not
+ * meaningful).  launch / collect split it like zh_attempt_launch / _collect.  The
+ * single-instance calls (zh_attempt, zh_propagate, zh_rhs, zh_steady_state) refuse a
+ * problem with several instances. */
+int zh_attempt_instances(zh_problem* p, zh_attempt_info* info, double* J);
+int zh_launch_instances(zh_problem* p, const zh_attempt_info* info);
+int zh_collect_instances(zh_problem* p, zh_attempt_info* info, double* J);
 
 /* One attempt of each of count problems: every launch, then every collect, in one
  * call (the multi-instance step of the batched and serving configurations; the

@@ -53,6 +53,13 @@ SIGNATURES = {
     "zh_abi_version": (C.c_int, []),
     "zh_last_error": (C.c_char_p, []),
     "zh_problem_create": (C.c_int, [C.POINTER(zh_dims), C.POINTER(C.c_void_p)]),
+    "zh_problem_create_instances": (C.c_int, [C.POINTER(zh_dims), C.c_int,
+                                               C.POINTER(C.c_void_p)]),
+    "zh_instances": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "zh_instance_pulse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp]),
+    "zh_attempt_instances": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info), _dp]),
+    "zh_launch_instances": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info)]),
+    "zh_collect_instances": (C.c_int, [C.c_void_p, C.POINTER(zh_attempt_info), _dp]),
     "zh_problem_destroy": (None, [C.c_void_p]),
     "zh_set_constants": (C.c_int, [C.c_void_p] + [_dp] * 8),
     "zh_set_hubbard": (C.c_int, [C.c_void_p, C.c_double, _dp]),

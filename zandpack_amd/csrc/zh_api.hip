@@ -94,6 +94,8 @@ struct EngineBase {
                          const int* nnz, const int* ks, const int* rows, const int* cols,
                          const double* vals) = 0;
   virtual void set_pulse(const PulseDesc& p) = 0;
+  virtual void set_instance_pulse(int g, const PulseDesc& p) = 0;
+  virtual int instances() const = 0;
   virtual void set_pulse_spline(int nknots, const double* x, const double* c) = 0;
   virtual void set_dissipator(const DissDesc& d, const double* sig_eq) = 0;
   virtual void set_overlap(const double* Sinv) = 0;
@@ -211,6 +213,10 @@ struct Engine : EngineBase {
   static constexpr int kRM3 = std::is_same<R, double>::value ? 3 : 0;
   static constexpr int kCGroup = 16;  // coupled-orbital column group (lead-aligned)
   int NK, n, NL, NX, NOT;
+  // Instances: NI independent problems of KI k blocks each stacked on the k axis (NK = NI
+  // KI), sharing the device mode order and every launch; each has its own control block
+  // (t, h, eps), pulse, stage parameters, error norm and acceptance.
+  int NI = 1, KI = 1;
   int M;        // modes per k in the reference layout, NL*NX*NOT
   int Md = 0;   // modes per k on the device (active block padded to kTile)
   int ML = 0;   // active block [0, ML); passive modes in [ML, Md)
@@ -227,7 +233,9 @@ struct Engine : EngineBase {
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   std::vector<void*> allocs;
   Tableau tab = dop54();
-  PulseDesc pulse;
+  std::vector<PulseDesc> pulse;  // per instance
+  PulseDesc* pulse_d = nullptr;  // device copies read by k_stage_params
+  bool pulse_dirty = true;
   DissDesc diss{ZH_DISS_NONE, 0.0, 0.0, 1.0};
   double2* sig_eq = nullptr;  // σ_eq of the dissipator (allocated on first use)
   // Drive terms H(t) = H0 + Σ_i f_i(t) M_i: the distinct entries (k, row, col) of the M_i,
@@ -368,7 +376,7 @@ struct Engine : EngineBase {
 
   long nbytes = 0;
   bool pending = false;  // an attempt was launched and not collected yet
-  zh_attempt_info pend_info;
+  std::vector<zh_attempt_info> pend_info;  // per instance
 
   template <typename T> T* dalloc(size_t count) {
     void* p = nullptr;
@@ -382,24 +390,28 @@ struct Engine : EngineBase {
     return (T*)p;
   }
 
-  Engine(const zh_dims& d)
-      : NK(d.NK), n(d.n), NL(d.NL), NX(d.NX), NOT(d.NOT), hbar(d.hbar), device(d.device) {
+  Engine(const zh_dims& d, int instances = 1)
+      : NK(d.NK * instances), n(d.n), NL(d.NL), NX(d.NX), NOT(d.NOT), NI(instances), KI(d.NK),
+        hbar(d.hbar), device(d.device) {
     M = NL * NX * NOT;
-    std::memset(&pulse, 0, sizeof(pulse));
-    pulse.hbar = hbar;
+    PulseDesc p0;
+    std::memset(&p0, 0, sizeof(p0));
+    p0.hbar = hbar;
+    pulse.assign(NI, p0);
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& a : aux) HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     for (auto& e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
-    HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl), hipHostMallocDefault));
-    std::memset(ctrl_h, 0, sizeof(Ctrl));
-    ctrl_h->norm_scale = 1.0;
+    HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl) * NI, hipHostMallocDefault));
+    std::memset(ctrl_h, 0, sizeof(Ctrl) * NI);
+    for (int g = 0; g < NI; ++g) ctrl_h[g].norm_scale = 1.0;
     HIPCHK(hipHostMalloc((void**)&J_h, sizeof(double) * NL * NK, hipHostMallocDefault));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    ctrl = dalloc<Ctrl>(1);
-    sp = dalloc<StageParams>(7);
+    ctrl = dalloc<Ctrl>(NI);
+    sp = dalloc<StageParams>(7 * (size_t)NI);  // stage-major: sp[s * NI + g]
+    pulse_d = dalloc<PulseDesc>(NI);
     Jd = dalloc<double>((size_t)NL * NK);
     Jpart = dalloc<double>((size_t)NL * NK * kCurChunks);
     HIPCHK(hipStreamSynchronize(stream));
@@ -528,7 +540,7 @@ struct Engine : EngineBase {
     }
     // room for the error-norm partials of the finest Ω tile candidate (32 x 32)
     const long t32 = (Md + 31) / 32;
-    te_part = dalloc<double>((size_t)2 * kTEBlocks + t32 * t32 * NK);
+    te_part = dalloc<double>((size_t)2 * kTEBlocks * NI + t32 * t32 * NK);
     Ap = dalloc<cs>((size_t)NK * Md * (n + nc));
     Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
     Aomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
@@ -1030,12 +1042,12 @@ struct Engine : EngineBase {
     if (ovl_model()) {
       hipLaunchKernelGGL((k_overlap_model<cs>), dim3(2048), dim3(256), 0, st, NK, n, nc, drive.nt,
                          (const double2*)SH0, (const double2*)SM, (const double2*)Sdev,
-                         (const double2*)(U != 0.0 ? dvec : nullptr), sps, Hcur, Bp);
+                         (const double2*)(U != 0.0 ? dvec : nullptr), sps, KI, Hcur, Bp);
       HIPCHK(hipGetLastError());
     } else if (drive.nt > 0) {
       DriveArgs<cs, R> a{(int)de_h.size(), n, nc, ell_R, de_k, de_r, de_c, de_slot, de_h0,
                          de_beg, de_term, de_coef, Hcur, Bp, h_real ? Hre : nullptr,
-                         h_real ? HreT : nullptr, h_sparse ? ell_val : nullptr, sps};
+                         h_real ? HreT : nullptr, h_sparse ? ell_val : nullptr, sps, KI};
       hipLaunchKernelGGL((k_drive_update<cs, R>), dim3(std::min<int>(1024, (a.E + 255) / 256)),
                          dim3(256), 0, st, a);
       HIPCHK(hipGetLastError());
@@ -1055,7 +1067,8 @@ struct Engine : EngineBase {
   }
 
   void set_hubbard(double U_, const double* n0_) override {
-    require(U_ == 0.0 || NK == 2, "the Hubbard model couples two spin blocks: NK must be 2");
+    require(U_ == 0.0 || KI == 2,
+            "the Hubbard model couples two spin blocks: NK must be 2 (per instance)");
     require(have_consts, "set the constants first", ZH_ESTATE);
     drop_graphs();
     U = U_;
@@ -1120,7 +1133,7 @@ struct Engine : EngineBase {
   }
   double norm_scale() const {
     if (!tol_scaled) return 1.0;
-    const double cnt = (double)NK * ((double)n * n + (double)M * n + (double)M * M);
+    const double cnt = (double)KI * ((double)n * n + (double)M * n + (double)M * M);
     return 1.0 / cnt;
   }
 
@@ -1187,13 +1200,26 @@ struct Engine : EngineBase {
   }
 
   void set_pulse(const PulseDesc& p) override {
+    for (int g = 0; g < NI; ++g) set_instance_pulse(g, p);
+  }
+  int instances() const override { return NI; }
+  void set_instance_pulse(int g, const PulseDesc& p) override {
+    require(g >= 0 && g < NI, "instance index out of range");
     require(p.kind != ZH_PULSE_SPLINE || spl_n >= 2, "set the pulse spline first", ZH_ESTATE);
-    drop_graphs();
-    pulse = p;
-    pulse.hbar = hbar;
-    pulse.sx = spl_x;
-    pulse.sc = spl_c;
-    pulse.ns = spl_n;
+    pulse[g] = p;
+    pulse[g].hbar = hbar;
+    pulse[g].sx = spl_x;
+    pulse[g].sc = spl_c;
+    pulse[g].ns = spl_n;
+    pulse_dirty = true;
+  }
+  // the pulses into pulse_d before a launch that reads them (stream order)
+  void upload_pulses() {
+    if (!pulse_dirty) return;
+    HIPCHK(hipMemcpyAsync(pulse_d, pulse.data(), sizeof(PulseDesc) * NI, hipMemcpyHostToDevice,
+                          stream));
+    HIPCHK(hipStreamSynchronize(stream));  // pulse is host memory that may change
+    pulse_dirty = false;
   }
 
   // knots and coefficients of ZH_PULSE_SPLINE (device copies; replaced on each call)
@@ -1211,9 +1237,12 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpyAsync(spl_c, c, sizeof(double) * 4 * (nk - 1), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
     spl_n = nk;
-    pulse.sx = spl_x;
-    pulse.sc = spl_c;
-    pulse.ns = spl_n;
+    for (auto& q : pulse) {
+      q.sx = spl_x;
+      q.sc = spl_c;
+      q.ns = spl_n;
+    }
+    pulse_dirty = true;
   }
 
   // ------------------------------------------------------------- transfers
@@ -1500,7 +1529,7 @@ struct Engine : EngineBase {
     g.Ap = Ap; g.ldAp = n + nc; g.sAp = (long)Md * (n + nc);
     g.Aomg = Aomg; g.Bomg = Bomg; g.Bp = Bp; g.sig = sig_in; g.cidx = cidx;
     g.dvec = dvec; g.n0 = n0; g.U = U; g.hbar = hbar; g.hubbard = hub && U != 0.0;
-    g.sp = sps; g.chip = chip; g.chim = chim; g.lead = lead; g.u = uvec; g.v = vvec;
+    g.sp = sps; g.ki = KI; g.chip = chip; g.chim = chim; g.lead = lead; g.u = uvec; g.v = vvec;
     hipLaunchKernelGGL(k_gather<cs>, dim3(512, 4), dim3(256), 0, stream, g);
     HIPCHK(hipGetLastError());
   }
@@ -1529,7 +1558,7 @@ struct Engine : EngineBase {
   // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
   void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, bool currents,
                  bool hub) {
-    const StageParams* sps = sp + s;
+    const StageParams* sps = sp + s * NI;
     timed(KC_GATHER, [&] {
       gather(sps, sig_in, hub);
       if (hub) apply_model(sps, stream);  // H(t_s, σ_s) of the attempts
@@ -1558,7 +1587,7 @@ struct Engine : EngineBase {
 
   // σ branch: Π_c, currents (stage 0), Hσ with the σ right-hand side.
   void stage_sigma(int s, const double2* sig_in, int kslot, bool currents, hipStream_t st) {
-    const StageParams* sps = sp + s;
+    const StageParams* sps = sp + s * NI;
     const double ih = 1.0 / hbar;
     if (nc > 0) {
       timed(KC_GEMM_PI, [&] {
@@ -1617,7 +1646,7 @@ struct Engine : EngineBase {
     timed(KC_SIGSYM, [&] {
       dim3 grid((n + 31) / 32, (n + 31) / 32, NK);
       hipLaunchKernelGGL(k_sigma_sym, grid, dim3(32, 8), 0, st, n, (const double2*)G,
-                         sig[kslot], sps, ih, sig_in,
+                         sig[kslot], sps, KI, ih, sig_in,
                          (const double2*)(diss.kind != ZH_DISS_NONE ? sig_eq : nullptr));
     });
     HIPCHK(hipGetLastError());
@@ -1625,7 +1654,7 @@ struct Engine : EngineBase {
 
   // Ψ branch: Ω_s ξ_c by linearity, then the Ψ right-hand side.
   void stage_psi(int s, const OmegaStage& os, int kslot, hipStream_t st) {
-    const StageParams* sps = sp + s;
+    const StageParams* sps = sp + s * NI;
     const double ih = 1.0 / hbar;
     cs* Wc = Wst;  // Ω_s ξ_c of this stage
     if (nc > 0 && os.wpart) {
@@ -1702,7 +1731,7 @@ struct Engine : EngineBase {
                                                           typename Cands<R>::psi())));
         }
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
-                     dvec_epi(), lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
+                     dvec_epi(), lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows(), KI};
         const size_t lds = (size_t)hpsi_rb * n * sizeof(cs);
         const long sAp = (long)Md * (n + nc);
         const dim3 grid((Md + hpsi_rb - 1) / hpsi_rb, NK);
@@ -1726,7 +1755,7 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
-                     dvec_epi(), lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows()};
+                     dvec_epi(), lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows(), KI};
         HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psiR())));
       } else {
@@ -1735,7 +1764,8 @@ struct Engine : EngineBase {
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
         EpiPsi<cs> e{psi[kslot], ovl ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
-                     dvec_epi(), lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md};
+                     dvec_epi(), lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md,
+                     KI};
         HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
       }
@@ -1818,7 +1848,7 @@ struct Engine : EngineBase {
         e.ynew = os.ynew;
         e.u = uvec; e.v = vvec; e.M = Md;
         e.ct_self = os.ct_self;
-        e.te_part = te_part + 2 * kTEBlocks;
+        e.te_part = te_part + 2 * kTEBlocks * NI;
         e.rtol = tol_rtol; e.atol = tol_atol; e.scaled = tol_scaled ? 1 : 0;
       };
       if (os.te) {
@@ -1864,7 +1894,7 @@ struct Engine : EngineBase {
           e.ynew = os.ynew;
           e.u = uvec; e.v = vvec; e.M = Md;
           e.ct_self = os.ct_self;
-          e.te_part = te_part + 2 * kTEBlocks;
+          e.te_part = te_part + 2 * kTEBlocks * NI;
           e.rtol = tol_rtol; e.atol = tol_atol; e.scaled = tol_scaled ? 1 : 0;
         };
         if (os.te) {
@@ -1915,26 +1945,32 @@ struct Engine : EngineBase {
         ts.rtol = tp.rtol = tol_rtol; ts.atol = tp.atol = tol_atol;
         ts.scaled = tp.scaled = 1;
       }
-      MapDense ms{NK * n, n, n, n, n};
-      MapDense mp{NK * Md, n, n, n, n};
-      hipLaunchKernelGGL((k_te<double2, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, ts,
-                         ms, te_part);
+      // per instance (blockIdx.y): KI k blocks of rows, kTEBlocks partials each
+      MapDense ms{KI * n, n, n, n, n};
+      MapDense mp{KI * Md, n, n, n, n};
+      const dim3 grid(kTEBlocks, NI);
+      hipLaunchKernelGGL((k_te<double2, MapDense, 6>), grid, dim3(256), 0, stream, ts, ms, te_part,
+                         (long)KI * n * n);
       bool done = false;
       if constexpr (sizeof(cs) == 8) {
         if (mp.pairs()) {
-          hipLaunchKernelGGL((k_te_pairs<cs, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream,
-                             tp, mp, te_part + kTEBlocks);
+          hipLaunchKernelGGL((k_te_pairs<cs, MapDense, 6>), grid, dim3(256), 0, stream, tp, mp,
+                             te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
           done = true;
         }
       }
       if (!done)
-        hipLaunchKernelGGL((k_te<cs, MapDense, 6>), dim3(kTEBlocks), dim3(256), 0, stream, tp,
-                           mp, te_part + kTEBlocks);
+        hipLaunchKernelGGL((k_te<cs, MapDense, 6>), grid, dim3(256), 0, stream, tp, mp,
+                           te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
     });
     HIPCHK(hipGetLastError());
     timed(KC_CTRL, [&] {
-      hipLaunchKernelGGL(k_controller, dim3(1), dim3(kCtrlThreads), 0, stream, (const double*)te_part,
-                         2 * kTEBlocks + te_omega_blocks(), ctrl);
+      // the Ω kernels' partials are one per workgroup, batch-major: KI blocks of each
+      // instance are contiguous
+      const int no = te_omega_blocks();
+      require(no % NI == 0, "error-norm partials do not split over the instances");
+      hipLaunchKernelGGL(k_controller, dim3(NI), dim3(kCtrlThreads), 0, stream,
+                         (const double*)te_part, kTEBlocks, no / NI, NI, ctrl);
     });
     HIPCHK(hipGetLastError());
   }
@@ -1942,8 +1978,9 @@ struct Engine : EngineBase {
   // The seven stages, error norm and controller of one attempt (graph body).
   void attempt_body() {
     timed(KC_PARAMS, [&] {
-      hipLaunchKernelGGL(k_stage_params, dim3(1), dim3(1), 0, stream, (const Ctrl*)ctrl, pulse,
-                         diss, drive, tab, NL, sp);
+      hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, (const Ctrl*)ctrl,
+                         (const PulseDesc*)pulse_d,
+                         diss, drive, tab, NL, NI, sp);
     });
     HIPCHK(hipGetLastError());
     for (int s = 0; s < 7; ++s) {
@@ -2062,15 +2099,17 @@ struct Engine : EngineBase {
       } restore{profile, tsel, profile, {tsel[0], tsel[1], tsel[2], tsel[3], tsel[4]}};
       const bool prof0 = profile;
       profile = true;
-      zh_attempt_info info;
-      std::memset(&info, 0, sizeof(info));
-      info.h = 1e-6;
-      info.eps = 0.0;  // never accepted: the resident state is untouched
+      zh_attempt_info info1;
+      std::memset(&info1, 0, sizeof(info1));
+      info1.h = 1e-6;
+      info1.eps = 0.0;  // never accepted: the resident state is untouched
+      std::vector<zh_attempt_info> infos(NI, info1);
+      zh_attempt_info* info = infos.data();
       for (int c = 0; c < kCandsOn; ++c) {
         for (int i = 0; i < TS_N; ++i) tsel[i] = c;
-        attempt(&info, nullptr);  // warm-up (code load)
+        attempt(info, nullptr);  // warm-up (code load)
         for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
-        for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
+        for (int r = 0; r < 2; ++r) attempt(info, nullptr);
         for (int i = 0; i < TS_N; ++i) {
           const int k = kc_of[i];
           const double t = kc_count[k] ? kc_ms[k] / kc_count[k] : 0.0;
@@ -2081,9 +2120,9 @@ struct Engine : EngineBase {
       // the streaming Ωξ kernel against the chosen Ωξ tile
       if (xi_ok && xi_req == 0) {
         xi_tuned = true;
-        attempt(&info, nullptr);
+        attempt(info, nullptr);
         for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
-        for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
+        for (int r = 0; r < 2; ++r) attempt(info, nullptr);
         const double t = kc_count[KC_GEMM_OMGXI] ? kc_ms[KC_GEMM_OMGXI] / kc_count[KC_GEMM_OMGXI]
                                                  : 1e300;
         xi_tuned = t < best[TS_OMGXI];
@@ -2096,9 +2135,9 @@ struct Engine : EngineBase {
         // per-attempt time of the Ω kernels, the Ωξ products and their sums, on the
         // chosen tiles unfused, then fused for each chunk width
         auto omega_side = [&]() {
-          attempt(&info, nullptr);
+          attempt(info, nullptr);
           for (int k = 0; k < KC_N; ++k) { kc_ms[k] = 0; kc_count[k] = 0; }
-          for (int r = 0; r < 2; ++r) attempt(&info, nullptr);
+          for (int r = 0; r < 2; ++r) attempt(info, nullptr);
           return 0.5 * (kc_ms[KC_GEMM_OMEGA] + kc_ms[KC_GEMM_OMGXI] + kc_ms[KC_WSUM]);
         };
         const double unfused = omega_side();
@@ -2138,12 +2177,15 @@ struct Engine : EngineBase {
     ensure_h0();
     prepare_model();
     if (ff && ff_dirty) { clear_ff(); ff_dirty = false; }
-    ctrl_h->t0 = info->t0;
-    ctrl_h->h = info->h;
-    ctrl_h->eps = info->eps;
-    ctrl_h->force = info->force_accept ? 1 : 0;
-    ctrl_h->norm_scale = norm_scale();
-    HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+    upload_pulses();
+    for (int g = 0; g < NI; ++g) {  // info[g]: instance g
+      ctrl_h[g].t0 = info[g].t0;
+      ctrl_h[g].h = info[g].h;
+      ctrl_h[g].eps = info[g].eps;
+      ctrl_h[g].force = info[g].force_accept ? 1 : 0;
+      ctrl_h[g].norm_scale = norm_scale();
+    }
+    HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl) * NI, hipMemcpyHostToDevice, stream));
     const int par = (iy0 == 7) ? 0 : 1;
     const bool fz = fusable();
     if (fz != fused_used) drop_graphs();
@@ -2167,9 +2209,9 @@ struct Engine : EngineBase {
       attempt_body();
     }
     HIPCHK(hipEventRecord(ev[2 * KC_N + 1], stream));
-    HIPCHK(hipMemcpyAsync(ctrl_h, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(ctrl_h, ctrl, sizeof(Ctrl) * NI, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(J_h, Jd, sizeof(double) * NL * NK, hipMemcpyDeviceToHost, stream));
-    pend_info = *info;
+    pend_info.assign(info, info + NI);
     pending = true;
   }
 
@@ -2193,22 +2235,53 @@ struct Engine : EngineBase {
     HIPCHK(hipEventElapsedTime(&ms, ev[2 * KC_N], ev[2 * KC_N + 1]));
     attempt_ms += ms;
     attempt_count += 1;
-    *info = pend_info;
-    info->te = ctrl_h->te;
-    info->h_next = ctrl_h->h_next;
-    info->accepted = ctrl_h->accept;
+    int nacc = 0;
+    for (int g = 0; g < NI; ++g) {
+      info[g] = pend_info[g];
+      info[g].te = ctrl_h[g].te;
+      info[g].h_next = ctrl_h[g].h_next;
+      info[g].accepted = ctrl_h[g].accept;
+      // several instances: one whose next step size is not finite is not committed
+      // (unless forced: a finished instance advanced with h = 0) and reports it
+      if (NI > 1 && info[g].accepted && !info[g].force_accept && !std::isfinite(info[g].h_next))
+        info[g].accepted = 0;
+      nacc += info[g].accepted ? 1 : 0;
+    }
     if (J) std::memcpy(J, J_h, sizeof(double) * NL * NK);
-    if (info->accepted && finite_next && !std::isfinite(info->h_next)) {
+    if (NI == 1 && info->accepted && finite_next && !std::isfinite(info->h_next)) {
       // not committed: the resident state stays at the step's start
       info->accepted = 0;
       throw Error{ZH_ENONFINITE, "non-finite next step size (error estimate " +
                                      std::to_string(info->te) + ") at t=" +
                                      std::to_string(info->t0) + "; the step is not committed"};
     }
-    if (info->accepted) {
-      std::swap(iy0, iyn);
-      y0xi_valid = true;  // the last stage left y_new ξ_c in the new y0's PredY slot
+    if (nacc == 0) return;
+    if (nacc < NI) {
+      // instances that stay: their step's start goes into the slot that becomes y0
+      for (int g = 0; g < NI; ++g)
+        if (!info[g].accepted) keep_instance(g);
+      HIPCHK(hipStreamSynchronize(stream));
     }
+    std::swap(iy0, iyn);
+    y0xi_valid = true;  // the last stage left y_new ξ_c in the new y0's PredY slot
+  }
+
+  // Instance g's y0 (σ, Ψ, Ω and its y0 ξ_c) copied over its block of the y_new slots,
+  // so that the slot swap of the others leaves g at its step's start.
+  void keep_instance(int g) {
+    const size_t k0 = (size_t)g * KI;
+    const size_t nn = (size_t)n * n, mn = (size_t)Md * n, mm = (size_t)Md * Md,
+                 mc = (size_t)Md * nc;
+    const int par = (iy0 == 7) ? 0 : 1;
+    HIPCHK(hipMemcpyAsync(sig[iyn] + k0 * nn, sig[iy0] + k0 * nn, sizeof(double2) * KI * nn,
+                          hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemcpyAsync(psi[iyn] + k0 * mn, psi[iy0] + k0 * mn, sizeof(cs) * KI * mn,
+                          hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemcpyAsync(omg[iyn] + k0 * mm, omg[iy0] + k0 * mm, sizeof(cs) * KI * mm,
+                          hipMemcpyDeviceToDevice, stream));
+    if (nc > 0)
+      HIPCHK(hipMemcpyAsync(PredY[1 - par] + k0 * mc, PredY[par] + k0 * mc, sizeof(cs) * KI * mc,
+                            hipMemcpyDeviceToDevice, stream));
   }
 
   void attempt(zh_attempt_info* info, double* J, bool finite_next = false) override {
@@ -2224,6 +2297,7 @@ struct Engine : EngineBase {
   // its coupling terms).  Writes (σ₀, Ψ₀, Ω₀) as the resident state.
   void steady_state(double t0, double tol, int maxiter, const zh_precond* pc, int* iters,
                     double* resid) override {
+    require(NI == 1, "the steady-state solver takes one instance per problem");
     require(!overlap, "the steady-state solver is defined for an orthogonal basis");
     require(have_consts && have_state, "steady state needs constants and a resident sigma",
             ZH_ESTATE);
@@ -2241,8 +2315,9 @@ struct Engine : EngineBase {
     ctrl_h->t0 = t0; ctrl_h->h = 1.0; ctrl_h->eps = 1.0; ctrl_h->force = 1;
     ctrl_h->norm_scale = 1.0;
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_stage_params, dim3(1), dim3(1), 0, stream, (const Ctrl*)ctrl, pulse,
-                       diss, drive, tab, NL, sp);
+    upload_pulses();
+    hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, (const Ctrl*)ctrl,
+                       (const PulseDesc*)pulse_d, diss, drive, tab, NL, NI, sp);
     HIPCHK(hipGetLastError());
     const StageParams* sp0 = sp;
     apply_model(sp0, stream);  // drive terms at t0 (an overlap is refused above)
@@ -2496,6 +2571,7 @@ struct Engine : EngineBase {
   void rhs(const double* Ht, const double* delta, const double* s, const double* p,
            const double* o, double* ds, double* dp, double* dom, double* J) override {
     require(have_consts, "constants not set", ZH_ESTATE);
+    require(NI == 1, "one evaluation takes one instance per problem");
     HIPCHK(hipSetDevice(device));
     install_h(Ht, false);
     h_dirty = true;
@@ -2562,7 +2638,8 @@ using namespace zh;
 
 struct zh_problem {
   std::unique_ptr<EngineBase> eng;
-  zh_dims dims;
+  zh_dims dims;       // dims.NK: k blocks of all instances together
+  int instances = 1;  // independent instances stacked on the k axis
 };
 
 // ---------------------------------------------------------- equilibrium density
@@ -2926,26 +3003,43 @@ extern "C" {
 int zh_abi_version(void) { return ZH_ABI_VERSION; }
 const char* zh_last_error(void) { return g_err.c_str(); }
 
+static void problem_create(const zh_dims* d, int instances, zh_problem** out) {
+  require(d && out, "null argument");
+  require(d->NK > 0 && d->n > 0 && d->NL > 0 && d->NX > 0 && d->NOT > 0, "dimensions must be positive");
+  require(instances >= 1, "at least one instance");
+  require((long)d->NK * instances <= (1 << 20), "too many k blocks");
+  require(d->NL <= ZH_MAX_LEADS, "too many leads");
+  require(d->precision == ZH_FP64 || d->precision == ZH_MIXED, "unknown precision");
+  require(d->hbar > 0, "hbar must be positive");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  require(d->device >= 0 && d->device < ndev, "no such HIP device");
+  auto* p = new zh_problem();
+  p->dims = *d;
+  p->dims.NK = d->NK * instances;
+  p->instances = instances;
+  try {
+    if (d->precision == ZH_FP64) p->eng.reset(new Engine<double>(*d, instances));
+    else p->eng.reset(new Engine<float>(*d, instances));
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
+}
+
 int zh_problem_create(const zh_dims* d, zh_problem** out) {
+  return zh_try([&] { problem_create(d, 1, out); });
+}
+
+int zh_problem_create_instances(const zh_dims* d, int instances, zh_problem** out) {
+  return zh_try([&] { problem_create(d, instances, out); });
+}
+
+int zh_instances(zh_problem* p, int* instances) {
   return zh_try([&] {
-    require(d && out, "null argument");
-    require(d->NK > 0 && d->n > 0 && d->NL > 0 && d->NX > 0 && d->NOT > 0, "dimensions must be positive");
-    require(d->NL <= ZH_MAX_LEADS, "too many leads");
-    require(d->precision == ZH_FP64 || d->precision == ZH_MIXED, "unknown precision");
-    require(d->hbar > 0, "hbar must be positive");
-    int ndev = 0;
-    HIPCHK(hipGetDeviceCount(&ndev));
-    require(d->device >= 0 && d->device < ndev, "no such HIP device");
-    auto* p = new zh_problem();
-    p->dims = *d;
-    try {
-      if (d->precision == ZH_FP64) p->eng.reset(new Engine<double>(*d));
-      else p->eng.reset(new Engine<float>(*d));
-    } catch (...) {
-      delete p;
-      throw;
-    }
-    *out = p;
+    require(p && instances, "null argument");
+    *instances = p->instances;
   });
 }
 
@@ -2980,22 +3074,30 @@ int zh_set_drive(zh_problem* p, int nterms, const int* kinds, const double* amps
   });
 }
 
+static PulseDesc pulse_desc(zh_problem* p, int kind, double amp, const double* par,
+                            const double* w) {
+  require(p && w && par, "null argument");
+  require(kind >= ZH_PULSE_NONE && kind <= ZH_PULSE_SPLINE, "unknown pulse kind");
+  require(!((kind == ZH_PULSE_GAUSS || kind == ZH_PULSE_BOX) && par[1] == 0.0), "zero pulse width");
+  require(!(kind == ZH_PULSE_RAMP && par[0] == 0.0), "zero ramp time");
+  require(!(kind == ZH_PULSE_GENERIC && par[0] == 0.0), "zero envelope width");
+  require(!(kind == ZH_PULSE_STAIRS && (par[2] == 0.0 || par[1] < 0.0 || par[1] > 1e6)),
+          "stairs: softness must be non-zero and the count in [0, 1e6]");
+  require(!(kind == ZH_PULSE_SPLINE && par[2] == 0.0), "zero window softness");
+  PulseDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.kind = kind; d.amp = amp; d.p1 = par[0]; d.p2 = par[1]; d.p3 = par[2]; d.p4 = par[3];
+  for (int a = 0; a < p->dims.NL; ++a) d.w[a] = w[a];
+  return d;
+}
+
 int zh_set_pulse_ex(zh_problem* p, int kind, double amp, const double* par, const double* w) {
-  return zh_try([&] {
-    require(p && w && par, "null argument");
-    require(kind >= ZH_PULSE_NONE && kind <= ZH_PULSE_SPLINE, "unknown pulse kind");
-    require(!((kind == ZH_PULSE_GAUSS || kind == ZH_PULSE_BOX) && par[1] == 0.0), "zero pulse width");
-    require(!(kind == ZH_PULSE_RAMP && par[0] == 0.0), "zero ramp time");
-    require(!(kind == ZH_PULSE_GENERIC && par[0] == 0.0), "zero envelope width");
-    require(!(kind == ZH_PULSE_STAIRS && (par[2] == 0.0 || par[1] < 0.0 || par[1] > 1e6)),
-            "stairs: softness must be non-zero and the count in [0, 1e6]");
-    require(!(kind == ZH_PULSE_SPLINE && par[2] == 0.0), "zero window softness");
-    PulseDesc d;
-    std::memset(&d, 0, sizeof(d));
-    d.kind = kind; d.amp = amp; d.p1 = par[0]; d.p2 = par[1]; d.p3 = par[2]; d.p4 = par[3];
-    for (int a = 0; a < p->dims.NL; ++a) d.w[a] = w[a];
-    p->eng->set_pulse(d);
-  });
+  return zh_try([&] { p->eng->set_pulse(pulse_desc(p, kind, amp, par, w)); });
+}
+
+int zh_instance_pulse(zh_problem* p, int instance, int kind, double amp, const double* par,
+                      const double* w) {
+  return zh_try([&] { p->eng->set_instance_pulse(instance, pulse_desc(p, kind, amp, par, w)); });
 }
 
 int zh_set_pulse_spline(zh_problem* p, int nknots, const double* x, const double* c) {
@@ -3119,9 +3221,15 @@ int zh_rhs(zh_problem* p, const double* Ht, const double* delta, const double* s
   });
 }
 
+static void one_instance(zh_problem* p) {
+  require(p->instances == 1, "a problem with several instances attempts all of them together "
+                             "(zh_attempt_instances)");
+}
+
 int zh_attempt(zh_problem* p, zh_attempt_info* info, double* J) {
   return zh_try([&] {
     require(p && info, "null argument");
+    one_instance(p);
     require(info->h > 0 && std::isfinite(info->h), "step size must be positive and finite");
     p->eng->attempt(info, J);
     if (!std::isfinite(info->te)) throw Error{ZH_ENONFINITE, "non-finite error estimate"};
@@ -3133,6 +3241,7 @@ int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int
                  long* n_steps, long* n_rejected, double* h_final) {
   return zh_try([&] {
     require(p && n_steps && n_rejected, "null argument");
+    one_instance(p);
     require(h0 > 0 && eps > 0, "h0 and eps must be positive");
     const int NLK = p->dims.NL * p->dims.NK;
     std::vector<double> Jtmp(NLK);
@@ -3180,6 +3289,7 @@ int zh_propagate(zh_problem* p, double t0, double t1, double h0, double eps, int
 int zh_attempt_launch(zh_problem* p, const zh_attempt_info* info) {
   return zh_try([&] {
     require(p && info, "null argument");
+    one_instance(p);
     require(info->h > 0 && std::isfinite(info->h), "step size must be positive and finite");
     p->eng->launch(info);
   });
@@ -3188,8 +3298,41 @@ int zh_attempt_launch(zh_problem* p, const zh_attempt_info* info) {
 int zh_attempt_collect(zh_problem* p, zh_attempt_info* info, double* J) {
   return zh_try([&] {
     require(p && info, "null argument");
+    one_instance(p);
     p->eng->collect(info, J);
     if (!std::isfinite(info->te)) throw Error{ZH_ENONFINITE, "non-finite error estimate"};
+  });
+}
+
+static void check_instance_infos(zh_problem* p, const zh_attempt_info* info) {
+  for (int g = 0; g < p->instances; ++g) {
+    const bool frozen = info[g].h == 0.0 && info[g].force_accept;
+    require(frozen || (info[g].h > 0 && std::isfinite(info[g].h)),
+            "instance " + std::to_string(g) + ": step size must be positive and finite (or 0 "
+            "with force_accept to hold a finished instance)");
+  }
+}
+
+int zh_launch_instances(zh_problem* p, const zh_attempt_info* info) {
+  return zh_try([&] {
+    require(p && info, "null argument");
+    check_instance_infos(p, info);
+    p->eng->launch(info);
+  });
+}
+
+int zh_collect_instances(zh_problem* p, zh_attempt_info* info, double* J) {
+  return zh_try([&] {
+    require(p && info, "null argument");
+    p->eng->collect(info, J);
+  });
+}
+
+int zh_attempt_instances(zh_problem* p, zh_attempt_info* info, double* J) {
+  return zh_try([&] {
+    require(p && info, "null argument");
+    check_instance_infos(p, info);
+    p->eng->attempt(info, J);
   });
 }
 

@@ -149,19 +149,24 @@ __device__ inline double pulse_value(const PulseDesc& p, double t) {
   }
 }
 
-// Stage times and biases of one attempt from the device control block.
-__global__ void k_stage_params(const Ctrl* ctrl, PulseDesc pulse, DissDesc diss, DriveDesc drive,
-                               Tableau tab, int NL, StageParams* sp) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double t0 = ctrl->t0, h = ctrl->h;
+// Stage times and biases of one attempt from the device control blocks, one workgroup
+// per instance g of a problem that carries G independent instances (each its own
+// control block, pulse and step): sp[s * G + g].
+__global__ void k_stage_params(const Ctrl* ctrl, const PulseDesc* pulses, DissDesc diss,
+                               DriveDesc drive, Tableau tab, int NL, int G, StageParams* sp) {
+  if (threadIdx.x != 0) return;
+  const int g = blockIdx.x;
+  const PulseDesc pulse = pulses[g];
+  const double t0 = ctrl[g].t0, h = ctrl[g].h;
   for (int s = 0; s < 7; ++s) {
+    StageParams& q = sp[s * G + g];
     const double ts = t0 + tab.c[s] * h;
-    sp[s].ts = ts;
-    sp[s].h = h;
+    q.ts = ts;
+    q.h = h;
     const double v = pulse_value(pulse, ts);
-    for (int a = 0; a < NL; ++a) sp[s].delta[a] = pulse.w[a] * v;
-    sp[s].eta = diss_rate(diss, ts);
-    for (int i = 0; i < ZH_MAX_DRIVE; ++i) sp[s].fdrive[i] = i < drive.nt ? drive_value(drive, i, ts) : 0.0;
+    for (int a = 0; a < NL; ++a) q.delta[a] = pulse.w[a] * v;
+    q.eta = diss_rate(diss, ts);
+    for (int i = 0; i < ZH_MAX_DRIVE; ++i) q.fdrive[i] = i < drive.nt ? drive_value(drive, i, ts) : 0.0;
   }
 }
 
@@ -179,14 +184,15 @@ struct DriveArgs {
   const int* term;
   const double2* coef;
   double2* Hcur; C* Bp; double* Hre; RT* HreT; double2* ell_val;
-  const StageParams* sp;
+  const StageParams* sp;  // the stage's parameters, one per instance of ki k blocks
+  int ki = 1 << 30;
 };
 template <typename C, typename RT>
 __global__ void k_drive_update(DriveArgs<C, RT> a) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < a.E; e += gridDim.x * blockDim.x) {
     double2 v = a.h0[e];
     for (int q = a.beg[e]; q < a.beg[e + 1]; ++q) {
-      const double f = a.sp->fdrive[a.term[q]];
+      const double f = a.sp[a.k[e] / a.ki].fdrive[a.term[q]];
       v.x += f * a.coef[q].x;
       v.y += f * a.coef[q].y;
     }
@@ -206,7 +212,7 @@ __global__ void k_drive_update(DriveArgs<C, RT> a) {
 template <typename C>
 __global__ void k_overlap_model(int NK, int n, int nc, int nt, const double2* SH0,
                                 const double2* SM, const double2* Sinv, const double2* dvec,
-                                const StageParams* sp, double2* Hcur, C* Bp) {
+                                const StageParams* sp, int ki, double2* Hcur, C* Bp) {
   const long nn = (long)n * n, total = (long)NK * nn;
   for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (long)gridDim.x * blockDim.x) {
@@ -214,7 +220,7 @@ __global__ void k_overlap_model(int NK, int n, int nc, int nt, const double2* SH
     const long r = rc / n, c = rc - r * n;
     double2 v = SH0[idx];
     for (int i = 0; i < nt; ++i) {
-      const double f = sp->fdrive[i];
+      const double f = sp[b / ki].fdrive[i];
       const double2 m = SM[(long)i * total + idx];
       v.x += f * m.x;
       v.y += f * m.y;
@@ -357,7 +363,8 @@ struct GatherArgs {
   const double* n0;                          // (NK, n)
   double U, hbar;
   int hubbard;
-  const StageParams* sp;
+  const StageParams* sp;                     // per instance of ki k blocks
+  int ki = 1 << 30;
   const double2* chip; const double2* chim;  // (NK, M)
   const int* lead;                           // (M)
   double2* u; double2* v;                    // (NK, M)
@@ -369,12 +376,12 @@ __global__ void k_gather(GatherArgs<C> g) {
   const long tid0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (blockIdx.y == 0) {
     const long total = (long)g.NK * g.M * g.nc;
-    const double h = g.sp->h;
     for (long idx = tid0; idx < total; idx += stride) {
       const int c = (int)(idx % g.nc);
       const long bm = idx / g.nc;                   // b*M + m
       const int b = (int)(bm / g.M);
       const long m = bm - (long)b * g.M;
+      const double h = g.sp[b / g.ki].h;
       const int o = g.cidx[c];  // -1: padding column of a lead group
       const double2 v = o < 0 ? make_double2(0.0, 0.0) : to_d2(g.Ap[b * g.sAp + m * g.ldAp + o]);
       g.Aomg[bm * 2 * g.nc + c] = from_d2<C>(make_double2(h * v.x, h * v.y));
@@ -382,9 +389,10 @@ __global__ void k_gather(GatherArgs<C> g) {
     }
   } else if (blockIdx.y == 3) {
     const long total = (long)g.NK * g.M;
-    const double s = g.sp->h / g.hbar;
     for (long idx = tid0; idx < total; idx += stride) {
-      const double d = g.sp->delta[g.lead[idx % g.M]];
+      const StageParams& q = g.sp[(int)(idx / g.M) / g.ki];
+      const double s = q.h / g.hbar;
+      const double d = q.delta[g.lead[idx % g.M]];
       const double2 a = g.chip[idx], c = g.chim[idx];
       g.u[idx] = make_double2(s * (a.x + d), s * a.y);
       g.v[idx] = make_double2(s * (c.x + d), s * c.y);
@@ -408,7 +416,8 @@ __global__ void k_gather(GatherArgs<C> g) {
       const int i = (int)(idx - (long)b * g.n);
       double2 d = make_double2(0.0, 0.0);
       if (g.hubbard) {
-        const int pb = g.NK - 1 - b;  // partner spin block
+        const int i0 = b - b % g.ki;          // the instance's first k block
+        const int pb = i0 + g.ki - 1 - (b - i0);  // partner spin block
         const double2 s = g.sig[((long)pb * g.n + i) * g.n + i];
         d = make_double2(g.U * (s.x - g.n0[(long)pb * g.n + i]), g.U * s.y);
       }
@@ -446,8 +455,9 @@ __global__ void k_transpose_h(int n, long ldBp_batch, const double2* H, C* Bp, d
 
 // dsig = (h/hbar) (G + G^H)   (Zandpack/mpi/zand:340-342, 384-387 with G = -iHσ + Π/ħ)
 // With sig_eq, also the dissipator term -h η(ts)(σ - σ_eq) of the stage input σ.
-__global__ void k_sigma_sym(int n, const double2* G, double2* out, const StageParams* sp,
-                            double inv_hbar, const double2* sig_in, const double2* sig_eq) {
+__global__ void k_sigma_sym(int n, const double2* G, double2* out, const StageParams* sps,
+                            int ki, double inv_hbar, const double2* sig_in,
+                            const double2* sig_eq) {
   __shared__ double2 tile[32][33];
   const int b = blockIdx.z;
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
@@ -459,6 +469,7 @@ __global__ void k_sigma_sym(int n, const double2* G, double2* out, const StagePa
     if (i < n && j < n) tile[r][tx] = Gb[(long)i * n + j];
   }
   __syncthreads();
+  const StageParams* sp = sps + b / ki;
   const double f = sp->h * inv_hbar;
   for (int r = ty; r < 32; r += 8) {
     const int i = by + r, j = bx + tx;
@@ -533,9 +544,21 @@ struct TEArgs {
 
 constexpr int kTEBlocks = 1024;
 
+// Instance blockIdx.y of a problem with several instances: its slots start istride
+// elements further on.
+template <typename C> __device__ inline void te_instance(TEArgs<C>& g, long istride) {
+  const long ib = blockIdx.y * istride;
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    if (g.k[j]) g.k[j] += ib;
+  if (g.y0) g.y0 += ib;
+  if (g.y1) g.y1 += ib;
+}
+
 // The slots with a non-zero error weight are packed in front (k[0..NW)).
 template <typename C, typename Map, int NW>
-__global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial) {
+__global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial, long istride) {
+  te_instance(g, istride);
   double acc = 0.0;
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
@@ -567,14 +590,16 @@ __global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // k_te for complex64 slots two elements per thread (16-byte loads), for maps whose rows
 // have even offsets and lengths.
 template <typename C, typename Map, int NW>
-__global__ __launch_bounds__(256) void k_te_pairs(TEArgs<C> g, Map mp, double* partial) {
+__global__ __launch_bounds__(256) void k_te_pairs(TEArgs<C> g, Map mp, double* partial,
+                                                  long istride) {
   static_assert(sizeof(C) == 8, "complex64 pairs");
+  te_instance(g, istride);
   double acc = 0.0;
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
@@ -613,17 +638,28 @@ __global__ __launch_bounds__(256) void k_te_pairs(TEArgs<C> g, Map mp, double* p
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // TE = sqrt(sum), accept iff TE < eps, h_next = 0.9 h (eps/TE)^0.2
 // (Zandpack/mpi/zand:396-407, Zandpack/mpi_RK4pars_dev.py:148-149).
 constexpr int kCtrlThreads = 1024;
-__global__ __launch_bounds__(kCtrlThreads) void k_controller(const double* partial, int np,
-                                                             Ctrl* ctrl) {
+// One workgroup per instance g: its partials are nb of σ (partial[g nb ..]), nb of Ψ
+// (partial[G nb + g nb ..]) and no of Ω (partial[2 G nb + g no ..]), summed as one
+// sequence in that order (with one instance, partial[0 .. 2 nb + no) in place).
+__global__ __launch_bounds__(kCtrlThreads) void k_controller(const double* partial, int nb,
+                                                             int no, int G, Ctrl* ctrls) {
   __shared__ double red[kCtrlThreads];
+  const int g = blockIdx.x;
+  Ctrl* ctrl = ctrls + g;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
+  const int np = 2 * nb + no;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    const long at = i < nb ? (long)g * nb + i
+                  : i < 2 * nb ? (long)G * nb + (long)g * nb + (i - nb)
+                               : 2L * G * nb + (long)g * no + (i - 2 * nb);
+    acc += partial[at];
+  }
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = kCtrlThreads / 2; s > 0; s >>= 1) {
@@ -730,10 +766,11 @@ struct EpiPsi {
   const double2* glp; const double2* chip;  // (NK, M)
   const double2* dvec;          // (NK, n)
   const int* lead; const int* cpos;
-  const StageParams* sp;
+  const StageParams* sp;        // per instance of ki k blocks
   int M, n, nc; double inv_hbar, inv_hbar2;
   int w_parts;  // split-K partials of W, layout (NK, w_parts, M, nc)
   int acc0_rows;
+  int ki = 1 << 30;
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
     const long bm = (long)b * M + row;
     if (acc0 && row < acc0_rows) {
@@ -742,8 +779,9 @@ struct EpiPsi {
       im += a0.y;
     }
     const double2 p = to_d2(Ap[bm * ldAp + col]);
+    const StageParams& q = sp[b / ki];
     double2 chi = chip[bm];
-    chi.x += sp->delta[lead[row]];
+    chi.x += q.delta[lead[row]];
     const double2 d = dvec[(long)b * n + col];
     const double2 cp = cmul(chi, p), dp = cmul(d, p);
     double vr = (re - cp.x + dp.x) * inv_hbar;
@@ -760,7 +798,7 @@ struct EpiPsi {
       vr += gx.x + wr * inv_hbar2;
       vi += gx.y + wi * inv_hbar2;
     }
-    const double h = sp->h;
+    const double h = q.h;
     out[bm * n + col] = from_d2<C>(make_double2(h * vi, -h * vr));
   }
 };
@@ -1213,14 +1251,14 @@ __global__ __launch_bounds__(256) void k_psi_ell(EpiPsi<C> e, long sAp, Ell h) {
     if (i < M) {
       const long bm = (long)b * M + i;
       chi = e.chip[bm];
-      chi.x += e.sp->delta[e.lead[i]];
+      chi.x += e.sp[b / e.ki].delta[e.lead[i]];
       g = e.glp[bm];
     }
     rchi[threadIdx.x] = chi;
     rglp[threadIdx.x] = g;
   }
   __syncthreads();
-  const double hs = e.sp->h;
+  const double hs = e.sp[b / e.ki].h;
   const int* col = h.col + (long)b * h.R * n;
   const double2* val = h.val + (long)b * h.R * n;
   constexpr int CPT = 2;  // columns per thread per pass

@@ -23,16 +23,20 @@ SPARSE = {"auto": 0, "dense": 1, "sparse": 2}
 class Propagator:
     def __init__(self, problem: AMEProblem, precision: str = "fp64", device: int = 0,
                  graph: bool = True, state=None, layout: str = "auto", autotune: bool = False,
-                 hproducts: str = "auto", antihermitian: str = "auto"):
+                 hproducts: str = "auto", antihermitian: str = "auto", _instances: int = 1):
         self.lib = N.lib()
         self.problem = problem
         self.precision = precision
         NK, n, NL, NX, NOT = problem.dims
         self.NK, self.n, self.NL, self.NX, self.NOT = NK, n, NL, NX, NOT
         self.M = NL * NX * NOT
-        d = N.zh_dims(NK, n, NL, NX, NOT, PRECISIONS[precision], float(problem.hbar), device)
+        G = _instances
+        d = N.zh_dims(NK // G, n, NL, NX, NOT, PRECISIONS[precision], float(problem.hbar), device)
         h = C.c_void_p()
-        N.check(self.lib.zh_problem_create(C.byref(d), C.byref(h)))
+        if G == 1:
+            N.check(self.lib.zh_problem_create(C.byref(d), C.byref(h)))
+        else:
+            N.check(self.lib.zh_problem_create_instances(C.byref(d), G, C.byref(h)))
         self.h = h
         N.check(self.lib.zh_set_layout(self.h, LAYOUTS[layout]))
         N.check(self.lib.zh_set_sparse(self.h, SPARSE[hproducts]))
@@ -432,3 +436,102 @@ def batch_attempt(props, t0s, hs, eps, force_accept=False):
         info = infos[i]
         out.append(dict(te=info.te, h_next=info.h_next, accepted=bool(info.accepted), J=Js[i]))
     return out
+
+
+class _Stacked:
+    """Problems of one shape stacked on the k axis (the constants of instance g are its
+    k blocks); pulses stay per instance."""
+
+    def __init__(self, problems):
+        p0 = problems[0]
+        for p in problems:
+            assert p.dims == p0.dims and p.hbar == p0.hbar, "instances must share one shape"
+            assert bool(p.hubbard_U) == bool(p0.hubbard_U) and p.hubbard_U == p0.hubbard_U
+            assert not p.drive and p.dissipator.kind == "none" and p.Sinv is None, \
+                "instances carry a pulse each; drive, dissipator and overlap stay single"
+        NK, n, NL, NX, NOT = p0.dims
+        G = len(problems)
+        self.dims = (NK * G, n, NL, NX, NOT)
+        self.hbar = p0.hbar
+        cat = lambda name: np.concatenate([np.asarray(getattr(p, name)) for p in problems])
+        for name in ("H0", "xi", "Ixi", "GL_P", "dGP", "dGM", "Xpp", "Xpm", "sigma0"):
+            setattr(self, name, cat(name))
+        self.hubbard_U = p0.hubbard_U
+        self.hubbard_n0 = cat("hubbard_n0") if p0.hubbard_U else None
+        self.pulse = p0.pulse
+        self.drive = None
+        self.dissipator = p0.dissipator
+        self.Sinv = None
+
+
+class Instances(Propagator):
+    """Several independent propagations of one shape in one native problem: every
+    kernel of an attempt runs once for all of them (zh_problem_create_instances).  Each
+    instance has its own pulse, time, step, tolerance, error norm and acceptance;
+    ``attempt`` takes one (t0, h, eps) per instance.  Results of an instance are bitwise
+    those of the same problem alone on the same tiles."""
+
+    def __init__(self, problems, precision: str = "fp64", device: int = 0, graph: bool = True,
+                 states=None, layout: str = "auto", autotune: bool = False,
+                 hproducts: str = "auto", antihermitian: str = "auto"):
+        self.problems = list(problems)
+        self.G = len(self.problems)
+        st = _Stacked(self.problems)
+        self.KI = self.problems[0].dims[0]
+        state = None
+        if states is not None:
+            state = tuple(np.concatenate([np.asarray(s[i]) for s in states]) for i in range(3))
+        super().__init__(st, precision, device, graph, state, layout, False, hproducts,
+                         antihermitian, _instances=self.G)
+        for g, p in enumerate(self.problems):
+            self.set_instance_pulse(g, p.pulse)
+        self.tiles = self.autotune() if autotune else None
+
+    def set_instance_pulse(self, g, pulse):
+        w = np.zeros(max(self.NL, 1))
+        w[: len(pulse.weights)] = pulse.weights[: self.NL]
+        w = np.ascontiguousarray(w[: self.NL])
+        assert pulse.kind != "spline", "spline pulses are shared: use set_pulse"
+        par = np.array([pulse.p1, pulse.p2, pulse.p3, pulse.p4], np.float64)
+        N.check(self.lib.zh_instance_pulse(self.h, int(g), N.PULSE_KINDS[pulse.kind],
+                                           float(pulse.amp), N.ptr(par), N.ptr(w)))
+
+    def instance_state(self, g, state=None):
+        """Instance g's (σ, Ψ, Ω) out of the stacked state (the resident one by default)."""
+        s, p, o = self.get_state() if state is None else state
+        k = slice(g * self.KI, (g + 1) * self.KI)
+        return s[k], p[k], o[k]
+
+    def _infos(self, t0, h, eps, force_accept):
+        G = self.G
+        vec = lambda v: np.broadcast_to(np.asarray(v, dtype=float), (G,))
+        t0, h, eps = vec(t0), vec(h), vec(eps)
+        force = np.broadcast_to(np.asarray(force_accept, dtype=bool), (G,))
+        arr = (N.zh_attempt_info * G)()
+        for g in range(G):
+            arr[g] = N.zh_attempt_info(float(t0[g]), float(h[g]), float(eps[g]),
+                                       1 if force[g] else 0, 0.0, 0.0, 0)
+        return arr
+
+    def _results(self, arr, J):
+        return [dict(te=arr[g].te, h_next=arr[g].h_next, accepted=bool(arr[g].accepted),
+                     J=J[:, g * self.KI:(g + 1) * self.KI])
+                for g in range(self.G)]
+
+    def attempt(self, t0, h, eps, force_accept=False):
+        """One attempt of every instance (per-instance or broadcast t0, h, eps); h = 0
+        with force_accept holds a finished instance.  Returns one dict per instance."""
+        arr = self._infos(t0, h, eps, force_accept)
+        J = np.empty((self.NL, self.NK))
+        N.check(self.lib.zh_attempt_instances(self.h, arr, N.ptr(J)))
+        return self._results(arr, J)
+
+    def launch(self, t0, h, eps, force_accept=False):
+        self._pending = self._infos(t0, h, eps, force_accept)
+        N.check(self.lib.zh_launch_instances(self.h, self._pending))
+
+    def collect(self):
+        arr = self._pending
+        J = np.empty((self.NL, self.NK))
+        N.check(self.lib.zh_collect_instances(self.h, arr, N.ptr(J)))
+        return self._results(arr, J)
