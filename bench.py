@@ -431,20 +431,37 @@ def steady_leg(p, tol=1e-10, maxiter=200):
             "tol": tol, "max_dpsi_rel": float(np.abs(dp).max() / scale)}
 
 
-def batched_leg(n_inst, steps, seed=0):
+def batched_leg(n_inst, steps, seed=0, workers=1):
     """Config 4: n_inst uneven propagations (n in {100..800}, M = 4n) advanced
-    together, each for ``steps`` accepted steps; aggregate steps/s."""
+    together, each for ``steps`` accepted steps; aggregate steps/s.  Requests of one n
+    share a shape and run as the instances of one problem (one launch per kernel class
+    for the group); the same workload one problem per request beside it.  The figures
+    with admission include the set-up and upload of every instance (on worker threads,
+    beside the stepping); the stepping figures admit everything first."""
     from zandpack_amd import builder
     from zandpack_amd.serving import Request, Server
     rng = np.random.default_rng(seed)
     ns = rng.choice(np.arange(100, 801, 100), size=n_inst)
-    reqs = [Request(i, builder.config_c4_instance(int(n), seed=seed * 1000 + i), t_end=60.0,
-                    eps=1e-6, h0=0.005, precision="mixed", max_steps=steps)
-            for i, n in enumerate(ns)]
-    st = Server(hbm_bytes=200 << 30, max_live=n_inst).run(reqs, realtime=False)
-    return {"instances": n_inst, "n_values": sorted(int(x) for x in ns), "steps_each": steps,
-            "steps_per_s": st["steps_per_s"], "wall_s": st["wall_s"], "failed": st["failed"],
-            "admit_s_on_worker": st["admit_s"], "precision": "mixed"}
+
+    def reqs():
+        return [Request(i, builder.config_c4_instance(int(n), seed=seed * 1000 + i),
+                        t_end=60.0, eps=1e-6, h0=0.005, precision="mixed", max_steps=steps)
+                for i, n in enumerate(ns)]
+
+    out = {"instances": n_inst, "n_values": sorted(int(x) for x in ns), "steps_each": steps,
+           "precision": "mixed"}
+    for name, g in (("grouped", 16), ("one_problem_each", 1)):
+        srv = Server(hbm_bytes=200 << 30, max_live=n_inst, max_group=g, admit_workers=workers)
+        st = srv.run(reqs(), realtime=False)
+        sp = Server(hbm_bytes=200 << 30, max_live=n_inst, max_group=g,
+                    admit_workers=workers).run(reqs(), realtime=False, preadmit=True)
+        out[name] = {"steps_per_s": st["steps_per_s"], "wall_s": st["wall_s"],
+                     "failed": st["failed"] + sp["failed"], "admit_s_on_workers": st["admit_s"],
+                     "stepping_steps_per_s": sp["steps_per_s"], "stepping_wall_s": sp["wall_s"],
+                     "groups": srv.groups}
+    out["steps_per_s"] = out["grouped"]["steps_per_s"]
+    out["stepping_steps_per_s"] = out["grouped"]["stepping_steps_per_s"]
+    return out
 
 
 def serving_leg(n_req, rate, t_scale, seed=0):

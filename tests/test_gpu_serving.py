@@ -106,3 +106,32 @@ def test_batch_attempt_matches_single(native):
         assert res[i]["te"] == one["te"]
     for p in a + b:
         p.close()
+
+
+def test_same_shape_requests_grouped(native):
+    """Requests of one shape are admitted together as the instances of one problem (one
+    launch per kernel class for all of them); each result is bitwise the same as the
+    request served alone, finished instances held while the others go on."""
+    from zandpack_amd.serving import Request, Server
+    say("serving grouped")
+
+    def reqs():
+        return [Request(i, builder.config_c4_instance(100, seed=20 + i), t_end=60.0, eps=1e-6,
+                        h0=0.005, precision="mixed", max_steps=3 + 2 * i) for i in range(5)]
+
+    alone = []
+    for r in reqs():
+        st = Server(max_group=1).run([r], realtime=False)
+        assert st["failed"] == 0
+        alone.append(st["done"][0])
+    srv = Server(max_group=16)
+    st = srv.run(reqs(), realtime=False)
+    assert st["failed"] == 0 and st["requests"] == 5
+    assert srv.groups == [5]
+    by_id = {r.rid: r for r in st["done"]}
+    for a in alone:
+        b = by_id[a.rid]
+        assert b.steps == a.steps == 3 + 2 * a.rid and b.rejected == a.rejected
+        np.testing.assert_array_equal(np.array(a.J), np.array(b.J))
+        np.testing.assert_array_equal(np.array(a.times), np.array(b.times))
+        np.testing.assert_array_equal(a.sigma, b.sigma)
