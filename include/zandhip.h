@@ -377,6 +377,16 @@ int zh_set_omgxi(zh_problem* p, int mode);
  * problem's layout allows it. */
 int zh_omgxi_info(zh_problem* p, int* out);
 
+/* How the Psi right-hand side runs with a row-compressed H (Zandpack/mpi/zand:746-778):
+ * 0 auto (default: one pass when the rows of H reach at most 160 orbitals around each
+ * 64-orbital tile, as tight-binding H do), 1 split (the dP xi_c sigma_c^T part as a
+ * matrix-core product into the k slot, then the gather kernel), 2 one pass (stage-Psi
+ * rows staged in LDS, that part formed beside the gather; ZH_EINVAL when H reaches
+ * too far).  Both agree to rounding.  No counterpart in the reference. */
+int zh_set_psi_kernel(zh_problem* p, int mode);
+/* *fused = 1 when the attempts run the one-pass Psi kernel. */
+int zh_psi_kernel_info(zh_problem* p, int* fused);
+
 /* Anti-Hermitian Omega: 0 auto (default), 1 off, 2 on.  dOmega_ij/dt (Zandpack/mpi/zand:
  * 784-809) maps Omega_ij -> -conj(Omega_ji) onto itself when every mode has
  * dGM_j Ixi_j = -conj(dGP_j xi_j) and Xpm_j = conj(Xpp_j) (the reference's Lorentzian

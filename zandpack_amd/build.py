@@ -9,7 +9,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 SRC = os.path.join(PKG, "csrc", "zh_api.hip")
 DEPS = [os.path.join(PKG, "csrc", f)
-        for f in ("zh_api.hip", "zh_gemm.hpp", "zh_kernels.hpp", "zh_scf.hpp", "zh_omega.hpp", "zh_omgxi.hpp")] + [
+        for f in ("zh_api.hip", "zh_gemm.hpp", "zh_kernels.hpp", "zh_scf.hpp", "zh_omega.hpp", "zh_omgxi.hpp", "zh_psi.hpp")] + [
     os.path.join(ROOT, "include", "zandhip.h")]
 LIB = os.path.join(PKG, "libzandhip.so")
 ARCH = os.environ.get("ZH_OFFLOAD_ARCH", "gfx950")

@@ -34,6 +34,7 @@
 #include "zh_scf.hpp"
 #include "zh_omega.hpp"
 #include "zh_omgxi.hpp"
+#include "zh_psi.hpp"
 
 namespace zh {
 
@@ -125,6 +126,8 @@ struct EngineBase {
   virtual void set_antihermitian(int mode) = 0;
   virtual int antihermitian_info() const = 0;
   virtual void set_omgxi(int mode) = 0;
+  virtual void set_psi_kernel(int mode) = 0;
+  virtual int psi_kernel_info() const = 0;
   virtual void omgxi_info(int* out) const = 0;
   bool use_graph = true;
   bool profile = false;
@@ -268,6 +271,14 @@ struct Engine : EngineBase {
   bool h_sparse = false;   // the H products run as gathers over the row-compressed H
   int ell_R = 0;           // entries per row of the row-compressed H
   int hpsi_rb = 0;         // mode rows per workgroup of the sparse ψHᵀ kernel
+  // one-pass sparse Ψ right-hand side (zh_psi.hpp): per 64-orbital tile the orbitals its
+  // H rows reach, [x, x + y); used when the widest fits kPsiFusedWMax
+  int2* psi_span = nullptr;
+  int psi_wmax = 0;
+  int psi_req = 0;         // 0 auto (one pass when it fits), 1 split, 2 one pass
+  bool psi_fused() const {
+    return h_sparse && !overlap && psi_wmax > 0 && psi_wmax <= kPsiFusedWMax && psi_req != 1;
+  }
   std::vector<double> H0h; // host copy of H0 (re-installed after an explicit rhs call)
   std::vector<int> perm;   // device mode -> reference mode (-1: inert padding mode)
   // Non-orthogonal basis (the nozand form): S⁻¹ multiplies H in both H products and
@@ -541,7 +552,10 @@ struct Engine : EngineBase {
     // room for the error-norm partials of the finest Ω tile candidate (32 x 32)
     const long t32 = (Md + 31) / 32;
     te_part = dalloc<double>((size_t)2 * kTEBlocks * NI + t32 * t32 * NK);
-    Ap = dalloc<cs>((size_t)NK * Md * (n + nc));
+    // (+ 16 zeroed elements: the one-pass Ψ kernel reads its ΔP ξ_c rows in chunks of 8
+    // that may run past the last row)
+    Ap = dalloc<cs>((size_t)NK * Md * (n + nc) + 16);
+    psi_span = dalloc<int2>((n + 63) / 64);
     Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
     Aomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
     Bomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
@@ -908,6 +922,25 @@ struct Engine : EngineBase {
         }
       HIPCHK(hipMemcpyAsync(ell_col, ell_col_h.data(), sizeof(int) * ell_col_h.size(),
                             hipMemcpyHostToDevice, stream));
+      // per 64-orbital tile: the orbitals the tile's rows of H reach (and the tile itself)
+      const int nt = (n + 63) / 64;
+      std::vector<int2> spn(nt);
+      psi_wmax = 0;
+      for (int ct = 0; ct < nt; ++ct) {
+        int lo = ct * 64, hi = std::min(n, ct * 64 + 64) - 1;
+        for (int b = 0; b < NK; ++b)
+          for (int r = 0; r < R_; ++r)
+            for (int o = ct * 64; o < std::min(n, ct * 64 + 64); ++o) {
+              const int c = ell_col_h[((size_t)b * R_ + r) * n + o];
+              lo = std::min(lo, c);
+              hi = std::max(hi, c);
+            }
+        spn[ct] = make_int2(lo, hi - lo + 1);
+        psi_wmax = std::max(psi_wmax, hi - lo + 1);
+      }
+      HIPCHK(hipMemcpyAsync(psi_span, spn.data(), sizeof(int2) * nt, hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipStreamSynchronize(stream));  // spn is read asynchronously
       HIPCHK(hipMemcpyAsync(ell_val, ell_val_h.data(), sizeof(double2) * ell_val_h.size(),
                             hipMemcpyHostToDevice, stream));
     }
@@ -1481,6 +1514,16 @@ struct Engine : EngineBase {
     xi_req = mode;
     drop_graphs();
   }
+  void set_psi_kernel(int mode) override {
+    require(mode >= 0 && mode <= 2, "Psi kernel: 0 auto, 1 split, 2 one pass");
+    if (mode == 2)
+      require(!h_sparse || (psi_wmax > 0 && psi_wmax <= kPsiFusedWMax),
+              "one-pass Psi kernel: the rows of H reach more than " +
+                  std::to_string(kPsiFusedWMax) + " orbitals per 64-orbital tile");
+    psi_req = mode;
+    drop_graphs();
+  }
+  int psi_kernel_info() const override { return psi_fused() ? 1 : 0; }
   void omgxi_info(int* out) const override {
     out[0] = xi_stream() && !fusable() ? 1 : 0;
     out[1] = xi_ok ? 1 : 0;
@@ -1631,8 +1674,13 @@ struct Engine : EngineBase {
                  ovl ? Qs : nullptr};
       if (h_sparse) {
         constexpr int rb = 8;
-        hipLaunchKernelGGL((k_hsig_ell<rb, EpiSigma>), dim3((n + 255) / 256, (n + rb - 1) / rb, NK),
-                           dim3(256), 0, st, sig_in, ell(), e);
+        const dim3 grid((n + 255) / 256, (n + rb - 1) / rb, NK);
+        if (ell_R == 4)
+          hipLaunchKernelGGL((k_hsig_ell<rb, 4, EpiSigma>), grid, dim3(256), 0, st, sig_in, ell(), e);
+        else if (ell_R == 8)
+          hipLaunchKernelGGL((k_hsig_ell<rb, 8, EpiSigma>), grid, dim3(256), 0, st, sig_in, ell(), e);
+        else
+          hipLaunchKernelGGL((k_hsig_ell<rb, 0, EpiSigma>), grid, dim3(256), 0, st, sig_in, ell(), e);
         HIPCHK(hipGetLastError());
       } else if (h_real) {  // real H: the A operand is Re H
         g.Areal = Hre; g.ldar = n; g.sAr = (long)n * n;
@@ -1719,7 +1767,29 @@ struct Engine : EngineBase {
       });
     }
     timed(KC_GEMM_PSI, [&] {
-      if (h_sparse) {
+      if (psi_fused() && !ovl) {
+        // one pass: stage-Ψ tile rows in LDS, ħ ΔP(σξ) on the VALU, gather, epilogue
+        PsiFusedArgs<cs> f{};
+        f.out = psi[kslot];
+        f.Ap = Ap; f.sAp = (long)Md * (n + nc); f.ldAp = n + nc;
+        f.Bc = Bp + (long)n * n; f.sBc = (long)(n + nc) * n;
+        f.xic = xic; f.W = Wc; f.glp = glp; f.chip = chip; f.dvec = dvec_epi();
+        f.lead = lead; f.cpos = cpos; f.t2 = t2; f.span = psi_span;
+        f.col = ell_col; f.val = ell_val; f.R = ell_R;
+        f.sp = sps; f.ki = KI;
+        f.Md = Md; f.n = n; f.nc = nc; f.acc0_rows = nc > 0 ? g0_rows() : 0;
+        f.inv_hbar = ih; f.inv_hbar2 = ih * ih;
+        constexpr int TR = 16;  // mode rows per workgroup (C3: 16 beat 8, 32 and 64)
+        const size_t lds = (size_t)TR * psi_wmax * sizeof(cs);
+        const dim3 grid((n + 63) / 64, (Md + TR - 1) / TR, NK);
+        if (ell_R == 4)
+          hipLaunchKernelGGL((k_psi_fused<cs, TR, 4>), grid, dim3(256), lds, st, f);
+        else if (ell_R == 8)
+          hipLaunchKernelGGL((k_psi_fused<cs, TR, 8>), grid, dim3(256), lds, st, f);
+        else
+          hipLaunchKernelGGL((k_psi_fused<cs, TR, 0>), grid, dim3(256), lds, st, f);
+        HIPCHK(hipGetLastError());
+      } else if (h_sparse) {
         // [ΔPξ_c] [ħ σ_cᵀ] into the k slot (MFMA, K = nc), then the gather ψHᵀ with
         // the Ψ epilogue adding it
         EpiStore<cs> e0{psi[kslot], n, (long)Md * n, 1.0, 0.0};
@@ -3419,6 +3489,20 @@ int zh_set_fusion(zh_problem* p, int mode) {
   return zh_try([&] {
     require(p, "null argument");
     p->eng->set_fusion(mode);
+  });
+}
+
+int zh_set_psi_kernel(zh_problem* p, int mode) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_psi_kernel(mode);
+  });
+}
+
+int zh_psi_kernel_info(zh_problem* p, int* fused) {
+  return zh_try([&] {
+    require(p && fused, "null argument");
+    *fused = p->eng->psi_kernel_info();
   });
 }
 

@@ -696,8 +696,11 @@ struct EpiSigma {
   int pi_parts;  // split-K partials of Π_c, layout (NK, pi_parts, n, nc)
   const double2* Q = nullptr;  // (NK, n, n) or null
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
+    apply(b, row, col, re, im, sig[((long)b * n + row) * n + col]);
+  }
+  // the same with the stage input's element s = σ[row, col] already loaded
+  __device__ inline void apply(int b, int row, int col, double re, double im, double2 s) const {
     const long off = ((long)b * n + row) * n + col;
-    const double2 s = sig[off];
     const double2 d = dvec[(long)b * n + row];
     const double2 ds = cmul(d, s);
     double gr = im + ds.y, gi = -re - ds.x;  // -i (acc + d s)
@@ -1332,29 +1335,57 @@ __global__ __launch_bounds__(256) void k_psi_ell(EpiPsi<C> e, long sAp, Ell h) {
 }
 
 // acc[o, j] = (H σ)[o, j] = Σ_r H_r(o) σ[col_r(o), j] for RB rows o per workgroup
-// and 256 consecutive columns j; acc goes to epi (the σ right-hand side).
-template <int RB, typename Epi>
-__global__ __launch_bounds__(256) void k_hsig_ell(const double2* sig, Ell e, Epi epi) {
+// and 256 consecutive columns j; acc goes to epi (the σ right-hand side).  With the
+// entries per row R known at compile time, a thread issues every gather of its RB
+// rows and their epilogue's σ elements before the first product (RB x (R + 1) loads
+// in flight); R = 0 takes e.R at run time, one row after another.
+template <int RB, int R, typename Epi>
+__global__ __launch_bounds__(256) void k_hsig_ell(const double2* __restrict__ sig, Ell e,
+                                                  Epi epi) {
   const int b = blockIdx.z;
   const int n = e.n;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int o0 = blockIdx.y * RB;
   if (j >= n) return;
-  const int* col = e.col + (long)b * e.R * n;
-  const double2* val = e.val + (long)b * e.R * n;
-  const double2* s = sig + (long)b * n * n;
-  for (int q = 0; q < RB; ++q) {
-    const int o = o0 + q;
-    if (o >= n) break;
-    double ar = 0.0, ai = 0.0;
-    for (int r = 0; r < e.R; ++r) {
-      const int c = col[(long)r * n + o];
-      const double2 v = val[(long)r * n + o];
-      const double2 x = s[(long)c * n + j];
-      ar += v.x * x.x - v.y * x.y;
-      ai += v.x * x.y + v.y * x.x;
+  const int* __restrict__ col = e.col + (long)b * e.R * n;
+  const double2* __restrict__ val = e.val + (long)b * e.R * n;
+  const double2* __restrict__ s = sig + (long)b * n * n + j;
+  if constexpr (R > 0) {
+    double2 x[RB][R], sv[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int o = min(o0 + q, n - 1);  // rows past n: loads kept in bounds, no output
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[q][r] = s[(long)col[(long)r * n + o] * n];
+      sv[q] = s[(long)o * n];
     }
-    epi(b, o, j, ar, ai);
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int o = o0 + q;
+      if (o >= n) break;
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double2 v = val[(long)r * n + o];
+        ar += v.x * x[q][r].x - v.y * x[q][r].y;
+        ai += v.x * x[q][r].y + v.y * x[q][r].x;
+      }
+      epi.apply(b, o, j, ar, ai, sv[q]);
+    }
+  } else {
+    for (int q = 0; q < RB; ++q) {
+      const int o = o0 + q;
+      if (o >= n) break;
+      double ar = 0.0, ai = 0.0;
+      for (int r = 0; r < e.R; ++r) {
+        const int c = col[(long)r * n + o];
+        const double2 v = val[(long)r * n + o];
+        const double2 x = s[(long)c * n];
+        ar += v.x * x.x - v.y * x.y;
+        ai += v.x * x.y + v.y * x.x;
+      }
+      epi(b, o, j, ar, ai);
+    }
   }
 }
 

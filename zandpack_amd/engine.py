@@ -189,6 +189,17 @@ class Propagator:
         the layout does not allow it); same results to rounding."""
         N.check(self.lib.zh_set_omgxi(self.h, {"auto": 0, "gemm": 1, "stream": 2}[mode]))
 
+    def set_psi_kernel(self, mode="auto"):
+        """How the Ψ right-hand side runs with a row-compressed H: "auto" (one pass when
+        the rows of H stay near their tile), "split" (matrix-core ΔPξ_cσ_cᵀ product, then
+        the gather kernel) or "fused" (one pass); same results to rounding."""
+        N.check(self.lib.zh_set_psi_kernel(self.h, {"auto": 0, "split": 1, "fused": 2}[mode]))
+
+    def psi_kernel_info(self):
+        v = C.c_int()
+        N.check(self.lib.zh_psi_kernel_info(self.h, C.byref(v)))
+        return "fused" if v.value else "split"
+
     def omgxi_info(self):
         """{"stream", "possible"}: whether the attempts run the streaming Ωξ kernel,
         whether the problem's layout allows it."""
