@@ -1781,14 +1781,15 @@ struct Engine : EngineBase {
         f.Md = Md; f.n = n; f.nc = nc; f.acc0_rows = nc > 0 ? g0_rows() : 0;
         f.inv_hbar = ih; f.inv_hbar2 = ih * ih;
         constexpr int TR = 16;  // mode rows per workgroup (C3: 16 beat 8, 32 and 64)
-        const size_t lds = (size_t)TR * psi_wmax * sizeof(cs);
+        constexpr bool XL = kPsiXL<cs>;
+        const size_t lds = (size_t)TR * (psi_wmax + (XL ? 64 : 0)) * sizeof(cs);
         const dim3 grid((n + 63) / 64, (Md + TR - 1) / TR, NK);
         if (ell_R == 4)
-          hipLaunchKernelGGL((k_psi_fused<cs, TR, 4>), grid, dim3(256), lds, st, f);
+          hipLaunchKernelGGL((k_psi_fused<cs, TR, 4, XL>), grid, dim3(256), lds, st, f);
         else if (ell_R == 8)
-          hipLaunchKernelGGL((k_psi_fused<cs, TR, 8>), grid, dim3(256), lds, st, f);
+          hipLaunchKernelGGL((k_psi_fused<cs, TR, 8, XL>), grid, dim3(256), lds, st, f);
         else
-          hipLaunchKernelGGL((k_psi_fused<cs, TR, 0>), grid, dim3(256), lds, st, f);
+          hipLaunchKernelGGL((k_psi_fused<cs, TR, 0, XL>), grid, dim3(256), lds, st, f);
         HIPCHK(hipGetLastError());
       } else if (h_sparse) {
         // [ΔPξ_c] [ħ σ_cᵀ] into the k slot (MFMA, K = nc), then the gather ψHᵀ with

@@ -8,11 +8,11 @@
 //   * loads its stage-Ψ rows over the orbitals the tile's H rows reach (the tile's
 //     columns and their neighbours, [lo, lo + W) from a host table: W <= WMAX) into LDS,
 //   * forms ħ ΔP_i (σ ξ_i)[o] = Σ_c (ΔP ξ_c)[i, c] (ħ σ_c)[c, o] over the row tile's
-//     lead orbitals on the VALU while those loads are in flight: (ΔP ξ_c)[i, c] is the
-//     same for every lane of a wave (its rows), (ħ σ_c)[c, o] is one coalesced load,
+//     lead orbitals on the matrix cores while those loads are in flight,
 //   * gathers (H ψ_i)[o] from the LDS rows and applies the epilogue,
-// so the stage Ψ is read once and k written once.  Thread layout: lane = orbital o of
-// the tile, wave w = rows [w TR/4, (w+1) TR/4) of the tile.
+// so the stage Ψ is read once and k written once.  Thread layout: wave w owns orbitals
+// [o0 + 16 w, o0 + 16 w + 16) of all 16 rows; a lane holds one orbital and 4 rows,
+// where the matrix-core product leaves them.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,6 +21,12 @@
 namespace zh {
 
 constexpr int kPsiFusedWMax = 160;  // widest gather range of a 64-orbital tile
+// Whether the product's result goes through LDS so that the gather and epilogue run
+// on wave-uniform rows (lane = orbital: scalar loads of the per-row values, 64-orbital
+// stores).  Measured at n = 1000, M = 4000: complex64 85 µs with it, 100 without;
+// complex128 116 µs without it, 128 with (the 16 KB more LDS per workgroup costs a
+// workgroup per CU).
+template <typename C> constexpr bool kPsiXL = sizeof(C) == 8;
 
 template <typename C>
 struct PsiFusedArgs {
@@ -39,29 +45,37 @@ struct PsiFusedArgs {
   double inv_hbar, inv_hbar2;
 };
 
-template <typename C, int TR, int RC>
+template <typename C, int TR, int RC, bool XL>
 __global__ __launch_bounds__(256) void k_psi_fused(PsiFusedArgs<C> a) {
+  static_assert(TR == 16, "one 16-row matrix-core block per workgroup");
   typedef decltype(C::x) T;
-  constexpr int RW = TR / 4;  // rows per wave
+  typedef Mfma<T> MF;
+  typedef typename MF::acc_t acc_t;
+  constexpr bool F64 = sizeof(T) == 8;
   extern __shared__ __align__(16) unsigned char psi_lds[];
   C* tile = reinterpret_cast<C*>(psi_lds);  // TR x W
   const int b = blockIdx.z;
   const int o0 = blockIdx.x * 64, m0 = blockIdx.y * TR;
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int lk = lane >> 4, lc = lane & 15;
   const int n = a.n, Md = a.Md;
-  const int o = o0 + lane;
-  const bool on = o < n;
   const int2 spn = a.span[blockIdx.x];
   const int lo = spn.x, W = spn.y;
+  // the lane's orbital in the product: column lc of the wave's 16-orbital block; in
+  // the gather and epilogue the same with XL false, else orbital o0 + lane (wave-
+  // uniform rows, the product exchanged through LDS)
+  const int o2 = o0 + 16 * w + lc;
+  const int o = XL ? o0 + lane : o2;
+  const bool on = o < n;
   const C* __restrict__ Ab = a.Ap + (long)b * a.sAp;
 
   // 1. the stage-Ψ rows of the tile over [lo, lo + W): batches of LB independent loads
-  //    per thread (all in flight), then their LDS stores
-  constexpr int LB = 16;
+  //    per thread; the first batch stays in flight over the matrix-core part
+  constexpr int LB = F64 ? 8 : 16;
   const int total = TR * W;
-  for (int e0 = 0; e0 < total; e0 += 256 * LB) {
-    C v[LB];
+  C v[LB];
+  auto load_batch = [&](int e0) {
 #pragma unroll
     for (int u = 0; u < LB; ++u) {
       const int e = e0 + u * 256 + t;
@@ -70,51 +84,71 @@ __global__ __launch_bounds__(256) void k_psi_fused(PsiFusedArgs<C> a) {
       v[u].x = 0; v[u].y = 0;
       if (e < total) v[u] = Ab[(long)i * a.ldAp + lo + x];
     }
+  };
+  auto store_batch = [&](int e0) {
 #pragma unroll
     for (int u = 0; u < LB; ++u) {
       const int e = e0 + u * 256 + t;
       if (e < total) tile[e] = v[u];
     }
-  }
+  };
+  load_batch(0);
 
-  // 2. ħ ΔP_i (σ ξ_i)[o] for the wave's rows, in chunks of KB modes: the orbital's KB
-  //    values of ħ σ_c in registers, each row's KB values of ΔP ξ_c the same for the
-  //    whole wave (rows past Md read the last row; they are never stored)
-  constexpr int KB = 8;
-  T ar[RW], ai[RW];
-#pragma unroll
-  for (int q = 0; q < RW; ++q) { ar[q] = 0; ai[q] = 0; }
+  // 2. ħ ΔP_i (σ ξ_i)[o] = Σ_c (ΔP ξ_c)[i, c] (ħ σ_c)[c, o] for the tile's 16 rows and
+  //    the wave's 16 orbitals on the matrix cores, over the row tile's coupled-orbital
+  //    range in steps of 16 modes: lane (lc, lk) holds modes k0 + 4 lk + j, j < 4, of
+  //    row m0 + lc (A) and of orbital o (B), and product j sums them over lk
+  acc_t cr = MF::zero(), ci = MF::zero(), c3 = MF::zero();
   if (m0 < a.acc0_rows) {
     const int2 kr = a.t2[m0 >> 6];
-    const C* __restrict__ Bb = a.Bc + (long)b * a.sBc;
-    const int i0 = m0 + RW * w;
-    const int oc = on ? o : n - 1;
-    for (int k0 = kr.x; k0 < kr.y; k0 += KB) {
-      C bv[KB];
+    const C* __restrict__ arow = Ab + (long)min(m0 + lc, Md - 1) * a.ldAp + n;
+    const C* __restrict__ bcol = a.Bc + (long)b * a.sBc + min(o2, n - 1);
+    for (int k0 = kr.x; k0 < kr.y; k0 += 16) {
+      C av[4], bv[4];
 #pragma unroll
-      for (int u = 0; u < KB; ++u) {
-        bv[u].x = 0; bv[u].y = 0;
-        if (k0 + u < kr.y) bv[u] = Bb[(long)(k0 + u) * n + oc];
+      for (int j = 0; j < 4; ++j) {
+        const int k = k0 + 4 * lk + j;
+        av[j].x = 0; av[j].y = 0; bv[j].x = 0; bv[j].y = 0;
+        if (k < kr.y) {
+          av[j] = arow[k];
+          bv[j] = bcol[(long)k * n];
+        }
       }
 #pragma unroll
-      for (int q = 0; q < RW; ++q) {
-        // one wave-uniform row of KB values (contiguous: wide scalar loads); modes past
-        // the range read on into the next row or the allocation's zeroed slack, times
-        // bv = 0
-        const C* __restrict__ arow = Ab + (long)min(i0 + q, Md - 1) * a.ldAp + n + k0;
-#pragma unroll
-        for (int u = 0; u < KB; ++u) {
-          const C av = arow[u];
-          ar[q] += av.x * bv[u].x - av.y * bv[u].y;
-          ai[q] += av.x * bv[u].y + av.y * bv[u].x;
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (F64) {  // three real products per complex one
+          c3 = MF::op(av[j].x + av[j].y, bv[j].x, c3);
+          ci = MF::op(av[j].x, bv[j].y - bv[j].x, ci);
+          cr = MF::op(-av[j].y, bv[j].x + bv[j].y, cr);
+        } else {
+          cr = MF::op(av[j].x, bv[j].x, cr);
+          ci = MF::op(av[j].x, bv[j].y, ci);
+          cr = MF::op(-av[j].y, bv[j].y, cr);
+          ci = MF::op(av[j].y, bv[j].x, ci);
         }
       }
     }
   }
+  store_batch(0);
+  if constexpr (XL) {
+    C* xl = tile + TR * W;  // TR x 64
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      C x;
+      x.x = F64 ? cr[q] + c3[q] : cr[q];
+      x.y = F64 ? ci[q] + c3[q] : ci[q];
+      xl[MF::row(lk, q) * 64 + 16 * w + lc] = x;
+    }
+  }
+  for (int e0 = 256 * LB; e0 < total; e0 += 256 * LB) {
+    load_batch(e0);
+    store_batch(e0);
+  }
   __syncthreads();
   if (!on) return;
 
-  // 3. (H ψ_i)[o] from the LDS rows, and the epilogue
+  // 3. (H ψ_i)[o] from the LDS rows, and the epilogue, for the lane's 4 rows of the
+  //    product's output layout
   const int R = RC > 0 ? RC : a.R;
   const int* __restrict__ colb = a.col + (long)b * a.R * n + o;
   const double2* __restrict__ valb = a.val + (long)b * a.R * n + o;
@@ -132,12 +166,21 @@ __global__ __launch_bounds__(256) void k_psi_fused(PsiFusedArgs<C> a) {
   const StageParams& sq = a.sp[b / a.ki];
   const double hs = sq.h;
 #pragma unroll
-  for (int q = 0; q < RW; ++q) {
-    const int rl = RW * w + q;
+  for (int q = 0; q < 4; ++q) {
+    const int rl = XL ? 4 * w + q : MF::row(lk, q);
     const int i = m0 + rl;
-    if (i >= Md) break;
+    if (XL && i >= Md) break;
+    if (i >= Md) continue;
     const C* row = tile + rl * W;
-    double re = (double)ar[q], im = (double)ai[q];
+    double re, im;
+    if constexpr (XL) {
+      const C x = tile[TR * W + rl * 64 + lane];
+      re = (double)x.x;
+      im = (double)x.y;
+    } else {
+      re = (double)(F64 ? cr[q] + c3[q] : cr[q]);
+      im = (double)(F64 ? ci[q] + c3[q] : ci[q]);
+    }
     if constexpr (RC > 0) {
 #pragma unroll
       for (int r = 0; r < RC; ++r) {
