@@ -205,7 +205,6 @@ __global__ __launch_bounds__(256, OCC) void k_xi_stream(XiArgs<C> a) {
   typedef XiGeo<C, NG> G;
   typedef decltype(C::x) T;
   typedef typename Mfma<T>::acc_t acc_t;
-  constexpr bool F64 = sizeof(T) == 8;
   static_assert(STAGES >= 2 && (STAGES - 2) * G::NL < 64, "pipeline depth");
   __shared__ __attribute__((aligned(1024))) unsigned char lds[STAGES * G::STAGE];
 
@@ -250,64 +249,80 @@ __global__ __launch_bounds__(256, OCC) void k_xi_stream(XiArgs<C> a) {
     }
   };
 
-  acc_t cr[NG], ci[NG], c3[F64 ? NG : 1];
+  constexpr bool M3 = sizeof(T) == 8;
+  acc_t cr[NG], ci[NG], c3[M3 ? NG : 1];
 #pragma unroll
   for (int t = 0; t < NG; ++t) {
     cr[t] = Mfma<T>::zero();
     ci[t] = Mfma<T>::zero();
-    if constexpr (F64) c3[t] = Mfma<T>::zero();
+    if constexpr (M3) c3[t] = Mfma<T>::zero();
   }
   const int rr = 16 * w + (lane & 15);
-  // one chunk from LDS: every operand of the chunk is read first (KC/4 mode steps x (1 +
-  // NG) values per lane), then the matrix-core products run back to back
-  auto compute = [&](const unsigned char* Ab, auto mirc) {
-    constexpr int MIR = decltype(mirc)::value;
-    constexpr int NS = G::KC / 4;
+  // the operands of one chunk in registers: KC/4 mode steps x (1 + NG) values per lane,
+  // the mirrored chunks' sign already applied
+  constexpr int NS = G::KC / 4;
+  struct Ops {
+    C av[NS];
+    C bv[NS][NG];
+  };
+  auto read = [&](int q, Ops& o) {
+    int kc0, mir;
+    xi_chunk<G::KC>(P, q0 + q, kc0, mir);
+    const unsigned char* Ab = lds + (q % STAGES) * G::STAGE;
     const unsigned char* Bb = Ab + G::A_BYTES;
-    C av[NS], bv[NS][NG];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int kl = 4 * s + (lane >> 4);
-      av[s] = *(const C*)(Ab + xi_lds_a<C, NG>(rr, kl, MIR));
+      if (mir) {
+        o.av[s] = *(const C*)(Ab + xi_lds_a<C, NG>(rr, kl, 1));
+        o.av[s].x = -o.av[s].x;
+      } else {
+        o.av[s] = *(const C*)(Ab + xi_lds_a<C, NG>(rr, kl, 0));
+      }
 #pragma unroll
       for (int t = 0; t < NG; ++t)
-        bv[s][t] = *(const C*)(Bb + xi_lds_b<C, NG>(kl, 16 * t + (lane & 15)));
+        o.bv[s][t] = *(const C*)(Bb + xi_lds_b<C, NG>(kl, 16 * t + (lane & 15)));
     }
+  };
+  // complex128: three real products per complex one,
+  //   re = (-ai)(br + bi) + (ar + ai) br,  im = ar (bi - br) + (ar + ai) br;
+  // complex64 keeps four (three were 12% faster at n = 1000, M = 4000, but their
+  // larger FP32 rounding moves the mixed error norm past 1e-4 of the FP64 one)
+  auto mma = [&](const Ops& o) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const T ar = MIR ? -av[s].x : av[s].x, ai = av[s].y;
-      if constexpr (F64) {  // three real products per complex one
-        const T sa = ar + ai, nai = -ai;
+      const T ar = o.av[s].x, ai = o.av[s].y;
+      const T nai = -ai;
+      if constexpr (M3) {
+        const T sa = ar + ai;
 #pragma unroll
         for (int t = 0; t < NG; ++t) {
-          c3[t] = Mfma<T>::op(sa, bv[s][t].x, c3[t]);
-          ci[t] = Mfma<T>::op(ar, bv[s][t].y - bv[s][t].x, ci[t]);
-          cr[t] = Mfma<T>::op(nai, bv[s][t].x + bv[s][t].y, cr[t]);
+          c3[t] = Mfma<T>::op(sa, o.bv[s][t].x, c3[t]);
+          ci[t] = Mfma<T>::op(ar, o.bv[s][t].y - o.bv[s][t].x, ci[t]);
+          cr[t] = Mfma<T>::op(nai, o.bv[s][t].x + o.bv[s][t].y, cr[t]);
         }
       } else {
-        const T nai = -ai;
 #pragma unroll
         for (int t = 0; t < NG; ++t) {
-          cr[t] = Mfma<T>::op(ar, bv[s][t].x, cr[t]);
-          ci[t] = Mfma<T>::op(ar, bv[s][t].y, ci[t]);
-          cr[t] = Mfma<T>::op(nai, bv[s][t].y, cr[t]);
-          ci[t] = Mfma<T>::op(ai, bv[s][t].x, ci[t]);
+          cr[t] = Mfma<T>::op(ar, o.bv[s][t].x, cr[t]);
+          ci[t] = Mfma<T>::op(ar, o.bv[s][t].y, ci[t]);
+          cr[t] = Mfma<T>::op(nai, o.bv[s][t].y, cr[t]);
+          ci[t] = Mfma<T>::op(ai, o.bv[s][t].x, ci[t]);
         }
       }
     }
   };
 
+  // chunk q: wait for it, barrier, read its operands, products, then the next loads
+  // into chunk q - 1's buffer, issued behind this chunk's products
   for (int q = 0; q < STAGES - 1 && q < nq; ++q) issue(q);
   for (int q = 0; q < nq; ++q) {
     if (q + STAGES - 2 < nq) xi_vmwait<(STAGES - 2) * G::NL>();
     else xi_vmwait<0>();
     xi_barrier();  // chunk q is in LDS for every wave; chunk q - 1's buffer is free
-    int kc0, mir;
-    xi_chunk<G::KC>(P, q0 + q, kc0, mir);
-    const unsigned char* Ab = lds + (q % STAGES) * G::STAGE;
-    if (mir) compute(Ab, std::integral_constant<int, 1>());
-    else compute(Ab, std::integral_constant<int, 0>());
-    // the next loads into chunk q - 1's buffer, issued behind this chunk's products
+    Ops o;
+    read(q, o);
+    mma(o);
     if (q + STAGES - 1 < nq) issue(q + STAGES - 1);
   }
 
@@ -320,8 +335,8 @@ __global__ __launch_bounds__(256, OCC) void k_xi_stream(XiArgs<C> a) {
       const int row = m0 + 16 * w + Mfma<T>::row(lane >> 4, r);
       if (row < a.Md && col < ncol) {
         C v;
-        v.x = F64 ? cr[t][r] + c3[F64 ? t : 0][r] : cr[t][r];
-        v.y = F64 ? ci[t][r] + c3[F64 ? t : 0][r] : ci[t][r];
+        v.x = M3 ? cr[t][r] + c3[M3 ? t : 0][r] : cr[t][r];
+        v.y = M3 ? ci[t][r] + c3[M3 ? t : 0][r] : ci[t][r];
         out[(long)row * a.nc + c0 + col] = v;
       }
     }
