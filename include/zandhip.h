@@ -387,6 +387,22 @@ int zh_set_psi_kernel(zh_problem* p, int mode);
 /* *fused = 1 when the attempts run the one-pass Psi kernel. */
 int zh_psi_kernel_info(zh_problem* p, int* fused);
 
+/* First same as last: 0 on (default), 1 off.  DOP54's seventh stage is evaluated at
+ * (t0 + h, y_new); after an accepted attempt its k_7 = h f(t0 + h, y_new) is kept in the
+ * k_1 slot and the next attempt (same state, step h') takes k_1 = (h'/h) k_7 instead of
+ * evaluating its first stage (Zandpack/mpi/zand:303-387 evaluates all seven stages,
+ * i == 0 included), when f is the same there: the stage parameters at its t0 (biases,
+ * dissipator rate, drive factors; f depends on t only through them) equal those of that
+ * seventh stage bit for bit, and nothing was changed in between (a new state, model,
+ * tiles or kernels).  The currents of the first stage then come from the seventh
+ * stage's y_new, the same state.  Decided per instance on the device; attempts of the
+ * fused mixed-precision Omega kernel always evaluate it.  No counterpart in the
+ * reference (a device schedule: same results to rounding). */
+int zh_set_fsal(zh_problem* p, int mode);
+/* *taken: instance attempts so far that took k_7 as their k_1; *on: 1 when the attempts
+ * may (mode on and the kernels support it).  Either pointer may be null. */
+int zh_fsal_info(zh_problem* p, long* taken, int* on);
+
 /* Anti-Hermitian Omega: 0 auto (default), 1 off, 2 on.  dOmega_ij/dt (Zandpack/mpi/zand:
  * 784-809) maps Omega_ij -> -conj(Omega_ji) onto itself when every mode has
  * dGM_j Ixi_j = -conj(dGP_j xi_j) and Xpm_j = conj(Xpp_j) (the reference's Lorentzian

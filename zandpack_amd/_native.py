@@ -105,6 +105,8 @@ SIGNATURES = {
     "zh_set_omgxi": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_set_psi_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_psi_kernel_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "zh_set_fsal": (C.c_int, [C.c_void_p, C.c_int]),
+    "zh_fsal_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_int)]),
     "zh_omgxi_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "zh_set_antihermitian": (C.c_int, [C.c_void_p, C.c_int]),
     "zh_antihermitian_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

@@ -128,6 +128,8 @@ struct EngineBase {
   virtual void set_omgxi(int mode) = 0;
   virtual void set_psi_kernel(int mode) = 0;
   virtual int psi_kernel_info() const = 0;
+  virtual void set_fsal(int mode) = 0;
+  virtual void fsal_info(long* taken, int* on) const = 0;
   virtual void omgxi_info(int* out) const = 0;
   bool use_graph = true;
   bool profile = false;
@@ -377,6 +379,17 @@ struct Engine : EngineBase {
   StageParams* sp = nullptr;
   double* Jd = nullptr;
   double* Jpart = nullptr;
+  // first same as last (the attempt's k_1 is the previous accepted attempt's k_7): per
+  // instance the device's decision (skip1, written by k_stage_params) and the currents
+  // the last stage formed from y_new (Jn); on the host, which instances may offer it
+  // at their next attempt and the step that made their k_7
+  int* skip1 = nullptr;
+  double* Jn = nullptr;
+  std::vector<char> fsal_ready;
+  std::vector<double> fsal_hprev;
+  int fsal_req = 0;       // 0 on, 1 off
+  long fsal_taken = 0;    // instance attempts that reused k_7
+  const int* skip_now = nullptr;  // skip1 while the first stage of an attempt is issued
   int iy0 = 7, iyn = 8;
   int tsel[TS_N] = {0, 0, 0, 0, 0};  // chosen candidate tile per product
   // pinned host mailboxes
@@ -425,6 +438,10 @@ struct Engine : EngineBase {
     pulse_d = dalloc<PulseDesc>(NI);
     Jd = dalloc<double>((size_t)NL * NK);
     Jpart = dalloc<double>((size_t)NL * NK * kCurChunks);
+    skip1 = dalloc<int>(NI);
+    Jn = dalloc<double>((size_t)NL * NK);
+    fsal_ready.assign(NI, 0);
+    fsal_hprev.assign(NI, 0.0);
     HIPCHK(hipStreamSynchronize(stream));
   }
 
@@ -448,7 +465,13 @@ struct Engine : EngineBase {
   void drop_graphs() {
     for (auto& g : graph)
       if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    fsal_reset();  // whatever changed, the next attempt forms its own k_1
   }
+  void fsal_reset() { std::fill(fsal_ready.begin(), fsal_ready.end(), 0); }
+  // First same as last applies to the unfused attempts: the last stage stores k_7 over
+  // the k_1 slot (Ω: in its epilogue; σ, Ψ: in the error-norm pass) and forms the
+  // currents at y_new.  DOP54: the last row of the tableau is its weights, c_7 = 1.
+  bool fsal_on() const { return fsal_req == 0 && !fused_used && tab.c[6] == 1.0; }
 
   // ---------------------------------------------------------------- set-up
   // Device mode order and coupled-orbital order, decided once from the constants.
@@ -1421,6 +1444,7 @@ struct Engine : EngineBase {
   void state_put(const double* s, const double* p, const double* o) override {
     require(have_consts, "set the constants first", ZH_ESTATE);
     HIPCHK(hipSetDevice(device));
+    fsal_reset();
     // a null Ω is zero, anti-Hermitian
     const bool anti_in = o ? (anti_ok && omega_antihermitian(o)) : true;
     require(anti_req != 2 || anti_in,
@@ -1524,6 +1548,15 @@ struct Engine : EngineBase {
     drop_graphs();
   }
   int psi_kernel_info() const override { return psi_fused() ? 1 : 0; }
+  void set_fsal(int mode) override {
+    require(mode == 0 || mode == 1, "first same as last: 0 on, 1 off");
+    fsal_req = mode;
+    drop_graphs();
+  }
+  void fsal_info(long* taken, int* on) const override {
+    if (taken) *taken = fsal_taken;
+    if (on) *on = fsal_on() ? 1 : 0;
+  }
   void omgxi_info(int* out) const override {
     out[0] = xi_stream() && !fusable() ? 1 : 0;
     out[1] = xi_ok ? 1 : 0;
@@ -1542,6 +1575,10 @@ struct Engine : EngineBase {
     if (!first)
       for (int j = 0; j < s; ++j)
         if (tab.a[s][j] != 0.0) { slot[nt] = j; coef[nt] = tab.a[s][j]; ++nt; }
+    // the k_1 term (slot 0) is scaled by the instance's r1 (first same as last), and
+    // the first stage's copy skips instances whose first stage is skipped
+    const int k1 = (nt > 0 && slot[0] == 0) ? 0 : -1;
+    const StageParams* sps = sp + s * NI;
     timed(KC_COMBINE, [&] {
       if (!first) {
         CombineArgs<double2> gs{};
@@ -1549,6 +1586,7 @@ struct Engine : EngineBase {
         gs.nterms = nt;
         for (int j = 0; j < nt; ++j) { gs.k[j] = sig[slot[j]]; gs.a[j] = coef[j]; }
         gs.d1 = sig_dst; gs.d2 = nullptr;
+        gs.sp = sps; gs.ki = KI; gs.k1 = k1; gs.rows_b = n;
         MapDense ms{NK * n, n, n, n, n};
         launch_combine(gs, ms, blocks, stream);
       }
@@ -1558,6 +1596,7 @@ struct Engine : EngineBase {
       for (int j = 0; j < nt; ++j) { gp.k[j] = psi[slot[j]]; gp.a[j] = coef[j]; }
       gp.d1 = Ap;
       gp.d2 = last ? psi[iyn] : nullptr;
+      gp.sp = sps; gp.ki = KI; gp.k1 = k1; gp.rows_b = Md;
       MapDense mp{NK * Md, n, n, n + nc, n};
       launch_combine(gp, mp, blocks, stream);
     });
@@ -1596,10 +1635,15 @@ struct Engine : EngineBase {
     // attempts: y0 ξ_c is resident in PredY[par] and the last stage leaves y_new ξ_c in
     // PredY[1 - par]; otherwise (a single evaluation) the first stage forms y0 ξ_c
     bool y0xi_resident = false;
+    // attempts: the stage's parameters (r1 of the k_1 term, k[k1]) and where k_s goes
+    // when not to its own slot (the last stage's k_7 over the k_1 slot)
+    const StageParams* sps = nullptr;
+    int k1 = -1;
+    cs* kout = nullptr;
   };
 
   // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
-  void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, bool currents,
+  void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, double* jout,
                  bool hub) {
     const StageParams* sps = sp + s * NI;
     timed(KC_GATHER, [&] {
@@ -1619,7 +1663,7 @@ struct Engine : EngineBase {
     }
     stage_omega(os, kslot, s_omg);
     stage_psi(s, os, kslot, s_psi);
-    stage_sigma(s, sig_in, kslot, currents, stream);
+    stage_sigma(s, sig_in, kslot, jout, stream);
     if (conc) {
       HIPCHK(hipEventRecord(ev_join[0], aux[0]));
       HIPCHK(hipEventRecord(ev_join[1], aux[1]));
@@ -1629,7 +1673,9 @@ struct Engine : EngineBase {
   }
 
   // σ branch: Π_c, currents (stage 0), Hσ with the σ right-hand side.
-  void stage_sigma(int s, const double2* sig_in, int kslot, bool currents, hipStream_t st) {
+  // jout: the currents of the stage input go there (the first stage: J of the attempt;
+  // the last: Jn, the next attempt's when it reuses k_7), or none
+  void stage_sigma(int s, const double2* sig_in, int kslot, double* jout, hipStream_t st) {
     const StageParams* sps = sp + s * NI;
     const double ih = 1.0 / hbar;
     if (nc > 0) {
@@ -1637,6 +1683,7 @@ struct Engine : EngineBase {
         // Π_c = ψᵀ Ixi_c  (n x Md) (Md x nc), FP64, split over K
         GemmArgs<R, double> g{n, nc, Md, Ap, n + nc, (long)Md * (n + nc), Ixic, nc,
                               (long)Md * nc, kPiParts};
+        g.skip = skip_now; g.skip_ki = KI;
         EpiStore<double2> e{Pic, nc, (long)n * nc, 1.0, 0.0};
         KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, 0};
         // FP64 MFMAs: three real products per complex tile product
@@ -1657,21 +1704,25 @@ struct Engine : EngineBase {
                                                                     KFull(), CandHsig())));
       });
     }
-    if (currents) {
-      // with no coupled orbitals the reductions run over empty ranges and give J = 0
+    if (jout) {
+      // with no coupled orbitals the reductions run over empty ranges and give J = 0; an
+      // instance whose first stage is skipped takes Jn
       timed(KC_CURRENTS, [&] {
         hipLaunchKernelGGL(k_currents<cs>, dim3(kCurChunks, NL, NK), dim3(256), 0, st, NK, Md,
                            nc, (const int*)lead, (const cs*)Ap, (long)(n + nc),
-                           (long)Md * (n + nc), (const int*)cidx, (const double2*)Ixic, Jpart);
+                           (long)Md * (n + nc), (const int*)cidx, (const double2*)Ixic, Jpart,
+                           sps, KI);
         hipLaunchKernelGGL(k_currents_final, dim3(1), dim3(((NL * NK + 63) / 64) * 64), 0, st,
-                           NL * NK, (const double*)Jpart, ih * ih, Jd);
+                           NL * NK, (const double*)Jpart, ih * ih, jout, NK, sps, KI,
+                           (const double*)Jn);
       });
       HIPCHK(hipGetLastError());
     }
     timed(KC_GEMM_HSIG, [&] {
       GemmArgs<double, double> g{n, n, n, Hcur, n, (long)n * n, sig_in, n, (long)n * n};
+      g.skip = skip_now; g.skip_ki = KI;
       EpiSigma e{G, sig_in, dvec_epi(), Pic, overlap ? cpos_none : cpos, n, nc, ih, kPiParts,
-                 ovl ? Qs : nullptr};
+                 ovl ? Qs : nullptr, skip_now, KI};
       if (h_sparse) {
         constexpr int rb = 1;  // rows per workgroup: more, shorter workgroups keep more
                                // gathers in flight (C3: 8 -> 1 rows, 50 -> 27 us)
@@ -1746,6 +1797,7 @@ struct Engine : EngineBase {
       w.W = (res && s == 6) ? PredY[1 - os.par] : Wst;
       for (int j = 0; j < 6; ++j) w.a[j] = (j < s) ? tab.a[s][j] : 0.0;
       w.s = s; w.parts = kWParts; w.NK = NK; w.M = Md; w.nc = nc;
+      w.sp = os.sps; w.ki = KI;
       timed(KC_WSUM, [&] {
         hipLaunchKernelGGL(k_wsum<cs>, dim3(1024), dim3(256), 0, st, w);
         HIPCHK(hipGetLastError());
@@ -1762,6 +1814,7 @@ struct Engine : EngineBase {
                            (const double2*)glp, (const cs*)xic, (const cs*)Wc, Yov, hbar);
         HIPCHK(hipGetLastError());
         GemmArgs<R, R> g{Md, n, nc, Yov, nc, (long)Md * nc, Bs, n, (long)nc * n};
+        g.skip = skip_now; g.skip_ki = KI;
         EpiStore<cs> e{psi[kslot], n, (long)Md * n, 1.0, 0.0};
         HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g, NK, e, st, KFull(),
                                                      typename Cands<R>::psi())));
@@ -1798,6 +1851,7 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
+          g0.skip = skip_now; g0.skip_ki = KI;
           HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g0, NK, e0, st,
                                                           kseg_g0(),
                                                           typename Cands<R>::psi())));
@@ -1820,12 +1874,14 @@ struct Engine : EngineBase {
         if (nc > 0) {
           GemmArgs<R, R> g0{Md, n, nc, Ap + n, n + nc, (long)Md * (n + nc), Bp + (long)n * n, n,
                             (long)(n + nc) * n};
+          g0.skip = skip_now; g0.skip_ki = KI;
           HIPCHK((launch_sel<R, R, R, false, false, kRM3>(tsel[TS_PSI], g0, NK, e0, st,
                                                           kseg_g0(),
                                                           typename Cands<R>::psi())));
         }
         GemmArgs<R, R> g{Md, n, n, Ap, n + nc, (long)Md * (n + nc), Bp, n, (long)(n + nc) * n};
         g.Breal = HreT; g.ldbr = n; g.sBr = (long)n * n;
+        g.skip = skip_now; g.skip_ki = KI;
         EpiPsi<cs> e{psi[kslot], nc > 0 ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
                      dvec_epi(), lead, cpos, sps, Md, n, nc, ih, ih * ih, 1, g0_rows(), KI};
         HIPCHK((launch_sel<R, R, R, false, false, 2>(tsel[TS_PSI], g, NK, e, st, KFull(),
@@ -1835,6 +1891,7 @@ struct Engine : EngineBase {
         // H0 is S⁻¹H and the coupled-column terms come in through acc0
         GemmArgs<R, R> g{Md, n, n + nc, Ap, n + nc, (long)Md * (n + nc), Bp, n,
                          (long)(n + nc) * n};
+        g.skip = skip_now; g.skip_ki = KI;
         EpiPsi<cs> e{psi[kslot], ovl ? psi[kslot] : nullptr, Ap, n + nc, xic, Wc, glp, chip,
                      dvec_epi(), lead, overlap ? cpos_none : cpos, sps, Md, n, nc, ih, ih * ih, 1, Md,
                      KI};
@@ -1909,9 +1966,11 @@ struct Engine : EngineBase {
       GemmArgs<R, R> g{Md, Md, 2 * nc, Aomg, 2 * nc, (long)Md * 2 * nc, Bomg, 2 * nc,
                        (long)Md * 2 * nc};
       g.xcd_order = 0;  // the skipped passive block would unbalance contiguous XCD bands
+      g.skip = skip_now; g.skip_ki = KI;
       auto fill = [&](auto& e) {
         e.tri = tri_on ? 1 : 0;
-        e.out = os.store_k ? omg[kslot] : nullptr;
+        e.out = os.kout ? os.kout : (os.store_k ? omg[kslot] : nullptr);
+        e.sp = os.sps; e.kper = KI; e.k1 = os.k1;
         e.y0 = os.y0;
         e.nterms = os.nterms;
         for (int j = 0; j < 6; ++j) { e.k[j] = os.k[j]; e.a[j] = os.a[j]; e.ct[j] = os.ct[j]; }
@@ -2003,13 +2062,22 @@ struct Engine : EngineBase {
       TEArgs<double2> ts{};
       TEArgs<cs> tp{};
       int nw = 0;
+      int last_w = -1;
       for (int j = 0; j < 7; ++j) {
         if (tab.ct[j] == 0.0) continue;
+        if (j == 0) ts.k1 = tp.k1 = nw;
         ts.k[nw] = sig[j]; tp.k[nw] = psi[j];
         ts.ct[nw] = tp.ct[nw] = tab.ct[j];
+        last_w = j;
         ++nw;
       }
       require(nw == 6, "error weights: expected six non-zero entries");
+      ts.sp = tp.sp = sp;  // the first stage's entries: r1 per instance
+      if (fsal_on()) {
+        // k_7 (the last weighted slot) over the k_1 slot: the next attempt's k_1
+        require(last_w == 6 && ts.k1 == 0, "first same as last: k_7 must carry an error weight");
+        ts.kdst = sig[0]; tp.kdst = psi[0];
+      }
       if (tol_scaled) {
         ts.y0 = sig[iy0]; ts.y1 = sig[iyn]; tp.y0 = psi[iy0]; tp.y1 = psi[iyn];
         ts.rtol = tp.rtol = tol_rtol; ts.atol = tp.atol = tol_atol;
@@ -2048,11 +2116,12 @@ struct Engine : EngineBase {
   // The seven stages, error norm and controller of one attempt (graph body).
   void attempt_body() {
     timed(KC_PARAMS, [&] {
-      hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, (const Ctrl*)ctrl,
+      hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, ctrl,
                          (const PulseDesc*)pulse_d,
-                         diss, drive, tab, NL, NI, sp);
+                         diss, drive, tab, NL, NI, sp, skip1);
     });
     HIPCHK(hipGetLastError());
+    const bool fs = fsal_on();
     for (int s = 0; s < 7; ++s) {
       const bool last = (s == 6);
       double2* sdst = last ? sig[iyn] : sig[9];
@@ -2060,17 +2129,21 @@ struct Engine : EngineBase {
       const double2* sin_ = (s == 0) ? sig[iy0] : sdst;
       OmegaStage os;
       os.y0 = omg[iy0];
+      os.sps = sp + s * NI;
       for (int j = 0; j < s; ++j)
         if (tab.a[s][j] != 0.0) {
+          if (j == 0) os.k1 = os.nterms;
           os.k[os.nterms] = omg[j];
           os.a[os.nterms] = tab.a[s][j];
           os.ct[os.nterms] = tab.ct[j];
           ++os.nterms;
         }
       if (last) {
-        // y_new = Ω_7; k_7 is only needed by the error norm, accumulated in place
+        // y_new = Ω_7; the error norm is accumulated in place; k_7 goes over the k_1 slot
+        // (each element once its k_1 is read) as the next attempt's k_1, or nowhere
         os.ynew = omg[iyn];
         os.store_k = false;
+        if (fs) os.kout = omg[0];
         os.te = true;
         os.ct_self = tab.ct[6];
         int covered = 0;
@@ -2080,7 +2153,11 @@ struct Engine : EngineBase {
       os.par = (iy0 == 7) ? 0 : 1;
       os.y0xi_resident = true;
       if (fused_used) os.wpart = Wpart[s % 2];
-      rhs_stage(s, sin_, os, s, s == 0, true);
+      // the first stage's kernels leave the instances that reuse k_7 alone; the currents
+      // of the first stage are J, those of the last (y_new) the next attempt's
+      skip_now = (s == 0) ? skip1 : nullptr;
+      rhs_stage(s, sin_, os, s, s == 0 ? Jd : ((last && fs) ? Jn : nullptr), true);
+      skip_now = nullptr;
     }
     if (fused_used) {
       // y_new ξ_c from the last stage's partials: the next attempt's y0 ξ_c if accepted
@@ -2248,18 +2325,21 @@ struct Engine : EngineBase {
     prepare_model();
     if (ff && ff_dirty) { clear_ff(); ff_dirty = false; }
     upload_pulses();
+    const int par = (iy0 == 7) ? 0 : 1;
+    const bool fz = fusable();
+    if (fz != fused_used) drop_graphs();
+    fused_used = fz;
+    const bool fs = fsal_on();
     for (int g = 0; g < NI; ++g) {  // info[g]: instance g
       ctrl_h[g].t0 = info[g].t0;
       ctrl_h[g].h = info[g].h;
       ctrl_h[g].eps = info[g].eps;
       ctrl_h[g].force = info[g].force_accept ? 1 : 0;
       ctrl_h[g].norm_scale = norm_scale();
+      ctrl_h[g].fsal = (fs && fsal_ready[g]) ? 1 : 0;
+      ctrl_h[g].hprev = fsal_hprev[g];
     }
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl) * NI, hipMemcpyHostToDevice, stream));
-    const int par = (iy0 == 7) ? 0 : 1;
-    const bool fz = fusable();
-    if (fz != fused_used) drop_graphs();
-    fused_used = fz;
     if (nc > 0 && !y0xi_valid) {
       compute_y0xi(par, stream);
       y0xi_valid = true;
@@ -2306,6 +2386,7 @@ struct Engine : EngineBase {
     attempt_ms += ms;
     attempt_count += 1;
     int nacc = 0;
+    const bool fs = fsal_on();
     for (int g = 0; g < NI; ++g) {
       info[g] = pend_info[g];
       info[g].te = ctrl_h[g].te;
@@ -2316,11 +2397,17 @@ struct Engine : EngineBase {
       if (NI > 1 && info[g].accepted && !info[g].force_accept && !std::isfinite(info[g].h_next))
         info[g].accepted = 0;
       nacc += info[g].accepted ? 1 : 0;
+      fsal_taken += ctrl_h[g].fsal_used ? 1 : 0;
+      // the last stage left k_7 = h f(t0 + h, y_new) in the k_1 slot: the next attempt
+      // may take it as its k_1 when this step is committed
+      fsal_ready[g] = (fs && info[g].accepted && info[g].h > 0.0) ? 1 : 0;
+      fsal_hprev[g] = info[g].h;
     }
     if (J) std::memcpy(J, J_h, sizeof(double) * NL * NK);
     if (NI == 1 && info->accepted && finite_next && !std::isfinite(info->h_next)) {
       // not committed: the resident state stays at the step's start
       info->accepted = 0;
+      fsal_ready[0] = 0;
       throw Error{ZH_ENONFINITE, "non-finite next step size (error estimate " +
                                      std::to_string(info->te) + ") at t=" +
                                      std::to_string(info->t0) + "; the step is not committed"};
@@ -2384,10 +2471,11 @@ struct Engine : EngineBase {
     // stage parameters at t0 with h = 1 (sp[0]: Δ(t0), the dissipator rate unused)
     ctrl_h->t0 = t0; ctrl_h->h = 1.0; ctrl_h->eps = 1.0; ctrl_h->force = 1;
     ctrl_h->norm_scale = 1.0;
+    ctrl_h->fsal = 0;
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
     upload_pulses();
-    hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, (const Ctrl*)ctrl,
-                       (const PulseDesc*)pulse_d, diss, drive, tab, NL, NI, sp);
+    hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, ctrl,
+                       (const PulseDesc*)pulse_d, diss, drive, tab, NL, NI, sp, skip1);
     HIPCHK(hipGetLastError());
     const StageParams* sp0 = sp;
     apply_model(sp0, stream);  // drive terms at t0 (an overlap is refused above)
@@ -2645,10 +2733,12 @@ struct Engine : EngineBase {
     HIPCHK(hipSetDevice(device));
     install_h(Ht, false);
     h_dirty = true;
+    fsal_reset();  // the k slots and stage parameters are used here
     StageParams st;
     std::memset(&st, 0, sizeof(st));
     st.ts = 0.0;
     st.h = 1.0;
+    st.r1 = 1.0;
     for (int a = 0; a < NL; ++a) st.delta[a] = delta[a];
     HIPCHK(hipMemcpyAsync(sp, &st, sizeof(st), hipMemcpyHostToDevice, stream));
     upload(sig[9], n, s, (long)NK * n, n);
@@ -2665,7 +2755,7 @@ struct Engine : EngineBase {
     tri_on = anti_ok && anti_req != 1 && omega_antihermitian(o);
     OmegaStage os;
     os.y0 = omg[9];
-    rhs_stage(0, sig[9], os, 0, J != nullptr, false);
+    rhs_stage(0, sig[9], os, 0, J ? Jd : nullptr, false);
     HIPCHK(hipStreamSynchronize(stream));
     ff_dirty = true;
     if (ds) download(ds, sig[0], (size_t)NK * n * n);
@@ -3510,6 +3600,20 @@ int zh_set_omgxi(zh_problem* p, int mode) {
   return zh_try([&] {
     require(p, "null argument");
     p->eng->set_omgxi(mode);
+  });
+}
+
+int zh_set_fsal(zh_problem* p, int mode) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->set_fsal(mode);
+  });
+}
+
+int zh_fsal_info(zh_problem* p, long* taken, int* on) {
+  return zh_try([&] {
+    require(p, "null argument");
+    p->eng->fsal_info(taken, on);
   });
 }
 

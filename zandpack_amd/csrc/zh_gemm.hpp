@@ -74,6 +74,9 @@ struct GemmArgs {
   int xcd_order = 1;  // XCD-aware tile order (off where tiles are skipped unevenly)
   const RA* Areal = nullptr; long ldar = 0; long sAr = 0;
   const RB* Breal = nullptr; long ldbr = 0; long sBr = 0;
+  // per instance of skip_ki batches: 1 = leave the batch alone (every workgroup of it
+  // returns at once; a first stage whose result is already known)
+  const int* skip = nullptr; int skip_ki = 1 << 30;
 };
 
 constexpr int kThreads = 256;
@@ -182,6 +185,12 @@ __global__ __launch_bounds__(kThreads, OCC) void zgemm_kernel(GemmArgs<RA, RB> g
   }
   constexpr bool RED = HasReduce<Epi>::value;
   const int blin = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (g.skip && g.skip[(blockIdx.z / g.splitk) / g.skip_ki]) {
+    if constexpr (RED) {
+      if (threadIdx.x == 0) epi.finish(blin, 0.0);
+    }
+    return;
+  }
   if (ks.skip(ty * BM, tx * BN, BM, BN)) {  // uniform over the workgroup
     if constexpr (RED) {
       if (threadIdx.x == 0) epi.finish(blin, 0.0);

@@ -14,6 +14,11 @@ struct StageParams {
   double delta[ZH_MAX_LEADS];
   double eta;  // dissipator rate at ts
   double fdrive[ZH_MAX_DRIVE];  // time factors of the drive terms at ts
+  // first same as last: k_1 of this attempt is the previous attempt's k_7, left in the
+  // k_1 slot and scaled by r1 = h / h_prev where it is used; skip (set in the first
+  // stage's entry only) tells the first stage's kernels to leave the instance alone
+  double r1;
+  int skip;
 };
 
 // Time factors f_i(t) of the drive terms H(t) = H0 + Σ_i f_i(t) M_i (the waveforms the
@@ -58,6 +63,12 @@ struct Ctrl {
   int accept;
   double te2, te, h_next;
   double norm_scale;  // TE = sqrt(norm_scale * sum): 1 (absolute norm) or 1/count (scaled)
+  // first same as last: fsal = 1 when the k_1 slot holds the k_7 of the previous attempt
+  // (accepted, step hprev > 0, nothing changed since); fsal_used reports whether this
+  // attempt took it (the stage parameters at t0 equal those of that k_7, bit for bit)
+  int fsal;
+  double hprev;
+  int fsal_used;
 };
 
 // Error weight of one complex entry in the scaled norm: 1/(atol + rtol max(|y0|, |y1|))²
@@ -152,22 +163,52 @@ __device__ inline double pulse_value(const PulseDesc& p, double t) {
 // Stage times and biases of one attempt from the device control blocks, one workgroup
 // per instance g of a problem that carries G independent instances (each its own
 // control block, pulse and step): sp[s * G + g].
-__global__ void k_stage_params(const Ctrl* ctrl, const PulseDesc* pulses, DissDesc diss,
-                               DriveDesc drive, Tableau tab, int NL, int G, StageParams* sp) {
+// The right-hand side depends on t only through these parameters: equal parameters
+// (bit for bit) give the same f(t, y).
+__device__ inline bool same_params(const StageParams& a, const StageParams& b, int NL,
+                                   int nt) {
+  bool eq = a.eta == b.eta;
+  for (int i = 0; i < NL; ++i) eq = eq && a.delta[i] == b.delta[i];
+  for (int i = 0; i < nt; ++i) eq = eq && a.fdrive[i] == b.fdrive[i];
+  return eq;
+}
+
+// Stage times and biases of one attempt from the device control blocks, one workgroup
+// per instance g of a problem that carries G independent instances (each its own
+// control block, pulse and step): sp[s * G + g].  First same as last: when the host
+// offers it (ctrl.fsal) and the parameters at t0 equal those of the previous attempt's
+// last stage (still in sp[6 G + g]), that stage's k_7 = h_prev f(t0, y0) stands in for
+// k_1 (skip1[g] = 1, r1 = h / h_prev); the tableau's last row is its weights and
+// c_7 = 1, so the last stage's input is this attempt's y0.
+__global__ void k_stage_params(Ctrl* ctrl, const PulseDesc* pulses, DissDesc diss,
+                               DriveDesc drive, Tableau tab, int NL, int G, StageParams* sp,
+                               int* skip1) {
   if (threadIdx.x != 0) return;
   const int g = blockIdx.x;
   const PulseDesc pulse = pulses[g];
   const double t0 = ctrl[g].t0, h = ctrl[g].h;
+  bool fsal = false;
+  double r1 = 1.0;
   for (int s = 0; s < 7; ++s) {
-    StageParams& q = sp[s * G + g];
+    StageParams q;
     const double ts = t0 + tab.c[s] * h;
     q.ts = ts;
     q.h = h;
     const double v = pulse_value(pulse, ts);
-    for (int a = 0; a < NL; ++a) q.delta[a] = pulse.w[a] * v;
+    for (int a = 0; a < ZH_MAX_LEADS; ++a) q.delta[a] = a < NL ? pulse.w[a] * v : 0.0;
     q.eta = diss_rate(diss, ts);
     for (int i = 0; i < ZH_MAX_DRIVE; ++i) q.fdrive[i] = i < drive.nt ? drive_value(drive, i, ts) : 0.0;
+    if (s == 0) {
+      const Ctrl& c = ctrl[g];
+      fsal = c.fsal && c.hprev > 0.0 && h > 0.0 && same_params(q, sp[6 * G + g], NL, drive.nt);
+      r1 = fsal ? h / c.hprev : 1.0;
+    }
+    q.r1 = r1;
+    q.skip = (s == 0 && fsal) ? 1 : 0;
+    sp[s * G + g] = q;
   }
+  skip1[g] = fsal ? 1 : 0;
+  ctrl[g].fsal_used = fsal ? 1 : 0;
 }
 
 // H(t_s) = H0 + Σ_i f_i(t_s) M_i written at the drive's entries into the Hamiltonian
@@ -251,6 +292,8 @@ struct MapDense {  // rows x cols block, source row stride lds, destination stri
 };
 
 // dst = y0 + sum_j a_j k_j   (Zandpack/mpi/zand:306-313, 722-735)
+// With sp: rows of instance (r / rows_b) / ki scale term k1 (the k_1 slot) by that
+// instance's r1, and skip the instance when its skip flag is set.
 template <typename C>
 struct CombineArgs {
   const C* y0;
@@ -259,7 +302,21 @@ struct CombineArgs {
   int nterms;
   C* d1;
   C* d2;
+  const StageParams* sp = nullptr;
+  int ki = 1 << 30, k1 = -1;
+  int rows_b = 1 << 30;
 };
+// the coefficients of one row's instance; false: the row is left alone
+template <typename C>
+__device__ inline bool combine_coefs(const CombineArgs<C>& g, int r, double* a) {
+#pragma unroll
+  for (int j = 0; j < 7; ++j) a[j] = g.a[j];
+  if (!g.sp) return true;
+  const StageParams& q = g.sp[(r / g.rows_b) / g.ki];
+  if (q.skip) return false;
+  if (g.k1 >= 0) a[g.k1] *= q.r1;
+  return true;
+}
 
 // NT = number of stage terms, a compile-time constant so that all NT+1 slot
 // loads of an element are in flight together.
@@ -267,6 +324,8 @@ template <typename C, typename Map, int NT>
 __global__ __launch_bounds__(256) void k_combine(CombineArgs<C> g, Map mp) {
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+    double a[7];
+    if (!combine_coefs(g, r, a)) continue;
     long src, o1, o2;
     const int cols = mp.row(r, src, o1, o2);
     for (int c = threadIdx.x; c < cols; c += blockDim.x) {
@@ -277,8 +336,8 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<C> g, Map mp) {
       double vr = (double)kv[0].x, vi = (double)kv[0].y;
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
-        vr += g.a[j] * (double)kv[j + 1].x;
-        vi += g.a[j] * (double)kv[j + 1].y;
+        vr += a[j] * (double)kv[j + 1].x;
+        vi += a[j] * (double)kv[j + 1].y;
       }
       const C out = from_d2<C>(make_double2(vr, vi));
       g.d1[o1 + c] = out;
@@ -294,6 +353,8 @@ __global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp)
   static_assert(sizeof(C) == 8, "complex64 pairs");
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+    double a[7];
+    if (!combine_coefs(g, r, a)) continue;
     long src, o1, o2;
     const int cols = mp.row(r, src, o1, o2);
     for (int c = 2 * threadIdx.x; c < cols; c += 2 * blockDim.x) {
@@ -305,10 +366,10 @@ __global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp)
       double v0r = kv[0].x, v0i = kv[0].y, v1r = kv[0].z, v1i = kv[0].w;
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
-        v0r += g.a[j] * (double)kv[j + 1].x;
-        v0i += g.a[j] * (double)kv[j + 1].y;
-        v1r += g.a[j] * (double)kv[j + 1].z;
-        v1i += g.a[j] * (double)kv[j + 1].w;
+        v0r += a[j] * (double)kv[j + 1].x;
+        v0i += a[j] * (double)kv[j + 1].y;
+        v1r += a[j] * (double)kv[j + 1].z;
+        v1i += a[j] * (double)kv[j + 1].w;
       }
       const v4f out = {(float)v0r, (float)v0i, (float)v1r, (float)v1i};
       *reinterpret_cast<v4f*>(g.d1 + o1 + c) = out;
@@ -381,6 +442,7 @@ __global__ void k_gather(GatherArgs<C> g) {
       const long bm = idx / g.nc;                   // b*M + m
       const int b = (int)(bm / g.M);
       const long m = bm - (long)b * g.M;
+      if (g.sp[b / g.ki].skip) continue;  // k_1 of the instance is already there
       const double h = g.sp[b / g.ki].h;
       const int o = g.cidx[c];  // -1: padding column of a lead group
       const double2 v = o < 0 ? make_double2(0.0, 0.0) : to_d2(g.Ap[b * g.sAp + m * g.ldAp + o]);
@@ -404,6 +466,7 @@ __global__ void k_gather(GatherArgs<C> g) {
       const long bc = idx / g.n;
       const int b = (int)(bc / g.nc);
       const int c = (int)(bc - (long)b * g.nc);
+      if (g.sp[b / g.ki].skip) continue;
       const int oc = g.cidx[c];
       double2 s = oc < 0 ? make_double2(0.0, 0.0) : g.sig[((long)b * g.n + o) * g.n + oc];
       s.x *= g.hbar; s.y *= g.hbar;
@@ -460,6 +523,7 @@ __global__ void k_sigma_sym(int n, const double2* G, double2* out, const StagePa
                             const double2* sig_eq) {
   __shared__ double2 tile[32][33];
   const int b = blockIdx.z;
+  if (sps[b / ki].skip) return;  // k_1 of the instance is already in its slot
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const double2* Gb = G + (long)b * n * n;
@@ -496,8 +560,10 @@ constexpr int kCurChunks = 64;
 // Π_α traces from the stage psi on the coupled orbitals (Ap[:, cidx]).
 template <typename C>
 __global__ void k_currents(int NK, int M, int nc, const int* lead, const C* Ap, long ldAp,
-                           long sAp, const int* cidx, const double2* Ixic, double* part) {
+                           long sAp, const int* cidx, const double2* Ixic, double* part,
+                           const StageParams* sp, int ki) {
   const int chunk = blockIdx.x, a = blockIdx.y, b = blockIdx.z;
+  if (sp[b / ki].skip) return;  // the currents at y0 came with the reused k_1
   const long total = (long)M * nc;
   const long per = (total + kCurChunks - 1) / kCurChunks;
   const long lo = chunk * per, hi = min(total, lo + per);
@@ -521,9 +587,16 @@ __global__ void k_currents(int NK, int M, int nc, const int* lead, const C* Ap, 
   if (threadIdx.x == 0) part[((long)a * NK + b) * kCurChunks + chunk] = red[0];
 }
 
-__global__ void k_currents_final(int nlk, const double* part, double inv_hbar2, double* J) {
+// J[a NK + b]; an instance whose k_1 is the previous attempt's k_7 takes the currents
+// that attempt's last stage formed from the same state (Jprev)
+__global__ void k_currents_final(int nlk, const double* part, double inv_hbar2, double* J,
+                                 int NK, const StageParams* sp, int ki, const double* Jprev) {
   const int i = threadIdx.x;
   if (i >= nlk) return;
+  if (sp[(i % NK) / ki].skip) {
+    J[i] = Jprev[i];
+    return;
+  }
   double s = 0.0;
   for (int c = 0; c < kCurChunks; ++c) s += part[(long)i * kCurChunks + c];
   J[i] = 2.0 * s * inv_hbar2;
@@ -540,7 +613,19 @@ struct TEArgs {
   const C* y1 = nullptr;
   double rtol = 0.0, atol = 0.0;
   int scaled = 0;
+  // first same as last: weight k1 (the k_1 slot) times the instance's r1 (sp: the first
+  // stage's entries, one per instance), and the last weighted slot (k_7) written over
+  // the k_1 slot kdst once read, as the next attempt's k_1
+  const StageParams* sp = nullptr;
+  int k1 = -1;
+  C* kdst = nullptr;
 };
+// the error weights of instance blockIdx.y
+template <typename C> __device__ inline void te_weights(const TEArgs<C>& g, double* ct) {
+#pragma unroll
+  for (int j = 0; j < 7; ++j) ct[j] = g.ct[j];
+  if (g.sp && g.k1 >= 0) ct[g.k1] *= g.sp[blockIdx.y].r1;
+}
 
 constexpr int kTEBlocks = 1024;
 
@@ -551,6 +636,7 @@ template <typename C> __device__ inline void te_instance(TEArgs<C>& g, long istr
 #pragma unroll
   for (int j = 0; j < 7; ++j)
     if (g.k[j]) g.k[j] += ib;
+  if (g.kdst) g.kdst += ib;
   if (g.y0) g.y0 += ib;
   if (g.y1) g.y1 += ib;
 }
@@ -559,6 +645,8 @@ template <typename C> __device__ inline void te_instance(TEArgs<C>& g, long istr
 template <typename C, typename Map, int NW>
 __global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial, long istride) {
   te_instance(g, istride);
+  double ct[7];
+  te_weights(g, ct);
   double acc = 0.0;
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
@@ -568,11 +656,12 @@ __global__ __launch_bounds__(256) void k_te(TEArgs<C> g, Map mp, double* partial
       C kv[NW];
 #pragma unroll
       for (int j = 0; j < NW; ++j) kv[j] = ld_stream(g.k[j] + src + c);
+      if (g.kdst) st_stream(g.kdst + src + c, kv[NW - 1]);
       double vr = 0.0, vi = 0.0;
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
-        vr += g.ct[j] * (double)kv[j].x;
-        vi += g.ct[j] * (double)kv[j].y;
+        vr += ct[j] * (double)kv[j].x;
+        vi += ct[j] * (double)kv[j].y;
       }
       if (g.scaled) {
         const C a = g.y0[src + c], b = g.y1[src + c];
@@ -600,6 +689,8 @@ __global__ __launch_bounds__(256) void k_te_pairs(TEArgs<C> g, Map mp, double* p
                                                   long istride) {
   static_assert(sizeof(C) == 8, "complex64 pairs");
   te_instance(g, istride);
+  double ct[7];
+  te_weights(g, ct);
   double acc = 0.0;
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
@@ -610,13 +701,15 @@ __global__ __launch_bounds__(256) void k_te_pairs(TEArgs<C> g, Map mp, double* p
 #pragma unroll
       for (int j = 0; j < NW; ++j)
         kv[j] = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(g.k[j] + src + c));
+      if (g.kdst)
+        __builtin_nontemporal_store(kv[NW - 1], reinterpret_cast<v4f*>(g.kdst + src + c));
       double v0r = 0.0, v0i = 0.0, v1r = 0.0, v1i = 0.0;
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
-        v0r += g.ct[j] * (double)kv[j].x;
-        v0i += g.ct[j] * (double)kv[j].y;
-        v1r += g.ct[j] * (double)kv[j].z;
-        v1i += g.ct[j] * (double)kv[j].w;
+        v0r += ct[j] * (double)kv[j].x;
+        v0i += ct[j] * (double)kv[j].y;
+        v1r += ct[j] * (double)kv[j].z;
+        v1i += ct[j] * (double)kv[j].w;
       }
       if (g.scaled) {
         const C a0 = g.y0[src + c], b0 = g.y1[src + c];
@@ -695,6 +788,8 @@ struct EpiSigma {
   int n, nc; double inv_hbar;
   int pi_parts;  // split-K partials of Π_c, layout (NK, pi_parts, n, nc)
   const double2* Q = nullptr;  // (NK, n, n) or null
+  const int* skip = nullptr;   // per instance of ki k blocks: leave it alone (k_hsig_ell)
+  int ki = 1 << 30;
   __device__ inline void operator()(int b, int row, int col, double re, double im) const {
     apply(b, row, col, re, im, sig[((long)b * n + row) * n + col]);
   }
@@ -829,6 +924,10 @@ struct EpiOmegaRK {
   double rtol, atol; int scaled;          // scaled norm (TE): weights from |y0|, |Ω_7|
   int tri = 0;  // only the block upper triangle is computed: entries outside the diagonal
                 // 64 x 64 blocks stand for two in the error norm
+  // first same as last: term k1 (the k_1 slot) weighted by the instance's r1 (sp: one
+  // entry per instance of ki k blocks)
+  const StageParams* sp = nullptr;
+  int kper = 1 << 30, k1 = -1;
 
   __device__ inline double tri_weight(int row, int col) const {
     return (tri && (row >> 6) != (col >> 6)) ? 2.0 : 1.0;
@@ -840,6 +939,7 @@ struct EpiOmegaRK {
                                const double* im, double& acc) const {
     long off[4];
     double sr[4], si[4], er[4], ei[4], yr0[4], yi0[4];
+    const double r1 = (sp && k1 >= 0) ? sp[b / kper].r1 : 1.0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       off[r] = ((long)b * M + (rows[r] < 0 ? 0 : rows[r])) * M + col;
@@ -850,14 +950,16 @@ struct EpiOmegaRK {
 #pragma unroll
     for (int j = 0; j < kMaxTerms; ++j)
       if (j < nterms) {
+        const double aj = j == k1 ? a[j] * r1 : a[j];
+        const double cj = j == k1 ? ct[j] * r1 : ct[j];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const double2 x = rows[r] < 0 ? make_double2(0.0, 0.0) : ld_nt(k[j] + off[r]);
-          sr[r] += a[j] * x.x;
-          si[r] += a[j] * x.y;
+          sr[r] += aj * x.x;
+          si[r] += aj * x.y;
           if constexpr (TE) {
-            er[r] += ct[j] * x.x;
-            ei[r] += ct[j] * x.y;
+            er[r] += cj * x.x;
+            ei[r] += cj * x.y;
           }
         }
       }
@@ -913,6 +1015,7 @@ struct EpiOmegaRK {
     }
     const long off = ((long)b * M + row) * M + col0;
     const bool vec = ncol == 4 && sizeof(C) == 8 && (off & 3) == 0;
+    const double r1 = (sp && k1 >= 0) ? sp[b / kper].r1 : 1.0;
     double sr[4], si[4], er[4], ei[4], yr0[4], yi0[4];
     load4(y0, off, ncol, vec, sr, si);
 #pragma unroll
@@ -925,13 +1028,15 @@ struct EpiOmegaRK {
       if (j < nterms) {
         double xr[4], xi[4];
         load4(k[j], off, ncol, vec, xr, xi);
+        const double aj = j == k1 ? a[j] * r1 : a[j];
+        const double cj = j == k1 ? ct[j] * r1 : ct[j];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          sr[r] += a[j] * xr[r];
-          si[r] += a[j] * xi[r];
+          sr[r] += aj * xr[r];
+          si[r] += aj * xi[r];
           if constexpr (TE) {
-            er[r] += ct[j] * xr[r];
-            ei[r] += ct[j] * xi[r];
+            er[r] += cj * xr[r];
+            ei[r] += cj * xi[r];
           }
         }
       }
@@ -1024,6 +1129,9 @@ struct WSumArgs {
   double a[6];
   int s, parts, NK, M, nc;
   int parts_pass = -1, ML = 0;
+  // first same as last: a[0] (the k_1 ξ_c term) times the instance's r1
+  const StageParams* sp = nullptr;
+  int ki = 1 << 30;
 };
 
 template <typename C>
@@ -1047,10 +1155,12 @@ __global__ __launch_bounds__(256) void k_wsum(WSumArgs<C> g) {
     } else {
       const double2 y = to_d2(g.pred[0][idx]);
       wr = y.x; wi = y.y;
+      const double r1 = g.sp ? g.sp[b / g.ki].r1 : 1.0;
       for (int j = 0; j < g.s; ++j) {
         const double2 x = (j + 1 == g.s) ? to_d2(ps) : to_d2(g.pred[j + 1][idx]);
-        wr += g.a[j] * x.x;
-        wi += g.a[j] * x.y;
+        const double aj = j == 0 ? g.a[0] * r1 : g.a[j];
+        wr += aj * x.x;
+        wi += aj * x.y;
       }
     }
     g.W[idx] = from_d2<C>(make_double2(wr, wi));
@@ -1240,6 +1350,7 @@ __global__ __launch_bounds__(256) void k_psi_ell(EpiPsi<C> e, long sAp, Ell h) {
   C* rows = reinterpret_cast<C*>(lds_raw);  // RB x n
   __shared__ double2 rchi[RB], rglp[RB];
   const int b = blockIdx.y;
+  if (e.sp[b / e.ki].skip) return;  // k_1 of the instance is already in its slot
   const int i0 = blockIdx.x * RB;
   const int n = h.n, M = e.M;
   for (int q = 0; q < RB; ++q) {
@@ -1343,6 +1454,7 @@ template <int RB, int R, typename Epi>
 __global__ __launch_bounds__(256) void k_hsig_ell(const double2* __restrict__ sig, Ell e,
                                                   Epi epi) {
   const int b = blockIdx.z;
+  if (epi.skip && epi.skip[b / epi.ki]) return;
   const int n = e.n;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int o0 = blockIdx.y * RB;

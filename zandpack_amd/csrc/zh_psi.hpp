@@ -55,6 +55,7 @@ __global__ __launch_bounds__(256) void k_psi_fused(PsiFusedArgs<C> a) {
   extern __shared__ __align__(16) unsigned char psi_lds[];
   C* tile = reinterpret_cast<C*>(psi_lds);  // TR x W
   const int b = blockIdx.z;
+  if (a.sp[b / a.ki].skip) return;  // k_1 of the instance is already in its slot
   const int o0 = blockIdx.x * 64, m0 = blockIdx.y * TR;
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);

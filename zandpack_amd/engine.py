@@ -200,6 +200,18 @@ class Propagator:
         N.check(self.lib.zh_psi_kernel_info(self.h, C.byref(v)))
         return "fused" if v.value else "split"
 
+    def set_fsal(self, mode="on"):
+        """First same as last: "on" (an attempt after an accepted one takes that one's k_7
+        as its k_1 where f is the same there) or "off" (every attempt evaluates all
+        seven stages); same results to rounding."""
+        N.check(self.lib.zh_set_fsal(self.h, {"on": 0, "off": 1}[mode]))
+
+    def fsal_info(self):
+        """{"taken": instance attempts that reused k_7, "on": whether attempts may}."""
+        t, o = C.c_long(), C.c_int()
+        N.check(self.lib.zh_fsal_info(self.h, C.byref(t), C.byref(o)))
+        return {"taken": t.value, "on": bool(o.value)}
+
     def omgxi_info(self):
         """{"stream", "possible"}: whether the attempts run the streaming Ωξ kernel,
         whether the problem's layout allows it."""

@@ -10,6 +10,9 @@ just that copy (one calibration for both precisions).  The tiles are
 autotuned first (those timing attempts show up in a trace as well); the last
 ``--attempts`` attempts run with the chosen tiles.  ``--serial`` runs them the way
 bench.py's per-class profile does (one stream, HIP events around each class).
+``--fsal off`` evaluates every stage of every attempt (with first same as last on, the
+first stage's dispatches of an attempt after an accepted one exit at once, which a
+per-dispatch average would count).
 """
 import argparse
 import sys
@@ -24,6 +27,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--serial", action="store_true",
                     help="per-class timing mode: one stream, no graph (as bench.py's kernel profile)")
+    ap.add_argument("--fsal", default="on", choices=["on", "off"],
+                    help="first same as last (off: every stage of every attempt evaluated)")
     a = ap.parse_args()
     from zandpack_amd import builder
     from zandpack_amd.engine import Propagator
@@ -38,6 +43,7 @@ def main():
             return 0
     p = builder.config_c3(seed=0)
     prop = Propagator(p, a.precision, graph=not a.no_graph, autotune=True)
+    prop.set_fsal(a.fsal)
     if a.serial:
         prop.set_profile(True)
     t, h = 0.0, 0.002
