@@ -2116,7 +2116,7 @@ struct Engine : EngineBase {
   // The seven stages, error norm and controller of one attempt (graph body).
   void attempt_body() {
     timed(KC_PARAMS, [&] {
-      hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, ctrl,
+      hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(64), 0, stream, ctrl,
                          (const PulseDesc*)pulse_d,
                          diss, drive, tab, NL, NI, sp, skip1);
     });
@@ -2474,7 +2474,7 @@ struct Engine : EngineBase {
     ctrl_h->fsal = 0;
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
     upload_pulses();
-    hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(1), 0, stream, ctrl,
+    hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(64), 0, stream, ctrl,
                        (const PulseDesc*)pulse_d, diss, drive, tab, NL, NI, sp, skip1);
     HIPCHK(hipGetLastError());
     const StageParams* sp0 = sp;

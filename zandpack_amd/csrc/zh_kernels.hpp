@@ -160,9 +160,6 @@ __device__ inline double pulse_value(const PulseDesc& p, double t) {
   }
 }
 
-// Stage times and biases of one attempt from the device control blocks, one workgroup
-// per instance g of a problem that carries G independent instances (each its own
-// control block, pulse and step): sp[s * G + g].
 // The right-hand side depends on t only through these parameters: equal parameters
 // (bit for bit) give the same f(t, y).
 __device__ inline bool same_params(const StageParams& a, const StageParams& b, int NL,
@@ -180,35 +177,51 @@ __device__ inline bool same_params(const StageParams& a, const StageParams& b, i
 // last stage (still in sp[6 G + g]), that stage's k_7 = h_prev f(t0, y0) stands in for
 // k_1 (skip1[g] = 1, r1 = h / h_prev); the tableau's last row is its weights and
 // c_7 = 1, so the last stage's input is this attempt's y0.
+__device__ inline StageParams stage_params_at(const PulseDesc& pulse, const DissDesc& diss,
+                                              const DriveDesc& drive, int NL, double ts,
+                                              double h) {
+  StageParams q;
+  q.ts = ts;
+  q.h = h;
+  const double v = pulse_value(pulse, ts);
+  for (int a = 0; a < ZH_MAX_LEADS; ++a) q.delta[a] = a < NL ? pulse.w[a] * v : 0.0;
+  q.eta = diss_rate(diss, ts);
+  for (int i = 0; i < ZH_MAX_DRIVE; ++i) q.fdrive[i] = i < drive.nt ? drive_value(drive, i, ts) : 0.0;
+  q.r1 = 1.0;
+  q.skip = 0;
+  return q;
+}
+// One workgroup of 64 threads per instance: thread 0 decides first same as last (before
+// the last stage's entry is replaced), then thread s < 7 writes stage s.
 __global__ void k_stage_params(Ctrl* ctrl, const PulseDesc* pulses, DissDesc diss,
                                DriveDesc drive, Tableau tab, int NL, int G, StageParams* sp,
                                int* skip1) {
-  if (threadIdx.x != 0) return;
-  const int g = blockIdx.x;
+  const int g = blockIdx.x, s = threadIdx.x;
   const PulseDesc pulse = pulses[g];
   const double t0 = ctrl[g].t0, h = ctrl[g].h;
-  bool fsal = false;
-  double r1 = 1.0;
-  for (int s = 0; s < 7; ++s) {
-    StageParams q;
-    const double ts = t0 + tab.c[s] * h;
-    q.ts = ts;
-    q.h = h;
-    const double v = pulse_value(pulse, ts);
-    for (int a = 0; a < ZH_MAX_LEADS; ++a) q.delta[a] = a < NL ? pulse.w[a] * v : 0.0;
-    q.eta = diss_rate(diss, ts);
-    for (int i = 0; i < ZH_MAX_DRIVE; ++i) q.fdrive[i] = i < drive.nt ? drive_value(drive, i, ts) : 0.0;
-    if (s == 0) {
-      const Ctrl& c = ctrl[g];
-      fsal = c.fsal && c.hprev > 0.0 && h > 0.0 && same_params(q, sp[6 * G + g], NL, drive.nt);
-      r1 = fsal ? h / c.hprev : 1.0;
+  __shared__ int fsal_s;
+  __shared__ double r1_s;
+  if (s == 0) {
+    const Ctrl& c = ctrl[g];
+    bool fsal = false;
+    if (c.fsal && c.hprev > 0.0 && h > 0.0) {
+      const StageParams q0 = stage_params_at(pulse, diss, drive, NL, t0 + tab.c[0] * h, h);
+      fsal = same_params(q0, sp[6 * G + g], NL, drive.nt);
     }
-    q.r1 = r1;
-    q.skip = (s == 0 && fsal) ? 1 : 0;
+    fsal_s = fsal ? 1 : 0;
+    r1_s = fsal ? h / c.hprev : 1.0;
+  }
+  __syncthreads();
+  if (s < 7) {
+    StageParams q = stage_params_at(pulse, diss, drive, NL, t0 + tab.c[s] * h, h);
+    q.r1 = r1_s;
+    q.skip = (s == 0) ? fsal_s : 0;
     sp[s * G + g] = q;
   }
-  skip1[g] = fsal ? 1 : 0;
-  ctrl[g].fsal_used = fsal ? 1 : 0;
+  if (s == 0) {
+    skip1[g] = fsal_s;
+    ctrl[g].fsal_used = fsal_s;
+  }
 }
 
 // H(t_s) = H0 + Σ_i f_i(t_s) M_i written at the drive's entries into the Hamiltonian
