@@ -387,6 +387,11 @@ struct Engine : EngineBase {
   double* Jn = nullptr;
   std::vector<char> fsal_ready;
   std::vector<double> fsal_hprev;
+  // where a next attempt may start and still take k_7: t0 + h (the last stage's time)
+  // always, t0 + h_next when the controller found the stage parameters equal there
+  std::vector<double> fsal_t7, fsal_tn;
+  std::vector<char> fsal_hn;
+  bool omit1_pending = false;  // the attempt in flight leaves out its first stage
   int fsal_req = 0;       // 0 on, 1 off
   long fsal_taken = 0;    // instance attempts that reused k_7
   const int* skip_now = nullptr;  // skip1 while the first stage of an attempt is issued
@@ -395,7 +400,8 @@ struct Engine : EngineBase {
   // pinned host mailboxes
   Ctrl* ctrl_h = nullptr;
   double* J_h = nullptr;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // by the parity of the y0 / y_new slots, and with / without the first stage
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev[2 * KC_N + 2];
 
   long nbytes = 0;
@@ -442,6 +448,9 @@ struct Engine : EngineBase {
     Jn = dalloc<double>((size_t)NL * NK);
     fsal_ready.assign(NI, 0);
     fsal_hprev.assign(NI, 0.0);
+    fsal_t7.assign(NI, 0.0);
+    fsal_tn.assign(NI, 0.0);
+    fsal_hn.assign(NI, 0);
     HIPCHK(hipStreamSynchronize(stream));
   }
 
@@ -1268,6 +1277,7 @@ struct Engine : EngineBase {
     pulse[g].sc = spl_c;
     pulse[g].ns = spl_n;
     pulse_dirty = true;
+    fsal_ready[g] = 0;  // its last k_7 was formed under the old pulse
   }
   // the pulses into pulse_d before a launch that reads them (stream order)
   void upload_pulses() {
@@ -2108,21 +2118,24 @@ struct Engine : EngineBase {
       const int no = te_omega_blocks();
       require(no % NI == 0, "error-norm partials do not split over the instances");
       hipLaunchKernelGGL(k_controller, dim3(NI), dim3(kCtrlThreads), 0, stream,
-                         (const double*)te_part, kTEBlocks, no / NI, NI, ctrl);
+                         (const double*)te_part, kTEBlocks, no / NI, NI, ctrl,
+                         (const PulseDesc*)pulse_d, diss, drive, NL,
+                         (const StageParams*)(sp + 6 * NI));
     });
     HIPCHK(hipGetLastError());
   }
 
   // The seven stages, error norm and controller of one attempt (graph body).
-  void attempt_body() {
+  // omit1: the first stage is left out (every instance takes k_7, first same as last)
+  void attempt_body(bool omit1) {
     timed(KC_PARAMS, [&] {
       hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(64), 0, stream, ctrl,
                          (const PulseDesc*)pulse_d,
-                         diss, drive, tab, NL, NI, sp, skip1);
+                         diss, drive, tab, NL, NI, sp, skip1, NK, (const double*)Jn, Jd);
     });
     HIPCHK(hipGetLastError());
     const bool fs = fsal_on();
-    for (int s = 0; s < 7; ++s) {
+    for (int s = omit1 ? 1 : 0; s < 7; ++s) {
       const bool last = (s == 6);
       double2* sdst = last ? sig[iyn] : sig[9];
       combine(s, s == 0, sdst, last);
@@ -2339,25 +2352,34 @@ struct Engine : EngineBase {
       ctrl_h[g].fsal = (fs && fsal_ready[g]) ? 1 : 0;
       ctrl_h[g].hprev = fsal_hprev[g];
     }
+    // every instance known to take k_7 (its start at the time of the last stage before,
+    // or where the controller found the same parameters): the first stage is left out of
+    // the attempt altogether, not just skipped by its kernels
+    bool omit1 = fs;
+    for (int g = 0; g < NI && omit1; ++g)
+      omit1 = fsal_ready[g] && info[g].h > 0.0 &&
+              (info[g].t0 == fsal_t7[g] || (fsal_hn[g] && info[g].t0 == fsal_tn[g]));
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl) * NI, hipMemcpyHostToDevice, stream));
     if (nc > 0 && !y0xi_valid) {
       compute_y0xi(par, stream);
       y0xi_valid = true;
     }
     HIPCHK(hipEventRecord(ev[2 * KC_N], stream));
+    const int gi = par + (omit1 ? 2 : 0);
     if (use_graph && !profile) {
-      if (!graph[par]) {
+      if (!graph[gi]) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        attempt_body();
+        attempt_body(omit1);
         HIPCHK(hipStreamEndCapture(stream, &g));
-        HIPCHK(hipGraphInstantiate(&graph[par], g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphInstantiate(&graph[gi], g, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(g));
       }
-      HIPCHK(hipGraphLaunch(graph[par], stream));
+      HIPCHK(hipGraphLaunch(graph[gi], stream));
     } else {
-      attempt_body();
+      attempt_body(omit1);
     }
+    omit1_pending = omit1;
     HIPCHK(hipEventRecord(ev[2 * KC_N + 1], stream));
     HIPCHK(hipMemcpyAsync(ctrl_h, ctrl, sizeof(Ctrl) * NI, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(J_h, Jd, sizeof(double) * NL * NK, hipMemcpyDeviceToHost, stream));
@@ -2398,10 +2420,18 @@ struct Engine : EngineBase {
         info[g].accepted = 0;
       nacc += info[g].accepted ? 1 : 0;
       fsal_taken += ctrl_h[g].fsal_used ? 1 : 0;
+      if (omit1_pending && !ctrl_h[g].fsal_used) {
+        fsal_reset();
+        throw Error{ZH_ESTATE, "first same as last: an attempt without its first stage found "
+                               "different stage parameters (its results are not valid)"};
+      }
       // the last stage left k_7 = h f(t0 + h, y_new) in the k_1 slot: the next attempt
       // may take it as its k_1 when this step is committed
       fsal_ready[g] = (fs && info[g].accepted && info[g].h > 0.0) ? 1 : 0;
       fsal_hprev[g] = info[g].h;
+      fsal_t7[g] = info[g].t0 + info[g].h;
+      fsal_tn[g] = info[g].t0 + info[g].h_next;
+      fsal_hn[g] = ctrl_h[g].fsal_hn ? 1 : 0;
     }
     if (J) std::memcpy(J, J_h, sizeof(double) * NL * NK);
     if (NI == 1 && info->accepted && finite_next && !std::isfinite(info->h_next)) {
@@ -2475,7 +2505,8 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpyAsync(ctrl, ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
     upload_pulses();
     hipLaunchKernelGGL(k_stage_params, dim3(NI), dim3(64), 0, stream, ctrl,
-                       (const PulseDesc*)pulse_d, diss, drive, tab, NL, NI, sp, skip1);
+                       (const PulseDesc*)pulse_d, diss, drive, tab, NL, NI, sp, skip1, NK,
+                       (const double*)Jn, Jd);
     HIPCHK(hipGetLastError());
     const StageParams* sp0 = sp;
     apply_model(sp0, stream);  // drive terms at t0 (an overlap is refused above)

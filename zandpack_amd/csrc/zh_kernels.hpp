@@ -69,6 +69,9 @@ struct Ctrl {
   int fsal;
   double hprev;
   int fsal_used;
+  // the controller's forecast: 1 when the stage parameters at t0 + h_next equal those of
+  // this attempt's last stage (a next attempt started there may reuse its k_7)
+  int fsal_hn;
 };
 
 // Error weight of one complex entry in the scaled norm: 1/(atol + rtol max(|y0|, |y1|))²
@@ -195,7 +198,7 @@ __device__ inline StageParams stage_params_at(const PulseDesc& pulse, const Diss
 // the last stage's entry is replaced), then thread s < 7 writes stage s.
 __global__ void k_stage_params(Ctrl* ctrl, const PulseDesc* pulses, DissDesc diss,
                                DriveDesc drive, Tableau tab, int NL, int G, StageParams* sp,
-                               int* skip1) {
+                               int* skip1, int NK, const double* Jn, double* J) {
   const int g = blockIdx.x, s = threadIdx.x;
   const PulseDesc pulse = pulses[g];
   const double t0 = ctrl[g].t0, h = ctrl[g].h;
@@ -222,6 +225,13 @@ __global__ void k_stage_params(Ctrl* ctrl, const PulseDesc* pulses, DissDesc dis
     skip1[g] = fsal_s;
     ctrl[g].fsal_used = fsal_s;
   }
+  // the currents at y0 of an instance that reuses k_7: those its last stage formed
+  const int KI = NK / G;
+  if (fsal_s)
+    for (int i = s; i < NL * KI; i += blockDim.x) {
+      const int a = i / KI, b = g * KI + i % KI;
+      J[a * NK + b] = Jn[a * NK + b];
+    }
 }
 
 // H(t_s) = H0 + Σ_i f_i(t_s) M_i written at the drive's entries into the Hamiltonian
@@ -754,7 +764,10 @@ constexpr int kCtrlThreads = 1024;
 // (partial[G nb + g nb ..]) and no of Ω (partial[2 G nb + g no ..]), summed as one
 // sequence in that order (with one instance, partial[0 .. 2 nb + no) in place).
 __global__ __launch_bounds__(kCtrlThreads) void k_controller(const double* partial, int nb,
-                                                             int no, int G, Ctrl* ctrls) {
+                                                             int no, int G, Ctrl* ctrls,
+                                                             const PulseDesc* pulses,
+                                                             DissDesc diss, DriveDesc drive,
+                                                             int NL, const StageParams* sp6) {
   __shared__ double red[kCtrlThreads];
   const int g = blockIdx.x;
   Ctrl* ctrl = ctrls + g;
@@ -779,6 +792,9 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller(const double* parti
     ctrl->te = te;
     ctrl->accept = (ctrl->force || te < ctrl->eps) ? 1 : 0;
     ctrl->h_next = 0.9 * ctrl->h * pow(ctrl->eps / te, 0.2);
+    const double tn = ctrl->t0 + ctrl->h_next;
+    const StageParams q = stage_params_at(pulses[g], diss, drive, NL, tn, ctrl->h_next);
+    ctrl->fsal_hn = same_params(q, sp6[g], NL, drive.nt) ? 1 : 0;
   }
 }
 
