@@ -234,8 +234,11 @@ struct Engine : EngineBase {
   // the σ branch (Π, currents, Hσ) are independent once the stage input is gathered:
   // they run on three streams (fork/join events, also inside the captured graph), so
   // MFMA-heavy and HBM-heavy kernels share the CUs.  All streams at default priority.
-  hipStream_t aux[2] = {nullptr, nullptr};
+  // aux[0]: Ω branch, aux[1]: Ψ branch, aux[2]: the next stage's Ωξ product, issued as
+  // soon as this stage's Ω kernel has written its k (attempts)
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  hipEvent_t ev_omg = nullptr, ev_xi[2] = {nullptr, nullptr};
   std::vector<void*> allocs;
   Tableau tab = dop54();
   std::vector<PulseDesc> pulse;  // per instance
@@ -315,6 +318,9 @@ struct Engine : EngineBase {
   double2 *uvec = nullptr, *vvec = nullptr;  // per-stage (h/ħ)(χ± + Δ) of the Ω epilogue
   double* n0 = nullptr;
   cs *Bp = nullptr, *Ap = nullptr, *Aomg = nullptr, *Bomg = nullptr, *xic = nullptr, *W = nullptr;
+  cs* W2 = nullptr;  // second buffer of the Ωξ partials: stage s's product goes to wpart(s)
+  cs* wpart(int s) const { return (s & 1) ? W2 : W; }
+  bool xi_early = false;  // attempt body being issued: each stage's product issued early
   cs* Pred[7] = {nullptr};  // y0 ξ_c and k_j ξ_c of the current attempt (Ω_s ξ_c by linearity)
   cs* Wst = nullptr;        // Ω_s ξ_c of the current stage
   // Fused Ω stage kernel (mixed precision): each stage's Ω kernel also writes k_s ξ_c
@@ -433,6 +439,8 @@ struct Engine : EngineBase {
     for (auto& a : aux) HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     for (auto& e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_omg, hipEventDisableTiming));
+    for (auto& e : ev_xi) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl) * NI, hipHostMallocDefault));
     std::memset(ctrl_h, 0, sizeof(Ctrl) * NI);
@@ -461,6 +469,9 @@ struct Engine : EngineBase {
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     for (auto& e : ev_join)
+      if (e) (void)hipEventDestroy(e);
+    if (ev_omg) (void)hipEventDestroy(ev_omg);
+    for (auto& e : ev_xi)
       if (e) (void)hipEventDestroy(e);
     for (auto& a : aux)
       if (a) (void)hipStreamDestroy(a);
@@ -572,6 +583,7 @@ struct Engine : EngineBase {
     Pic = dalloc<double2>((size_t)NK * kPiParts * n * nc);
     xic = dalloc<cs>(mc);
     W = dalloc<cs>(mc * kWParts);
+    W2 = dalloc<cs>(mc * kWParts);
     for (auto& p : Pred) p = dalloc<cs>(mc);
     Wst = dalloc<cs>(mc);
     for (auto& q : PredY) q = dalloc<cs>(mc);
@@ -1652,6 +1664,18 @@ struct Engine : EngineBase {
     cs* kout = nullptr;
   };
 
+  // Ωξ product of stage t (X = the k of stage t - 1, omg[t - 1]) into wpart(t), on aux[2]
+  // after the work on `after` (the kernel that wrote it); ev_xi[t & 1] marks it done.
+  void issue_xi(int t, hipStream_t after, bool conc) {
+    hipStream_t sx = conc ? aux[2] : stream;
+    if (conc) {
+      HIPCHK(hipEventRecord(ev_omg, after));
+      HIPCHK(hipStreamWaitEvent(sx, ev_omg, 0));
+    }
+    timed(KC_GEMM_OMGXI, [&] { omgxi_run<kRM3>(omg[t - 1], sx, true, wpart(t)); });
+    if (conc) HIPCHK(hipEventRecord(ev_xi[t & 1], sx));
+  }
+
   // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
   void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, double* jout,
                  bool hub) {
@@ -1672,6 +1696,9 @@ struct Engine : EngineBase {
       s_psi = aux[1];
     }
     stage_omega(os, kslot, s_omg);
+    // the next stage's Ωξ product of this stage's k, as soon as it is written: it overlaps
+    // the rest of this stage and the next one's stage combination and gathers
+    if (xi_early && s + 1 < 7) issue_xi(s + 1, s_omg, conc);
     stage_psi(s, os, kslot, s_psi);
     stage_sigma(s, sig_in, kslot, jout, stream);
     if (conc) {
@@ -1799,9 +1826,15 @@ struct Engine : EngineBase {
       // y_new ξ_c and goes to PredY[1 - par], the next attempt's y0 ξ_c if this one is
       // accepted.
       const bool res = os.y0xi_resident;
-      timed(KC_GEMM_OMGXI, [&] { omgxi_run<kRM3>((s == 0) ? os.y0 : omg[s - 1], st); });
       WSumArgs<cs> w{};
-      w.part = W;
+      if (xi_early && s >= 1) {
+        // issued as soon as the previous stage's Ω kernel wrote its k (issue_xi)
+        if (!profile) HIPCHK(hipStreamWaitEvent(st, ev_xi[s & 1], 0));
+        w.part = wpart(s);
+      } else {
+        timed(KC_GEMM_OMGXI, [&] { omgxi_run<kRM3>((s == 0) ? os.y0 : omg[s - 1], st); });
+        w.part = W;
+      }
       for (int j = 0; j < 7; ++j) w.pred[j] = Pred[j];
       if (res) w.pred[0] = PredY[os.par];
       w.W = (res && s == 6) ? PredY[1 - os.par] : Wst;
@@ -1915,11 +1948,11 @@ struct Engine : EngineBase {
   // X ξ_c (Md x Md)(Md x nc) into the split-K partials W (X = y0 or a k slot).  With Ω
   // held as its block upper triangle each row block also runs the mirrored part,
   // -X[J < I, I]^H ξ_c, in the same K loop (KSegTri).
-  template <int NG> void xi_launch(const cs* X, hipStream_t st) {
+  template <int NG> void xi_launch(const cs* X, hipStream_t st, cs* part) {
     XiArgs<cs> a{};
     a.X = X; a.sX = (long)Md * Md;
     a.xi = xic; a.sXi = (long)Md * nc;
-    a.part = W; a.sPart = (long)Md * nc;
+    a.part = part; a.sPart = (long)Md * nc;
     a.Md = Md; a.nc = nc; a.parts = kWParts; a.tri = tri_on ? 1 : 0; a.ML = ML; a.ff = ff ? 1 : 0;
     a.nitem = (int)xi_items.size();
     for (int i = 0; i < a.nitem; ++i) a.item[i] = xi_items[i];
@@ -1932,18 +1965,21 @@ struct Engine : EngineBase {
   }
   // stream: the attempts' products may take the streaming kernel (the steady-state
   // operator keeps the four-multiplication GEMM)
-  template <int RM> void omgxi_run(const cs* X, hipStream_t st, bool stream = true) {
+  // part: the partials' buffer (W unless given)
+  template <int RM> void omgxi_run(const cs* X, hipStream_t st, bool stream = true,
+                                   cs* part = nullptr) {
+    if (!part) part = W;
     if (stream && xi_stream()) {
       require(xi_ng >= 1 && xi_ng <= 4 && (int)lrange_h.size() == NL && NL <= kXiMaxLeads,
               "streaming Omega xi_c kernel: bad column tiles");
-      if (xi_ng == 1) xi_launch<1>(X, st);
-      else if (xi_ng == 2) xi_launch<2>(X, st);
-      else if (xi_ng == 3) xi_launch<3>(X, st);
-      else xi_launch<4>(X, st);
+      if (xi_ng == 1) xi_launch<1>(X, st, part);
+      else if (xi_ng == 2) xi_launch<2>(X, st, part);
+      else if (xi_ng == 3) xi_launch<3>(X, st, part);
+      else xi_launch<4>(X, st, part);
       return;
     }
     GemmArgs<R, R> g{Md, nc, Md, X, Md, (long)Md * Md, xic, nc, (long)Md * nc, kWParts};
-    EpiStore<cs> e{W, nc, (long)Md * nc, 1.0, 0.0};
+    EpiStore<cs> e{part, nc, (long)Md * nc, 1.0, 0.0};
     KSegLead ks{gmask, lrange, (nc + kCGroup - 1) / kCGroup, ML, ff ? 1 : 0};
     if (!tri_on) {
       HIPCHK((launch_sel<R, R, R, false, false, RM>(tsel[TS_OMGXI], g, NK, e, st, ks,
@@ -2135,6 +2171,13 @@ struct Engine : EngineBase {
     });
     HIPCHK(hipGetLastError());
     const bool fs = fsal_on();
+    // each stage's Ωξ product issued as soon as its X is written (unfused attempts)
+    xi_early = !fused_used && nc > 0;
+    struct Off {  // also on error
+      bool& f;
+      ~Off() { f = false; }
+    } off{xi_early};
+    if (xi_early && omit1) issue_xi(1, stream, !profile);  // X = k_1 (the last k_7) is there
     for (int s = omit1 ? 1 : 0; s < 7; ++s) {
       const bool last = (s == 6);
       double2* sdst = last ? sig[iyn] : sig[9];
@@ -2172,6 +2215,9 @@ struct Engine : EngineBase {
       rhs_stage(s, sin_, os, s, s == 0 ? Jd : ((last && fs) ? Jn : nullptr), true);
       skip_now = nullptr;
     }
+    // the early products' stream joins the main one (its last product, the seventh
+    // stage's, was waited for by that stage's Ψ branch)
+    if (xi_early && !profile) HIPCHK(hipStreamWaitEvent(stream, ev_xi[6 & 1], 0));
     if (fused_used) {
       // y_new ξ_c from the last stage's partials: the next attempt's y0 ξ_c if accepted
       const OmegaFusedArgs fa = fused_args(kFusedChunk[tsel[TS_OMEGA]]);
