@@ -1590,7 +1590,10 @@ struct Engine : EngineBase {
 
   // Stage inputs of σ and Ψ (Ω's is formed inside its own right-hand side kernel);
   // terms with a zero coefficient are left out.
-  void combine(int s, bool first, double2* sig_dst, bool last) {
+  // do_sig / do_psi: the σ and the Ψ part (the attempts issue σ's for stage s on the main
+  // stream right after stage s - 1's σ branch, ahead of that stage's join)
+  void combine(int s, bool first, double2* sig_dst, bool last, bool do_sig = true,
+               bool do_psi = true) {
     const int blocks = 2048;
     int nt = 0, slot[6];
     double coef[6];
@@ -1602,7 +1605,7 @@ struct Engine : EngineBase {
     const int k1 = (nt > 0 && slot[0] == 0) ? 0 : -1;
     const StageParams* sps = sp + s * NI;
     timed(KC_COMBINE, [&] {
-      if (!first) {
+      if (!first && do_sig) {
         CombineArgs<double2> gs{};
         gs.y0 = sig[iy0];
         gs.nterms = nt;
@@ -1612,6 +1615,7 @@ struct Engine : EngineBase {
         MapDense ms{NK * n, n, n, n, n};
         launch_combine(gs, ms, blocks, stream);
       }
+      if (!do_psi) return;
       CombineArgs<cs> gp{};
       gp.y0 = psi[iy0];
       gp.nterms = nt;
@@ -1678,7 +1682,7 @@ struct Engine : EngineBase {
 
   // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
   void rhs_stage(int s, const double2* sig_in, const OmegaStage& os, int kslot, double* jout,
-                 bool hub) {
+                 bool hub, int sig_next = -1) {
     const StageParams* sps = sp + s * NI;
     timed(KC_GATHER, [&] {
       gather(sps, sig_in, hub);
@@ -1701,6 +1705,10 @@ struct Engine : EngineBase {
     if (xi_early && s + 1 < 7) issue_xi(s + 1, s_omg, conc);
     stage_psi(s, os, kslot, s_psi);
     stage_sigma(s, sig_in, kslot, jout, stream);
+    // the σ stage input of the next stage, as soon as this stage's σ derivative is there
+    // (this stage's other branches do not read σ's stage input)
+    if (sig_next > 0)
+      combine(sig_next, false, sig_next == 6 ? sig[iyn] : sig[9], sig_next == 6, true, false);
     if (conc) {
       HIPCHK(hipEventRecord(ev_join[0], aux[0]));
       HIPCHK(hipEventRecord(ev_join[1], aux[1]));
@@ -2178,10 +2186,12 @@ struct Engine : EngineBase {
       ~Off() { f = false; }
     } off{xi_early};
     if (xi_early && omit1) issue_xi(1, stream, !profile);  // X = k_1 (the last k_7) is there
-    for (int s = omit1 ? 1 : 0; s < 7; ++s) {
+    const int s0 = omit1 ? 1 : 0;
+    for (int s = s0; s < 7; ++s) {
       const bool last = (s == 6);
       double2* sdst = last ? sig[iyn] : sig[9];
-      combine(s, s == 0, sdst, last);
+      // σ's part of the stage combination was issued by the stage before (rhs_stage)
+      combine(s, s == 0, sdst, last, s == s0, true);
       const double2* sin_ = (s == 0) ? sig[iy0] : sdst;
       OmegaStage os;
       os.y0 = omg[iy0];
@@ -2212,7 +2222,8 @@ struct Engine : EngineBase {
       // the first stage's kernels leave the instances that reuse k_7 alone; the currents
       // of the first stage are J, those of the last (y_new) the next attempt's
       skip_now = (s == 0) ? skip1 : nullptr;
-      rhs_stage(s, sin_, os, s, s == 0 ? Jd : ((last && fs) ? Jn : nullptr), true);
+      rhs_stage(s, sin_, os, s, s == 0 ? Jd : ((last && fs) ? Jn : nullptr), true,
+                last ? -1 : s + 1);
       skip_now = nullptr;
     }
     // the early products' stream joins the main one (its last product, the seventh
