@@ -234,11 +234,9 @@ struct Engine : EngineBase {
   // the σ branch (Π, currents, Hσ) are independent once the stage input is gathered:
   // they run on three streams (fork/join events, also inside the captured graph), so
   // MFMA-heavy and HBM-heavy kernels share the CUs.  All streams at default priority.
-  // aux[0]: Ω branch, aux[1]: Ψ branch, aux[2]: the next stage's Ωξ product, issued as
-  // soon as this stage's Ω kernel has written its k (attempts)
-  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+  // the Ω and Ψ branches of a stage (they take turns, rhs_stage)
+  hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
-  hipEvent_t ev_omg = nullptr, ev_xi[2] = {nullptr, nullptr};
   std::vector<void*> allocs;
   Tableau tab = dop54();
   std::vector<PulseDesc> pulse;  // per instance
@@ -439,8 +437,6 @@ struct Engine : EngineBase {
     for (auto& a : aux) HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     for (auto& e : ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&ev_omg, hipEventDisableTiming));
-    for (auto& e : ev_xi) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     HIPCHK(hipHostMalloc((void**)&ctrl_h, sizeof(Ctrl) * NI, hipHostMallocDefault));
     std::memset(ctrl_h, 0, sizeof(Ctrl) * NI);
@@ -470,9 +466,7 @@ struct Engine : EngineBase {
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     for (auto& e : ev_join)
       if (e) (void)hipEventDestroy(e);
-    if (ev_omg) (void)hipEventDestroy(ev_omg);
-    for (auto& e : ev_xi)
-      if (e) (void)hipEventDestroy(e);
+
     for (auto& a : aux)
       if (a) (void)hipStreamDestroy(a);
 
@@ -1668,16 +1662,10 @@ struct Engine : EngineBase {
     cs* kout = nullptr;
   };
 
-  // Ωξ product of stage t (X = the k of stage t - 1, omg[t - 1]) into wpart(t), on aux[2]
-  // after the work on `after` (the kernel that wrote it); ev_xi[t & 1] marks it done.
-  void issue_xi(int t, hipStream_t after, bool conc) {
-    hipStream_t sx = conc ? aux[2] : stream;
-    if (conc) {
-      HIPCHK(hipEventRecord(ev_omg, after));
-      HIPCHK(hipStreamWaitEvent(sx, ev_omg, 0));
-    }
-    timed(KC_GEMM_OMGXI, [&] { omgxi_run<kRM3>(omg[t - 1], sx, true, wpart(t)); });
-    if (conc) HIPCHK(hipEventRecord(ev_xi[t & 1], sx));
+  // Ωξ product of stage t (X = the k of stage t - 1, omg[t - 1]) into wpart(t), on st:
+  // the stream of the kernel that wrote X, which stage t's Ψ branch then continues
+  void issue_xi(int t, hipStream_t st) {
+    timed(KC_GEMM_OMGXI, [&] { omgxi_run<kRM3>(omg[t - 1], st, true, wpart(t)); });
   }
 
   // Right-hand side of stage s from (sig_in, Ap, Ω stage os) into slot kslot.
@@ -1689,29 +1677,32 @@ struct Engine : EngineBase {
       if (hub) apply_model(sps, stream);  // H(t_s, σ_s) of the attempts
     });
     // the three branches (serial on the main stream when profiling, so that the
-    // per-class event timings stay meaningful)
+    // per-class event timings stay meaningful).  The two side streams take turns: stage
+    // s's Ω kernel runs on aux[s & 1] and is followed there by the next stage's Ωξ
+    // product of its k (early, overlapping the rest of this stage and the next one's
+    // stage combination and gathers), which that stage's Ψ branch continues; stage s's
+    // Ψ branch runs on the other one, after this stage's product.  Three streams in all,
+    // one hardware queue each.
     const bool conc = !profile;
     hipStream_t s_psi = stream, s_omg = stream;
     if (conc) {
+      s_omg = aux[s & 1];
+      s_psi = aux[(s + 1) & 1];
       HIPCHK(hipEventRecord(ev_fork, stream));
-      HIPCHK(hipStreamWaitEvent(aux[0], ev_fork, 0));
-      HIPCHK(hipStreamWaitEvent(aux[1], ev_fork, 0));
-      s_omg = aux[0];
-      s_psi = aux[1];
+      HIPCHK(hipStreamWaitEvent(s_omg, ev_fork, 0));
+      HIPCHK(hipStreamWaitEvent(s_psi, ev_fork, 0));
     }
     stage_omega(os, kslot, s_omg);
-    // the next stage's Ωξ product of this stage's k, as soon as it is written: it overlaps
-    // the rest of this stage and the next one's stage combination and gathers
-    if (xi_early && s + 1 < 7) issue_xi(s + 1, s_omg, conc);
+    if (conc) HIPCHK(hipEventRecord(ev_join[0], s_omg));  // the join does not wait for
+    if (xi_early && s + 1 < 7) issue_xi(s + 1, s_omg);    // the early product
     stage_psi(s, os, kslot, s_psi);
+    if (conc) HIPCHK(hipEventRecord(ev_join[1], s_psi));
     stage_sigma(s, sig_in, kslot, jout, stream);
     // the σ stage input of the next stage, as soon as this stage's σ derivative is there
     // (this stage's other branches do not read σ's stage input)
     if (sig_next > 0)
       combine(sig_next, false, sig_next == 6 ? sig[iyn] : sig[9], sig_next == 6, true, false);
     if (conc) {
-      HIPCHK(hipEventRecord(ev_join[0], aux[0]));
-      HIPCHK(hipEventRecord(ev_join[1], aux[1]));
       HIPCHK(hipStreamWaitEvent(stream, ev_join[0], 0));
       HIPCHK(hipStreamWaitEvent(stream, ev_join[1], 0));
     }
@@ -1836,8 +1827,8 @@ struct Engine : EngineBase {
       const bool res = os.y0xi_resident;
       WSumArgs<cs> w{};
       if (xi_early && s >= 1) {
-        // issued as soon as the previous stage's Ω kernel wrote its k (issue_xi)
-        if (!profile) HIPCHK(hipStreamWaitEvent(st, ev_xi[s & 1], 0));
+        // issued as soon as the previous stage's Ω kernel wrote its k, earlier on this
+        // stream (issue_xi)
         w.part = wpart(s);
       } else {
         timed(KC_GEMM_OMGXI, [&] { omgxi_run<kRM3>((s == 0) ? os.y0 : omg[s - 1], st); });
@@ -2185,7 +2176,16 @@ struct Engine : EngineBase {
       bool& f;
       ~Off() { f = false; }
     } off{xi_early};
-    if (xi_early && omit1) issue_xi(1, stream, !profile);  // X = k_1 (the last k_7) is there
+    if (xi_early && omit1) {
+      // X = k_1 (the last attempt's k_7) is there; stage 1's Ψ branch runs on aux[0]
+      hipStream_t st = stream;
+      if (!profile) {
+        st = aux[0];
+        HIPCHK(hipEventRecord(ev_fork, stream));
+        HIPCHK(hipStreamWaitEvent(st, ev_fork, 0));
+      }
+      issue_xi(1, st);
+    }
     const int s0 = omit1 ? 1 : 0;
     for (int s = s0; s < 7; ++s) {
       const bool last = (s == 6);
@@ -2226,9 +2226,7 @@ struct Engine : EngineBase {
                 last ? -1 : s + 1);
       skip_now = nullptr;
     }
-    // the early products' stream joins the main one (its last product, the seventh
-    // stage's, was waited for by that stage's Ψ branch)
-    if (xi_early && !profile) HIPCHK(hipStreamWaitEvent(stream, ev_xi[6 & 1], 0));
+    // every early product is followed on its stream by a Ψ branch the joins wait for
     if (fused_used) {
       // y_new ξ_c from the last stage's partials: the next attempt's y0 ξ_c if accepted
       const OmegaFusedArgs fa = fused_args(kFusedChunk[tsel[TS_OMEGA]]);
