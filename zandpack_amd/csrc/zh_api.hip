@@ -316,6 +316,13 @@ struct Engine : EngineBase {
   double2 *uvec = nullptr, *vvec = nullptr;  // per-stage (h/ħ)(χ± + Δ) of the Ω epilogue
   double* n0 = nullptr;
   cs *Bp = nullptr, *Ap = nullptr, *Aomg = nullptr, *Bomg = nullptr, *xic = nullptr, *W = nullptr;
+  // the Ψ stage input: two buffers, stage s of an attempt in apb(s) (Ap points at the
+  // current one; Ap0 outside attempts), so that a stage's Ψ kernel can form the next
+  // stage's input while its own is still read
+  cs *Ap0 = nullptr, *Ap1 = nullptr;
+  cs* apb(int s) const { return (s & 1) ? Ap1 : Ap0; }
+  bool psi_nx = false;  // attempt body being issued: the one-pass Ψ kernel of stage s >= 1
+                        // forms stage s + 1's Ψ input
   cs* W2 = nullptr;  // second buffer of the Ωξ partials: stage s's product goes to wpart(s)
   cs* wpart(int s) const { return (s & 1) ? W2 : W; }
   bool xi_early = false;  // attempt body being issued: each stage's product issued early
@@ -590,9 +597,8 @@ struct Engine : EngineBase {
     // room for the error-norm partials of the finest Ω tile candidate (32 x 32)
     const long t32 = (Md + 31) / 32;
     te_part = dalloc<double>((size_t)2 * kTEBlocks * NI + t32 * t32 * NK);
-    // (+ 16 zeroed elements: the one-pass Ψ kernel reads its ΔP ξ_c rows in chunks of 8
-    // that may run past the last row)
-    Ap = dalloc<cs>((size_t)NK * Md * (n + nc) + 16);
+    Ap = Ap0 = dalloc<cs>((size_t)NK * Md * (n + nc));
+    Ap1 = dalloc<cs>((size_t)NK * Md * (n + nc));
     psi_span = dalloc<int2>((n + 63) / 64);
     Bp = dalloc<cs>((size_t)NK * (n + nc) * n);
     Aomg = dalloc<cs>((size_t)NK * Md * 2 * nc);
@@ -762,8 +768,10 @@ struct Engine : EngineBase {
       // strided device copies (a strided copy from pageable host memory goes row by row)
       HIPCHK(hipMemcpyAsync(W, apr.data(), sizeof(cs) * mc, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(W + mc, bol.data(), sizeof(cs) * mc, hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpy2DAsync(Ap + n, sizeof(cs) * (n + nc), W, sizeof(cs) * nc, sizeof(cs) * nc,
-                              (size_t)NK * Md, hipMemcpyDeviceToDevice, stream));
+      for (cs* a : {Ap0, Ap1})
+        HIPCHK(hipMemcpy2DAsync(a + n, sizeof(cs) * (n + nc), W, sizeof(cs) * nc,
+                                sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyDeviceToDevice,
+                                stream));
       HIPCHK(hipMemcpy2DAsync(Aomg + nc, sizeof(cs) * 2 * nc, W, sizeof(cs) * nc,
                               sizeof(cs) * nc, (size_t)NK * Md, hipMemcpyDeviceToDevice, stream));
       HIPCHK(hipMemcpy2DAsync(Bomg, sizeof(cs) * 2 * nc, W + mc, sizeof(cs) * nc,
@@ -1875,16 +1883,37 @@ struct Engine : EngineBase {
         f.sp = sps; f.ki = KI;
         f.Md = Md; f.n = n; f.nc = nc; f.acc0_rows = nc > 0 ? g0_rows() : 0;
         f.inv_hbar = ih; f.inv_hbar2 = ih * ih;
+        if (psi_nx && s >= 1 && s + 1 < 7) {
+          // stage s + 1's input y0 + Σ_j a_{s+1,j} k_j, this stage's k from registers
+          const int t = s + 1;
+          f.nx_out = apb(t);
+          f.nx_y = (t == 6) ? psi[iyn] : nullptr;
+          f.nx_y0 = psi[iy0];
+          for (int j = 0; j < t; ++j)
+            if (tab.a[t][j] != 0.0) {
+              require(f.nx_n < 6, "stage combination: too many terms");
+              if (j == s) f.nx_self = f.nx_n;
+              if (j == 0) f.nx_k1 = f.nx_n;
+              f.nx_k[f.nx_n] = psi[j];
+              f.nx_a[f.nx_n] = tab.a[t][j];
+              ++f.nx_n;
+            }
+        }
         constexpr int TR = 16;  // mode rows per workgroup (C3: 16 beat 8, 32 and 64)
         constexpr bool XL = kPsiXL<cs>;
         const size_t lds = (size_t)TR * (psi_wmax + (XL ? 64 : 0)) * sizeof(cs);
         const dim3 grid((n + 63) / 64, (Md + TR - 1) / TR, NK);
-        if (ell_R == 4)
-          hipLaunchKernelGGL((k_psi_fused<cs, TR, 4, XL>), grid, dim3(256), lds, st, f);
-        else if (ell_R == 8)
-          hipLaunchKernelGGL((k_psi_fused<cs, TR, 8, XL>), grid, dim3(256), lds, st, f);
-        else
-          hipLaunchKernelGGL((k_psi_fused<cs, TR, 0, XL>), grid, dim3(256), lds, st, f);
+        auto go = [&](auto nxt) {
+          constexpr bool NXT = decltype(nxt)::value;
+          if (ell_R == 4)
+            hipLaunchKernelGGL((k_psi_fused<cs, TR, 4, XL, NXT>), grid, dim3(256), lds, st, f);
+          else if (ell_R == 8)
+            hipLaunchKernelGGL((k_psi_fused<cs, TR, 8, XL, NXT>), grid, dim3(256), lds, st, f);
+          else
+            hipLaunchKernelGGL((k_psi_fused<cs, TR, 0, XL, NXT>), grid, dim3(256), lds, st, f);
+        };
+        if (f.nx_out) go(std::true_type());
+        else go(std::false_type());
         HIPCHK(hipGetLastError());
       } else if (h_sparse) {
         // [ΔPξ_c] [ħ σ_cᵀ] into the k slot (MFMA, K = nc), then the gather ψHᵀ with
@@ -2187,11 +2216,22 @@ struct Engine : EngineBase {
       issue_xi(1, st);
     }
     const int s0 = omit1 ? 1 : 0;
+    // the one-pass Ψ kernel of stage s >= 1 forms stage s + 1's Ψ input (complex128; at
+    // n = 1000, M = 4000: FP64 226 -> 235 steps/s, while the complex64 kernel, shorter and
+    // more latency-bound, lost more than the separate combination cost: 337 -> 322)
+    psi_nx = psi_fused() && !(overlap && nc > 0) && sizeof(cs) == 16;
+    struct Back {  // Ap back to its buffer outside attempts, also on error
+      Engine& e;
+      ~Back() { e.Ap = e.Ap0; e.psi_nx = false; }
+    } back{*this};
     for (int s = s0; s < 7; ++s) {
       const bool last = (s == 6);
       double2* sdst = last ? sig[iyn] : sig[9];
-      // σ's part of the stage combination was issued by the stage before (rhs_stage)
-      combine(s, s == 0, sdst, last, s == s0, true);
+      Ap = apb(s);
+      // σ's part of the stage combination was issued by the stage before (rhs_stage),
+      // and Ψ's too from stage 2 on with the one-pass Ψ kernel
+      const bool psi_done = psi_nx && s - 1 >= 1 && s - 1 >= s0;
+      combine(s, s == 0, sdst, last, s == s0, !psi_done);
       const double2* sin_ = (s == 0) ? sig[iy0] : sdst;
       OmegaStage os;
       os.y0 = omg[iy0];

@@ -43,9 +43,19 @@ struct PsiFusedArgs {
   const StageParams* sp; int ki;
   int Md, n, nc, acc0_rows;
   double inv_hbar, inv_hbar2;
+  // the next stage's input, formed here (the stage combination's Ψ part): nx_out (Ap
+  // layout) and nx_y (y_new, the last stage's input, or null) = y0 + Σ_j nx_a[j] k_j over
+  // nx_n slots nx_k[j] (term nx_self is this kernel's k, taken from registers; term nx_k1,
+  // the k_1 slot, weighted by the instance's r1); none when nx_out is null
+  C* nx_out = nullptr;
+  C* nx_y = nullptr;
+  const C* nx_y0 = nullptr;
+  const C* nx_k[6] = {nullptr};
+  double nx_a[6] = {0};
+  int nx_n = 0, nx_self = -1, nx_k1 = -1;
 };
 
-template <typename C, int TR, int RC, bool XL>
+template <typename C, int TR, int RC, bool XL, bool NXT>
 __global__ __launch_bounds__(256) void k_psi_fused(PsiFusedArgs<C> a) {
   static_assert(TR == 16, "one 16-row matrix-core block per workgroup");
   typedef decltype(C::x) T;
@@ -210,7 +220,27 @@ __global__ __launch_bounds__(256) void k_psi_fused(PsiFusedArgs<C> a) {
       vr += gx.x + wv.x * a.inv_hbar2;
       vi += gx.y + wv.y * a.inv_hbar2;
     }
-    st_stream(a.out + bm * n + o, from_d2<C>(make_double2(hs * vi, -hs * vr)));
+    const C kv = from_d2<C>(make_double2(hs * vi, -hs * vr));
+    st_stream(a.out + bm * n + o, kv);
+    if constexpr (NXT) {
+      // the next stage's input at (i, o), the terms in the stage combination's order
+      const long e = bm * n + o;
+      C x[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        if (j < a.nx_n) x[j] = j == a.nx_self ? kv : ld_stream(a.nx_k[j] + e);
+      double2 y = to_d2(ld_stream(a.nx_y0 + e));
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        if (j < a.nx_n) {
+          const double aj = j == a.nx_k1 ? a.nx_a[j] * sq.r1 : a.nx_a[j];
+          y.x += aj * (double)x[j].x;
+          y.y += aj * (double)x[j].y;
+        }
+      const C yc = from_d2<C>(y);
+      st_stream(a.nx_out + (long)b * a.sAp + (long)i * a.ldAp + o, yc);
+      if (a.nx_y) st_stream(a.nx_y + e, yc);
+    }
   }
 }
 
