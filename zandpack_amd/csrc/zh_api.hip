@@ -323,6 +323,15 @@ struct Engine : EngineBase {
   cs* apb(int s) const { return (s & 1) ? Ap1 : Ap0; }
   bool psi_nx = false;  // attempt body being issued: the one-pass Ψ kernel of stage s >= 1
                         // forms stage s + 1's Ψ input
+  // attempt body being issued: the last stage's input pass also leaves the error norm's
+  // weighted sum of k_1 .. k_6 (E6) in the free k_2 slot, so that the error-norm pass
+  // reads E6 and k_7 only (σ; Ψ in FP64, where E6 keeps the norm's precision)
+  bool e6_sig = false, e6_psi = false;
+  bool e6_ok() const {
+    bool ok = tab.a[6][1] == 0.0 && tab.ct[1] == 0.0 && tab.ct[6] != 0.0;
+    for (int j = 0; j < 6; ++j) ok = ok && ((tab.a[6][j] != 0.0) == (tab.ct[j] != 0.0));
+    return ok;
+  }
   cs* W2 = nullptr;  // second buffer of the Ωξ partials: stage s's product goes to wpart(s)
   cs* wpart(int s) const { return (s & 1) ? W2 : W; }
   bool xi_early = false;  // attempt body being issued: each stage's product issued early
@@ -1614,6 +1623,10 @@ struct Engine : EngineBase {
         for (int j = 0; j < nt; ++j) { gs.k[j] = sig[slot[j]]; gs.a[j] = coef[j]; }
         gs.d1 = sig_dst; gs.d2 = nullptr;
         gs.sp = sps; gs.ki = KI; gs.k1 = k1; gs.rows_b = n;
+        if (s == 6 && e6_sig) {
+          gs.e_out = sig[1];
+          for (int j = 0; j < nt; ++j) gs.ect[j] = tab.ct[slot[j]];
+        }
         MapDense ms{NK * n, n, n, n, n};
         launch_combine(gs, ms, blocks, stream);
       }
@@ -1625,6 +1638,10 @@ struct Engine : EngineBase {
       gp.d1 = Ap;
       gp.d2 = last ? psi[iyn] : nullptr;
       gp.sp = sps; gp.ki = KI; gp.k1 = k1; gp.rows_b = Md;
+      if (s == 6 && e6_psi) {
+        gp.e_out = psi[1];
+        for (int j = 0; j < nt; ++j) gp.ect[j] = tab.ct[slot[j]];
+      }
       MapDense mp{NK * Md, n, n, n + nc, n};
       launch_combine(gp, mp, blocks, stream);
     });
@@ -1896,8 +1913,10 @@ struct Engine : EngineBase {
               if (j == 0) f.nx_k1 = f.nx_n;
               f.nx_k[f.nx_n] = psi[j];
               f.nx_a[f.nx_n] = tab.a[t][j];
+              f.nx_ct[f.nx_n] = tab.ct[j];
               ++f.nx_n;
             }
+          if (t == 6 && e6_psi) f.nx_e = psi[1];
         }
         constexpr int TR = 16;  // mode rows per workgroup (C3: 16 beat 8, 32 and 64)
         constexpr bool XL = kPsiXL<cs>;
@@ -2152,6 +2171,14 @@ struct Engine : EngineBase {
         require(last_w == 6 && ts.k1 == 0, "first same as last: k_7 must carry an error weight");
         ts.kdst = sig[0]; tp.kdst = psi[0];
       }
+      // E6 (weighted sum of k_1 .. k_6, r1 applied) in the k_2 slot, then k_7
+      auto two = [&](auto& t, auto* e6, auto* k7) {
+        t.k[0] = e6; t.k[1] = k7;
+        t.ct[0] = 1.0; t.ct[1] = tab.ct[6];
+        t.k1 = -1;
+      };
+      if (e6_sig) two(ts, sig[1], sig[6]);
+      if (e6_psi) two(tp, psi[1], psi[6]);
       if (tol_scaled) {
         ts.y0 = sig[iy0]; ts.y1 = sig[iyn]; tp.y0 = psi[iy0]; tp.y1 = psi[iyn];
         ts.rtol = tp.rtol = tol_rtol; ts.atol = tp.atol = tol_atol;
@@ -2161,8 +2188,12 @@ struct Engine : EngineBase {
       MapDense ms{KI * n, n, n, n, n};
       MapDense mp{KI * Md, n, n, n, n};
       const dim3 grid(kTEBlocks, NI);
-      hipLaunchKernelGGL((k_te<double2, MapDense, 6>), grid, dim3(256), 0, stream, ts, ms, te_part,
-                         (long)KI * n * n);
+      if (e6_sig)
+        hipLaunchKernelGGL((k_te<double2, MapDense, 2>), grid, dim3(256), 0, stream, ts, ms,
+                           te_part, (long)KI * n * n);
+      else
+        hipLaunchKernelGGL((k_te<double2, MapDense, 6>), grid, dim3(256), 0, stream, ts, ms,
+                           te_part, (long)KI * n * n);
       bool done = false;
       if constexpr (sizeof(cs) == 8) {
         if (mp.pairs()) {
@@ -2171,9 +2202,14 @@ struct Engine : EngineBase {
           done = true;
         }
       }
-      if (!done)
-        hipLaunchKernelGGL((k_te<cs, MapDense, 6>), grid, dim3(256), 0, stream, tp, mp,
-                           te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
+      if (!done) {
+        if (e6_psi)
+          hipLaunchKernelGGL((k_te<cs, MapDense, 2>), grid, dim3(256), 0, stream, tp, mp,
+                             te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
+        else
+          hipLaunchKernelGGL((k_te<cs, MapDense, 6>), grid, dim3(256), 0, stream, tp, mp,
+                             te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
+      }
     });
     HIPCHK(hipGetLastError());
     timed(KC_CTRL, [&] {
@@ -2220,9 +2256,11 @@ struct Engine : EngineBase {
     // n = 1000, M = 4000: FP64 226 -> 235 steps/s, while the complex64 kernel, shorter and
     // more latency-bound, lost more than the separate combination cost: 337 -> 322)
     psi_nx = psi_fused() && !(overlap && nc > 0) && sizeof(cs) == 16;
+    e6_sig = e6_ok();
+    e6_psi = e6_sig && sizeof(cs) == 16;
     struct Back {  // Ap back to its buffer outside attempts, also on error
       Engine& e;
-      ~Back() { e.Ap = e.Ap0; e.psi_nx = false; }
+      ~Back() { e.Ap = e.Ap0; e.psi_nx = e.e6_sig = e.e6_psi = false; }
     } back{*this};
     for (int s = s0; s < 7; ++s) {
       const bool last = (s == 6);

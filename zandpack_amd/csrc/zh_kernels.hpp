@@ -328,16 +328,20 @@ struct CombineArgs {
   const StageParams* sp = nullptr;
   int ki = 1 << 30, k1 = -1;
   int rows_b = 1 << 30;
+  // also e_out = Σ_j ect[j] k_j over the same terms (slot layout), or none: the last
+  // stage's input reads the k_j the error norm weights, all but k_7
+  C* e_out = nullptr;
+  double ect[7] = {0};
 };
 // the coefficients of one row's instance; false: the row is left alone
 template <typename C>
-__device__ inline bool combine_coefs(const CombineArgs<C>& g, int r, double* a) {
+__device__ inline bool combine_coefs(const CombineArgs<C>& g, int r, double* a, double* e) {
 #pragma unroll
-  for (int j = 0; j < 7; ++j) a[j] = g.a[j];
+  for (int j = 0; j < 7; ++j) { a[j] = g.a[j]; e[j] = g.ect[j]; }
   if (!g.sp) return true;
   const StageParams& q = g.sp[(r / g.rows_b) / g.ki];
   if (q.skip) return false;
-  if (g.k1 >= 0) a[g.k1] *= q.r1;
+  if (g.k1 >= 0) { a[g.k1] *= q.r1; e[g.k1] *= q.r1; }
   return true;
 }
 
@@ -347,8 +351,8 @@ template <typename C, typename Map, int NT>
 __global__ __launch_bounds__(256) void k_combine(CombineArgs<C> g, Map mp) {
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
-    double a[7];
-    if (!combine_coefs(g, r, a)) continue;
+    double a[7], ec[7];
+    if (!combine_coefs(g, r, a, ec)) continue;
     long src, o1, o2;
     const int cols = mp.row(r, src, o1, o2);
     for (int c = threadIdx.x; c < cols; c += blockDim.x) {
@@ -365,6 +369,15 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs<C> g, Map mp) {
       const C out = from_d2<C>(make_double2(vr, vi));
       g.d1[o1 + c] = out;
       if (g.d2) g.d2[o2 + c] = out;
+      if (g.e_out) {
+        double er = 0.0, ei = 0.0;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          er += ec[j] * (double)kv[j + 1].x;
+          ei += ec[j] * (double)kv[j + 1].y;
+        }
+        st_stream(g.e_out + src + c, from_d2<C>(make_double2(er, ei)));
+      }
     }
   }
 }
@@ -376,8 +389,8 @@ __global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp)
   static_assert(sizeof(C) == 8, "complex64 pairs");
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
-    double a[7];
-    if (!combine_coefs(g, r, a)) continue;
+    double a[7], ec[7];  // (e_out goes through k_combine)
+    if (!combine_coefs(g, r, a, ec)) continue;
     long src, o1, o2;
     const int cols = mp.row(r, src, o1, o2);
     for (int c = 2 * threadIdx.x; c < cols; c += 2 * blockDim.x) {
@@ -404,7 +417,7 @@ __global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp)
 template <typename C, typename Map>
 inline void launch_combine(const CombineArgs<C>& g, const Map& mp, int blocks, hipStream_t s) {
   if constexpr (sizeof(C) == 8) {
-    if (mp.pairs()) {
+    if (mp.pairs() && !g.e_out) {
       switch (g.nterms) {
 #define ZH_COMBINE_CASE(N)                                                                   \
   case N:                                                                                    \

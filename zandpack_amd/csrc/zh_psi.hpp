@@ -53,6 +53,10 @@ struct PsiFusedArgs {
   const C* nx_k[6] = {nullptr};
   double nx_a[6] = {0};
   int nx_n = 0, nx_self = -1, nx_k1 = -1;
+  // and nx_e = Σ_j nx_ct[j] k_j over the same terms (the last stage's input: the error
+  // norm's weighted sum of all but k_7), or none
+  C* nx_e = nullptr;
+  double nx_ct[6] = {0};
 };
 
 template <typename C, int TR, int RC, bool XL, bool NXT>
@@ -240,6 +244,17 @@ __global__ __launch_bounds__(256) void k_psi_fused(PsiFusedArgs<C> a) {
       const C yc = from_d2<C>(y);
       st_stream(a.nx_out + (long)b * a.sAp + (long)i * a.ldAp + o, yc);
       if (a.nx_y) st_stream(a.nx_y + e, yc);
+      if (a.nx_e) {
+        double2 u = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+          if (j < a.nx_n) {
+            const double cj = j == a.nx_k1 ? a.nx_ct[j] * sq.r1 : a.nx_ct[j];
+            u.x += cj * (double)x[j].x;
+            u.y += cj * (double)x[j].y;
+          }
+        st_stream(a.nx_e + e, from_d2<C>(u));
+      }
     }
   }
 }
