@@ -327,6 +327,9 @@ struct Engine : EngineBase {
   // weighted sum of k_1 .. k_6 (E6) in the free k_2 slot, so that the error-norm pass
   // reads E6 and k_7 only (σ; Ψ in FP64, where E6 keeps the norm's precision)
   bool e6_sig = false, e6_psi = false;
+  // with E6 the last stage's σ / Ψ derivative k_7 goes straight over the k_1 slot (no
+  // later pass reads k_1), as the next attempt's k_1 (first same as last)
+  bool k7_sig0 = false, k7_psi0 = false;
   bool e6_ok() const {
     bool ok = tab.a[6][1] == 0.0 && tab.ct[1] == 0.0 && tab.ct[6] != 0.0;
     for (int j = 0; j < 6; ++j) ok = ok && ((tab.a[6][j] != 0.0) == (tab.ct[j] != 0.0));
@@ -1720,9 +1723,9 @@ struct Engine : EngineBase {
     stage_omega(os, kslot, s_omg);
     if (conc) HIPCHK(hipEventRecord(ev_join[0], s_omg));  // the join does not wait for
     if (xi_early && s + 1 < 7) issue_xi(s + 1, s_omg);    // the early product
-    stage_psi(s, os, kslot, s_psi);
+    stage_psi(s, os, (s == 6 && k7_psi0) ? 0 : kslot, s_psi);
     if (conc) HIPCHK(hipEventRecord(ev_join[1], s_psi));
-    stage_sigma(s, sig_in, kslot, jout, stream);
+    stage_sigma(s, sig_in, (s == 6 && k7_sig0) ? 0 : kslot, jout, stream);
     // the σ stage input of the next stage, as soon as this stage's σ derivative is there
     // (this stage's other branches do not read σ's stage input)
     if (sig_next > 0)
@@ -2177,8 +2180,10 @@ struct Engine : EngineBase {
         t.ct[0] = 1.0; t.ct[1] = tab.ct[6];
         t.k1 = -1;
       };
-      if (e6_sig) two(ts, sig[1], sig[6]);
-      if (e6_psi) two(tp, psi[1], psi[6]);
+      if (e6_sig) two(ts, sig[1], sig[k7_sig0 ? 0 : 6]);
+      if (e6_psi) two(tp, psi[1], psi[k7_psi0 ? 0 : 6]);
+      if (k7_sig0) ts.kdst = nullptr;  // k_7 is there already
+      if (k7_psi0) tp.kdst = nullptr;
       if (tol_scaled) {
         ts.y0 = sig[iy0]; ts.y1 = sig[iyn]; tp.y0 = psi[iy0]; tp.y1 = psi[iyn];
         ts.rtol = tp.rtol = tol_rtol; ts.atol = tp.atol = tol_atol;
@@ -2258,9 +2263,14 @@ struct Engine : EngineBase {
     psi_nx = psi_fused() && !(overlap && nc > 0) && sizeof(cs) == 16;
     e6_sig = e6_ok();
     e6_psi = e6_sig && sizeof(cs) == 16;
+    k7_sig0 = fs && e6_sig;
+    k7_psi0 = fs && e6_psi;
     struct Back {  // Ap back to its buffer outside attempts, also on error
       Engine& e;
-      ~Back() { e.Ap = e.Ap0; e.psi_nx = e.e6_sig = e.e6_psi = false; }
+      ~Back() {
+        e.Ap = e.Ap0;
+        e.psi_nx = e.e6_sig = e.e6_psi = e.k7_sig0 = e.k7_psi0 = false;
+      }
     } back{*this};
     for (int s = s0; s < 7; ++s) {
       const bool last = (s == 6);
