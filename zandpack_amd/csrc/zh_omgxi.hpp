@@ -284,6 +284,21 @@ __global__ __launch_bounds__(256, OCC) void k_xi_stream(XiArgs<C> a) {
         o.bv[s][t] = *(const C*)(Bb + xi_lds_b<C, NG>(kl, 16 * t + (lane & 15)));
     }
   };
+  // the same with the chunk's sign known at compile time
+  auto read_sign = [&](int q, Ops& o, auto mirc) {
+    constexpr int MIR = decltype(mirc)::value;
+    const unsigned char* Ab = lds + (q % STAGES) * G::STAGE;
+    const unsigned char* Bb = Ab + G::A_BYTES;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int kl = 4 * s + (lane >> 4);
+      o.av[s] = *(const C*)(Ab + xi_lds_a<C, NG>(rr, kl, MIR));
+      if (MIR) o.av[s].x = -o.av[s].x;
+#pragma unroll
+      for (int t = 0; t < NG; ++t)
+        o.bv[s][t] = *(const C*)(Bb + xi_lds_b<C, NG>(kl, 16 * t + (lane & 15)));
+    }
+  };
   // complex128: three real products per complex one,
   //   re = (-ai)(br + bi) + (ar + ai) br,  im = ar (bi - br) + (ar + ai) br;
   // complex64 keeps four (three were 12% faster at n = 1000, M = 4000, but their
@@ -321,8 +336,22 @@ __global__ __launch_bounds__(256, OCC) void k_xi_stream(XiArgs<C> a) {
     else xi_vmwait<0>();
     xi_barrier();  // chunk q is in LDS for every wave; chunk q - 1's buffer is free
     Ops o;
-    read(q, o);
-    mma(o);
+    if constexpr (M3) {
+      read(q, o);
+      mma(o);
+    } else {
+      // complex64: one copy of the reads and products per sign (at n = 1000, M = 4000:
+      // 116 against 120 us; complex128 runs out of registers that way, 212 against 132)
+      int kc0, mir;
+      xi_chunk<G::KC>(P, q0 + q, kc0, mir);
+      if (mir) {
+        read_sign(q, o, std::integral_constant<int, 1>());
+        mma(o);
+      } else {
+        read_sign(q, o, std::integral_constant<int, 0>());
+        mma(o);
+      }
+    }
     if (q + STAGES - 1 < nq) issue(q + STAGES - 1);
   }
 
