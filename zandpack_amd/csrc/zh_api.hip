@@ -338,6 +338,9 @@ struct Engine : EngineBase {
   cs* W2 = nullptr;  // second buffer of the Ωξ partials: stage s's product goes to wpart(s)
   cs* wpart(int s) const { return (s & 1) ? W2 : W; }
   bool xi_early = false;  // attempt body being issued: each stage's product issued early
+  // attempt body being issued: the last stage issues the σ and Ψ error-norm partials on
+  // its σ and Ψ branches' streams (te_launch), the attempt ends with the controller alone
+  bool te_stage = false;
   cs* Pred[7] = {nullptr};  // y0 ξ_c and k_j ξ_c of the current attempt (Ω_s ξ_c by linearity)
   cs* Wst = nullptr;        // Ω_s ξ_c of the current stage
   // Fused Ω stage kernel (mixed precision): each stage's Ω kernel also writes k_s ξ_c
@@ -1724,8 +1727,10 @@ struct Engine : EngineBase {
     if (conc) HIPCHK(hipEventRecord(ev_join[0], s_omg));  // the join does not wait for
     if (xi_early && s + 1 < 7) issue_xi(s + 1, s_omg);    // the early product
     stage_psi(s, os, (s == 6 && k7_psi0) ? 0 : kslot, s_psi);
+    if (te_stage && s == 6) te_launch(false, true, s_psi);
     if (conc) HIPCHK(hipEventRecord(ev_join[1], s_psi));
     stage_sigma(s, sig_in, (s == 6 && k7_sig0) ? 0 : kslot, jout, stream);
+    if (te_stage && s == 6) te_launch(true, false, stream);
     // the σ stage input of the next stage, as soon as this stage's σ derivative is there
     // (this stage's other branches do not read σ's stage input)
     if (sig_next > 0)
@@ -2151,7 +2156,10 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
   }
 
-  void te_and_control() {
+  // Error-norm partials of σ (do_sig) and Ψ (do_psi) on st.  The attempts issue each on
+  // its branch's stream right after the last stage's σ / Ψ derivative, so that they
+  // overlap the last Ω kernel instead of following it.
+  void te_launch(bool do_sig, bool do_psi, hipStream_t st) {
     timed(KC_TE, [&] {
       // slots with a non-zero error weight (b5 - b4 vanishes for k_2)
       // (the Ω part is accumulated by the last stage's Ω kernel)
@@ -2193,30 +2201,37 @@ struct Engine : EngineBase {
       MapDense ms{KI * n, n, n, n, n};
       MapDense mp{KI * Md, n, n, n, n};
       const dim3 grid(kTEBlocks, NI);
-      if (e6_sig)
-        hipLaunchKernelGGL((k_te<double2, MapDense, 2>), grid, dim3(256), 0, stream, ts, ms,
-                           te_part, (long)KI * n * n);
-      else
-        hipLaunchKernelGGL((k_te<double2, MapDense, 6>), grid, dim3(256), 0, stream, ts, ms,
-                           te_part, (long)KI * n * n);
+      if (do_sig) {
+        if (e6_sig)
+          hipLaunchKernelGGL((k_te<double2, MapDense, 2>), grid, dim3(256), 0, st, ts, ms,
+                             te_part, (long)KI * n * n);
+        else
+          hipLaunchKernelGGL((k_te<double2, MapDense, 6>), grid, dim3(256), 0, st, ts, ms,
+                             te_part, (long)KI * n * n);
+      }
+      if (!do_psi) return;
       bool done = false;
       if constexpr (sizeof(cs) == 8) {
         if (mp.pairs()) {
-          hipLaunchKernelGGL((k_te_pairs<cs, MapDense, 6>), grid, dim3(256), 0, stream, tp, mp,
+          hipLaunchKernelGGL((k_te_pairs<cs, MapDense, 6>), grid, dim3(256), 0, st, tp, mp,
                              te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
           done = true;
         }
       }
       if (!done) {
         if (e6_psi)
-          hipLaunchKernelGGL((k_te<cs, MapDense, 2>), grid, dim3(256), 0, stream, tp, mp,
+          hipLaunchKernelGGL((k_te<cs, MapDense, 2>), grid, dim3(256), 0, st, tp, mp,
                              te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
         else
-          hipLaunchKernelGGL((k_te<cs, MapDense, 6>), grid, dim3(256), 0, stream, tp, mp,
+          hipLaunchKernelGGL((k_te<cs, MapDense, 6>), grid, dim3(256), 0, st, tp, mp,
                              te_part + (size_t)kTEBlocks * NI, (long)KI * Md * n);
       }
     });
     HIPCHK(hipGetLastError());
+  }
+
+  // The controller over the σ, Ψ and Ω error-norm partials, on the main stream.
+  void control() {
     timed(KC_CTRL, [&] {
       // the Ω kernels' partials are one per workgroup, batch-major: KI blocks of each
       // instance are contiguous
@@ -2269,9 +2284,10 @@ struct Engine : EngineBase {
       Engine& e;
       ~Back() {
         e.Ap = e.Ap0;
-        e.psi_nx = e.e6_sig = e.e6_psi = e.k7_sig0 = e.k7_psi0 = false;
+        e.psi_nx = e.e6_sig = e.e6_psi = e.k7_sig0 = e.k7_psi0 = e.te_stage = false;
       }
     } back{*this};
+    te_stage = true;
     for (int s = s0; s < 7; ++s) {
       const bool last = (s == 6);
       double2* sdst = last ? sig[iyn] : sig[9];
@@ -2326,7 +2342,7 @@ struct Engine : EngineBase {
       });
       HIPCHK(hipGetLastError());
     }
-    te_and_control();
+    control();
   }
 
   void ensure_h0() {

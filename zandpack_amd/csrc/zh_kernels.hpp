@@ -204,19 +204,19 @@ __global__ void k_stage_params(Ctrl* ctrl, const PulseDesc* pulses, DissDesc dis
   const double t0 = ctrl[g].t0, h = ctrl[g].h;
   __shared__ int fsal_s;
   __shared__ double r1_s;
+  StageParams q;
+  if (s < 7) q = stage_params_at(pulse, diss, drive, NL, t0 + tab.c[s] * h, h);
   if (s == 0) {
+    // the first stage's parameters against those of the last attempt's seventh stage
+    // (read here, rewritten below after the barrier)
     const Ctrl& c = ctrl[g];
-    bool fsal = false;
-    if (c.fsal && c.hprev > 0.0 && h > 0.0) {
-      const StageParams q0 = stage_params_at(pulse, diss, drive, NL, t0 + tab.c[0] * h, h);
-      fsal = same_params(q0, sp[6 * G + g], NL, drive.nt);
-    }
+    const bool fsal = c.fsal && c.hprev > 0.0 && h > 0.0 && same_params(q, sp[6 * G + g], NL,
+                                                                        drive.nt);
     fsal_s = fsal ? 1 : 0;
     r1_s = fsal ? h / c.hprev : 1.0;
   }
   __syncthreads();
   if (s < 7) {
-    StageParams q = stage_params_at(pulse, diss, drive, NL, t0 + tab.c[s] * h, h);
     q.r1 = r1_s;
     q.skip = (s == 0) ? fsal_s : 0;
     sp[s * G + g] = q;
