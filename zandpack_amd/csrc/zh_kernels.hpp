@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp)
   static_assert(sizeof(C) == 8, "complex64 pairs");
   const int nr = mp.nrows();
   for (int r = blockIdx.x; r < nr; r += gridDim.x) {
-    double a[7], ec[7];  // (e_out goes through k_combine)
+    double a[7], ec[7];
     if (!combine_coefs(g, r, a, ec)) continue;
     long src, o1, o2;
     const int cols = mp.row(r, src, o1, o2);
@@ -410,6 +410,18 @@ __global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp)
       const v4f out = {(float)v0r, (float)v0i, (float)v1r, (float)v1i};
       *reinterpret_cast<v4f*>(g.d1 + o1 + c) = out;
       if (g.d2) *reinterpret_cast<v4f*>(g.d2 + o2 + c) = out;
+      if (g.e_out) {
+        double e0r = 0.0, e0i = 0.0, e1r = 0.0, e1i = 0.0;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          e0r += ec[j] * (double)kv[j + 1].x;
+          e0i += ec[j] * (double)kv[j + 1].y;
+          e1r += ec[j] * (double)kv[j + 1].z;
+          e1i += ec[j] * (double)kv[j + 1].w;
+        }
+        const v4f e = {(float)e0r, (float)e0i, (float)e1r, (float)e1i};
+        __builtin_nontemporal_store(e, reinterpret_cast<v4f*>(g.e_out + src + c));
+      }
     }
   }
 }
@@ -417,7 +429,7 @@ __global__ __launch_bounds__(256) void k_combine_pairs(CombineArgs<C> g, Map mp)
 template <typename C, typename Map>
 inline void launch_combine(const CombineArgs<C>& g, const Map& mp, int blocks, hipStream_t s) {
   if constexpr (sizeof(C) == 8) {
-    if (mp.pairs() && !g.e_out) {
+    if (mp.pairs()) {
       switch (g.nterms) {
 #define ZH_COMBINE_CASE(N)                                                                   \
   case N:                                                                                    \
