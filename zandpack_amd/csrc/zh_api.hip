@@ -2280,7 +2280,7 @@ struct Engine : EngineBase {
     const int s0 = omit1 ? 1 : 0;
     // the one-pass Ψ kernel of stage s >= 1 forms stage s + 1's Ψ input (complex128; at
     // n = 1000, M = 4000: FP64 226 -> 235 steps/s, while the complex64 kernel, shorter and
-    // more latency-bound, lost more than the separate combination cost: 337 -> 322)
+    // more latency-bound, lost more than the separate combination cost: 349 -> 333)
     psi_nx = psi_fused() && !(overlap && nc > 0) && sizeof(cs) == 16;
     e6_sig = e6_ok();
     e6_psi = e6_sig;
