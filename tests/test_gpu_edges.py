@@ -100,6 +100,33 @@ def test_edge_attempt(native, case):
     prop.close()
 
 
+@pytest.mark.parametrize("case", list(CASES))
+def test_edge_attempt_mixed(native, case):
+    """The same attempt in mixed precision (odd orbital counts take the one-element
+    stage combination and error-norm kernels in complex64, E6 included), then a second
+    one that reuses k_7 (t += h): error norms within 1e-3, the new state within 2e-5
+    of the oracle's."""
+    from zandpack_amd.engine import Propagator
+    p = CASES[case]()
+    y0 = _rand_state(p, 11, scale=0.02)
+    say("edge attempt mixed", case, p.dims)
+    prop = Propagator(p, "mixed", state=y0)
+    t0, h = 0.2, 0.02
+    NL = p.dims[2]
+    ks, TE, J0, y1 = O.dop54_attempt(p.consts(), y0, t0, h, p.H0, dH_of(p), p.bias, NL)
+    r = prop.attempt(t0, h, 1.0, force_accept=True)
+    assert abs(r["te"] - TE) <= 1e-3 * TE, (r["te"], TE)
+    for g, w in zip(prop.get_state(), y1):
+        assert rel(g, w) < 2e-5
+    ks, TE2, J1, y2 = O.dop54_attempt(p.consts(), y1, t0 + h, h, p.H0, dH_of(p), p.bias, NL)
+    r = prop.attempt(t0 + h, h, 1.0, force_accept=True)
+    assert abs(r["te"] - TE2) <= 1e-3 * TE2, (r["te"], TE2)
+    if TE2 > 0:
+        for g, w in zip(prop.get_state(), y2):
+            assert rel(g, w) < 2e-5
+    prop.close()
+
+
 def test_uncoupled_is_von_neumann(native):
     """ξ = 0: σ(t) = U σ₀ U† with U = exp(-iHt/ħ), J = 0, Ψ and Ω stay zero."""
     from zandpack_amd.engine import Propagator
