@@ -1613,7 +1613,9 @@ struct Engine : EngineBase {
   // stream right after stage s - 1's σ branch, ahead of that stage's join)
   void combine(int s, bool first, double2* sig_dst, bool last, bool do_sig = true,
                bool do_psi = true) {
-    const int blocks = 2048;
+    // one workgroup per row up to 4096 (at n = 1000, M = 4000 the concurrent attempts ran
+    // 0.6% faster with 4096 than with 2048 for the complex64 Ψ rows; 8192 and 16384 the same)
+    auto blocks = [](long rows) { return (int)std::max(1L, std::min(4096L, rows)); };
     int nt = 0, slot[6];
     double coef[6];
     if (!first)
@@ -1636,7 +1638,7 @@ struct Engine : EngineBase {
           for (int j = 0; j < nt; ++j) gs.ect[j] = tab.ct[slot[j]];
         }
         MapDense ms{NK * n, n, n, n, n};
-        launch_combine(gs, ms, blocks, stream);
+        launch_combine(gs, ms, blocks((long)NK * n), stream);
       }
       if (!do_psi) return;
       CombineArgs<cs> gp{};
@@ -1651,7 +1653,7 @@ struct Engine : EngineBase {
         for (int j = 0; j < nt; ++j) gp.ect[j] = tab.ct[slot[j]];
       }
       MapDense mp{NK * Md, n, n, n + nc, n};
-      launch_combine(gp, mp, blocks, stream);
+      launch_combine(gp, mp, blocks((long)NK * Md), stream);
     });
     HIPCHK(hipGetLastError());
   }
