@@ -2022,9 +2022,13 @@ struct Engine : EngineBase {
     for (int i = 0; i < a.nitem; ++i) a.item[i] = xi_items[i];
     for (int l = 0; l < NL; ++l) a.lr[l] = lrange_h[l];
     const dim3 grid((Md + 63) / 64, a.nitem, NK * kWParts);
-    // 3 chunk buffers (at most 48 KB of LDS per workgroup), 3 workgroups per CU (both
-    // element types; measured at n = 1000, M = 4000 against 4 buffers at 2 per CU)
-    hipLaunchKernelGGL((k_xi_stream<cs, NG, 3, 3>), grid, dim3(256), 0, st, a);
+    // 4 chunk buffers (at most 64 KB of LDS per workgroup), 2 workgroups per CU (both
+    // element types).  At n = 1000, M = 4000 the product alone is faster with 3 buffers
+    // at 3 per CU (complex64 117 against 134 µs, complex128 187 against 212), but in the
+    // attempts, where it shares the CUs with the HBM-bound kernels of the other branches,
+    // 2 per CU gave 1.2% (mixed) and 0.6% (FP64) more steps/s; 1 per CU with 4 or 6
+    // buffers lost (FP64 225 against 245 steps/s)
+    hipLaunchKernelGGL((k_xi_stream<cs, NG, 4, 2>), grid, dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
   }
   // stream: the attempts' products may take the streaming kernel (the steady-state
